@@ -1,0 +1,234 @@
+/*
+ * libsteam_hip — C ABI of the MI355X (gfx950) field-generation hot path of
+ * STEAM: world-keyed Philox noise + CMS extremal-Levy variates, turbulon
+ * kernel synthesis, the periodic-x/y zero-padded-z FFT overlap-add
+ * convolution, the flux-cascade advance and the periodic trilinear regrid.
+ *
+ * Conventions
+ *   - Every pointer argument named *_dev is a device pointer.  Host pointers
+ *     are only the small descriptor structs and ledgers passed by address.
+ *   - Every call is asynchronous on the given hipStream_t (NULL = default
+ *     stream) unless its comment says it synchronises.
+ *   - Every call returns a status (STEAM_OK = 0).  steam_last_error() gives a
+ *     thread-local message for the last failing call on the calling thread.
+ *     The Python layer maps the codes to the reference's exception types:
+ *       STEAM_EINVAL   -> ValueError
+ *       STEAM_ENOMEM   -> MemoryError
+ *       STEAM_EDEVICE  -> RuntimeError
+ *       STEAM_ENOTIMPL -> NotImplementedError
+ *   - Arrays are C order (x slowest, z fastest), float32 unless stated.
+ *
+ * Reference interfaces replaced (paths relative to the reference repo):
+ *   steam/noise.py, steam/turbulons.py, steam/utils.py, steam/simulate.py,
+ *   steam/streaming.py — cited per entry point below.
+ */
+#ifndef STEAM_HIP_H
+#define STEAM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* steam_stream_t; /* == hipStream_t */
+
+#define STEAM_OK 0
+#define STEAM_EINVAL 1
+#define STEAM_ENOMEM 2
+#define STEAM_EDEVICE 3
+#define STEAM_ENOTIMPL 4
+
+/* Library version (major*10000 + minor*100 + patch). */
+int steam_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* steam_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * (a) keyed noise
+ * ---------------------------------------------------------------------- */
+
+/* Philox4x32-10 uniforms over an index box, both lanes in C order over
+ * (ix, iy, iz).  Replaces keyed_uniforms (steam/noise.py:103-130) with the
+ * counter/key layout of _uniform_pair (steam/noise.py:88-100).  ix/iy/iz are
+ * int64 device arrays of world indices; lane0/lane1 hold nx*ny*nz floats. */
+int steam_keyed_uniforms(const int64_t* ix_dev, int64_t nx,
+                         const int64_t* iy_dev, int64_t ny,
+                         const int64_t* iz_dev, int64_t nz, uint32_t key0,
+                         uint32_t key1, uint32_t key2, float* lane0_dev,
+                         float* lane1_dev, steam_stream_t stream);
+
+/* Raw Philox4x32-10 blocks: counters (n x 4 uint32, device), key (k0, k1);
+ * out receives n x 4 uint32.  The round function of _philox4x32_10
+ * (steam/noise.py:67-85), exposed for known-answer checks. */
+int steam_philox4x32_10(const uint32_t* ctr_dev, int64_t n, uint32_t key0,
+                        uint32_t key1, uint32_t* out_dev, steam_stream_t stream);
+
+/* CMS transform of two uniform lanes, in place on phi (R is consumed too).
+ * Replaces _levy_from_uniforms (steam/simulate.py:441-479). */
+int steam_levy_from_uniforms(double alpha, float* phi_dev, float* r_dev,
+                             int64_t n, steam_stream_t stream);
+
+/* Where an array sits in its class's world grid (steam/simulate.py:520-551)
+ * plus the sub-lattice factors. */
+typedef struct steam_region {
+  uint32_t key[3];        /* class key (noise.class_key) */
+  int64_t origin[3];      /* world index of local (0,0,0) */
+  int64_t world_shape[3]; /* world grid; x/y wrap modulo it, z never wraps */
+  int32_t sparsity[3];    /* (s_x, s_y, s_z) */
+} steam_region;
+
+/* Sparse keyed Levy field: draw at every centre (world index divisible by s,
+ * x/y wrapped first), 0 elsewhere.  Replaces _keyed_sparse_levy
+ * (steam/simulate.py:554-598); out is (nx, ny, nz). */
+int steam_keyed_sparse_levy(int64_t nx, int64_t ny, int64_t nz,
+                            const steam_region* region, double alpha,
+                            float* out_dev, steam_stream_t stream);
+
+/* Unit-mean multiplier noise exp(gamma)-1 (0 off-centre), the construction
+ * shared by _advance_flux (steam/simulate.py:2146-2161) and
+ * _multiplier_noise (steam/streaming.py:822-847).  per_class_scale and shift
+ * are the float32 values the reference casts. */
+typedef struct steam_noise_params {
+  steam_region region;
+  double alpha;
+  float per_class_scale;
+  float shift;
+  int32_t n_zero;
+  int32_t zero_bottom;
+  int32_t zero_top;
+} steam_noise_params;
+
+int steam_multiplier_noise(int64_t nx, int64_t ny, int64_t nz,
+                           const steam_noise_params* noise, float* out_dev,
+                           steam_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * (b) turbulon kernel
+ * ---------------------------------------------------------------------- */
+
+#define STEAM_SHAPE_MEXICAN_HAT 0
+#define STEAM_SHAPE_MORLET_OMEGA0_6 1
+
+/* Half-widths ceil(support_factor*k/d) of the kernel (host-side, no GPU). */
+int steam_envelope_halfwidths(double k, double dx, double dy, double dz,
+                              double support_factor, int64_t half[3]);
+
+/* Centred turbulon kernel of shape (2hx+1, 2hy+1, 2hz+1) with the
+ * Gaussian-weighted zero-sum correction; the two correction sums are taken
+ * in float64.  Replaces _turbulon_envelope (steam/simulate.py:2503-2556) and
+ * the shape registry (steam/turbulons.py:20-48).  Synchronises. */
+int steam_turbulon_envelope(double k, double dx, double dy, double dz,
+                            double support_factor, int shape_id,
+                            float* out_dev, steam_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * (c) FFT overlap-add convolution: periodic x/y, zero-padded z, centred
+ *     'same' output (steam/utils.py:222-242, 293-427).
+ * ---------------------------------------------------------------------- */
+
+typedef struct steam_conv_plan steam_conv_plan;
+
+#define STEAM_PREC_PLAIN 0 /* float32 FFT, float32 overlap-add */
+#define STEAM_PREC_KAHAN 1 /* float32 FFT, compensated overlap-add */
+
+/* Plan the operator for field (nx, ny, nz) and kernel (kx, ky, kz).  n_fft is
+ * the z transform length (even; 0 = let the planner choose).  A kernel wider
+ * than the field in x/y is folded as fold_kernel_to_field does. */
+int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz, int64_t kx,
+                           int64_t ky, int64_t kz, int64_t n_fft,
+                           int precision, steam_conv_plan** plan);
+int steam_conv_plan_destroy(steam_conv_plan* plan);
+
+typedef struct steam_conv_info {
+  int64_t n_fft;           /* z transform length */
+  int64_t n_blocks;        /* overlap-add blocks along z */
+  int64_t block_len;       /* input planes per block */
+  int64_t workspace_bytes; /* caller-allocated scratch for execute */
+  int64_t spectrum_bytes;  /* caller-allocated kernel spectrum */
+} steam_conv_info;
+int steam_conv_plan_info(const steam_conv_plan* plan, steam_conv_info* info);
+
+/* Kernel spectrum of kernel_dev (kx, ky, kz) into spectrum_dev. */
+int steam_conv_set_kernel(const steam_conv_plan* plan, const float* kernel_dev,
+                          void* spectrum_dev, void* workspace_dev,
+                          steam_stream_t stream);
+
+/* out = field (*) kernel.  in_dev and out_dev may alias. */
+int steam_conv_execute(const steam_conv_plan* plan, const float* in_dev,
+                       float* out_dev, const void* spectrum_dev,
+                       void* workspace_dev, steam_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * (d) flux-cascade advance (steam/simulate.py:2066-2228)
+ * ---------------------------------------------------------------------- */
+
+/* The realized means of one advance as additive (total, count) pairs, plus
+ * the clip count (steam/ledger.py:170-202). */
+typedef struct steam_flux_ledger {
+  double entering_total;
+  int64_t entering_count;
+  double noise_abs_total;
+  int64_t noise_abs_count;
+  double realized_total;
+  int64_t realized_count;
+  int64_t n_clipped;
+  float rescale;   /* float32(entering_mean / realized_mean) */
+  int32_t flatten; /* 1 when the realized mean was 0 (flux set flat) */
+} steam_flux_ledger;
+
+#define STEAM_MEASURE 0 /* take every reduction, fill *ledger */
+#define STEAM_APPLY 1   /* inject the reductions from *ledger */
+
+/* One fused class advance of flux (nx, ny, nz) in place:
+ *   f = sigma_in * flux                 (lazily applied previous rescale)
+ *   noise = multiplier noise            (regenerated from counters)
+ *   flux  = max(f + kernel (*) (noise f), 0)
+ *   flux *= rescale                      (unless defer_rescale)
+ * window = {x0, x1, y0, y1} restricts the volume means and the noise mean to
+ * the domain (steam/simulate.py:2052-2063); NULL = whole array.
+ * sk_dev (optional) receives S_k = noise f / <|noise|>.  With defer_rescale
+ * the flux is left unscaled and the factor is only reported in *ledger (the
+ * caller folds it into the next read).  Synchronises iff ledger_out != NULL
+ * in measure mode (the ledger is copied back to the host). */
+int steam_flux_advance(const steam_conv_plan* plan, const void* spectrum_dev,
+                       void* workspace_dev, float* flux_dev, float sigma_in,
+                       const steam_noise_params* noise, const int64_t* window,
+                       int mode, steam_flux_ledger* ledger, float* sk_dev,
+                       int defer_rescale, steam_stream_t stream);
+
+/* Device-side copy of the last advance's ledger (inside workspace_dev). */
+int steam_flux_ledger_read(const steam_conv_plan* plan,
+                           const void* workspace_dev, steam_flux_ledger* out,
+                           steam_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * (d) regrid: periodic trilinear upsample (steam/utils.py:435-534) and its
+ *     tiled window form (steam/streaming.py:132-242).
+ * ---------------------------------------------------------------------- */
+
+/* dst[x, y, z] for target cells [x0, x0+wx) x [y0, y0+wy) x [0, tz) of the
+ * zoom of src (sx, sy, sz) to (tx, ty, tz), each source value first
+ * multiplied by float32 scale.  periodic = {px, py, pz}.  dst is (wx, wy, tz).
+ */
+int steam_regrid_trilinear(const float* src_dev, int64_t sx, int64_t sy,
+                           int64_t sz, int64_t tx, int64_t ty, int64_t tz,
+                           int64_t x0, int64_t wx, int64_t y0, int64_t wy,
+                           const int32_t* periodic, float scale,
+                           float* dst_dev, steam_stream_t stream);
+
+/* Elementwise flux *= scale (or = flat when flatten) over n values. */
+int steam_scale_inplace(float* data_dev, int64_t n, float scale,
+                        steam_stream_t stream);
+
+/* Deterministic float64 sum of a float32 array over a window of its first
+ * two axes (x0,x1,y0,y1; NULL = whole): total and element count. */
+int steam_sum_f64(const float* data_dev, int64_t nx, int64_t ny, int64_t nz,
+                  const int64_t* window, double* total_out,
+                  steam_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STEAM_HIP_H */

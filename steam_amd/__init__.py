@@ -1,0 +1,28 @@
+"""steam_amd — the STEAM field-generation hot path on AMD MI355X (gfx950).
+
+World-keyed Philox4x32-10 noise with CMS extremal-Levy variates, turbulon
+kernel synthesis, the periodic-x/y zero-padded-z FFT overlap-add convolution
+and the multiplicative flux-cascade advance, as hand-written HIP kernels
+(libsteam_hip.so) behind the reference's Python API:
+
+    steam_amd.noise       keyed_uniforms, class_key, center_indices
+    steam_amd.turbulons   SHAPES, mexican_hat, morlet_omega0_6
+    steam_amd.utils       convolve_fft_xy_oa_z, fold_kernel_to_field,
+                          zoom_trilinear
+    steam_amd.simulate    CONVOLVE, NoiseRegion, _keyed_sparse_levy,
+                          _levy_from_uniforms, _turbulon_envelope,
+                          _compute_all_grids, _advance_flux,
+                          simulate_flux_only
+    steam_amd.ledger      MeanReduction, FluxAdvanceLedger, RunLedger
+    steam_amd.streaming   tile geometry, tiled regrid, tiled flux cascade
+    steam_amd.batch       many uneven realisations per launch
+    steam_amd.serve       continuous-batching tile server
+"""
+
+__version__ = "0.1.0"
+
+from . import _lib  # noqa: F401  (binds torch's HIP runtime first)
+
+
+def library_version():
+    return _lib.load().steam_version()

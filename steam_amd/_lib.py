@@ -1,0 +1,157 @@
+"""ctypes binding of libsteam_hip.so (the C ABI declared in include/steam_hip.h).
+
+The library is loaded after torch so that both share one HIP runtime (torch
+ships its own libamdhip64.so.7 and the dynamic linker reuses it for ours).
+There is no fallback: a missing library or a missing GPU raises.
+"""
+
+import ctypes
+import os
+import threading
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsteam_hip.so")
+
+STEAM_OK = 0
+STEAM_EINVAL = 1
+STEAM_ENOMEM = 2
+STEAM_EDEVICE = 3
+STEAM_ENOTIMPL = 4
+
+PREC_PLAIN = 0
+PREC_KAHAN = 1
+MEASURE = 0
+APPLY = 1
+SHAPE_IDS = {'mexican_hat': 0, 'morlet_omega0_6': 1}
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_U32 = ctypes.c_uint32
+_I32 = ctypes.c_int32
+_F32 = ctypes.c_float
+_F64 = ctypes.c_double
+
+
+class Region(ctypes.Structure):
+    _fields_ = [("key", _U32 * 3), ("origin", _I64 * 3),
+                ("world_shape", _I64 * 3), ("sparsity", _I32 * 3)]
+
+
+class NoiseParams(ctypes.Structure):
+    _fields_ = [("region", Region), ("alpha", _F64),
+                ("per_class_scale", _F32), ("shift", _F32),
+                ("n_zero", _I32), ("zero_bottom", _I32), ("zero_top", _I32)]
+
+
+class ConvInfo(ctypes.Structure):
+    _fields_ = [("n_fft", _I64), ("n_blocks", _I64), ("block_len", _I64),
+                ("workspace_bytes", _I64), ("spectrum_bytes", _I64)]
+
+
+class FluxLedger(ctypes.Structure):
+    _fields_ = [("entering_total", _F64), ("entering_count", _I64),
+                ("noise_abs_total", _F64), ("noise_abs_count", _I64),
+                ("realized_total", _F64), ("realized_count", _I64),
+                ("n_clipped", _I64), ("rescale", _F32), ("flatten", _I32)]
+
+
+# (name, restype, argtypes)
+_SIGNATURES = [
+    ("steam_version", ctypes.c_int, []),
+    ("steam_last_error", ctypes.c_char_p, []),
+    ("steam_keyed_uniforms", ctypes.c_int,
+     [_P, _I64, _P, _I64, _P, _I64, _U32, _U32, _U32, _P, _P, _P]),
+    ("steam_philox4x32_10", ctypes.c_int, [_P, _I64, _U32, _U32, _P, _P]),
+    ("steam_levy_from_uniforms", ctypes.c_int, [_F64, _P, _P, _I64, _P]),
+    ("steam_keyed_sparse_levy", ctypes.c_int,
+     [_I64, _I64, _I64, ctypes.POINTER(Region), _F64, _P, _P]),
+    ("steam_multiplier_noise", ctypes.c_int,
+     [_I64, _I64, _I64, ctypes.POINTER(NoiseParams), _P, _P]),
+    ("steam_envelope_halfwidths", ctypes.c_int,
+     [_F64, _F64, _F64, _F64, _F64, ctypes.POINTER(_I64)]),
+    ("steam_turbulon_envelope", ctypes.c_int,
+     [_F64, _F64, _F64, _F64, _F64, ctypes.c_int, _P, _P]),
+    ("steam_conv_plan_create", ctypes.c_int,
+     [_I64, _I64, _I64, _I64, _I64, _I64, _I64, ctypes.c_int,
+      ctypes.POINTER(_P)]),
+    ("steam_conv_plan_destroy", ctypes.c_int, [_P]),
+    ("steam_conv_plan_info", ctypes.c_int, [_P, ctypes.POINTER(ConvInfo)]),
+    ("steam_conv_set_kernel", ctypes.c_int, [_P, _P, _P, _P, _P]),
+    ("steam_conv_execute", ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    ("steam_flux_advance", ctypes.c_int,
+     [_P, _P, _P, _P, _F32, ctypes.POINTER(NoiseParams), ctypes.POINTER(_I64),
+      ctypes.c_int, ctypes.POINTER(FluxLedger), _P, ctypes.c_int, _P]),
+    ("steam_flux_ledger_read", ctypes.c_int,
+     [_P, _P, ctypes.POINTER(FluxLedger), _P]),
+    ("steam_regrid_trilinear", ctypes.c_int,
+     [_P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64,
+      ctypes.POINTER(_I32), _F32, _P, _P]),
+    ("steam_scale_inplace", ctypes.c_int, [_P, _I64, _F32, _P]),
+    ("steam_sum_f64", ctypes.c_int,
+     [_P, _I64, _I64, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_F64), _P]),
+]
+
+EXPORTED = tuple(name for name, _, _ in _SIGNATURES)
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load():
+    """The bound library (loads on first use; raises if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"steam_amd: {LIB_PATH} is missing. Build it first "
+                f"(python steam_amd/csrc/build.py or __graft_entry__.build()); "
+                f"there is no CPU fallback.")
+        lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        for name, res, args in _SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+_EXC = {STEAM_EINVAL: ValueError, STEAM_ENOMEM: MemoryError,
+        STEAM_EDEVICE: RuntimeError, STEAM_ENOTIMPL: NotImplementedError}
+
+
+def check(rc, what=""):
+    """Map a status code to the reference's exception types."""
+    if rc == STEAM_OK:
+        return
+    msg = load().steam_last_error().decode(errors="replace")
+    raise _EXC.get(rc, RuntimeError)(f"{what}: {msg}" if what else msg)
+
+
+def require_gpu(device='cuda'):
+    """Validate a device request: the hot path runs on the MI355X only."""
+    dev = torch.device(device) if not isinstance(device, torch.device) else device
+    if dev.type != 'cuda':
+        raise ValueError(
+            f"steam_amd runs the field-generation hot path on the GPU only; "
+            f"got device={device!r}. Pass device='cuda'.")
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            "steam_amd: device='cuda' requested but torch.cuda is unavailable "
+            "(no ROCm GPU visible); there is no CPU fallback.")
+    return dev
+
+
+def stream_handle(stream=None):
+    """hipStream_t of a torch stream (default: the current stream)."""
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None

@@ -1,0 +1,62 @@
+"""Build libsteam_hip.so for gfx950 with hipcc (in-tree, no JIT cache).
+
+Usage: python steam_amd/csrc/build.py [-j N] [--resource-usage]
+"""
+
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+OBJ = os.path.join(HERE, "_build")
+TARGET = os.path.join(PKG, "libsteam_hip.so")
+SOURCES = ["util.hip", "noise.hip", "envelope.hip", "regrid.hip", "conv.hip",
+           "batch.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def _compile(src, extra):
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(HERE, src), "-o", obj]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    return src, obj, res
+
+
+def build(jobs=4, resource_usage=False, verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
+    sources = [s for s in SOURCES if os.path.exists(os.path.join(HERE, s))]
+    objs = []
+    failed = []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as pool:
+        for src, obj, res in pool.map(lambda s: _compile(s, extra), sources):
+            if verbose or resource_usage or res.returncode:
+                sys.stderr.write(res.stderr)
+            if res.returncode:
+                failed.append(src)
+            objs.append(obj)
+    if failed:
+        raise RuntimeError(f"hipcc failed for {failed}")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o",
+           TARGET + ".tmp"]
+    subprocess.run(cmd, check=True)
+    os.replace(TARGET + ".tmp", TARGET)
+    return TARGET
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", type=int, default=4)
+    ap.add_argument("--resource-usage", action="store_true")
+    ap.add_argument("-v", action="store_true")
+    a = ap.parse_args()
+    print(build(a.j, a.resource_usage, a.v))

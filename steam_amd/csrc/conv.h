@@ -1,0 +1,120 @@
+// Descriptors shared by the convolution / flux-advance passes and their
+// batched launches.
+#pragma once
+
+#include "fft.h"
+
+namespace steam {
+
+// Geometry of one convolution operator instance.
+struct ConvGeom {
+  int nx, ny, nz;       // field
+  int kx, ky, kz;       // kernel after folding (kx <= nx, ky <= ny)
+  int offx, offy;       // kernel placement: kp[x] gathers taps i = x+off mod nx
+  int cz;               // (kz-1)/2: 'same' trim offset along z
+  int nfft, n2, nzc;    // z transform: real length, complex half, kept bins
+  int L, nblocks;       // overlap-add block length and count along z
+  int Tz, logTz;        // lines per workgroup in the z passes
+  int Ty;               // lines per workgroup in the y pass
+  int Tx, logTx;        // lines per workgroup in the x pass
+  int nyt_z, nyt_x;     // y-tiles per x line (z passes), per plane (x pass)
+  int wg_z, wg_y, wg_x; // workgroups per pass
+  AxisPlan ax, ay, az;
+  const float2* wr2c;   // n2+1 entries exp(-2 pi i k / nfft)
+};
+
+// Keyed multiplier-noise parameters as the kernels use them.
+struct NoiseDev {
+  uint32_t key0, key1, key2;
+  int64_t ox, oy, oz;       // world origin of local (0,0,0)
+  int64_t wx, wy;           // world extent (x/y wrap)
+  int sx, sy, sz;           // sparsity
+  CmsConst cms;
+  float pcs, shift;         // per-class scale, shift (float32)
+  int n_zero, zero_bottom, zero_top;
+};
+
+// Device-side ledger written by the finalize kernel.
+struct DevLedger {
+  double entering_total, entering_count;
+  double noise_abs_total, noise_abs_count;
+  double realized_total, realized_count;
+  double n_clipped;
+  float rescale;
+  int flatten;
+  float entering_mean_f;
+  float inv_mean_abs;
+  int have_mean_abs;
+  int pad;
+};
+
+enum PassMode { MODE_PLAIN = 0, MODE_FLUX = 1, MODE_KERNEL = 2 };
+
+struct PassDesc {
+  ConvGeom g;
+  float2* S;            // spectral workspace [nblocks][nzc][nx][ny]
+  const float2* K;      // kernel spectrum [nzc][nx][ny]
+  const float* in;      // plain input / kernel taps
+  float* out;           // plain output / flux state (in place)
+  int mode;
+  int kahan;
+  // flux advance
+  float sigma_in;
+  int apply;            // 1: reductions injected from ledger_in
+  NoiseDev noise;
+  int win[4];           // x0, x1, y0, y1
+  float* sk;            // optional S_k output
+  double* part_z;       // [wg_z][3]: entering, |noise|, nonzero count
+  double* part_o;       // [wg_z][2]: realized, clipped count
+  DevLedger* led;
+  DevLedger led_in;     // apply mode values
+  float kscale;         // kernel-spectrum normalisation (MODE_KERNEL)
+  int wg_base[6];       // batched: first workgroup of z, y, x, zinv, post,
+                        // finalize passes
+};
+
+}  // namespace steam
+
+namespace steam {
+
+// Per-line part of the keyed multiplier noise: the wrapped world x/y of the
+// line and whether it lies on the x/y centre sub-lattice.
+struct LineNoise {
+  int64_t wx, wy;
+  int xy_center;
+  int zphase;  // world z of local z=0, modulo s_z
+};
+
+__device__ __forceinline__ LineNoise noise_line(const NoiseDev& nd, int64_t x,
+                                               int64_t y) {
+  LineNoise ln;
+  ln.wx = pmod(x + nd.ox, nd.wx);
+  ln.wy = pmod(y + nd.oy, nd.wy);
+  ln.xy_center = (ln.wx % nd.sx == 0) && (ln.wy % nd.sy == 0);
+  ln.zphase = (int)pmod(nd.oz, nd.sz);
+  return ln;
+}
+
+// Unit-mean multiplier noise exp(gamma)-1 at local level z of a line of an
+// array with nz levels; 0 off-centre and in the zeroed boundary levels.
+// Follows steam/simulate.py:2146-2161 (zero levels, scale, off-centre mask,
+// shift, expm1) on the keyed draw of steam/simulate.py:554-598 (centres are
+// world sites divisible by s after the x/y wrap; z never wraps).
+__device__ __forceinline__ float noise_at(const NoiseDev& nd,
+                                         const LineNoise& ln, int z, int nz) {
+  if (!ln.xy_center) return 0.f;
+  if (nd.n_zero > 0) {
+    if (nd.zero_bottom && z < nd.n_zero) return 0.f;
+    if (nd.zero_top && z >= nz - nd.n_zero) return 0.f;
+  }
+  if (nd.sz > 1 && (z + ln.zphase) % nd.sz != 0) return 0.f;
+  float u0, u1;
+  uniform_pair(ln.wx, ln.wy, (int64_t)z + nd.oz, nd.key0, nd.key1, nd.key2, u0,
+               u1);
+  float gam = cms_f32(u0, u1, nd.cms);
+  float g = __fmul_rn(gam, nd.pcs);
+  if (g == 0.f) return 0.f;
+  return expm1f(__fsub_rn(g, nd.shift));
+}
+
+}  // namespace steam

@@ -1,0 +1,906 @@
+// FFT overlap-add convolution (periodic x/y, zero-padded z) and the fused
+// flux-cascade advance, for gfx950.
+//
+// Operator (steam/utils.py:293-427): out = field (*) kernel with circular
+// x/y over the whole extent, linear z with zero padding, kernel centred at
+// ((kx-1)/2, (ky-1)/2, (kz-1)/2) and folded onto the field when wider than
+// it (steam/utils.py:222-242).  Overlap-add runs along z over blocks of
+// L = nfft - kz + 1 input planes; x and y keep their exact periods.
+//
+// Five passes per operator application, all through LDS:
+//   zfwd  real z-lines -> half-length complex FFT -> r2c split -> planes
+//         S[block][kz][x][y] (the flux advance generates the keyed noise,
+//         multiplies it by the entering flux and takes its partial sums here)
+//   ypass contiguous y-lines, forward
+//   xpass x-lines (T-wide column tiles): forward, * kernel spectrum, inverse
+//   ypass inverse
+//   zinv  c2r per block, overlap-add of all blocks in LDS, one coalesced
+//         write per output line (the flux advance adds the increment, clips
+//         and takes the realized partial sums here)
+// Partial sums are per workgroup and reduced in a fixed order by one
+// workgroup, so results do not depend on launch composition.
+#include <math.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "conv.h"
+
+namespace steam {
+
+// ---------------------------------------------------------------------------
+// descriptor sources: one instance by value, or a batch in device memory
+// ---------------------------------------------------------------------------
+struct SingleSrc {
+  PassDesc d;
+  __device__ __forceinline__ const PassDesc& get(int, int&) const { return d; }
+};
+
+struct BatchSrc {
+  const PassDesc* d;
+  int n;
+  __device__ __forceinline__ const PassDesc& get(int pass, int& wg) const {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (d[mid].wg_base[pass] <= wg)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    wg -= d[lo].wg_base[pass];
+    return d[lo];
+  }
+};
+
+enum { PASS_Z = 0, PASS_Y = 1, PASS_X = 2, PASS_ZINV = 3, PASS_POST = 4,
+       PASS_FIN = 5 };
+
+// workgroup sum of up to 3 doubles, result valid in thread 0 (fixed order)
+template <int NV>
+__device__ void block_sum(double (&v)[NV], double* red) {
+  const int tid = threadIdx.x;
+  for (int i = 0; i < NV; ++i) red[i * FFT_NTH + tid] = v[i];
+  __syncthreads();
+  for (int s = FFT_NTH / 2; s > 0; s >>= 1) {
+    if (tid < s)
+      for (int i = 0; i < NV; ++i)
+        red[i * FFT_NTH + tid] += red[i * FFT_NTH + tid + s];
+    __syncthreads();
+  }
+  if (tid == 0)
+    for (int i = 0; i < NV; ++i) v[i] = red[i * FFT_NTH];
+}
+
+// kp[x, y, z] of the rolled, folded kernel: taps i = x + offx (mod nx),
+// j = y + offy (mod ny), summed x-fold first then y-fold.
+__device__ __forceinline__ float kernel_gather(const PassDesc& d, int x, int y,
+                                               int z) {
+  const ConvGeom& g = d.g;
+  int i0 = (int)pmod((int64_t)x + g.offx, g.nx);
+  int j0 = (int)pmod((int64_t)y + g.offy, g.ny);
+  float acc = 0.f;
+  for (int j = j0; j < g.ky; j += g.ny) {
+    float s = 0.f;
+    for (int i = i0; i < g.kx; i += g.nx) s += d.in[((int64_t)i * g.ky + j) * g.kz + z];
+    acc += s;
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// zfwd: real z-lines -> spectral planes
+// ---------------------------------------------------------------------------
+template <class Src, int MODE, int RS>
+__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void zfwd_kernel(Src src) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_Z, wg);
+  const ConvGeom& g = d.g;
+  extern __shared__ float2 smem[];
+  const int tid = threadIdx.x;
+  const int x = wg / g.nyt_z;
+  const int y0 = (wg - x * g.nyt_z) * g.Tz;
+  const int ldt = g.n2 + 1;
+  float* sr = reinterpret_cast<float*>(smem);
+  const int nzin = (MODE == MODE_KERNEL) ? g.kz : g.nz;
+  const int nblk = (MODE == MODE_KERNEL) ? 1 : g.nblocks;
+  const int L = (MODE == MODE_KERNEL) ? g.nfft : g.L;
+  double acc[3] = {0.0, 0.0, 0.0};
+  bool xin = false;
+  if (MODE == MODE_FLUX) xin = x >= d.win[0] && x < d.win[1];
+
+  for (int b = 0; b < nblk; ++b) {
+    const int z0 = b * L;
+    for (int t = 0; t < g.Tz; ++t) {
+      const int y = y0 + t;
+      const bool yv = y < g.ny;
+      const int64_t line = ((int64_t)x * g.ny + y) * nzin;
+      LineNoise ln;
+      bool win = false;
+      if (MODE == MODE_FLUX && yv) {
+        ln = noise_line(d.noise, x, y);
+        win = xin && y >= d.win[2] && y < d.win[3];
+      }
+      for (int m = tid; m < g.nfft; m += FFT_NTH) {
+        const int z = z0 + m;
+        float v = 0.f;
+        if (yv && m < L && z < nzin) {
+          if (MODE == MODE_PLAIN) {
+            v = d.in[line + z];
+          } else if (MODE == MODE_FLUX) {
+            float f = __fmul_rn(d.sigma_in, d.out[line + z]);
+            float nse = noise_at(d.noise, ln, z, g.nz);
+            float p = __fmul_rn(nse, f);
+            if (win) {
+              acc[0] += (double)f;
+              acc[1] += (double)fabsf(nse);
+              acc[2] += (nse != 0.f) ? 1.0 : 0.0;
+            }
+            if (d.sk) d.sk[line + z] = p;
+            v = p;
+          } else {
+            v = kernel_gather(d, x, y, z);
+          }
+        }
+        sr[t * 2 * ldt + m] = v;
+      }
+    }
+    __syncthreads();
+    fft_lines<false, RS>(smem, g.az, g.Tz, g.logTz, 1, ldt, false, tid);
+    // r2c split: X2[k] = (Z[k] + conj Z[n2-k]) - i W^k (Z[k] - conj Z[n2-k])
+    const int nh = g.n2 / 2 + 1;
+    for (int e = tid; e < g.Tz * nh; e += FFT_NTH) {
+      const int t = e / nh;
+      const int k = e - t * nh;
+      float2* Lp = smem + t * ldt;
+      if (k == 0) {
+        float2 z0v = Lp[0];
+        Lp[0] = make_float2(2.f * (z0v.x + z0v.y), 0.f);
+        Lp[g.n2] = make_float2(2.f * (z0v.x - z0v.y), 0.f);
+      } else {
+        const int km = g.n2 - k;
+        float2 zk = Lp[k], zm = Lp[km];
+        float2 w = g.wr2c[k];
+        float2 A = make_float2(zk.x + zm.x, zk.y - zm.y);
+        float2 B = make_float2(zk.x - zm.x, zk.y + zm.y);
+        float2 wB = cmul(w, B);
+        Lp[k] = make_float2(A.x + wB.y, A.y - wB.x);
+        if (km != k) {
+          float2 A2 = make_float2(zm.x + zk.x, zm.y - zk.y);
+          float2 B2 = make_float2(zm.x - zk.x, zm.y + zk.y);
+          float2 wB2 = cmul(make_float2(-w.x, w.y), B2);
+          Lp[km] = make_float2(A2.x + wB2.y, A2.y - wB2.x);
+        }
+      }
+    }
+    __syncthreads();
+    float2* Sb = d.S + (int64_t)b * g.nzc * g.nx * g.ny;
+    for (int e = tid; e < g.nzc * g.Tz; e += FFT_NTH) {
+      const int kz = e >> g.logTz;
+      const int t = e & (g.Tz - 1);
+      const int y = y0 + t;
+      if (y < g.ny) Sb[((int64_t)kz * g.nx + x) * g.ny + y] = smem[t * ldt + kz];
+    }
+    __syncthreads();
+  }
+  if (MODE == MODE_FLUX && !d.apply) {
+    __shared__ double red[3 * FFT_NTH];
+    block_sum<3>(acc, red);
+    if (tid == 0) {
+      d.part_z[wg * 3 + 0] = acc[0];
+      d.part_z[wg * 3 + 1] = acc[1];
+      d.part_z[wg * 3 + 2] = acc[2];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ypass: contiguous y-lines (Ty per workgroup), in place
+// ---------------------------------------------------------------------------
+template <class Src, int RS>
+__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void ypass_kernel(Src src, int inv) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_Y, wg);
+  const ConvGeom& g = d.g;
+  extern __shared__ float2 smem[];
+  const int tid = threadIdx.x;
+  const int P = (d.mode == MODE_KERNEL) ? g.nzc : g.nblocks * g.nzc;
+  const int64_t lines = (int64_t)P * g.nx;
+  const int64_t l0 = (int64_t)wg * g.Ty;
+  const int ny = g.ny;
+  for (int t = 0; t < g.Ty; ++t) {
+    const int64_t l = l0 + t;
+    const float2* srcl = d.S + l * ny;
+    for (int n = tid; n < ny; n += FFT_NTH)
+      smem[t * ny + n] = (l < lines) ? srcl[n] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lines<false, RS>(smem, g.ay, g.Ty, 0, 1, ny, inv != 0, tid);
+  for (int t = 0; t < g.Ty; ++t) {
+    const int64_t l = l0 + t;
+    if (l >= lines) break;
+    float2* dstl = d.S + l * ny;
+    for (int n = tid; n < ny; n += FFT_NTH) dstl[n] = smem[t * ny + n];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// xpass: x-lines in column tiles of Tx; XMODE 0: fwd * K inv, 1: fwd * scale
+// ---------------------------------------------------------------------------
+template <class Src, int XMODE, int RS>
+__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void xpass_kernel(Src src) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_X, wg);
+  const ConvGeom& g = d.g;
+  extern __shared__ float2 smem[];
+  const int tid = threadIdx.x;
+  const int p = wg / g.nyt_x;
+  const int y0 = (wg - p * g.nyt_x) * g.Tx;
+  const int nx = g.nx, ny = g.ny, Tx = g.Tx, logTx = g.logTx;
+  float2* Sp = d.S + (int64_t)p * nx * ny;
+  const int ne = nx * Tx;
+  for (int e = tid; e < ne; e += FFT_NTH) {
+    const int t = e & (Tx - 1);
+    const int n = e >> logTx;
+    const int y = y0 + t;
+    smem[e] = (y < ny) ? Sp[(int64_t)n * ny + y] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lines<true, RS>(smem, g.ax, Tx, logTx, Tx, 1, false, tid);
+  if (XMODE == 0) {
+    const int kzp = p % g.nzc;
+    const float2* Kp = d.K + (int64_t)kzp * nx * ny;
+    for (int e = tid; e < ne; e += FFT_NTH) {
+      const int t = e & (Tx - 1);
+      const int n = e >> logTx;
+      const int y = y0 + t;
+      if (y < ny) smem[e] = cmul(smem[e], Kp[(int64_t)n * ny + y]);
+    }
+    __syncthreads();
+    fft_lines<true, RS>(smem, g.ax, Tx, logTx, Tx, 1, true, tid);
+  } else {
+    for (int e = tid; e < ne; e += FFT_NTH) smem[e] = cscale(smem[e], d.kscale);
+    __syncthreads();
+  }
+  for (int e = tid; e < ne; e += FFT_NTH) {
+    const int t = e & (Tx - 1);
+    const int n = e >> logTx;
+    const int y = y0 + t;
+    if (y < ny) Sp[(int64_t)n * ny + y] = smem[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// zinv: planes -> c2r per block -> overlap-add in LDS -> epilogue
+// ---------------------------------------------------------------------------
+template <class Src, int MODE, int RS>
+__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void zinv_kernel(Src src) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_ZINV, wg);
+  const ConvGeom& g = d.g;
+  extern __shared__ float2 smem[];
+  const int tid = threadIdx.x;
+  const int x = wg / g.nyt_z;
+  const int y0 = (wg - x * g.nyt_z) * g.Tz;
+  const int ldt = g.n2 + 1;
+  const int nz = g.nz;
+  float* sr = reinterpret_cast<float*>(smem);
+  float* accz = reinterpret_cast<float*>(smem + g.Tz * ldt);
+  float* comp = accz + g.Tz * nz;
+  const bool kahan = d.kahan != 0;
+  for (int e = tid; e < g.Tz * nz; e += FFT_NTH) {
+    accz[e] = 0.f;
+    if (kahan) comp[e] = 0.f;
+  }
+  for (int b = 0; b < g.nblocks; ++b) {
+    const float2* Sb = d.S + (int64_t)b * g.nzc * g.nx * g.ny;
+    for (int e = tid; e < g.nzc * g.Tz; e += FFT_NTH) {
+      const int kz = e >> g.logTz;
+      const int t = e & (g.Tz - 1);
+      const int y = y0 + t;
+      smem[t * ldt + kz] = (y < g.ny) ? Sb[((int64_t)kz * g.nx + x) * g.ny + y]
+                                      : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    // c2r merge: Z[k] = (Y[k] + conj Y[n2-k]) + i conj(W^k) (Y[k] - conj Y[n2-k])
+    const int nh = g.n2 / 2 + 1;
+    for (int e = tid; e < g.Tz * nh; e += FFT_NTH) {
+      const int t = e / nh;
+      const int k = e - t * nh;
+      float2* Lp = smem + t * ldt;
+      if (k == 0) {
+        float2 Y0 = Lp[0], YN = Lp[g.n2];
+        float2 A = make_float2(Y0.x + YN.x, Y0.y - YN.y);
+        float2 B = make_float2(Y0.x - YN.x, Y0.y + YN.y);
+        Lp[0] = make_float2(A.x - B.y, A.y + B.x);
+      } else {
+        const int km = g.n2 - k;
+        float2 Yk = Lp[k], Ym = Lp[km];
+        float2 w = g.wr2c[k];
+        float2 A = make_float2(Yk.x + Ym.x, Yk.y - Ym.y);
+        float2 B = make_float2(Yk.x - Ym.x, Yk.y + Ym.y);
+        float2 cB = cmul(make_float2(w.x, -w.y), B);  // conj(W^k) B
+        Lp[k] = make_float2(A.x - cB.y, A.y + cB.x);  // A + i cB
+        if (km != k) {
+          float2 A2 = make_float2(Ym.x + Yk.x, Ym.y - Yk.y);
+          float2 B2 = make_float2(Ym.x - Yk.x, Ym.y + Yk.y);
+          float2 cB2 = cmul(make_float2(-w.x, -w.y), B2);  // conj(W^{n2-k}) = -W^k
+          Lp[km] = make_float2(A2.x - cB2.y, A2.y + cB2.x);
+        }
+      }
+    }
+    __syncthreads();
+    fft_lines<false, RS>(smem, g.az, g.Tz, g.logTz, 1, ldt, true, tid);
+    const int zb = b * g.L - g.cz;
+    for (int t = 0; t < g.Tz; ++t) {
+      for (int m = tid; m < g.nfft; m += FFT_NTH) {
+        const int zo = zb + m;
+        if (zo >= 0 && zo < nz) {
+          const float v = sr[t * 2 * ldt + m];
+          const int ia = t * nz + zo;
+          if (!kahan) {
+            accz[ia] += v;
+          } else {
+            float s = accz[ia];
+            float yv = v - comp[ia];
+            float tt = s + yv;
+            comp[ia] = (tt - s) - yv;
+            accz[ia] = tt;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double acc[2] = {0.0, 0.0};
+  const bool xin = (MODE == MODE_FLUX) && x >= d.win[0] && x < d.win[1];
+  for (int t = 0; t < g.Tz; ++t) {
+    const int y = y0 + t;
+    if (y >= g.ny) break;
+    const int64_t line = ((int64_t)x * g.ny + y) * nz;
+    if (MODE == MODE_PLAIN) {
+      for (int z = tid; z < nz; z += FFT_NTH) d.out[line + z] = accz[t * nz + z];
+    } else {
+      const bool win = xin && y >= d.win[2] && y < d.win[3];
+      for (int z = tid; z < nz; z += FFT_NTH) {
+        float f = __fmul_rn(d.sigma_in, d.out[line + z]);
+        f = __fadd_rn(f, accz[t * nz + z]);
+        if (f < 0.f) {
+          acc[1] += 1.0;
+          f = 0.f;
+        }
+        if (win) acc[0] += (double)f;
+        d.out[line + z] = f;
+      }
+    }
+  }
+  if (MODE == MODE_FLUX && !d.apply) {
+    __shared__ double red[2 * FFT_NTH];
+    block_sum<2>(acc, red);
+    if (tid == 0) {
+      d.part_o[wg * 2 + 0] = acc[0];
+      d.part_o[wg * 2 + 1] = acc[1];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// finalize: fixed-order reduction of the partials into the ledger
+// ---------------------------------------------------------------------------
+template <class Src>
+__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void finalize_kernel(Src src) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_FIN, wg);
+  const ConvGeom& g = d.g;
+  const int tid = threadIdx.x;
+  __shared__ double red[5 * FFT_NTH];
+  DevLedger L;
+  if (!d.apply) {
+    double v[5] = {0, 0, 0, 0, 0};
+    for (int i = tid; i < g.wg_z; i += FFT_NTH) {
+      v[0] += d.part_z[i * 3 + 0];
+      v[1] += d.part_z[i * 3 + 1];
+      v[2] += d.part_z[i * 3 + 2];
+      v[3] += d.part_o[i * 2 + 0];
+      v[4] += d.part_o[i * 2 + 1];
+    }
+    block_sum<5>(v, red);
+    if (tid != 0) return;
+    const double cnt = (double)(d.win[1] - d.win[0]) * (double)(d.win[3] - d.win[2]) *
+                       (double)g.nz;
+    L.entering_total = v[0];
+    L.entering_count = cnt;
+    L.noise_abs_total = v[1];
+    L.noise_abs_count = v[2];
+    L.realized_total = v[3];
+    L.realized_count = cnt;
+    L.n_clipped = v[4];
+  } else {
+    if (tid != 0) return;
+    L = d.led_in;
+  }
+  const double em = L.entering_total / fmax(L.entering_count, 1.0);
+  const double rm = L.realized_total / fmax(L.realized_count, 1.0);
+  const double ma = L.noise_abs_total / fmax(L.noise_abs_count, 1.0);
+  if (rm > 0.0) {
+    L.rescale = (float)(em / rm);
+    L.flatten = 0;
+  } else {
+    L.rescale = 1.0f;
+    L.flatten = 1;
+  }
+  L.entering_mean_f = (float)em;
+  L.have_mean_abs = ma > 0.0;
+  L.inv_mean_abs = ma > 0.0 ? (float)(1.0 / ma) : 1.0f;
+  L.pad = 0;
+  *d.led = L;
+}
+
+// ---------------------------------------------------------------------------
+// post: apply the rescale to the flux and the 1/<|noise|> scale to S_k
+// ---------------------------------------------------------------------------
+template <class Src>
+__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void post_kernel(Src src, int do_rescale) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_POST, wg);
+  const ConvGeom& g = d.g;
+  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+  const DevLedger* L = d.led;
+  const float rs = L->rescale;
+  const int flat = L->flatten;
+  const float em = L->entering_mean_f;
+  const float inv = L->inv_mean_abs;
+  const int have = L->have_mean_abs;
+  const int64_t i0 = ((int64_t)wg * FFT_NTH + threadIdx.x) * 4;
+  for (int q = 0; q < 4; ++q) {
+    const int64_t i = i0 + q;
+    if (i >= n) break;
+    if (do_rescale) d.out[i] = flat ? em : __fmul_rn(d.out[i], rs);
+    if (d.sk && have) d.sk[i] = __fmul_rn(d.sk[i], inv);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side: planning
+// ---------------------------------------------------------------------------
+static bool is_smooth7(int64_t n) {
+  for (int p : {2, 3, 5, 7})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+static void factor_axis(int n, AxisPlan& ap) {
+  ap.n = n;
+  ap.nst = 0;
+  if (n <= 1) return;
+  if (!is_smooth7(n)) {
+    ap.radix[ap.nst++] = 0;
+    return;
+  }
+  int m = n;
+  while (m % 8 == 0 && ap.nst < FFT_MAXST) { ap.radix[ap.nst++] = 8; m /= 8; }
+  while (m % 4 == 0 && ap.nst < FFT_MAXST) { ap.radix[ap.nst++] = 4; m /= 4; }
+  while (m % 2 == 0 && ap.nst < FFT_MAXST) { ap.radix[ap.nst++] = 2; m /= 2; }
+  for (int p : {3, 5, 7})
+    while (m % p == 0 && ap.nst < FFT_MAXST) { ap.radix[ap.nst++] = p; m /= p; }
+}
+
+static int floor_pow2(int64_t v) {
+  int p = 1;
+  while ((int64_t)p * 2 <= v) p *= 2;
+  return p;
+}
+static int ilog2(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
+}  // namespace steam
+
+using namespace steam;
+
+struct steam_conv_plan {
+  ConvGeom g;        // field operator
+  ConvGeom gk;       // kernel-spectrum build (one block)
+  int precision;
+  float2* tw_dev;    // twiddle tables
+  int64_t nx, ny, nz, kx, ky, kz;
+  int64_t s_bytes, pz_off, po_off, led_off, ws_bytes, spec_bytes;
+  size_t lds_z, lds_zinv, lds_y, lds_x;
+  int wg_post;
+};
+
+static const int64_t kMaxAxis = FFT_LDSE;  // longest x/y line one workgroup holds
+
+static int64_t align256(int64_t v) { return (v + 255) & ~(int64_t)255; }
+
+static void set_tiles(ConvGeom& g, int nz_acc, int kahan, size_t& lds_z,
+                      size_t& lds_zinv, size_t& lds_y, size_t& lds_x) {
+  const int ldt = g.n2 + 1;
+  int Tz = std::min(16, floor_pow2(std::max<int64_t>(1, FFT_LDSE / ldt)));
+  Tz = std::min(Tz, 1 << ilog2(g.ny));
+  while (Tz > 1 && (size_t)Tz * ldt * 8 + (size_t)Tz * nz_acc * 4 * (1 + kahan) >
+                       64 * 1024)
+    Tz /= 2;
+  g.Tz = Tz;
+  g.logTz = ilog2(Tz);
+  g.Ty = std::max(1, std::min(16, floor_pow2(FFT_LDSE / g.ny)));
+  int Tx = std::min(16, floor_pow2(std::max<int64_t>(1, FFT_LDSE / g.nx)));
+  Tx = std::min(Tx, 1 << ilog2(g.ny));
+  g.Tx = std::max(1, Tx);
+  g.logTx = ilog2(g.Tx);
+  g.nyt_z = (g.ny + g.Tz - 1) / g.Tz;
+  g.nyt_x = (g.ny + g.Tx - 1) / g.Tx;
+  const int64_t P = (int64_t)g.nblocks * g.nzc;
+  g.wg_z = g.nx * g.nyt_z;
+  g.wg_y = (int)((P * g.nx + g.Ty - 1) / g.Ty);
+  g.wg_x = (int)(P * g.nyt_x);
+  lds_z = (size_t)g.Tz * ldt * 8;
+  lds_zinv = lds_z + (size_t)g.Tz * nz_acc * 4 * (1 + kahan);
+  lds_y = (size_t)g.Ty * g.ny * 8;
+  lds_x = (size_t)g.nx * g.Tx * 8;
+}
+
+// choose the z transform length: minimise spectral planes nblocks*(n/2+1)
+static int64_t choose_nfft(int64_t nz, int64_t kz) {
+  int64_t best = -1, best_cost = INT64_MAX;
+  int64_t lo = std::max<int64_t>(2, kz);
+  int64_t hi = std::max<int64_t>(lo, 2 * (nz + kz));
+  for (int64_t n = lo; n <= hi && n / 2 + 1 <= FFT_LDSE; ++n) {
+    if (n % 2 || !is_smooth7(n / 2)) continue;
+    int64_t L = n - kz + 1;
+    if (L < 1) continue;
+    int64_t nb = (nz + L - 1) / L;
+    int64_t cost = nb * (n / 2 + 1);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = n;
+    }
+  }
+  return best;
+}
+
+extern "C" int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz,
+                                      int64_t kx, int64_t ky, int64_t kz,
+                                      int64_t n_fft, int precision,
+                                      steam_conv_plan** out) {
+  if (!out) return set_error(STEAM_EINVAL, "plan pointer is NULL");
+  *out = nullptr;
+  if (nx < 1 || ny < 1 || nz < 1 || kx < 1 || ky < 1 || kz < 1)
+    return set_error(STEAM_EINVAL, "conv plan: all sizes must be positive");
+  if (nx > kMaxAxis || ny > kMaxAxis)
+    return set_error(STEAM_ENOTIMPL,
+                     "conv plan: x/y extents above 4096 cells are not supported "
+                     "by one workgroup's LDS line; tile the domain");
+  if (precision != STEAM_PREC_PLAIN && precision != STEAM_PREC_KAHAN)
+    return set_error(STEAM_EINVAL, "conv plan: unknown precision");
+  if (n_fft == 0) n_fft = choose_nfft(nz, kz);
+  if (n_fft < 2 || n_fft % 2 || n_fft < kz || n_fft / 2 + 1 > FFT_LDSE)
+    return set_error(STEAM_EINVAL,
+                     "conv plan: n_fft must be even, >= kz and <= 8190");
+  if (nz > 16384) return set_error(STEAM_ENOTIMPL, "conv plan: nz above 16384");
+
+  steam_conv_plan* p = new steam_conv_plan();
+  p->nx = nx; p->ny = ny; p->nz = nz; p->kx = kx; p->ky = ky; p->kz = kz;
+  p->precision = precision;
+  ConvGeom& g = p->g;
+  memset(&g, 0, sizeof(g));
+  g.nx = (int)nx; g.ny = (int)ny; g.nz = (int)nz;
+  g.kx = (int)kx; g.ky = (int)ky; g.kz = (int)kz;
+  // kernel placement (fold_kernel_to_field then roll by -(k'-1)/2)
+  g.offx = (kx <= nx) ? (int)((kx - 1) / 2)
+                      : (int)(kx / 2 + (nx - 1) / 2 - nx / 2);
+  g.offy = (ky <= ny) ? (int)((ky - 1) / 2)
+                      : (int)(ky / 2 + (ny - 1) / 2 - ny / 2);
+  g.cz = (int)((kz - 1) / 2);
+  g.nfft = (int)n_fft;
+  g.n2 = (int)(n_fft / 2);
+  g.nzc = g.n2 + 1;
+  g.L = (int)(n_fft - kz + 1);
+  g.nblocks = (int)((nz + g.L - 1) / g.L);
+  factor_axis((int)nx, g.ax);
+  factor_axis((int)ny, g.ay);
+  factor_axis(g.n2, g.az);
+  const int kah = precision == STEAM_PREC_KAHAN ? 1 : 0;
+  set_tiles(g, (int)nz, kah, p->lds_z, p->lds_zinv, p->lds_y, p->lds_x);
+  if (p->lds_zinv > 64 * 1024) {
+    delete p;
+    return set_error(STEAM_ENOTIMPL, "conv plan: z line too long for LDS");
+  }
+  // kernel build geometry: one block covering kz taps
+  p->gk = g;
+  p->gk.nblocks = 1;
+  p->gk.L = g.nfft;
+  {
+    size_t a, b, c, d2;
+    set_tiles(p->gk, 1, 0, a, b, c, d2);
+  }
+
+  // twiddles: [x: nx][y: ny][z: n2][r2c: n2+1]
+  const int64_t ntw = nx + ny + g.n2 + g.n2 + 1;
+  std::vector<float2> tw(ntw);
+  auto fill = [&](int64_t off, int64_t n, int64_t period) {
+    for (int64_t m = 0; m < n; ++m) {
+      double a = -2.0 * M_PI * (double)m / (double)period;
+      tw[off + m] = make_float2((float)cos(a), (float)sin(a));
+    }
+  };
+  fill(0, nx, nx);
+  fill(nx, ny, ny);
+  fill(nx + ny, g.n2, g.n2);
+  fill(nx + ny + g.n2, g.n2 + 1, g.nfft);
+  hipError_t e = hipMalloc(&p->tw_dev, ntw * sizeof(float2));
+  if (e != hipSuccess) {
+    delete p;
+    return check_hip(e, "hipMalloc(twiddles)");
+  }
+  e = hipMemcpy(p->tw_dev, tw.data(), ntw * sizeof(float2), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(p->tw_dev);
+    delete p;
+    return check_hip(e, "hipMemcpy(twiddles)");
+  }
+  g.ax.tw = p->tw_dev;
+  g.ay.tw = p->tw_dev + nx;
+  g.az.tw = p->tw_dev + nx + ny;
+  g.wr2c = p->tw_dev + nx + ny + g.n2;
+  p->gk.ax.tw = g.ax.tw;
+  p->gk.ay.tw = g.ay.tw;
+  p->gk.az.tw = g.az.tw;
+  p->gk.wr2c = g.wr2c;
+
+  p->s_bytes = align256((int64_t)g.nblocks * g.nzc * nx * ny * 8);
+  p->pz_off = p->s_bytes;
+  p->po_off = p->pz_off + align256((int64_t)g.wg_z * 3 * 8);
+  p->led_off = p->po_off + align256((int64_t)g.wg_z * 2 * 8);
+  p->ws_bytes = p->led_off + align256(sizeof(DevLedger));
+  p->spec_bytes = align256((int64_t)g.nzc * nx * ny * 8);
+  p->wg_post = (int)((nx * ny * nz + 4 * FFT_NTH - 1) / (4 * FFT_NTH));
+  *out = p;
+  return STEAM_OK;
+}
+
+extern "C" int steam_conv_plan_destroy(steam_conv_plan* p) {
+  if (!p) return STEAM_OK;
+  if (p->tw_dev) (void)hipFree(p->tw_dev);
+  delete p;
+  return STEAM_OK;
+}
+
+extern "C" int steam_conv_plan_info(const steam_conv_plan* p,
+                                    steam_conv_info* info) {
+  if (!p || !info) return set_error(STEAM_EINVAL, "NULL plan or info");
+  info->n_fft = p->g.nfft;
+  info->n_blocks = p->g.nblocks;
+  info->block_len = p->g.L;
+  info->workspace_bytes = p->ws_bytes;
+  info->spectrum_bytes = p->spec_bytes;
+  return STEAM_OK;
+}
+
+namespace steam {
+
+static PassDesc base_desc(const steam_conv_plan* p, void* ws) {
+  PassDesc d;
+  memset(&d, 0, sizeof(d));
+  d.g = p->g;
+  char* w = (char*)ws;
+  d.S = (float2*)w;
+  d.part_z = (double*)(w + p->pz_off);
+  d.part_o = (double*)(w + p->po_off);
+  d.led = (DevLedger*)(w + p->led_off);
+  d.kahan = p->precision == STEAM_PREC_KAHAN;
+  d.win[0] = 0;
+  d.win[1] = (int)p->nx;
+  d.win[2] = 0;
+  d.win[3] = (int)p->ny;
+  d.sigma_in = 1.0f;
+  return d;
+}
+
+#define STEAM_RS_DISPATCH(rs, ...)          \
+  switch (rs) {                             \
+    case 0: { constexpr int RS_ = 0; __VA_ARGS__; } break; \
+    case 1: { constexpr int RS_ = 1; __VA_ARGS__; } break; \
+    default: { constexpr int RS_ = 2; __VA_ARGS__; } break; \
+  }
+
+template <int MODE>
+static int run_operator(const steam_conv_plan* p, const PassDesc& d,
+                        hipStream_t s) {
+  SingleSrc src{d};
+  const ConvGeom& g = d.g;
+  const int rsx = radix_set(g.ax), rsy = radix_set(g.ay), rsz = radix_set(g.az);
+  STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zfwd_kernel<SingleSrc, MODE, RS_>),
+                                            dim3(g.wg_z), dim3(FFT_NTH), p->lds_z, s, src));
+  STEAM_LAUNCH_CHECK("zfwd_kernel");
+  STEAM_RS_DISPATCH(rsy, hipLaunchKernelGGL((ypass_kernel<SingleSrc, RS_>), dim3(g.wg_y),
+                                            dim3(FFT_NTH), p->lds_y, s, src, 0));
+  STEAM_LAUNCH_CHECK("ypass_kernel");
+  STEAM_RS_DISPATCH(rsx, hipLaunchKernelGGL((xpass_kernel<SingleSrc, 0, RS_>), dim3(g.wg_x),
+                                            dim3(FFT_NTH), p->lds_x, s, src));
+  STEAM_LAUNCH_CHECK("xpass_kernel");
+  STEAM_RS_DISPATCH(rsy, hipLaunchKernelGGL((ypass_kernel<SingleSrc, RS_>), dim3(g.wg_y),
+                                            dim3(FFT_NTH), p->lds_y, s, src, 1));
+  STEAM_LAUNCH_CHECK("ypass_kernel(inv)");
+  STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zinv_kernel<SingleSrc, MODE, RS_>),
+                                            dim3(g.wg_z), dim3(FFT_NTH), p->lds_zinv, s, src));
+  STEAM_LAUNCH_CHECK("zinv_kernel");
+  return STEAM_OK;
+}
+
+}  // namespace steam
+
+extern "C" int steam_conv_set_kernel(const steam_conv_plan* p,
+                                     const float* kernel_dev,
+                                     void* spectrum_dev, void* workspace_dev,
+                                     steam_stream_t stream) {
+  (void)workspace_dev;
+  if (!p || !kernel_dev || !spectrum_dev)
+    return set_error(STEAM_EINVAL, "steam_conv_set_kernel: NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  PassDesc d;
+  memset(&d, 0, sizeof(d));
+  d.g = p->gk;
+  d.mode = MODE_KERNEL;
+  d.S = (float2*)spectrum_dev;
+  d.in = kernel_dev;
+  d.kscale = (float)(1.0 / (4.0 * (double)p->g.nfft * (double)p->nx * (double)p->ny));
+  SingleSrc src{d};
+  const ConvGeom& g = d.g;
+  const int rsx = radix_set(g.ax), rsy = radix_set(g.ay), rsz = radix_set(g.az);
+  STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zfwd_kernel<SingleSrc, MODE_KERNEL, RS_>),
+                                            dim3(g.wg_z), dim3(FFT_NTH), p->lds_z, s, src));
+  STEAM_LAUNCH_CHECK("zfwd_kernel(kernel)");
+  STEAM_RS_DISPATCH(rsy, hipLaunchKernelGGL((ypass_kernel<SingleSrc, RS_>), dim3(g.wg_y),
+                                            dim3(FFT_NTH), p->lds_y, s, src, 0));
+  STEAM_LAUNCH_CHECK("ypass_kernel(kernel)");
+  STEAM_RS_DISPATCH(rsx, hipLaunchKernelGGL((xpass_kernel<SingleSrc, 1, RS_>), dim3(g.wg_x),
+                                            dim3(FFT_NTH), p->lds_x, s, src));
+  STEAM_LAUNCH_CHECK("xpass_kernel(kernel)");
+  return STEAM_OK;
+}
+
+extern "C" int steam_conv_execute(const steam_conv_plan* p, const float* in_dev,
+                                  float* out_dev, const void* spectrum_dev,
+                                  void* workspace_dev, steam_stream_t stream) {
+  if (!p || !in_dev || !out_dev || !spectrum_dev || !workspace_dev)
+    return set_error(STEAM_EINVAL, "steam_conv_execute: NULL argument");
+  PassDesc d = base_desc(p, workspace_dev);
+  d.mode = MODE_PLAIN;
+  d.K = (const float2*)spectrum_dev;
+  d.in = in_dev;
+  d.out = out_dev;
+  return run_operator<MODE_PLAIN>(p, d, (hipStream_t)stream);
+}
+
+namespace steam {
+
+NoiseDev make_noise_dev(const steam_noise_params* np) {
+  NoiseDev n;
+  memset(&n, 0, sizeof(n));
+  n.key0 = np->region.key[0];
+  n.key1 = np->region.key[1];
+  n.key2 = np->region.key[2];
+  n.ox = np->region.origin[0];
+  n.oy = np->region.origin[1];
+  n.oz = np->region.origin[2];
+  n.wx = np->region.world_shape[0];
+  n.wy = np->region.world_shape[1];
+  n.sx = np->region.sparsity[0];
+  n.sy = np->region.sparsity[1];
+  n.sz = np->region.sparsity[2];
+  n.cms = make_cms(np->alpha);
+  n.pcs = np->per_class_scale;
+  n.shift = np->shift;
+  n.n_zero = np->n_zero;
+  n.zero_bottom = np->zero_bottom;
+  n.zero_top = np->zero_top;
+  return n;
+}
+
+int check_noise_params(const steam_noise_params* np) {
+  if (!np) return set_error(STEAM_EINVAL, "noise params are NULL");
+  for (int a = 0; a < 3; ++a)
+    if (np->region.sparsity[a] < 1)
+      return set_error(STEAM_EINVAL, "sparsity factors must be positive");
+  if (np->region.world_shape[0] < 1 || np->region.world_shape[1] < 1)
+    return set_error(STEAM_EINVAL, "world shape must be positive in x and y");
+  if (!(np->alpha > 0.0 && np->alpha <= 2.0))
+    return set_error(STEAM_EINVAL, "alpha must lie in (0, 2]");
+  return STEAM_OK;
+}
+
+static DevLedger ledger_to_dev(const steam_flux_ledger* h) {
+  DevLedger L;
+  memset(&L, 0, sizeof(L));
+  L.entering_total = h->entering_total;
+  L.entering_count = (double)h->entering_count;
+  L.noise_abs_total = h->noise_abs_total;
+  L.noise_abs_count = (double)h->noise_abs_count;
+  L.realized_total = h->realized_total;
+  L.realized_count = (double)h->realized_count;
+  L.n_clipped = (double)h->n_clipped;
+  return L;
+}
+
+static void ledger_to_host(const DevLedger& L, steam_flux_ledger* h) {
+  h->entering_total = L.entering_total;
+  h->entering_count = (int64_t)L.entering_count;
+  h->noise_abs_total = L.noise_abs_total;
+  h->noise_abs_count = (int64_t)L.noise_abs_count;
+  h->realized_total = L.realized_total;
+  h->realized_count = (int64_t)L.realized_count;
+  h->n_clipped = (int64_t)L.n_clipped;
+  h->rescale = L.rescale;
+  h->flatten = L.flatten;
+}
+
+}  // namespace steam
+
+extern "C" int steam_flux_advance(const steam_conv_plan* p,
+                                  const void* spectrum_dev, void* workspace_dev,
+                                  float* flux_dev, float sigma_in,
+                                  const steam_noise_params* noise,
+                                  const int64_t* window, int mode,
+                                  steam_flux_ledger* ledger, float* sk_dev,
+                                  int defer_rescale, steam_stream_t stream) {
+  if (!p || !spectrum_dev || !workspace_dev || !flux_dev)
+    return set_error(STEAM_EINVAL, "steam_flux_advance: NULL argument");
+  int rc = check_noise_params(noise);
+  if (rc) return rc;
+  if (mode != STEAM_MEASURE && mode != STEAM_APPLY)
+    return set_error(STEAM_EINVAL, "steam_flux_advance: unknown mode");
+  if (mode == STEAM_APPLY && !ledger)
+    return set_error(STEAM_EINVAL, "steam_flux_advance: apply mode needs a ledger");
+  hipStream_t s = (hipStream_t)stream;
+  PassDesc d = base_desc(p, workspace_dev);
+  d.mode = MODE_FLUX;
+  d.K = (const float2*)spectrum_dev;
+  d.out = flux_dev;
+  d.sigma_in = sigma_in;
+  d.noise = make_noise_dev(noise);
+  d.sk = sk_dev;
+  d.apply = mode == STEAM_APPLY;
+  if (window) {
+    if (window[0] < 0 || window[1] > p->nx || window[0] >= window[1] ||
+        window[2] < 0 || window[3] > p->ny || window[2] >= window[3])
+      return set_error(STEAM_EINVAL, "steam_flux_advance: window out of range");
+    for (int i = 0; i < 4; ++i) d.win[i] = (int)window[i];
+  }
+  if (d.apply) d.led_in = ledger_to_dev(ledger);
+  rc = run_operator<MODE_FLUX>(p, d, s);
+  if (rc) return rc;
+  SingleSrc src{d};
+  hipLaunchKernelGGL((finalize_kernel<SingleSrc>), dim3(1), dim3(FFT_NTH), 0, s, src);
+  STEAM_LAUNCH_CHECK("finalize_kernel");
+  if (!defer_rescale || sk_dev) {
+    hipLaunchKernelGGL((post_kernel<SingleSrc>), dim3(p->wg_post), dim3(FFT_NTH), 0,
+                       s, src, defer_rescale ? 0 : 1);
+    STEAM_LAUNCH_CHECK("post_kernel");
+  }
+  if (ledger && mode == STEAM_MEASURE) {
+    DevLedger L;
+    STEAM_HIP(hipMemcpyAsync(&L, d.led, sizeof(L), hipMemcpyDeviceToHost, s));
+    STEAM_HIP(hipStreamSynchronize(s));
+    ledger_to_host(L, ledger);
+  }
+  return STEAM_OK;
+}
+
+extern "C" int steam_flux_ledger_read(const steam_conv_plan* p,
+                                      const void* workspace_dev,
+                                      steam_flux_ledger* out,
+                                      steam_stream_t stream) {
+  if (!p || !workspace_dev || !out)
+    return set_error(STEAM_EINVAL, "steam_flux_ledger_read: NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  DevLedger L;
+  STEAM_HIP(hipMemcpyAsync(&L, (const char*)workspace_dev + p->led_off, sizeof(L),
+                           hipMemcpyDeviceToHost, s));
+  STEAM_HIP(hipStreamSynchronize(s));
+  ledger_to_host(L, out);
+  return STEAM_OK;
+}

@@ -1,0 +1,249 @@
+// LDS-resident multi-line Stockham FFT for gfx950 (64-wide wavefronts).
+//
+// A workgroup of NTH threads holds T lines of length N (N*T <= LDSE complex
+// values) in LDS and transforms all of them together.  Mixed radix
+// 2/3/4/5/7/8 stages (Stockham autosort, natural order in and out); a length
+// with a prime factor above 7 is transformed by one direct-DFT stage.  Each
+// stage stages its butterflies' inputs in registers, synchronises, and
+// writes the outputs back in place, so one LDS buffer suffices.
+//
+// Element (line t, index n) lives at buf[n*ldn + t*ldt].  TFAST selects the
+// thread mapping: consecutive threads take consecutive t (lines interleaved
+// in LDS, [n][t] layout) or consecutive n within a line ([t][n] layout);
+// either way consecutive lanes touch consecutive LDS words.
+#pragma once
+
+#include "common.h"
+
+namespace steam {
+
+constexpr int FFT_NTH = 256;              // threads per FFT workgroup
+constexpr int FFT_EMAX = 16;              // complex values per thread
+constexpr int FFT_LDSE = FFT_NTH * FFT_EMAX;  // 4096 complex = 32 KiB
+constexpr int FFT_MAXST = 14;
+
+struct AxisPlan {
+  int n;                   // transform length
+  int nst;                 // number of stages
+  int radix[FFT_MAXST];    // 2,3,4,5,7,8; 0 = direct DFT of the line
+  const float2* tw;        // n entries exp(-2 pi i m / n)
+};
+
+// constant rotations for the odd radices: cos/sin(2 pi m / R)
+__device__ __forceinline__ float2 rot_const(int R, int m) {
+  // tables for R = 3, 5, 7 (m < R)
+  const float c3[3] = {1.0f, -0.5f, -0.5f};
+  const float s3[3] = {0.0f, 0.86602540378443865f, -0.86602540378443865f};
+  const float c5[5] = {1.0f, 0.30901699437494742f, -0.80901699437494742f,
+                       -0.80901699437494742f, 0.30901699437494742f};
+  const float s5[5] = {0.0f, 0.95105651629515357f, 0.58778525229247313f,
+                       -0.58778525229247313f, -0.95105651629515357f};
+  const float c7[7] = {1.0f,
+                       0.62348980185873353f,
+                       -0.22252093395631440f,
+                       -0.90096886790241913f,
+                       -0.90096886790241913f,
+                       -0.22252093395631440f,
+                       0.62348980185873353f};
+  const float s7[7] = {0.0f,
+                       0.78183148246802981f,
+                       0.97492791218182361f,
+                       0.43388373911755812f,
+                       -0.43388373911755812f,
+                       -0.97492791218182361f,
+                       -0.78183148246802981f};
+  if (R == 3) return make_float2(c3[m], s3[m]);
+  if (R == 5) return make_float2(c5[m], s5[m]);
+  return make_float2(c7[m], s7[m]);
+}
+
+// In-register DFT of size R; forward uses exp(-2 pi i rq/R).
+template <int R>
+__device__ __forceinline__ void dft_small(float2* v, bool inv) {
+  if constexpr (R == 2) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]), t3 = csub(v[1], v[3]);
+    // forward: y1 = t1 - i t3, y3 = t1 + i t3
+    float2 mt3 = inv ? make_float2(-t3.y, t3.x) : make_float2(t3.y, -t3.x);
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, mt3);
+    v[3] = csub(t1, mt3);
+  } else if constexpr (R == 8) {
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    dft_small<4>(e, inv);
+    dft_small<4>(o, inv);
+    const float c = 0.70710678118654752f;
+    float sg = inv ? 1.0f : -1.0f;
+    // w^q = exp(sg * i pi q / 4)
+    float2 w1 = make_float2(c, sg * c);
+    float2 w3 = make_float2(-c, sg * c);
+    float2 o1 = cmul(o[1], w1);
+    float2 o2 = inv ? make_float2(-o[2].y, o[2].x) : make_float2(o[2].y, -o[2].x);
+    float2 o3 = cmul(o[3], w3);
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+  } else {
+    float2 y[R];
+    float sg = inv ? 1.0f : -1.0f;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      float2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = rot_const(R, (r * q) % R);
+        w.y *= sg;
+        acc = cadd(acc, cmul(v[r], w));
+      }
+      y[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = y[q];
+  }
+}
+
+template <bool TFAST>
+__device__ __forceinline__ void line_index(int b, int T, int logT, int NR,
+                                           int& t, int& j) {
+  if (TFAST) {
+    t = b & (T - 1);
+    j = b >> logT;
+  } else {
+    t = b / NR;
+    j = b - t * NR;
+  }
+}
+
+template <int R, bool TFAST>
+__device__ __noinline__ void fft_stage(float2* buf, int N, int T, int logT, int ldn,
+                          int ldt, int Ns, const float2* __restrict__ tw,
+                          bool inv, int tid) {
+  constexpr int MAXB = (FFT_EMAX + R - 1) / R;
+  const int NR = N / R;
+  const int nb = NR * T;
+  float2 v[MAXB][R];
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    int b = tid + i * FFT_NTH;
+    if (b < nb) {
+      int t, j;
+      line_index<TFAST>(b, T, logT, NR, t, j);
+      const float2* src = buf + j * ldn + t * ldt;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = src[r * NR * ldn];
+    }
+  }
+  __syncthreads();
+  const int step = N / (Ns * R);
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    int b = tid + i * FFT_NTH;
+    if (b < nb) {
+      int t, j;
+      line_index<TFAST>(b, T, logT, NR, t, j);
+      int k = j % Ns;
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[r * k * step];
+          if (inv) w.y = -w.y;
+          v[i][r] = cmul(v[i][r], w);
+        }
+      }
+      dft_small<R>(v[i], inv);
+      int base = (j - k) * R + k;
+      float2* dst = buf + base * ldn + t * ldt;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[r * Ns * ldn] = v[i][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Whole-line direct DFT (lengths with a prime factor > 7).
+template <bool TFAST>
+__device__ __noinline__ void fft_direct(float2* buf, int N, int T, int logT, int ldn,
+                           int ldt, const float2* __restrict__ tw, bool inv,
+                           int tid) {
+  float2 o[FFT_EMAX];
+  const int ne = N * T;
+#pragma unroll
+  for (int i = 0; i < FFT_EMAX; ++i) {
+    int e = tid + i * FFT_NTH;
+    if (e < ne) {
+      int t, q;
+      line_index<TFAST>(e, T, logT, N, t, q);
+      const float2* src = buf + t * ldt;
+      float2 acc = make_float2(0.f, 0.f);
+      int m = 0;
+      for (int n = 0; n < N; ++n) {
+        float2 w = tw[m];
+        if (inv) w.y = -w.y;
+        acc = cadd(acc, cmul(src[n * ldn], w));
+        m += q;
+        if (m >= N) m -= N;
+      }
+      o[i] = acc;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < FFT_EMAX; ++i) {
+    int e = tid + i * FFT_NTH;
+    if (e < ne) {
+      int t, q;
+      line_index<TFAST>(e, T, logT, N, t, q);
+      buf[q * ldn + t * ldt] = o[i];
+    }
+  }
+  __syncthreads();
+}
+
+// Radix sets a kernel is instantiated for: 0 = {2,4,8}, 1 = {2,3,4,5,8},
+// 2 = every stage kind.  Stages are separate (non-inlined) functions so each
+// gets its own register allocation.
+inline int radix_set(const AxisPlan& ap) {
+  int rs = 0;
+  for (int s = 0; s < ap.nst; ++s) {
+    int r = ap.radix[s];
+    if (r == 3 || r == 5) rs = rs < 1 ? 1 : rs;
+    if (r == 7 || r == 0) rs = 2;
+  }
+  return rs;
+}
+
+// Transform T lines of length ap.n held in LDS.  Caller synchronises before.
+template <bool TFAST, int RSET>
+__device__ void fft_lines(float2* buf, const AxisPlan& ap, int T, int logT,
+                          int ldn, int ldt, bool inv, int tid) {
+  int Ns = 1;
+  for (int s = 0; s < ap.nst; ++s) {
+    int R = ap.radix[s];
+    switch (R) {
+      case 2: fft_stage<2, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+      case 3: if (RSET >= 1) fft_stage<3, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+      case 4: fft_stage<4, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+      case 5: if (RSET >= 1) fft_stage<5, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+      case 7: if (RSET >= 2) fft_stage<7, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+      case 8: fft_stage<8, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+      default:
+        if (RSET >= 2) fft_direct<TFAST>(buf, ap.n, T, logT, ldn, ldt, ap.tw, inv, tid);
+        R = ap.n;
+        break;
+    }
+    Ns *= R;
+  }
+}
+
+}  // namespace steam
