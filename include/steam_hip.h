@@ -146,6 +146,7 @@ typedef struct steam_conv_info {
   int64_t block_len;       /* input planes per block */
   int64_t workspace_bytes; /* caller-allocated scratch for execute */
   int64_t spectrum_bytes;  /* caller-allocated kernel spectrum */
+  int64_t spectral_elems;  /* complex values per spectral pass */
 } steam_conv_info;
 int steam_conv_plan_info(const steam_conv_plan* plan, steam_conv_info* info);
 
@@ -190,11 +191,16 @@ typedef struct steam_flux_ledger {
  * sk_dev (optional) receives S_k = noise f / <|noise|>.  With defer_rescale
  * the flux is left unscaled and the factor is only reported in *ledger (the
  * caller folds it into the next read).  Synchronises iff ledger_out != NULL
- * in measure mode (the ledger is copied back to the host). */
+ * in measure mode (the ledger is copied back to the host).
+ * chain_pending = 1 takes the pending rescale (or the flatten of an
+ * all-clipped class) from the ledger the previous advance left in this
+ * workspace instead of sigma_in, so consecutive deferred advances on one
+ * flux need no host round trip. */
 int steam_flux_advance(const steam_conv_plan* plan, const void* spectrum_dev,
                        void* workspace_dev, float* flux_dev, float sigma_in,
-                       const steam_noise_params* noise, const int64_t* window,
-                       int mode, steam_flux_ledger* ledger, float* sk_dev,
+                       int chain_pending, const steam_noise_params* noise,
+                       const int64_t* window, int mode,
+                       steam_flux_ledger* ledger, float* sk_dev,
                        int defer_rescale, steam_stream_t stream);
 
 /* Device-side copy of the last advance's ledger (inside workspace_dev). */
@@ -226,6 +232,25 @@ int steam_scale_inplace(float* data_dev, int64_t n, float scale,
 int steam_sum_f64(const float* data_dev, int64_t nx, int64_t ny, int64_t nz,
                   const int64_t* window, double* total_out,
                   steam_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * instrumentation: per-kernel-family timing of this library's own launches,
+ * by HIP events recorded on each launch's stream.
+ * ---------------------------------------------------------------------- */
+
+#define STEAM_KF_ZFWD 0     /* z forward pass (noise + r2c) */
+#define STEAM_KF_YPASS 1    /* y passes (forward and inverse) */
+#define STEAM_KF_XPASS 2    /* x pass: forward, spectral multiply, inverse */
+#define STEAM_KF_ZINV 3     /* z inverse pass (c2r, overlap-add, epilogue) */
+#define STEAM_KF_FINALIZE 4 /* ledger reduction */
+#define STEAM_KF_POST 5     /* rescale / S_k scale */
+#define STEAM_KF_COUNT 6
+
+/* Start (on != 0) or stop recording; starting clears earlier records. */
+int steam_profile_enable(int on);
+/* Synchronise on the recorded events and report one family's launch count
+ * and summed duration in milliseconds. */
+int steam_profile_read(int family, int64_t* launches, double* total_ms);
 
 #ifdef __cplusplus
 }

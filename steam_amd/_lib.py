@@ -47,7 +47,8 @@ class NoiseParams(ctypes.Structure):
 
 class ConvInfo(ctypes.Structure):
     _fields_ = [("n_fft", _I64), ("n_blocks", _I64), ("block_len", _I64),
-                ("workspace_bytes", _I64), ("spectrum_bytes", _I64)]
+                ("workspace_bytes", _I64), ("spectrum_bytes", _I64),
+                ("spectral_elems", _I64)]
 
 
 class FluxLedger(ctypes.Structure):
@@ -81,8 +82,9 @@ _SIGNATURES = [
     ("steam_conv_set_kernel", ctypes.c_int, [_P, _P, _P, _P, _P]),
     ("steam_conv_execute", ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     ("steam_flux_advance", ctypes.c_int,
-     [_P, _P, _P, _P, _F32, ctypes.POINTER(NoiseParams), ctypes.POINTER(_I64),
-      ctypes.c_int, ctypes.POINTER(FluxLedger), _P, ctypes.c_int, _P]),
+     [_P, _P, _P, _P, _F32, ctypes.c_int, ctypes.POINTER(NoiseParams),
+      ctypes.POINTER(_I64), ctypes.c_int, ctypes.POINTER(FluxLedger), _P,
+      ctypes.c_int, _P]),
     ("steam_flux_ledger_read", ctypes.c_int,
      [_P, _P, ctypes.POINTER(FluxLedger), _P]),
     ("steam_regrid_trilinear", ctypes.c_int,
@@ -91,7 +93,13 @@ _SIGNATURES = [
     ("steam_scale_inplace", ctypes.c_int, [_P, _I64, _F32, _P]),
     ("steam_sum_f64", ctypes.c_int,
      [_P, _I64, _I64, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_F64), _P]),
+    ("steam_profile_enable", ctypes.c_int, [ctypes.c_int]),
+    ("steam_profile_read", ctypes.c_int,
+     [ctypes.c_int, ctypes.POINTER(_I64), ctypes.POINTER(_F64)]),
 ]
+
+KERNEL_FAMILIES = {'zfwd': 0, 'ypass': 1, 'xpass': 2, 'zinv': 3,
+                   'finalize': 4, 'post': 5}
 
 EXPORTED = tuple(name for name, _, _ in _SIGNATURES)
 

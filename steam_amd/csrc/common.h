@@ -31,6 +31,24 @@ int check_hip(hipError_t e, const char* what);
   } while (0)
 
 // ---------------------------------------------------------------------------
+// instrumentation (no-ops unless steam_profile_enable(1) was called)
+// ---------------------------------------------------------------------------
+extern bool g_profiling;
+void prof_begin(int family, hipStream_t s);
+void prof_end(int family, hipStream_t s);
+
+struct ProfScope {
+  int fam;
+  hipStream_t s;
+  ProfScope(int f, hipStream_t st) : fam(f), s(st) {
+    if (g_profiling) prof_begin(fam, s);
+  }
+  ~ProfScope() {
+    if (g_profiling) prof_end(fam, s);
+  }
+};
+
+// ---------------------------------------------------------------------------
 // Philox4x32-10 (steam/noise.py:48-85).  Counter (ix, iy, iz, key2), key
 // (key0, key1).  Words are carried the way the reference carries them: as
 // uint64 values whose products wrap mod 2^64.  Lattice indices in [0, 2^32)

@@ -60,6 +60,7 @@ struct PassDesc {
   int kahan;
   // flux advance
   float sigma_in;
+  const DevLedger* pend;  // chained: pending rescale of the previous advance
   int apply;            // 1: reductions injected from ledger_in
   NoiseDev noise;
   int win[4];           // x0, x1, y0, y1

@@ -73,6 +73,26 @@ __device__ void block_sum(double (&v)[NV], double* red) {
     for (int i = 0; i < NV; ++i) v[i] = red[i * FFT_NTH];
 }
 
+// The entering flux value of a stored cell: the pending rescale of the
+// previous advance (sigma_in, or chained from the ledger it left on the
+// device) applied on read, or the flat entering mean of an all-clipped class.
+struct Pending {
+  float sig, em;
+  int flat;
+};
+__device__ __forceinline__ Pending pending_of(const PassDesc& d) {
+  Pending p{d.sigma_in, 0.f, 0};
+  if (d.pend) {
+    p.flat = d.pend->flatten;
+    p.sig = d.pend->rescale;
+    p.em = d.pend->entering_mean_f;
+  }
+  return p;
+}
+__device__ __forceinline__ float entering_value(const Pending& p, float raw) {
+  return p.flat ? p.em : __fmul_rn(p.sig, raw);
+}
+
 // kp[x, y, z] of the rolled, folded kernel: taps i = x + offx (mod nx),
 // j = y + offy (mod ny), summed x-fold first then y-fold.
 __device__ __forceinline__ float kernel_gather(const PassDesc& d, int x, int y,
@@ -108,7 +128,11 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
   const int L = (MODE == MODE_KERNEL) ? g.nfft : g.L;
   double acc[3] = {0.0, 0.0, 0.0};
   bool xin = false;
-  if (MODE == MODE_FLUX) xin = x >= d.win[0] && x < d.win[1];
+  Pending pd{1.f, 0.f, 0};
+  if (MODE == MODE_FLUX) {
+    xin = x >= d.win[0] && x < d.win[1];
+    pd = pending_of(d);
+  }
 
   for (int b = 0; b < nblk; ++b) {
     const int z0 = b * L;
@@ -129,7 +153,7 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
           if (MODE == MODE_PLAIN) {
             v = d.in[line + z];
           } else if (MODE == MODE_FLUX) {
-            float f = __fmul_rn(d.sigma_in, d.out[line + z]);
+            float f = entering_value(pd, d.out[line + z]);
             float nse = noise_at(d.noise, ln, z, g.nz);
             float p = __fmul_rn(nse, f);
             if (win) {
@@ -355,6 +379,8 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
   }
   double acc[2] = {0.0, 0.0};
   const bool xin = (MODE == MODE_FLUX) && x >= d.win[0] && x < d.win[1];
+  Pending pd{1.f, 0.f, 0};
+  if (MODE == MODE_FLUX) pd = pending_of(d);
   for (int t = 0; t < g.Tz; ++t) {
     const int y = y0 + t;
     if (y >= g.ny) break;
@@ -364,7 +390,7 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
     } else {
       const bool win = xin && y >= d.win[2] && y < d.win[3];
       for (int z = tid; z < nz; z += FFT_NTH) {
-        float f = __fmul_rn(d.sigma_in, d.out[line + z]);
+        float f = entering_value(pd, d.out[line + z]);
         f = __fadd_rn(f, accz[t * nz + z]);
         if (f < 0.f) {
           acc[1] += 1.0;
@@ -677,6 +703,7 @@ extern "C" int steam_conv_plan_info(const steam_conv_plan* p,
   info->block_len = p->g.L;
   info->workspace_bytes = p->ws_bytes;
   info->spectrum_bytes = p->spec_bytes;
+  info->spectral_elems = (int64_t)p->g.nblocks * p->g.nzc * p->nx * p->ny;
   return STEAM_OK;
 }
 
@@ -713,20 +740,35 @@ static int run_operator(const steam_conv_plan* p, const PassDesc& d,
   SingleSrc src{d};
   const ConvGeom& g = d.g;
   const int rsx = radix_set(g.ax), rsy = radix_set(g.ay), rsz = radix_set(g.az);
-  STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zfwd_kernel<SingleSrc, MODE, RS_>),
-                                            dim3(g.wg_z), dim3(FFT_NTH), p->lds_z, s, src));
+  {
+    ProfScope ps(STEAM_KF_ZFWD, s);
+    STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zfwd_kernel<SingleSrc, MODE, RS_>),
+                                              dim3(g.wg_z), dim3(FFT_NTH), p->lds_z, s, src));
+  }
   STEAM_LAUNCH_CHECK("zfwd_kernel");
-  STEAM_RS_DISPATCH(rsy, hipLaunchKernelGGL((ypass_kernel<SingleSrc, RS_>), dim3(g.wg_y),
-                                            dim3(FFT_NTH), p->lds_y, s, src, 0));
+  {
+    ProfScope ps(STEAM_KF_YPASS, s);
+    STEAM_RS_DISPATCH(rsy, hipLaunchKernelGGL((ypass_kernel<SingleSrc, RS_>), dim3(g.wg_y),
+                                              dim3(FFT_NTH), p->lds_y, s, src, 0));
+  }
   STEAM_LAUNCH_CHECK("ypass_kernel");
-  STEAM_RS_DISPATCH(rsx, hipLaunchKernelGGL((xpass_kernel<SingleSrc, 0, RS_>), dim3(g.wg_x),
-                                            dim3(FFT_NTH), p->lds_x, s, src));
+  {
+    ProfScope ps(STEAM_KF_XPASS, s);
+    STEAM_RS_DISPATCH(rsx, hipLaunchKernelGGL((xpass_kernel<SingleSrc, 0, RS_>), dim3(g.wg_x),
+                                              dim3(FFT_NTH), p->lds_x, s, src));
+  }
   STEAM_LAUNCH_CHECK("xpass_kernel");
-  STEAM_RS_DISPATCH(rsy, hipLaunchKernelGGL((ypass_kernel<SingleSrc, RS_>), dim3(g.wg_y),
-                                            dim3(FFT_NTH), p->lds_y, s, src, 1));
+  {
+    ProfScope ps(STEAM_KF_YPASS, s);
+    STEAM_RS_DISPATCH(rsy, hipLaunchKernelGGL((ypass_kernel<SingleSrc, RS_>), dim3(g.wg_y),
+                                              dim3(FFT_NTH), p->lds_y, s, src, 1));
+  }
   STEAM_LAUNCH_CHECK("ypass_kernel(inv)");
-  STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zinv_kernel<SingleSrc, MODE, RS_>),
-                                            dim3(g.wg_z), dim3(FFT_NTH), p->lds_zinv, s, src));
+  {
+    ProfScope ps(STEAM_KF_ZINV, s);
+    STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zinv_kernel<SingleSrc, MODE, RS_>),
+                                              dim3(g.wg_z), dim3(FFT_NTH), p->lds_zinv, s, src));
+  }
   STEAM_LAUNCH_CHECK("zinv_kernel");
   return STEAM_OK;
 }
@@ -843,6 +885,7 @@ static void ledger_to_host(const DevLedger& L, steam_flux_ledger* h) {
 extern "C" int steam_flux_advance(const steam_conv_plan* p,
                                   const void* spectrum_dev, void* workspace_dev,
                                   float* flux_dev, float sigma_in,
+                                  int chain_pending,
                                   const steam_noise_params* noise,
                                   const int64_t* window, int mode,
                                   steam_flux_ledger* ledger, float* sk_dev,
@@ -861,6 +904,8 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
   d.K = (const float2*)spectrum_dev;
   d.out = flux_dev;
   d.sigma_in = sigma_in;
+  d.pend = chain_pending ? (const DevLedger*)((const char*)workspace_dev + p->led_off)
+                         : nullptr;
   d.noise = make_noise_dev(noise);
   d.sk = sk_dev;
   d.apply = mode == STEAM_APPLY;
@@ -874,9 +919,13 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
   rc = run_operator<MODE_FLUX>(p, d, s);
   if (rc) return rc;
   SingleSrc src{d};
-  hipLaunchKernelGGL((finalize_kernel<SingleSrc>), dim3(1), dim3(FFT_NTH), 0, s, src);
+  {
+    ProfScope ps(STEAM_KF_FINALIZE, s);
+    hipLaunchKernelGGL((finalize_kernel<SingleSrc>), dim3(1), dim3(FFT_NTH), 0, s, src);
+  }
   STEAM_LAUNCH_CHECK("finalize_kernel");
   if (!defer_rescale || sk_dev) {
+    ProfScope ps(STEAM_KF_POST, s);
     hipLaunchKernelGGL((post_kernel<SingleSrc>), dim3(p->wg_post), dim3(FFT_NTH), 0,
                        s, src, defer_rescale ? 0 : 1);
     STEAM_LAUNCH_CHECK("post_kernel");

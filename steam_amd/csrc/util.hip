@@ -3,9 +3,51 @@
 
 #include "common.h"
 
+#include <mutex>
+#include <vector>
+
 namespace steam {
 
 static thread_local std::string g_last_error;
+
+bool g_profiling = false;
+struct ProfRec {
+  int fam;
+  hipEvent_t a, b;
+};
+static std::mutex g_prof_mu;
+static std::vector<ProfRec> g_prof;
+static std::vector<hipEvent_t> g_pending;  // begin events awaiting their end
+
+void prof_begin(int fam, hipStream_t s) {
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  (void)hipEventRecord(e, s);
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_pending.push_back(e);
+  (void)fam;
+}
+
+void prof_end(int fam, hipStream_t s) {
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  (void)hipEventRecord(e, s);
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (g_pending.empty()) return;
+  hipEvent_t a = g_pending.back();
+  g_pending.pop_back();
+  g_prof.push_back({fam, a, e});
+}
+
+static void prof_clear() {
+  for (auto& r : g_prof) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  g_prof.clear();
+  for (auto e : g_pending) (void)hipEventDestroy(e);
+  g_pending.clear();
+}
 
 int set_error(int code, const std::string& msg) {
   g_last_error = msg;
@@ -65,6 +107,32 @@ __global__ __launch_bounds__(256) void part_sum_kernel(const double* part,
 }  // namespace steam
 
 using namespace steam;
+
+extern "C" int steam_profile_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (on) prof_clear();
+  g_profiling = on != 0;
+  return STEAM_OK;
+}
+
+extern "C" int steam_profile_read(int family, int64_t* launches,
+                                  double* total_ms) {
+  if (!launches || !total_ms) return set_error(STEAM_EINVAL, "profile_read: NULL");
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  int64_t n = 0;
+  double ms = 0.0;
+  for (auto& r : g_prof) {
+    if (r.fam != family) continue;
+    STEAM_HIP(hipEventSynchronize(r.b));
+    float t = 0.f;
+    STEAM_HIP(hipEventElapsedTime(&t, r.a, r.b));
+    ms += t;
+    ++n;
+  }
+  *launches = n;
+  *total_ms = ms;
+  return STEAM_OK;
+}
 
 extern "C" int steam_version(void) { return 1 * 10000 + 0 * 100 + 0; }
 

@@ -285,7 +285,7 @@ def advance_flux_device(flux_t, region, kernel_t, flux_noise_scale,
                         zero_bottom=False, zero_top=False, window=None,
                         ledger=None, alpha=None, sigma_in=1.0, sk_t=None,
                         defer_rescale=False, precision='plain', spec=None,
-                        op=None, sync_ledger=True):
+                        op=None, sync_ledger=True, chain_pending=False):
     """Fused class advance of a device-resident flux (float32 tensor, in
     place).  ``sigma_in`` is a pending float32 rescale applied on read;
     with ``defer_rescale`` the new rescale is left pending (returned in the
@@ -311,7 +311,8 @@ def advance_flux_device(flux_t, region, kernel_t, flux_noise_scale,
     want_host = ledger is None and sync_ledger
     _lib.check(lib.steam_flux_advance(
         op.plan, _lib.ptr(spec), _lib.ptr(ws), _lib.ptr(flux_t),
-        float(np.float32(sigma_in)), ctypes.byref(p), _native_window(window),
+        float(np.float32(sigma_in)), 1 if chain_pending else 0, ctypes.byref(p),
+        _native_window(window),
         mode, ctypes.byref(L) if (want_host or ledger is not None) else None,
         _lib.ptr(sk_t), 1 if defer_rescale else 0, _lib.stream_handle()),
         "steam_flux_advance")

@@ -1,0 +1,163 @@
+"""GPU parity of the FFT overlap-add convolution and the kernel synthesis /
+regrid against the float64 oracle."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import model as M
+
+pytestmark = pytest.mark.gpu
+
+# rel-L2 bound of the float32 FFT operator against the float64 oracle, and
+# the elementwise bound the reference's own test uses (rtol = atol = 1e-4).
+CONV_REL_L2 = 1e-6
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("fs,ks", [
+    ((8, 8, 8), (3, 3, 3)), ((16, 16, 10), (5, 5, 5)), ((12, 14, 8), (7, 5, 3)),
+    ((10, 10, 6), (3, 7, 5)), ((20, 20, 15), (9, 9, 7)), ((10, 12, 8), (5, 3, 7)),
+    ((4, 5, 8), (9, 7, 3)), ((4, 4, 6), (9, 9, 3)), ((13, 11, 9), (3, 5, 3)),
+    ((6, 7, 5), (4, 2, 4)), ((1, 8, 9), (3, 3, 3)), ((10, 12, 40), (5, 3, 7)),
+    ((16, 16, 60), (9, 9, 11)), ((96, 96, 48), (17, 17, 17)),
+    ((64, 64, 32), (17, 17, 17)), ((26, 22, 30), (13, 13, 13)),
+])
+def test_convolution_matches_oracle(fs, ks):
+    from steam_amd.utils import convolve_fft_xy_oa_z
+    rng = np.random.default_rng(42)
+    f = rng.standard_normal(fs).astype(np.float32)
+    k = rng.standard_normal(ks).astype(np.float32)
+    out = convolve_fft_xy_oa_z(f, k)
+    assert out.dtype == np.float32 and out.shape == fs
+    ref = M.conv_fft(f, k)
+    assert _rel(out, ref) < CONV_REL_L2
+    np.testing.assert_allclose(out, ref, rtol=1e-4, atol=1e-4)
+    if np.prod(fs) * np.prod(ks) < 5e6:
+        np.testing.assert_allclose(ref, M.conv_direct(f, k), rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("n_fft", [16, 24, 32, 64])
+def test_overlap_add_block_lengths_agree(n_fft):
+    """Several z blocks (including a last partial one) give the single-block
+    answer; the compensated overlap-add agrees too."""
+    from steam_amd.utils import ConvOperator
+    rng = np.random.default_rng(3)
+    f = torch.from_numpy(rng.standard_normal((12, 10, 50)).astype(np.float32)).cuda()
+    k = torch.from_numpy(rng.standard_normal((5, 5, 9)).astype(np.float32)).cuda()
+    ref = M.conv_fft(f.cpu().numpy(), k.cpu().numpy())
+    for prec in ('plain', 'kahan'):
+        op = ConvOperator(tuple(f.shape), tuple(k.shape), n_fft=n_fft, precision=prec)
+        assert op.n_blocks == -(-50 // (n_fft - 8))
+        out = torch.empty_like(f)
+        op.execute(f, out, op.spectrum(k))
+        assert _rel(out.cpu().numpy(), ref) < CONV_REL_L2
+
+
+def test_identity_zero_and_shift():
+    from steam_amd.utils import convolve_fft_xy_oa_z
+    rng = np.random.default_rng(30)
+    f = rng.standard_normal((10, 10, 8)).astype(np.float32)
+    k = np.zeros((3, 3, 3), np.float32)
+    k[1, 1, 1] = 1.0
+    np.testing.assert_allclose(convolve_fft_xy_oa_z(f, k), f, atol=1e-6)
+    np.testing.assert_array_equal(
+        convolve_fft_xy_oa_z(np.zeros((8, 8, 6), np.float32), k), 0.0)
+    k = rng.standard_normal((5, 5, 5)).astype(np.float32)
+    a = convolve_fft_xy_oa_z(np.roll(f, 3, axis=0), k)
+    np.testing.assert_allclose(a, np.roll(convolve_fft_xy_oa_z(f, k), 3, axis=0),
+                               atol=1e-5)
+    a = convolve_fft_xy_oa_z(np.roll(f, 5, axis=1), k)
+    np.testing.assert_allclose(a, np.roll(convolve_fft_xy_oa_z(f, k), 5, axis=1),
+                               atol=1e-5)
+
+
+def test_device_tensor_in_device_tensor_out():
+    from steam_amd.utils import convolve_fft_xy_oa_z
+    rng = np.random.default_rng(8)
+    f = torch.from_numpy(rng.standard_normal((16, 16, 20)).astype(np.float32)).cuda()
+    k = torch.from_numpy(rng.standard_normal((5, 5, 5)).astype(np.float32)).cuda()
+    out = convolve_fft_xy_oa_z(f, k)
+    assert isinstance(out, torch.Tensor) and out.is_cuda
+    assert _rel(out.cpu().numpy(), M.conv_fft(f.cpu().numpy(), k.cpu().numpy())) < CONV_REL_L2
+
+
+def test_cpu_device_request_raises():
+    from steam_amd.utils import convolve_fft_xy_oa_z
+    with pytest.raises(ValueError):
+        convolve_fft_xy_oa_z(np.zeros((4, 4, 4), np.float32),
+                             np.ones((1, 1, 1), np.float32), device='cpu')
+
+
+def test_fold_kernel_matches_oracle():
+    from steam_amd.utils import fold_kernel_to_field
+    rng = np.random.default_rng(70)
+    k = rng.standard_normal((9, 7, 5)).astype(np.float32)
+    np.testing.assert_allclose(fold_kernel_to_field(k, (4, 5, 8)),
+                               M.fold_kernel_to_field(k, (4, 5, 8)), atol=1e-6)
+
+
+@pytest.mark.parametrize("s,sf,shape", [(1, 3, 'mexican_hat'), (2, 3, 'mexican_hat'),
+                                        (1, 4, 'mexican_hat'), (1, 3, 'morlet_omega0_6'),
+                                        (4, 4, 'mexican_hat')])
+def test_envelope_matches_oracle(s, sf, shape):
+    from steam_amd.simulate import _turbulon_envelope
+    d = 1 / (2 * s)
+    e = _turbulon_envelope(1, d, d, d, support_factor=sf, shape=shape)
+    ref = M.turbulon_envelope(1, d, d, d, support_factor=sf, shape=shape,
+                              dtype=np.float64)
+    assert e.shape == ref.shape
+    np.testing.assert_allclose(e, ref, atol=2e-6)
+    assert abs(float(e.astype(np.float64).sum())) < 1e-4
+
+
+@pytest.mark.parametrize("sparsity,peak", [(1, 2.96781718), (2, 3.0)])
+def test_envelope_reference_constants(sparsity, peak):
+    from steam_amd.simulate import _turbulon_envelope
+    d = 500.0 / (2 * sparsity)
+    e = _turbulon_envelope(500.0, d, d, d, support_factor=5)
+    assert e.max() == pytest.approx(peak, abs=1e-5)
+    assert abs(float(e.sum())) < 1e-5
+    assert abs(float(e[0, 0, 0])) < 1e-20
+
+
+def test_unknown_shape_raises():
+    from steam_amd.simulate import _turbulon_envelope
+    with pytest.raises(ValueError):
+        _turbulon_envelope(1, .5, .5, .5, shape='box')
+
+
+@pytest.mark.parametrize("si,so", [((8, 4, 4), (16, 4, 4)), ((8, 4, 4), (12, 4, 4)),
+                                   ((4, 4, 5), (8, 8, 7)), ((11, 8, 8), (16, 11, 10)),
+                                   ((2, 3, 4), (3, 4, 5)), ((64, 64, 20), (128, 128, 27))])
+def test_zoom_matches_oracle(si, so):
+    from steam_amd.utils import zoom_trilinear
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal(si).astype(np.float32)
+    np.testing.assert_allclose(zoom_trilinear(f, so), M.zoom_trilinear(f, so),
+                               atol=3e-6)
+
+
+def test_zoom_refusals():
+    from steam_amd.utils import zoom_trilinear, VOLUME_GUARD_CELLS
+    with pytest.raises(NotImplementedError, match="downsamples"):
+        zoom_trilinear(np.zeros((8, 4, 4), np.float32), (4, 4, 4))
+    n = int(VOLUME_GUARD_CELLS ** (1 / 3)) + 1
+    with pytest.raises(NotImplementedError, match="larger working array"):
+        zoom_trilinear(np.zeros((n, n, n), np.float32), (2 * n + 1, 2 * n + 1, n))
+
+
+def test_regrid_window_equals_full_zoom():
+    """Target windows (the tiled regrid) are slices of the full resample."""
+    from steam_amd.utils import regrid
+    rng = np.random.default_rng(2)
+    src = torch.from_numpy(rng.standard_normal((11, 9, 6)).astype(np.float32)).cuda()
+    full = regrid(src, (33, 18, 8)).cpu().numpy()
+    for win in [(0, 5, 0, 4), (7, 33, 3, 18), (30, 33, 0, 18)]:
+        part = regrid(src, (33, 18, 8), window=win).cpu().numpy()
+        np.testing.assert_array_equal(part, full[win[0]:win[1], win[2]:win[3]])

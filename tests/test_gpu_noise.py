@@ -1,0 +1,118 @@
+"""GPU parity of the keyed-noise family against the oracle (bit-exact for the
+integer hashing and uniforms; ulp-level for the float32 CMS transform)."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import model as M
+from oracle import native as N
+
+from test_oracle import PHILOX4X32_10_KAT
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("counter,key,expected", PHILOX4X32_10_KAT)
+def test_philox_known_answers_on_gpu(counter, key, expected):
+    from steam_amd.noise import _philox4x32_10
+    assert _philox4x32_10(*counter, *key) == expected
+
+
+def test_philox_blocks_match_oracle_bitwise():
+    from steam_amd.noise import philox_blocks
+    rng = np.random.default_rng(5)
+    ctr = rng.integers(0, 2**32, size=(4096, 4), dtype=np.uint64).astype(np.uint32)
+    k0, k1 = 0x12345678, 0x9ABCDEF0
+    out = torch.empty((4096, 4), dtype=torch.int32, device='cuda')
+    philox_blocks(torch.from_numpy(ctr.view(np.int32)).cuda(), k0, k1, out)
+    got = out.cpu().numpy().view(np.uint32)
+    for i in range(0, 4096, 97):
+        assert tuple(int(v) for v in got[i]) == N.philox4x32_10(*ctr[i], k0, k1)
+
+
+@pytest.mark.parametrize("box", [
+    (np.arange(40), np.arange(33), np.arange(17)),
+    (np.array([2, 37, 9]), np.array([5]), np.array([21, 0])),
+    (np.arange(1000, 1003), np.arange(70000, 70004), np.arange(5)),
+])
+def test_keyed_uniforms_bit_exact(box):
+    from steam_amd.noise import keyed_uniforms
+    key = M.class_key(np.random.SeedSequence(11))
+    n = len(box[0]) * len(box[1]) * len(box[2])
+    l0 = np.empty(n, np.float32)
+    l1 = np.empty(n, np.float32)
+    keyed_uniforms(*box, *key, l0, l1)
+    o0, o1 = M.keyed_uniforms(*box, key)
+    np.testing.assert_array_equal(l0, o0.ravel())
+    np.testing.assert_array_equal(l1, o1.ravel())
+
+
+def test_negative_world_index_takes_the_reference_uint64_path():
+    """A negative world z (an elevated nest's halo) is cast to uint64 as the
+    reference's numba kernel does; the GPU wide path reproduces it."""
+    from steam_amd.noise import keyed_uniforms
+    key = M.class_key(np.random.SeedSequence(3))
+    box = (np.arange(3), np.arange(2), np.array([-4, -1, 0, 2]))
+    l0 = np.empty(24, np.float32)
+    l1 = np.empty(24, np.float32)
+    keyed_uniforms(*box, *key, l0, l1)
+    o0, o1 = M.keyed_uniforms(*box, key)
+    np.testing.assert_array_equal(l0, o0.ravel())
+    np.testing.assert_array_equal(l1, o1.ravel())
+
+
+@pytest.mark.parametrize("alpha", [1.5, 1.8, 2.0])
+def test_cms_matches_fp64_oracle(alpha):
+    """float32 CMS on the device against the float64 CMS of the same uniforms:
+    the bulk within a few ulp, the whole field at rel-L2 < 1e-5."""
+    from steam_amd.simulate import _levy_from_uniforms
+    key = M.class_key(np.random.SeedSequence(7))
+    u0, u1 = M.keyed_uniforms(np.arange(64), np.arange(64), np.arange(32), key)
+    phi = u0.ravel().copy()
+    _levy_from_uniforms(alpha, phi, u1.ravel().copy())
+    ref = M.levy_from_uniforms(alpha, u0.ravel(), u1.ravel())
+    rel = np.abs(phi - ref) / np.maximum(np.abs(ref), 1e-3)
+    assert np.median(rel) < 2e-7
+    assert np.quantile(rel, 0.999) < 5e-5
+    assert np.linalg.norm(phi - ref) / np.linalg.norm(ref) < 1e-5
+
+
+@pytest.mark.parametrize("sparsity", [(1, 1, 1), (2, 2, 2), (2, 3, 1), (4, 4, 2)])
+@pytest.mark.parametrize("origin", [(0, 0, 0), (7, 5, 3), (13, 1, 0), (30, 33, 2)])
+def test_sparse_levy_is_the_world_restriction(sparsity, origin):
+    """Reference tests/test_world_keyed_noise.py:61-118: a region's draws are
+    bit-equal to the world field's at those sites, including across a wrap."""
+    from steam_amd.simulate import NoiseRegion, _keyed_sparse_levy
+    world_shape = (36, 40, 24)
+    key = M.class_key(np.random.SeedSequence(17))
+    world = _keyed_sparse_levy(world_shape, sparsity, 1.8,
+                               NoiseRegion(key, (0, 0, 0), world_shape))
+    shape = (16, 14, 12)
+    region = _keyed_sparse_levy(shape, sparsity, 1.8,
+                                NoiseRegion(key, origin, world_shape))
+    ix = (np.arange(shape[0]) + origin[0]) % world_shape[0]
+    iy = (np.arange(shape[1]) + origin[1]) % world_shape[1]
+    iz = np.arange(shape[2]) + origin[2]
+    assert iz.max() < world_shape[2]
+    np.testing.assert_array_equal(region, world[np.ix_(ix, iy, iz)])
+    oracle = M.keyed_sparse_levy(shape, sparsity, 1.8, key, origin, world_shape)
+    np.testing.assert_array_equal(region == 0, oracle == 0)
+
+
+def test_multiplier_noise_matches_oracle():
+    from steam_amd.simulate import _multiplier_noise
+    key = M.class_key(np.random.SeedSequence(23))
+    shape, origin, world = (20, 18, 16), (-3, 30, 0), (36, 36, 16)
+    pcs, shift = np.float32(0.27826), np.float32(0.27826 ** 1.8 / 0.8)
+    got = _multiplier_noise(shape, origin, world, key, (1, 2, 1), pcs, shift, 2,
+                            True, True)
+    g = M.keyed_sparse_levy(shape, (1, 2, 1), 1.8, key, origin, world)
+    g[:, :, :2] = 0
+    g[:, :, -2:] = 0
+    g *= float(pcs)
+    off = g == 0
+    ref = np.expm1(g - float(shift))
+    ref[off] = 0
+    np.testing.assert_array_equal(got == 0, ref == 0)
+    np.testing.assert_allclose(got, ref, rtol=2e-5, atol=1e-6)
