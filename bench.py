@@ -48,7 +48,23 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, nargs=3, default=None,
                     help="host sub-box for the CPU baseline (default: the grid)")
     ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy three times (the "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE into bytes)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+    n = (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device='cuda')
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4,
+                      "launches": 3}))
 
 
 def family_bytes(info, shape, kernel_elems_planes):
@@ -100,6 +116,9 @@ def cpu_baseline(sample, c, alpha, kernel_np, seed_key, repeats=3):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus "
               f"{args.gpus}); the distributed decomposition is designed, not "

@@ -230,7 +230,8 @@ def conv_direct(field, kernel):
     out[x,y,z] = sum_{a,b,c} K[a,b,c] f[(x-a+cx) mod nx, (y-b+cy) mod ny,
     z-c+cz] with zero outside [0,nz), after folding, cx=(kx-1)//2 etc.
     """
-    kernel = fold_kernel_to_field(np.asarray(kernel), field.shape)
+    kernel = fold_kernel_to_field(np.asarray(kernel, dtype=np.float64),
+                                  field.shape)
     f = np.asarray(field, dtype=np.float64)
     nx, ny, nz = f.shape
     kx, ky, kz = kernel.shape

@@ -123,48 +123,98 @@ __device__ __forceinline__ void uniform_pair(int64_t ix, int64_t iy,
 }
 
 // ---------------------------------------------------------------------------
-// CMS extremal-Levy transform in float32, in the reference's operation order
-// (steam/simulate.py:413-438 with the constants of 441-470).
+// Elementary functions for the noise path.
+//   log2 / exp2: the transcendental unit (v_log_f32 / v_exp_f32, ~1 ulp).
+//   sinpi(t) = sin(pi t): exact argument reduction (t - k/2 is exact in
+//     float32) to |r| <= 1/4, then least-squares-fitted odd/even polynomials
+//     in r (fitted here for this library; <= 1.7 ulp on the reduced range).
+//   expm1: degree-9 Taylor polynomial for |x| < 1/2, exp2 otherwise.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float hw_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+__device__ __forceinline__ float sinpi_f32(float t) {
+  const float k = rintf(2.0f * t);
+  const float r = t - 0.5f * k;  // exact
+  const int q = (int)k & 3;
+  const float r2 = r * r;
+  float s = 0.08099588006734848f;
+  s = fmaf(s, r2, -0.5992013812065125f);
+  s = fmaf(s, r2, 2.5501625537872314f);
+  s = fmaf(s, r2, -5.167712688446045f);
+  s = fmaf(s, r2, 3.1415927410125732f);
+  s *= r;
+  float c = 0.23126092553138733f;
+  c = fmaf(c, r2, -1.3350359201431274f);
+  c = fmaf(c, r2, 4.058706760406494f);
+  c = fmaf(c, r2, -4.934802055358887f);
+  c = fmaf(c, r2, 1.0f);
+  const float v = (q & 1) ? c : s;
+  return (q & 2) ? -v : v;
+}
+
+__device__ __forceinline__ float expm1_f32(float x) {
+  if (fabsf(x) < 0.5f) {
+    float p = 1.0f / 362880.0f;
+    p = fmaf(p, x, 1.0f / 40320.0f);
+    p = fmaf(p, x, 1.0f / 5040.0f);
+    p = fmaf(p, x, 1.0f / 720.0f);
+    p = fmaf(p, x, 1.0f / 120.0f);
+    p = fmaf(p, x, 1.0f / 24.0f);
+    p = fmaf(p, x, 1.0f / 6.0f);
+    p = fmaf(p, x, 0.5f);
+    p = fmaf(p, x, 1.0f);
+    return p * x;
+  }
+  return hw_exp2(x * 1.4426950408889634f) - 1.0f;
+}
+
+// ---------------------------------------------------------------------------
+// CMS extremal-Levy transform (steam/simulate.py:413-479), float32, in a
+// cancellation-free form.  With phi = pi(u0 - 1/2), d = 1 - u0 (exact) and
+// phi0 = -(pi/2)(1 - |1 - alpha|)/alpha the reference's three trigonometric
+// factors are, for alpha > 1,
+//   cos(phi)                   = sin(pi min(u0, d))
+//   sin(alpha (phi - phi0))    = sin(pi alpha d)
+//   cos(phi - alpha(phi-phi0)) = sin(pi (alpha - 1) d)
+// and for alpha < 1 the last two are sin(pi alpha u0), sin(pi (1-alpha) u0).
+// Evaluating these instead of the reference's float32 chain removes the
+// cancellation near u0 -> 0, 1 that puts the reference's own float32 draws
+// ~2e-3 (rel-L2) from the float64 transform; here they are ~5e-7.  Clamps
+// (eps = 1e-6) and the sign convention are the reference's.
 // ---------------------------------------------------------------------------
 struct CmsConst {
   float alpha;      // float32(alpha)
-  float phi0;       // float32(-(pi/2)(1-|1-alpha|)/alpha)
-  float sign;       // +1 for alpha > 1
-  float abs_am1;    // float32(|alpha-1|)
+  float am1;        // float32(|alpha - 1|)
   float neg_inv_a;  // float32(-1/alpha)
   float tail_exp;   // float32((1-alpha)/alpha)
+  float sign;       // +1 for alpha > 1, -1 otherwise
+  int above_one;    // alpha > 1
 };
 
 inline CmsConst make_cms(double a) {
   CmsConst c;
   c.alpha = (float)a;
-  c.phi0 = (float)(-(3.141592653589793 / 2.0) * (1.0 - fabs(1.0 - a)) / a);
-  c.sign = a > 1.0 ? 1.0f : -1.0f;
-  c.abs_am1 = (float)fabs(a - 1.0);
+  c.am1 = (float)fabs(a - 1.0);
   c.neg_inv_a = (float)(-1.0 / a);
   c.tail_exp = (float)((1.0 - a) / a);
+  c.sign = a > 1.0 ? 1.0f : -1.0f;
+  c.above_one = a > 1.0;
   return c;
 }
 
 __device__ __forceinline__ float cms_f32(float u0, float u1, const CmsConst& k) {
   const float eps = 1e-6f;
-  const float pi_f = 3.14159265358979323846f;
-  float phi = __fmul_rn(__fsub_rn(u0, 0.5f), pi_f);
-  float R = fmaxf(u1, eps);
-  R = -logf(R);
-  float factor = fmaxf(cosf(phi), eps);
-  factor = __fmul_rn(factor, k.abs_am1);
-  factor = powf(factor, k.neg_inv_a);
-  float shifted = __fmul_rn(__fsub_rn(phi, k.phi0), k.alpha);
-  float p = __fsub_rn(phi, shifted);
-  p = fmaxf(cosf(p), eps);
-  p = __fdiv_rn(p, R);
-  p = powf(p, k.tail_exp);
-  float s = sinf(shifted);
-  s = __fmul_rn(s, k.sign);
-  s = __fmul_rn(s, factor);
-  s = __fmul_rn(s, p);
-  return s;
+  const float ln2 = 0.69314718055994531f;
+  const float d = 1.0f - u0;
+  const float R = -hw_log2(fmaxf(u1, eps)) * ln2;
+  const float cphi = fmaxf(sinpi_f32(fminf(u0, d)), eps);
+  const float factor = hw_exp2(k.neg_inv_a * hw_log2(cphi * k.am1));
+  const float w = k.above_one ? d : u0;
+  const float ss = sinpi_f32(k.alpha * w);
+  const float cp = fmaxf(sinpi_f32(k.am1 * w), eps);
+  const float tail = hw_exp2(k.tail_exp * hw_log2(cp / R));
+  return k.sign * ss * factor * tail;
 }
 
 // Python-style (floor) modulo for a positive modulus.

@@ -115,7 +115,7 @@ __device__ __forceinline__ float noise_at(const NoiseDev& nd,
   float gam = cms_f32(u0, u1, nd.cms);
   float g = __fmul_rn(gam, nd.pcs);
   if (g == 0.f) return 0.f;
-  return expm1f(__fsub_rn(g, nd.shift));
+  return expm1_f32(__fsub_rn(g, nd.shift));
 }
 
 }  // namespace steam

@@ -62,20 +62,28 @@ def test_negative_world_index_takes_the_reference_uint64_path():
     np.testing.assert_array_equal(l1, o1.ravel())
 
 
-@pytest.mark.parametrize("alpha", [1.5, 1.8, 2.0])
+@pytest.mark.parametrize("alpha", [1.5, 1.6, 1.8, 1.95, 2.0, 0.7])
 def test_cms_matches_fp64_oracle(alpha):
-    """float32 CMS on the device against the float64 CMS of the same uniforms:
-    the bulk within a few ulp, the whole field at rel-L2 < 1e-5."""
+    """float32 CMS on the device against the float64 CMS of the same uniforms
+    (2M draws): rel-L2 below 2e-6, the bulk within a few ulp.  The device
+    evaluates the transform in a cancellation-free form, so it is far closer
+    to float64 than the reference's own float32 operation order (checked on
+    the same draws: that order sits at ~1e-3 for alpha < 2)."""
     from steam_amd.simulate import _levy_from_uniforms
     key = M.class_key(np.random.SeedSequence(7))
-    u0, u1 = M.keyed_uniforms(np.arange(64), np.arange(64), np.arange(32), key)
+    u0, u1 = M.keyed_uniforms(np.arange(128), np.arange(128), np.arange(128), key)
     phi = u0.ravel().copy()
     _levy_from_uniforms(alpha, phi, u1.ravel().copy())
     ref = M.levy_from_uniforms(alpha, u0.ravel(), u1.ravel())
     rel = np.abs(phi - ref) / np.maximum(np.abs(ref), 1e-3)
-    assert np.median(rel) < 2e-7
-    assert np.quantile(rel, 0.999) < 5e-5
-    assert np.linalg.norm(phi - ref) / np.linalg.norm(ref) < 1e-5
+    assert np.median(rel) < 3e-7
+    assert np.quantile(rel, 0.999) < 1e-4
+    gpu_l2 = np.linalg.norm(phi - ref) / np.linalg.norm(ref)
+    assert gpu_l2 < 2e-6
+    if alpha < 2.0:
+        ref_order = M.levy_from_uniforms(alpha, u0.ravel(), u1.ravel(),
+                                         dtype=np.float32)
+        assert gpu_l2 < 0.01 * np.linalg.norm(ref_order - ref) / np.linalg.norm(ref)
 
 
 @pytest.mark.parametrize("sparsity", [(1, 1, 1), (2, 2, 2), (2, 3, 1), (4, 4, 2)])
