@@ -82,21 +82,22 @@ __device__ __forceinline__ void philox_fast(uint32_t& c0, uint32_t& c1,
   }
 }
 
-__device__ __noinline__ void philox_wide(uint64_t c[4], uint64_t k0,
-                                         uint64_t k1) {
+__device__ __forceinline__ void philox_wide(uint64_t& c0, uint64_t& c1,
+                                            uint64_t& c2, uint64_t& c3,
+                                            uint64_t k0, uint64_t k1) {
   for (int r = 0; r < 10; ++r) {
     if (r > 0) {
       k0 = (k0 + PHILOX_W0) & 0xFFFFFFFFull;
       k1 = (k1 + PHILOX_W1) & 0xFFFFFFFFull;
     }
-    uint64_t p0 = (uint64_t)PHILOX_M0 * c[0];
-    uint64_t p1 = (uint64_t)PHILOX_M1 * c[2];
-    uint64_t n0 = (p1 >> 32) ^ c[1] ^ k0;
-    uint64_t n2 = (p0 >> 32) ^ c[3] ^ k1;
-    c[0] = n0;
-    c[1] = p1 & 0xFFFFFFFFull;
-    c[2] = n2;
-    c[3] = p0 & 0xFFFFFFFFull;
+    const uint64_t p0 = (uint64_t)PHILOX_M0 * c0;
+    const uint64_t p1 = (uint64_t)PHILOX_M1 * c2;
+    const uint64_t n0 = (p1 >> 32) ^ c1 ^ k0;
+    const uint64_t n2 = (p0 >> 32) ^ c3 ^ k1;
+    c0 = n0;
+    c1 = p1 & 0xFFFFFFFFull;
+    c2 = n2;
+    c3 = p0 & 0xFFFFFFFFull;
   }
 }
 
@@ -115,10 +116,10 @@ __device__ __forceinline__ void uniform_pair(int64_t ix, int64_t iy,
     u0 = (float)(c0 >> 8) * scale;
     u1 = (float)(c1 >> 8) * scale;
   } else {
-    uint64_t c[4] = {(uint64_t)ix, (uint64_t)iy, (uint64_t)iz, key2};
-    philox_wide(c, key0, key1);
-    u0 = (float)(c[0] >> 8) * scale;
-    u1 = (float)(c[1] >> 8) * scale;
+    uint64_t c0 = (uint64_t)ix, c1 = (uint64_t)iy, c2 = (uint64_t)iz, c3 = key2;
+    philox_wide(c0, c1, c2, c3, key0, key1);
+    u0 = (float)(c0 >> 8) * scale;
+    u1 = (float)(c1 >> 8) * scale;
   }
 }
 
@@ -188,6 +189,7 @@ struct CmsConst {
   float am1;        // float32(|alpha - 1|)
   float neg_inv_a;  // float32(-1/alpha)
   float tail_exp;   // float32((1-alpha)/alpha)
+  float c0;         // -log2|alpha-1|/alpha - tail_exp * log2(ln 2)
   float sign;       // +1 for alpha > 1, -1 otherwise
   int above_one;    // alpha > 1
 };
@@ -198,23 +200,26 @@ inline CmsConst make_cms(double a) {
   c.am1 = (float)fabs(a - 1.0);
   c.neg_inv_a = (float)(-1.0 / a);
   c.tail_exp = (float)((1.0 - a) / a);
+  c.c0 = (float)(-log2(fabs(a - 1.0)) / a - (1.0 - a) / a * log2(log(2.0)));
   c.sign = a > 1.0 ? 1.0f : -1.0f;
   c.above_one = a > 1.0;
   return c;
 }
 
+// X = sign sin(alpha(phi-phi0)) (cos(phi)|alpha-1|)^(-1/alpha)
+//       (cos(phi - alpha(phi-phi0)) / R)^((1-alpha)/alpha),  R = -ln u1,
+// with the two powers folded into one exp2 of a sum of log2 terms.
 __device__ __forceinline__ float cms_f32(float u0, float u1, const CmsConst& k) {
   const float eps = 1e-6f;
-  const float ln2 = 0.69314718055994531f;
   const float d = 1.0f - u0;
-  const float R = -hw_log2(fmaxf(u1, eps)) * ln2;
-  const float cphi = fmaxf(sinpi_f32(fminf(u0, d)), eps);
-  const float factor = hw_exp2(k.neg_inv_a * hw_log2(cphi * k.am1));
   const float w = k.above_one ? d : u0;
+  const float lu = hw_log2(fmaxf(u1, eps));          // log2 u1 < 0
+  const float lr = hw_log2(-lu);                      // log2(R / ln 2)
+  const float lc = hw_log2(fmaxf(sinpi_f32(fminf(u0, d)), eps));
+  const float lp = hw_log2(fmaxf(sinpi_f32(k.am1 * w), eps));
   const float ss = sinpi_f32(k.alpha * w);
-  const float cp = fmaxf(sinpi_f32(k.am1 * w), eps);
-  const float tail = hw_exp2(k.tail_exp * hw_log2(cp / R));
-  return k.sign * ss * factor * tail;
+  const float e = fmaf(k.neg_inv_a, lc, fmaf(k.tail_exp, lp - lr, k.c0));
+  return k.sign * ss * hw_exp2(e);
 }
 
 // Python-style (floor) modulo for a positive modulus.

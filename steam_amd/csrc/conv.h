@@ -15,13 +15,24 @@ struct ConvGeom {
   int nfft, n2, nzc;    // z transform: real length, complex half, kept bins
   int L, nblocks;       // overlap-add block length and count along z
   int Tz, logTz;        // lines per workgroup in the z passes
-  int Ty;               // lines per workgroup in the y pass
+  int Ty, logTy;        // lines per workgroup in the y pass
   int Tx, logTx;        // lines per workgroup in the x pass
   int nyt_z, nyt_x;     // y-tiles per x line (z passes), per plane (x pass)
   int wg_z, wg_y, wg_x; // workgroups per pass
+  int reg_y, reg_x;     // log2 length of the register FFT path (0: generic)
+  int Tr;               // columns per workgroup in the register x pass
   AxisPlan ax, ay, az;
   const float2* wr2c;   // n2+1 entries exp(-2 pi i k / nfft)
 };
+
+// Workgroup index remapped so that consecutive tiles land on one XCD
+// (workgroups are dealt round-robin over the 8 XCDs; each has its own L2):
+// neighbouring column tiles that share 128-byte lines then share a cache.
+__device__ __forceinline__ int xcd_remap(int w, int n) {
+  const int q = n >> 3, r = n & 7;
+  const int i = w & 7, slot = w >> 3;
+  return i * q + (i < r ? i : r) + slot;
+}
 
 // Keyed multiplier-noise parameters as the kernels use them.
 struct NoiseDev {
@@ -79,20 +90,25 @@ struct PassDesc {
 namespace steam {
 
 // Per-line part of the keyed multiplier noise: the wrapped world x/y of the
-// line and whether it lies on the x/y centre sub-lattice.
+// line and whether it lies on the x/y centre sub-lattice.  World coordinates
+// are 32-bit here (the host rejects worlds or origins beyond +-2^30).
 struct LineNoise {
-  int64_t wx, wy;
+  int wx, wy;
   int xy_center;
   int zphase;  // world z of local z=0, modulo s_z
 };
 
-__device__ __forceinline__ LineNoise noise_line(const NoiseDev& nd, int64_t x,
-                                               int64_t y) {
+__device__ __forceinline__ int pmod32(int a, int n) {
+  const int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+__device__ __forceinline__ LineNoise noise_line(const NoiseDev& nd, int x, int y) {
   LineNoise ln;
-  ln.wx = pmod(x + nd.ox, nd.wx);
-  ln.wy = pmod(y + nd.oy, nd.wy);
-  ln.xy_center = (ln.wx % nd.sx == 0) && (ln.wy % nd.sy == 0);
-  ln.zphase = (int)pmod(nd.oz, nd.sz);
+  ln.wx = pmod32(x + (int)nd.ox, (int)nd.wx);
+  ln.wy = pmod32(y + (int)nd.oy, (int)nd.wy);
+  ln.xy_center = (nd.sx == 1 || ln.wx % nd.sx == 0) && (nd.sy == 1 || ln.wy % nd.sy == 0);
+  ln.zphase = pmod32((int)nd.oz, nd.sz);
   return ln;
 }
 
@@ -110,8 +126,8 @@ __device__ __forceinline__ float noise_at(const NoiseDev& nd,
   }
   if (nd.sz > 1 && (z + ln.zphase) % nd.sz != 0) return 0.f;
   float u0, u1;
-  uniform_pair(ln.wx, ln.wy, (int64_t)z + nd.oz, nd.key0, nd.key1, nd.key2, u0,
-               u1);
+  uniform_pair(ln.wx, ln.wy, (int64_t)(z + (int)nd.oz), nd.key0, nd.key1,
+               nd.key2, u0, u1);
   float gam = cms_f32(u0, u1, nd.cms);
   float g = __fmul_rn(gam, nd.pcs);
   if (g == 0.f) return 0.f;

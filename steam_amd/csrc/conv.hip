@@ -26,6 +26,12 @@
 #include <vector>
 
 #include "conv.h"
+#include "fft_reg.h"
+
+// z-pass FFT stages inlined (false) or called out of line (true)
+#ifndef Z_STAGE_CALLS
+#define Z_STAGE_CALLS false
+#endif
 
 namespace steam {
 
@@ -113,7 +119,7 @@ __device__ __forceinline__ float kernel_gather(const PassDesc& d, int x, int y,
 // zfwd: real z-lines -> spectral planes
 // ---------------------------------------------------------------------------
 template <class Src, int MODE, int RS>
-__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void zfwd_kernel(Src src) {
+__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) void zfwd_kernel(Src src) {
   int wg = blockIdx.x;
   const PassDesc& d = src.get(PASS_Z, wg);
   const ConvGeom& g = d.g;
@@ -121,7 +127,7 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
   const int tid = threadIdx.x;
   const int x = wg / g.nyt_z;
   const int y0 = (wg - x * g.nyt_z) * g.Tz;
-  const int ldt = g.n2 + 1;
+  const int ldn = g.Tz + 1;  // element (line t, index n) at n*ldn + t
   float* sr = reinterpret_cast<float*>(smem);
   const int nzin = (MODE == MODE_KERNEL) ? g.kz : g.nz;
   const int nblk = (MODE == MODE_KERNEL) ? 1 : g.nblocks;
@@ -167,34 +173,34 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
             v = kernel_gather(d, x, y, z);
           }
         }
-        sr[t * 2 * ldt + m] = v;
+        sr[((m >> 1) * ldn + t) * 2 + (m & 1)] = v;
       }
     }
     __syncthreads();
-    fft_lines<false, RS>(smem, g.az, g.Tz, g.logTz, 1, ldt, false, tid);
+    fft_lines<true, RS, Z_STAGE_CALLS>(smem, g.az, g.Tz, g.logTz, ldn, 1, false, tid);
     // r2c split: X2[k] = (Z[k] + conj Z[n2-k]) - i W^k (Z[k] - conj Z[n2-k])
     const int nh = g.n2 / 2 + 1;
     for (int e = tid; e < g.Tz * nh; e += FFT_NTH) {
-      const int t = e / nh;
-      const int k = e - t * nh;
-      float2* Lp = smem + t * ldt;
+      const int t = e & (g.Tz - 1);
+      const int k = e >> g.logTz;
+      float2* Lp = smem + t;
       if (k == 0) {
         float2 z0v = Lp[0];
         Lp[0] = make_float2(2.f * (z0v.x + z0v.y), 0.f);
-        Lp[g.n2] = make_float2(2.f * (z0v.x - z0v.y), 0.f);
+        Lp[g.n2 * ldn] = make_float2(2.f * (z0v.x - z0v.y), 0.f);
       } else {
         const int km = g.n2 - k;
-        float2 zk = Lp[k], zm = Lp[km];
+        float2 zk = Lp[k * ldn], zm = Lp[km * ldn];
         float2 w = g.wr2c[k];
         float2 A = make_float2(zk.x + zm.x, zk.y - zm.y);
         float2 B = make_float2(zk.x - zm.x, zk.y + zm.y);
         float2 wB = cmul(w, B);
-        Lp[k] = make_float2(A.x + wB.y, A.y - wB.x);
+        Lp[k * ldn] = make_float2(A.x + wB.y, A.y - wB.x);
         if (km != k) {
           float2 A2 = make_float2(zm.x + zk.x, zm.y - zk.y);
           float2 B2 = make_float2(zm.x - zk.x, zm.y + zk.y);
           float2 wB2 = cmul(make_float2(-w.x, w.y), B2);
-          Lp[km] = make_float2(A2.x + wB2.y, A2.y - wB2.x);
+          Lp[km * ldn] = make_float2(A2.x + wB2.y, A2.y - wB2.x);
         }
       }
     }
@@ -204,7 +210,7 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
       const int kz = e >> g.logTz;
       const int t = e & (g.Tz - 1);
       const int y = y0 + t;
-      if (y < g.ny) Sb[((int64_t)kz * g.nx + x) * g.ny + y] = smem[t * ldt + kz];
+      if (y < g.ny) Sb[((int64_t)kz * g.nx + x) * g.ny + y] = smem[kz * ldn + t];
     }
     __syncthreads();
   }
@@ -233,19 +239,20 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
   const int64_t lines = (int64_t)P * g.nx;
   const int64_t l0 = (int64_t)wg * g.Ty;
   const int ny = g.ny;
+  const int ldn = g.Ty + 1;
   for (int t = 0; t < g.Ty; ++t) {
     const int64_t l = l0 + t;
     const float2* srcl = d.S + l * ny;
     for (int n = tid; n < ny; n += FFT_NTH)
-      smem[t * ny + n] = (l < lines) ? srcl[n] : make_float2(0.f, 0.f);
+      smem[n * ldn + t] = (l < lines) ? srcl[n] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lines<false, RS>(smem, g.ay, g.Ty, 0, 1, ny, inv != 0, tid);
+  fft_lines<true, RS>(smem, g.ay, g.Ty, g.logTy, ldn, 1, inv != 0, tid);
   for (int t = 0; t < g.Ty; ++t) {
     const int64_t l = l0 + t;
     if (l >= lines) break;
     float2* dstl = d.S + l * ny;
-    for (int n = tid; n < ny; n += FFT_NTH) dstl[n] = smem[t * ny + n];
+    for (int n = tid; n < ny; n += FFT_NTH) dstl[n] = smem[n * ldn + t];
   }
 }
 
@@ -254,7 +261,7 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
 // ---------------------------------------------------------------------------
 template <class Src, int XMODE, int RS>
 __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void xpass_kernel(Src src) {
-  int wg = blockIdx.x;
+  int wg = xcd_remap(blockIdx.x, gridDim.x);
   const PassDesc& d = src.get(PASS_X, wg);
   const ConvGeom& g = d.g;
   extern __shared__ float2 smem[];
@@ -296,10 +303,81 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
 }
 
 // ---------------------------------------------------------------------------
+// register-FFT y pass (power-of-two ny): P = ny/8 threads per contiguous
+// line, 256/P lines per workgroup, HBM -> registers -> HBM
+// ---------------------------------------------------------------------------
+template <class Src, int LOGN>
+__global__ __launch_bounds__(256) void ypass_reg_kernel(Src src, int inv) {
+  using F = RegFFT<LOGN>;
+  constexpr int P = F::P, LPW = 256 / P;
+  __shared__ float2 lds[LPW * F::LDSN];
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_Y, wg);
+  const ConvGeom& g = d.g;
+  const int tid = threadIdx.x;
+  const int q = tid % P, lw = tid / P;
+  const int planes = (d.mode == MODE_KERNEL) ? g.nzc : g.nblocks * g.nzc;
+  const int64_t lines = (int64_t)planes * g.nx;
+  const int64_t line = (int64_t)wg * LPW + lw;
+  const bool valid = line < lines;
+  float2* Sl = d.S + (valid ? line : 0) * F::N;
+  float2 v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = valid ? Sl[q + m * P] : make_float2(0.f, 0.f);
+  fft_reg<LOGN>(v, q, lds, 1, lw * F::LDSN, g.ay.tw, inv != 0);
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) Sl[q + m * P] = v[m];
+  }
+}
+
+// register-FFT x pass (power-of-two nx): P = nx/8 threads per x-line, Tr
+// adjacent y columns per workgroup (lane t = column, so each row access is a
+// Tr*8-byte run); XMODE 0: forward, * kernel spectrum, inverse; 1: forward,
+// * scale.
+template <class Src, int XMODE, int LOGN>
+__global__ __launch_bounds__(1024) void xpass_reg_kernel(Src src) {
+  using F = RegFFT<LOGN>;
+  constexpr int P = F::P;
+  constexpr int T = (1024 / P < 8) ? 1024 / P : 8;
+  __shared__ float2 lds[T * F::LDSN];
+  int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const PassDesc& d = src.get(PASS_X, wg);
+  const ConvGeom& g = d.g;
+  const int tid = threadIdx.x;
+  const int t = tid % T, q = tid / T;
+  const int nyt = (g.ny + T - 1) / T;
+  const int p = wg / nyt;
+  const int y = (wg - p * nyt) * T + t;
+  const bool valid = y < g.ny;
+  const int nx = g.nx, ny = g.ny;
+  float2* Sp = d.S + (int64_t)p * nx * ny + (valid ? y : 0);
+  float2 v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m)
+    v[m] = valid ? Sp[(int64_t)(q + m * P) * ny] : make_float2(0.f, 0.f);
+  fft_reg<LOGN>(v, q, lds, T, t, g.ax.tw, false);
+  if (XMODE == 0) {
+    const float2* Kp = d.K + (int64_t)(p % g.nzc) * nx * ny + (valid ? y : 0);
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      v[m] = cmul(v[m], valid ? Kp[(int64_t)(q + m * P) * ny] : make_float2(0.f, 0.f));
+    fft_reg<LOGN>(v, q, lds, T, t, g.ax.tw, true);
+  } else {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = cscale(v[m], d.kscale);
+  }
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) Sp[(int64_t)(q + m * P) * ny] = v[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // zinv: planes -> c2r per block -> overlap-add in LDS -> epilogue
 // ---------------------------------------------------------------------------
 template <class Src, int MODE, int RS>
-__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void zinv_kernel(Src src) {
+__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) void zinv_kernel(Src src) {
   int wg = blockIdx.x;
   const PassDesc& d = src.get(PASS_ZINV, wg);
   const ConvGeom& g = d.g;
@@ -307,10 +385,10 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
   const int tid = threadIdx.x;
   const int x = wg / g.nyt_z;
   const int y0 = (wg - x * g.nyt_z) * g.Tz;
-  const int ldt = g.n2 + 1;
+  const int ldn = g.Tz + 1;
   const int nz = g.nz;
   float* sr = reinterpret_cast<float*>(smem);
-  float* accz = reinterpret_cast<float*>(smem + g.Tz * ldt);
+  float* accz = reinterpret_cast<float*>(smem + (g.n2 + 1) * ldn);
   float* comp = accz + g.Tz * nz;
   const bool kahan = d.kahan != 0;
   for (int e = tid; e < g.Tz * nz; e += FFT_NTH) {
@@ -323,45 +401,45 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
       const int kz = e >> g.logTz;
       const int t = e & (g.Tz - 1);
       const int y = y0 + t;
-      smem[t * ldt + kz] = (y < g.ny) ? Sb[((int64_t)kz * g.nx + x) * g.ny + y]
+      smem[kz * ldn + t] = (y < g.ny) ? Sb[((int64_t)kz * g.nx + x) * g.ny + y]
                                       : make_float2(0.f, 0.f);
     }
     __syncthreads();
     // c2r merge: Z[k] = (Y[k] + conj Y[n2-k]) + i conj(W^k) (Y[k] - conj Y[n2-k])
     const int nh = g.n2 / 2 + 1;
     for (int e = tid; e < g.Tz * nh; e += FFT_NTH) {
-      const int t = e / nh;
-      const int k = e - t * nh;
-      float2* Lp = smem + t * ldt;
+      const int t = e & (g.Tz - 1);
+      const int k = e >> g.logTz;
+      float2* Lp = smem + t;
       if (k == 0) {
-        float2 Y0 = Lp[0], YN = Lp[g.n2];
+        float2 Y0 = Lp[0], YN = Lp[g.n2 * ldn];
         float2 A = make_float2(Y0.x + YN.x, Y0.y - YN.y);
         float2 B = make_float2(Y0.x - YN.x, Y0.y + YN.y);
         Lp[0] = make_float2(A.x - B.y, A.y + B.x);
       } else {
         const int km = g.n2 - k;
-        float2 Yk = Lp[k], Ym = Lp[km];
+        float2 Yk = Lp[k * ldn], Ym = Lp[km * ldn];
         float2 w = g.wr2c[k];
         float2 A = make_float2(Yk.x + Ym.x, Yk.y - Ym.y);
         float2 B = make_float2(Yk.x - Ym.x, Yk.y + Ym.y);
         float2 cB = cmul(make_float2(w.x, -w.y), B);  // conj(W^k) B
-        Lp[k] = make_float2(A.x - cB.y, A.y + cB.x);  // A + i cB
+        Lp[k * ldn] = make_float2(A.x - cB.y, A.y + cB.x);  // A + i cB
         if (km != k) {
           float2 A2 = make_float2(Ym.x + Yk.x, Ym.y - Yk.y);
           float2 B2 = make_float2(Ym.x - Yk.x, Ym.y + Yk.y);
           float2 cB2 = cmul(make_float2(-w.x, -w.y), B2);  // conj(W^{n2-k}) = -W^k
-          Lp[km] = make_float2(A2.x - cB2.y, A2.y + cB2.x);
+          Lp[km * ldn] = make_float2(A2.x - cB2.y, A2.y + cB2.x);
         }
       }
     }
     __syncthreads();
-    fft_lines<false, RS>(smem, g.az, g.Tz, g.logTz, 1, ldt, true, tid);
+    fft_lines<true, RS, Z_STAGE_CALLS>(smem, g.az, g.Tz, g.logTz, ldn, 1, true, tid);
     const int zb = b * g.L - g.cz;
     for (int t = 0; t < g.Tz; ++t) {
       for (int m = tid; m < g.nfft; m += FFT_NTH) {
         const int zo = zb + m;
         if (zo >= 0 && zo < nz) {
-          const float v = sr[t * 2 * ldt + m];
+          const float v = sr[((m >> 1) * ldn + t) * 2 + (m & 1)];
           const int ia = t * nz + zo;
           if (!kahan) {
             accz[ia] += v;
@@ -510,6 +588,17 @@ static void factor_axis(int n, AxisPlan& ap) {
   while (m % 2 == 0 && ap.nst < FFT_MAXST) { ap.radix[ap.nst++] = 2; m /= 2; }
   for (int p : {3, 5, 7})
     while (m % p == 0 && ap.nst < FFT_MAXST) { ap.radix[ap.nst++] = p; m /= p; }
+  if (m != 1) {  // more factors than stage slots: one direct-DFT stage
+    ap.nst = 1;
+    ap.radix[0] = 0;
+  }
+}
+
+// log2 of a power of two in [64, 2048], else 0 (no register FFT path)
+static int reg_log2(int n) {
+  for (int l = 6; l <= 11; ++l)
+    if (n == (1 << l)) return l;
+  return 0;
 }
 
 static int floor_pow2(int64_t v) {
@@ -544,15 +633,16 @@ static int64_t align256(int64_t v) { return (v + 255) & ~(int64_t)255; }
 
 static void set_tiles(ConvGeom& g, int nz_acc, int kahan, size_t& lds_z,
                       size_t& lds_zinv, size_t& lds_y, size_t& lds_x) {
-  const int ldt = g.n2 + 1;
-  int Tz = std::min(16, floor_pow2(std::max<int64_t>(1, FFT_LDSE / ldt)));
+  const int nzc = g.n2 + 1;
+  int Tz = std::min(16, floor_pow2(std::max<int64_t>(1, FFT_LDSE / nzc)));
   Tz = std::min(Tz, 1 << ilog2(g.ny));
-  while (Tz > 1 && (size_t)Tz * ldt * 8 + (size_t)Tz * nz_acc * 4 * (1 + kahan) >
+  while (Tz > 1 && (size_t)(Tz + 1) * nzc * 8 + (size_t)Tz * nz_acc * 4 * (1 + kahan) >
                        64 * 1024)
     Tz /= 2;
   g.Tz = Tz;
   g.logTz = ilog2(Tz);
   g.Ty = std::max(1, std::min(16, floor_pow2(FFT_LDSE / g.ny)));
+  g.logTy = ilog2(g.Ty);
   int Tx = std::min(16, floor_pow2(std::max<int64_t>(1, FFT_LDSE / g.nx)));
   Tx = std::min(Tx, 1 << ilog2(g.ny));
   g.Tx = std::max(1, Tx);
@@ -560,12 +650,26 @@ static void set_tiles(ConvGeom& g, int nz_acc, int kahan, size_t& lds_z,
   g.nyt_z = (g.ny + g.Tz - 1) / g.Tz;
   g.nyt_x = (g.ny + g.Tx - 1) / g.Tx;
   const int64_t P = (int64_t)g.nblocks * g.nzc;
+  g.reg_y = reg_log2(g.ny);
+  g.reg_x = reg_log2(g.nx);
   g.wg_z = g.nx * g.nyt_z;
-  g.wg_y = (int)((P * g.nx + g.Ty - 1) / g.Ty);
-  g.wg_x = (int)(P * g.nyt_x);
-  lds_z = (size_t)g.Tz * ldt * 8;
+  if (g.reg_y) {
+    const int lpw = 256 / (g.ny / 8);
+    g.wg_y = (int)((P * g.nx + lpw - 1) / lpw);
+  } else {
+    g.wg_y = (int)((P * g.nx + g.Ty - 1) / g.Ty);
+  }
+  if (g.reg_x) {
+    const int pth = g.nx / 8;
+    g.Tr = (1024 / pth < 8) ? 1024 / pth : 8;
+    g.wg_x = (int)(P * ((g.ny + g.Tr - 1) / g.Tr));
+  } else {
+    g.Tr = 0;
+    g.wg_x = (int)(P * g.nyt_x);
+  }
+  lds_z = (size_t)(g.Tz + 1) * nzc * 8;
   lds_zinv = lds_z + (size_t)g.Tz * nz_acc * 4 * (1 + kahan);
-  lds_y = (size_t)g.Ty * g.ny * 8;
+  lds_y = (size_t)(g.Ty + 1) * g.ny * 8;
   lds_x = (size_t)g.nx * g.Tx * 8;
 }
 
@@ -734,12 +838,52 @@ static PassDesc base_desc(const steam_conv_plan* p, void* ws) {
     default: { constexpr int RS_ = 2; __VA_ARGS__; } break; \
   }
 
+#define STEAM_LOGN_DISPATCH(logn, ...)                     \
+  switch (logn) {                                          \
+    case 6: { constexpr int LN_ = 6; __VA_ARGS__; } break;  \
+    case 7: { constexpr int LN_ = 7; __VA_ARGS__; } break;  \
+    case 8: { constexpr int LN_ = 8; __VA_ARGS__; } break;  \
+    case 9: { constexpr int LN_ = 9; __VA_ARGS__; } break;  \
+    case 10: { constexpr int LN_ = 10; __VA_ARGS__; } break; \
+    default: { constexpr int LN_ = 11; __VA_ARGS__; } break; \
+  }
+
+template <class SrcT>
+static void launch_ypass(const ConvGeom& g, size_t lds_y, hipStream_t s,
+                         const SrcT& src, int inv) {
+  if (g.reg_y) {
+    STEAM_LOGN_DISPATCH(g.reg_y, hipLaunchKernelGGL((ypass_reg_kernel<SrcT, LN_>),
+                                                    dim3(g.wg_y), dim3(256), 0, s,
+                                                    src, inv));
+  } else {
+    const int rsy = radix_set(g.ay);
+    STEAM_RS_DISPATCH(rsy, hipLaunchKernelGGL((ypass_kernel<SrcT, RS_>), dim3(g.wg_y),
+                                              dim3(FFT_NTH), lds_y, s, src, inv));
+  }
+}
+
+template <class SrcT, int XMODE>
+static void launch_xpass(const ConvGeom& g, size_t lds_x, hipStream_t s,
+                         const SrcT& src) {
+  if (g.reg_x) {
+    const int threads = (g.nx / 8) * g.Tr;
+    STEAM_LOGN_DISPATCH(g.reg_x, hipLaunchKernelGGL((xpass_reg_kernel<SrcT, XMODE, LN_>),
+                                                    dim3(g.wg_x), dim3(threads), 0,
+                                                    s, src));
+  } else {
+    const int rsx = radix_set(g.ax);
+    STEAM_RS_DISPATCH(rsx, hipLaunchKernelGGL((xpass_kernel<SrcT, XMODE, RS_>),
+                                              dim3(g.wg_x), dim3(FFT_NTH), lds_x, s,
+                                              src));
+  }
+}
+
 template <int MODE>
 static int run_operator(const steam_conv_plan* p, const PassDesc& d,
                         hipStream_t s) {
   SingleSrc src{d};
   const ConvGeom& g = d.g;
-  const int rsx = radix_set(g.ax), rsy = radix_set(g.ay), rsz = radix_set(g.az);
+  const int rsz = radix_set(g.az);
   {
     ProfScope ps(STEAM_KF_ZFWD, s);
     STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zfwd_kernel<SingleSrc, MODE, RS_>),
@@ -748,20 +892,17 @@ static int run_operator(const steam_conv_plan* p, const PassDesc& d,
   STEAM_LAUNCH_CHECK("zfwd_kernel");
   {
     ProfScope ps(STEAM_KF_YPASS, s);
-    STEAM_RS_DISPATCH(rsy, hipLaunchKernelGGL((ypass_kernel<SingleSrc, RS_>), dim3(g.wg_y),
-                                              dim3(FFT_NTH), p->lds_y, s, src, 0));
+    launch_ypass(g, p->lds_y, s, src, 0);
   }
   STEAM_LAUNCH_CHECK("ypass_kernel");
   {
     ProfScope ps(STEAM_KF_XPASS, s);
-    STEAM_RS_DISPATCH(rsx, hipLaunchKernelGGL((xpass_kernel<SingleSrc, 0, RS_>), dim3(g.wg_x),
-                                              dim3(FFT_NTH), p->lds_x, s, src));
+    launch_xpass<SingleSrc, 0>(g, p->lds_x, s, src);
   }
   STEAM_LAUNCH_CHECK("xpass_kernel");
   {
     ProfScope ps(STEAM_KF_YPASS, s);
-    STEAM_RS_DISPATCH(rsy, hipLaunchKernelGGL((ypass_kernel<SingleSrc, RS_>), dim3(g.wg_y),
-                                              dim3(FFT_NTH), p->lds_y, s, src, 1));
+    launch_ypass(g, p->lds_y, s, src, 1);
   }
   STEAM_LAUNCH_CHECK("ypass_kernel(inv)");
   {
@@ -792,15 +933,13 @@ extern "C" int steam_conv_set_kernel(const steam_conv_plan* p,
   d.kscale = (float)(1.0 / (4.0 * (double)p->g.nfft * (double)p->nx * (double)p->ny));
   SingleSrc src{d};
   const ConvGeom& g = d.g;
-  const int rsx = radix_set(g.ax), rsy = radix_set(g.ay), rsz = radix_set(g.az);
+  const int rsz = radix_set(g.az);
   STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zfwd_kernel<SingleSrc, MODE_KERNEL, RS_>),
                                             dim3(g.wg_z), dim3(FFT_NTH), p->lds_z, s, src));
   STEAM_LAUNCH_CHECK("zfwd_kernel(kernel)");
-  STEAM_RS_DISPATCH(rsy, hipLaunchKernelGGL((ypass_kernel<SingleSrc, RS_>), dim3(g.wg_y),
-                                            dim3(FFT_NTH), p->lds_y, s, src, 0));
+  launch_ypass(g, p->lds_y, s, src, 0);
   STEAM_LAUNCH_CHECK("ypass_kernel(kernel)");
-  STEAM_RS_DISPATCH(rsx, hipLaunchKernelGGL((xpass_kernel<SingleSrc, 1, RS_>), dim3(g.wg_x),
-                                            dim3(FFT_NTH), p->lds_x, s, src));
+  launch_xpass<SingleSrc, 1>(g, p->lds_x, s, src);
   STEAM_LAUNCH_CHECK("xpass_kernel(kernel)");
   return STEAM_OK;
 }
@@ -845,6 +984,12 @@ NoiseDev make_noise_dev(const steam_noise_params* np) {
 
 int check_noise_params(const steam_noise_params* np) {
   if (!np) return set_error(STEAM_EINVAL, "noise params are NULL");
+  for (int a = 0; a < 3; ++a)
+    if (np->region.origin[a] < -(1ll << 30) || np->region.origin[a] > (1ll << 30) ||
+        np->region.world_shape[a] > (1ll << 30))
+      return set_error(STEAM_EINVAL, "world origin/shape beyond +-2^30 cells");
+  if (np->alpha == 1.0)
+    return set_error(STEAM_EINVAL, "alpha = 1 is outside the generator's domain");
   for (int a = 0; a < 3; ++a)
     if (np->region.sparsity[a] < 1)
       return set_error(STEAM_EINVAL, "sparsity factors must be positive");

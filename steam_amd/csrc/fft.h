@@ -20,7 +20,7 @@ namespace steam {
 constexpr int FFT_NTH = 256;              // threads per FFT workgroup
 constexpr int FFT_EMAX = 16;              // complex values per thread
 constexpr int FFT_LDSE = FFT_NTH * FFT_EMAX;  // 4096 complex = 32 KiB
-constexpr int FFT_MAXST = 14;
+constexpr int FFT_MAXST = 6;   // stages per length (factor 8 first)
 
 struct AxisPlan {
   int n;                   // transform length
@@ -126,7 +126,7 @@ __device__ __forceinline__ void line_index(int b, int T, int logT, int NR,
 }
 
 template <int R, bool TFAST>
-__device__ __noinline__ void fft_stage(float2* buf, int N, int T, int logT, int ldn,
+__device__ __forceinline__ void fft_stage(float2* buf, int N, int T, int logT, int ldn,
                           int ldt, int Ns, const float2* __restrict__ tw,
                           bool inv, int tid) {
   constexpr int MAXB = (FFT_EMAX + R - 1) / R;
@@ -171,6 +171,14 @@ __device__ __noinline__ void fft_stage(float2* buf, int N, int T, int logT, int 
   __syncthreads();
 }
 
+template <int R, bool TFAST>
+__device__ __noinline__ void fft_stage_call(float2* buf, int N, int T, int logT,
+                                           int ldn, int ldt, int Ns,
+                                           const float2* __restrict__ tw,
+                                           bool inv, int tid) {
+  fft_stage<R, TFAST>(buf, N, T, logT, ldn, ldt, Ns, tw, inv, tid);
+}
+
 // Whole-line direct DFT (lengths with a prime factor > 7).
 template <bool TFAST>
 __device__ __noinline__ void fft_direct(float2* buf, int N, int T, int logT, int ldn,
@@ -211,8 +219,10 @@ __device__ __noinline__ void fft_direct(float2* buf, int N, int T, int logT, int
 }
 
 // Radix sets a kernel is instantiated for: 0 = {2,4,8}, 1 = {2,3,4,5,8},
-// 2 = every stage kind.  Stages are separate (non-inlined) functions so each
-// gets its own register allocation.
+// 2 = every stage kind.  The stage loop is unrolled into fixed slots so the
+// register allocator sees straight-line code (a rolled loop around the radix
+// switch keeps every case's registers live); the rare radix-7 / direct set
+// calls out-of-line stages instead.
 inline int radix_set(const AxisPlan& ap) {
   int rs = 0;
   for (int s = 0; s < ap.nst; ++s) {
@@ -224,25 +234,36 @@ inline int radix_set(const AxisPlan& ap) {
 }
 
 // Transform T lines of length ap.n held in LDS.  Caller synchronises before.
-template <bool TFAST, int RSET>
-__device__ void fft_lines(float2* buf, const AxisPlan& ap, int T, int logT,
-                          int ldn, int ldt, bool inv, int tid) {
+template <bool TFAST, int RSET, bool CALLS = false>
+__device__ __forceinline__ void fft_lines(float2* buf, const AxisPlan& ap, int T,
+                                          int logT, int ldn, int ldt, bool inv,
+                                          int tid) {
   int Ns = 1;
-  for (int s = 0; s < ap.nst; ++s) {
-    int R = ap.radix[s];
-    switch (R) {
-      case 2: fft_stage<2, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-      case 3: if (RSET >= 1) fft_stage<3, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-      case 4: fft_stage<4, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-      case 5: if (RSET >= 1) fft_stage<5, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-      case 7: if (RSET >= 2) fft_stage<7, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-      case 8: fft_stage<8, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-      default:
-        if (RSET >= 2) fft_direct<TFAST>(buf, ap.n, T, logT, ldn, ldt, ap.tw, inv, tid);
-        R = ap.n;
-        break;
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    if (s >= ap.nst) break;
+    const int R = ap.radix[s];
+    if (RSET < 2 && !CALLS) {
+      switch (R) {
+        case 2: fft_stage<2, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 4: fft_stage<4, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 8: fft_stage<8, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 3: if (RSET >= 1) fft_stage<3, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 5: if (RSET >= 1) fft_stage<5, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        default: break;
+      }
+    } else {
+      switch (R) {
+        case 2: fft_stage_call<2, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 3: fft_stage_call<3, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 4: fft_stage_call<4, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 5: fft_stage_call<5, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 7: fft_stage_call<7, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 8: fft_stage_call<8, TFAST>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        default: fft_direct<TFAST>(buf, ap.n, T, logT, ldn, ldt, ap.tw, inv, tid); break;
+      }
     }
-    Ns *= R;
+    Ns *= (R == 0) ? ap.n : R;
   }
 }
 

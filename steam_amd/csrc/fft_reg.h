@@ -1,0 +1,91 @@
+// Register-resident radix-8 Stockham FFT for power-of-two lines
+// (N = 2^LOGN, 64 <= N <= 2048).
+//
+// A line is held by P = N/8 threads; thread q holds the 8 values with natural
+// indices q + m P (m = 0..7), which is both the input pattern of every stage
+// and the output pattern of the last one.  So the first stage reads straight
+// from HBM and the last stage's results are written straight back, both
+// coalesced across q; only the NST - 1 intermediate exchanges go through LDS.
+// Stages are radix 8, with a final radix-4 or radix-2 stage when LOGN is not a
+// multiple of 3 (each thread then does 2 or 4 butterflies of that stage).
+// LDS index i is stored at i + i/8 (one pad slot per 8 values), which makes
+// the strided writes of the early stages bank-conflict free.
+#pragma once
+
+#include "fft.h"
+
+namespace steam {
+
+template <int LOGN>
+struct RegFFT {
+  static constexpr int N = 1 << LOGN;
+  static constexpr int P = N / 8;
+  static constexpr int N8 = LOGN / 3;
+  static constexpr int TAIL = 1 << (LOGN % 3);
+  static constexpr int NST = N8 + (TAIL > 1 ? 1 : 0);
+  static constexpr int LDSN = N + N / 8;  // padded line length
+};
+
+__device__ __forceinline__ int pad8(int i) { return i + (i >> 3); }
+
+// Transform the line held in v (thread q of the line).  LDS element i of the
+// line lives at lds[pad8(i) * lstride + loff].  Every thread of the
+// workgroup must call this (it synchronises the workgroup between stages).
+template <int LOGN>
+__device__ __forceinline__ void fft_reg(float2 (&v)[8], int q, float2* lds,
+                                        int lstride, int loff,
+                                        const float2* __restrict__ tw,
+                                        bool inv) {
+  using F = RegFFT<LOGN>;
+  constexpr int N = F::N, P = F::P;
+#pragma unroll
+  for (int s = 0; s < F::N8; ++s) {
+    const int Ns = 1 << (3 * s);
+    const int k = q & (Ns - 1);
+    if (s > 0) {
+      const int step = N / (Ns * 8);
+#pragma unroll
+      for (int r = 1; r < 8; ++r) {
+        float2 w = tw[r * k * step];
+        if (inv) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    }
+    dft_small<8>(v, inv);
+    const bool last = (s == F::NST - 1);
+    if (!last) {
+      const int base = (q - k) * 8 + k;
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 8; ++r) lds[pad8(base + r * Ns) * lstride + loff] = v[r];
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = lds[pad8(q + m * P) * lstride + loff];
+    }
+  }
+  if constexpr (F::TAIL > 1) {
+    constexpr int R = F::TAIL;
+    constexpr int NB = 8 / R;
+    // butterfly b: j = q + b P (j < Ns = N/R), inputs v[b + NB r]
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = q + b * P;
+      float2 u[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 x = v[b + NB * r];
+        if (r > 0) {
+          float2 w = tw[r * j];
+          if (inv) w.y = -w.y;
+          x = cmul(x, w);
+        }
+        u[r] = x;
+      }
+      dft_small<R>(u, inv);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + NB * r] = u[r];
+    }
+  }
+}
+
+}  // namespace steam

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void field_noise_kernel(NoiseDev nd,
   for (int64_t l = blockIdx.x; l < lines; l += gridDim.x) {
     const int64_t x = l / ny;
     const int64_t y = l - x * ny;
-    LineNoise ln = noise_line(nd, x, y);
+    LineNoise ln = noise_line(nd, (int)x, (int)y);
     float* o = out + l * nz;
     for (int64_t z = threadIdx.x; z < nz; z += blockDim.x) {
       float v;
@@ -74,7 +74,8 @@ __global__ __launch_bounds__(256) void field_noise_kernel(NoiseDev nd,
         v = 0.f;
         if (ln.xy_center && (nd.sz == 1 || ((int)z + ln.zphase) % nd.sz == 0)) {
           float u0, u1;
-          uniform_pair(ln.wx, ln.wy, z + nd.oz, nd.key0, nd.key1, nd.key2, u0, u1);
+          uniform_pair(ln.wx, ln.wy, (int64_t)((int)z + (int)nd.oz), nd.key0,
+                       nd.key1, nd.key2, u0, u1);
           v = cms_f32(u0, u1, nd.cms);
         }
       }
