@@ -75,7 +75,8 @@ def family_bytes(info, shape, kernel_elems_planes):
     E = info.spectral_elems
     K = kernel_elems_planes
     return {
-        "zfwd": 4 * N + 8 * E,          # read flux, write spectral planes
+        "noise": 4 * N,                 # write the noise field
+        "zfwd": 8 * N + 8 * E,          # read flux + noise, write planes
         "ypass": 16 * E,                # read + write planes
         "xpass": 16 * E + 8 * K,        # read + write planes, read kernel spectrum
         "zinv": 8 * E + 8 * N,          # read planes, read + write flux
@@ -146,18 +147,59 @@ def main():
     info = _lib.ConvInfo()
     lib.steam_conv_plan_info(op.plan, ctypes.byref(info))
 
-    def step(i):
-        S.advance_flux_device(flux, region, kern, c, 1, (1, 1, 1), alpha=alpha,
+    total = args.warmup + args.steps
+    seeds = np.random.SeedSequence(0).spawn(total + 1)
+    regions = [S.NoiseRegion(S.class_key(sq), (0, 0, 0), shape) for sq in seeds]
+
+    # serial form: each advance generates its own noise
+    def serial_step(i):
+        S.advance_flux_device(flux, regions[i], kern, c, 1, (1, 1, 1), alpha=alpha,
                               defer_rescale=True, sync_ledger=False, op=op,
                               spec=spec, chain_pending=i > 0)
 
     for i in range(args.warmup):
-        step(i)
+        serial_step(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        serial_step(args.warmup + i)
+    torch.cuda.synchronize()
+    serial_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+
+    # pipelined form: class i+1's noise (flux-independent, VALU-bound) is
+    # generated on a side stream into the other of two buffers while class i
+    # runs its memory-bound FFT passes on the main stream
+    main = torch.cuda.current_stream()
+    side = torch.cuda.Stream()
+    bufs = [torch.empty(shape, dtype=torch.float32, device='cuda') for _ in range(2)]
+    ev_noise = [torch.cuda.Event() for _ in range(2)]
+    ev_adv = [torch.cuda.Event() for _ in range(2)]
+
+    def gen(i):
+        b = i % 2
+        if i >= 2:
+            side.wait_event(ev_adv[b])
+        S.generate_noise_device(op, regions[i], c, 1, (1, 1, 1), bufs[b],
+                                alpha=alpha, stream=side)
+        ev_noise[b].record(side)
+
+    def piped_step(i):
+        b = i % 2
+        gen(i + 1)
+        main.wait_event(ev_noise[b])
+        S.advance_flux_device(flux, regions[i], kern, c, 1, (1, 1, 1), alpha=alpha,
+                              defer_rescale=True, sync_ledger=False, op=op,
+                              spec=spec, chain_pending=True, noise_t=bufs[b])
+        ev_adv[b].record(main)
+
+    gen(0)
+    for i in range(args.warmup):
+        piped_step(i)
     torch.cuda.synchronize()
     _lib.check(lib.steam_profile_enable(1))
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(args.warmup + i)
+    for i in range(args.warmup, total):
+        piped_step(i)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     fam = {}
@@ -228,6 +270,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_dropin_voxels_per_s": plain,
+        "serial_resident_voxels_per_s": voxels / (serial_ms * 1e-3),
+        "serial_resident_ms_per_step": serial_ms,
         "kernels": fam,
     }
     print(json.dumps(out))

@@ -181,27 +181,35 @@ typedef struct steam_flux_ledger {
 #define STEAM_MEASURE 0 /* take every reduction, fill *ledger */
 #define STEAM_APPLY 1   /* inject the reductions from *ledger */
 
-/* One fused class advance of flux (nx, ny, nz) in place:
- *   f = sigma_in * flux                 (lazily applied previous rescale)
+#define STEAM_DEFER_RESCALE 1 /* leave the new rescale pending (in the ledger) */
+#define STEAM_CHAIN_PENDING 2 /* read the pending rescale from the ledger the
+                                 previous advance left in this workspace */
+
+/* Multiplier-noise field of one advance, (nx, ny, nz) floats into out_dev
+ * (the VALU-bound half of the advance).  It does not read the flux, so a
+ * caller may compute it ahead on another stream and pass it to
+ * steam_flux_advance as noise_dev. */
+int steam_flux_noise(const steam_conv_plan* plan, const steam_noise_params* noise,
+                     float* out_dev, steam_stream_t stream);
+
+/* One class advance of flux (nx, ny, nz) in place
+ * (steam/simulate.py:2066-2228):
+ *   f = sigma_in * flux                 (pending rescale applied on read)
  *   noise = multiplier noise            (regenerated from counters)
  *   flux  = max(f + kernel (*) (noise f), 0)
- *   flux *= rescale                      (unless defer_rescale)
+ *   flux *= rescale                      (unless STEAM_DEFER_RESCALE)
  * window = {x0, x1, y0, y1} restricts the volume means and the noise mean to
  * the domain (steam/simulate.py:2052-2063); NULL = whole array.
- * sk_dev (optional) receives S_k = noise f / <|noise|>.  With defer_rescale
- * the flux is left unscaled and the factor is only reported in *ledger (the
- * caller folds it into the next read).  Synchronises iff ledger_out != NULL
- * in measure mode (the ledger is copied back to the host).
- * chain_pending = 1 takes the pending rescale (or the flatten of an
- * all-clipped class) from the ledger the previous advance left in this
- * workspace instead of sigma_in, so consecutive deferred advances on one
- * flux need no host round trip. */
+ * sk_dev (optional) receives S_k = noise f / <|noise|>.  noise_dev
+ * (optional) is the class's noise field from steam_flux_noise; NULL
+ * generates it here.  flags: see above.  Synchronises iff ledger != NULL in
+ * measure mode (the ledger is copied back to the host). */
 int steam_flux_advance(const steam_conv_plan* plan, const void* spectrum_dev,
                        void* workspace_dev, float* flux_dev, float sigma_in,
-                       int chain_pending, const steam_noise_params* noise,
+                       const steam_noise_params* noise, const float* noise_dev,
                        const int64_t* window, int mode,
-                       steam_flux_ledger* ledger, float* sk_dev,
-                       int defer_rescale, steam_stream_t stream);
+                       steam_flux_ledger* ledger, float* sk_dev, int flags,
+                       steam_stream_t stream);
 
 /* Device-side copy of the last advance's ledger (inside workspace_dev). */
 int steam_flux_ledger_read(const steam_conv_plan* plan,
@@ -244,7 +252,8 @@ int steam_sum_f64(const float* data_dev, int64_t nx, int64_t ny, int64_t nz,
 #define STEAM_KF_ZINV 3     /* z inverse pass (c2r, overlap-add, epilogue) */
 #define STEAM_KF_FINALIZE 4 /* ledger reduction */
 #define STEAM_KF_POST 5     /* rescale / S_k scale */
-#define STEAM_KF_COUNT 6
+#define STEAM_KF_NOISE 6    /* multiplier-noise field */
+#define STEAM_KF_COUNT 7
 
 /* Start (on != 0) or stop recording; starting clears earlier records. */
 int steam_profile_enable(int on);

@@ -81,8 +81,9 @@ _SIGNATURES = [
     ("steam_conv_plan_info", ctypes.c_int, [_P, ctypes.POINTER(ConvInfo)]),
     ("steam_conv_set_kernel", ctypes.c_int, [_P, _P, _P, _P, _P]),
     ("steam_conv_execute", ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    ("steam_flux_noise", ctypes.c_int, [_P, ctypes.POINTER(NoiseParams), _P, _P]),
     ("steam_flux_advance", ctypes.c_int,
-     [_P, _P, _P, _P, _F32, ctypes.c_int, ctypes.POINTER(NoiseParams),
+     [_P, _P, _P, _P, _F32, ctypes.POINTER(NoiseParams), _P,
       ctypes.POINTER(_I64), ctypes.c_int, ctypes.POINTER(FluxLedger), _P,
       ctypes.c_int, _P]),
     ("steam_flux_ledger_read", ctypes.c_int,
@@ -99,7 +100,9 @@ _SIGNATURES = [
 ]
 
 KERNEL_FAMILIES = {'zfwd': 0, 'ypass': 1, 'xpass': 2, 'zinv': 3,
-                   'finalize': 4, 'post': 5}
+                   'finalize': 4, 'post': 5, 'noise': 6}
+DEFER_RESCALE = 1
+CHAIN_PENDING = 2
 
 EXPORTED = tuple(name for name, _, _ in _SIGNATURES)
 

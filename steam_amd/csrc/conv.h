@@ -76,13 +76,14 @@ struct PassDesc {
   NoiseDev noise;
   int win[4];           // x0, x1, y0, y1
   float* sk;            // optional S_k output
+  float* nfield;        // multiplier-noise field [nx][ny][nz] (workspace)
   double* part_z;       // [wg_z][3]: entering, |noise|, nonzero count
   double* part_o;       // [wg_z][2]: realized, clipped count
   DevLedger* led;
   DevLedger led_in;     // apply mode values
   float kscale;         // kernel-spectrum normalisation (MODE_KERNEL)
-  int wg_base[6];       // batched: first workgroup of z, y, x, zinv, post,
-                        // finalize passes
+  int wg_base[7];       // batched: first workgroup of z, y, x, zinv, post,
+                        // finalize, noise passes
 };
 
 }  // namespace steam

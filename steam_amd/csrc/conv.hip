@@ -34,6 +34,13 @@
 #endif
 
 namespace steam {
+// z passes: 512 threads holding 8 values each (same tile as 256 x 16, half
+// the registers per thread)
+constexpr int Z_NTH = 512;
+constexpr int Z_EM = 8;
+}
+
+namespace steam {
 
 // ---------------------------------------------------------------------------
 // descriptor sources: one instance by value, or a batch in device memory
@@ -61,22 +68,22 @@ struct BatchSrc {
 };
 
 enum { PASS_Z = 0, PASS_Y = 1, PASS_X = 2, PASS_ZINV = 3, PASS_POST = 4,
-       PASS_FIN = 5 };
+       PASS_FIN = 5, PASS_NOISE = 6 };
 
-// workgroup sum of up to 3 doubles, result valid in thread 0 (fixed order)
-template <int NV>
+// workgroup sum of NV doubles over NT threads, result valid in thread 0
+// (fixed order: independent of launch composition)
+template <int NV, int NT>
 __device__ void block_sum(double (&v)[NV], double* red) {
   const int tid = threadIdx.x;
-  for (int i = 0; i < NV; ++i) red[i * FFT_NTH + tid] = v[i];
+  for (int i = 0; i < NV; ++i) red[i * NT + tid] = v[i];
   __syncthreads();
-  for (int s = FFT_NTH / 2; s > 0; s >>= 1) {
+  for (int s = NT / 2; s > 0; s >>= 1) {
     if (tid < s)
-      for (int i = 0; i < NV; ++i)
-        red[i * FFT_NTH + tid] += red[i * FFT_NTH + tid + s];
+      for (int i = 0; i < NV; ++i) red[i * NT + tid] += red[i * NT + tid + s];
     __syncthreads();
   }
   if (tid == 0)
-    for (int i = 0; i < NV; ++i) v[i] = red[i * FFT_NTH];
+    for (int i = 0; i < NV; ++i) v[i] = red[i * NT];
 }
 
 // The entering flux value of a stored cell: the pending rescale of the
@@ -116,10 +123,35 @@ __device__ __forceinline__ float kernel_gather(const PassDesc& d, int x, int y,
 }
 
 // ---------------------------------------------------------------------------
+// noise: the unit-mean multiplier noise field of one advance (VALU-bound:
+// Philox + CMS + expm1 per centre).  Independent of the flux, so it may run
+// ahead of the advance on another stream.  4 lines per workgroup, threads
+// along z (coalesced stores).
+// ---------------------------------------------------------------------------
+template <class Src>
+__global__ __launch_bounds__(256) void noise_field_kernel(Src src) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_NOISE, wg);
+  const ConvGeom& g = d.g;
+  const int nz = g.nz;
+  const int64_t lines = (int64_t)g.nx * g.ny;
+  const int lpw = 4;
+  for (int l = 0; l < lpw; ++l) {
+    const int64_t line = (int64_t)wg * lpw + l;
+    if (line >= lines) break;
+    const int x = (int)(line / g.ny);
+    const int y = (int)(line - (int64_t)x * g.ny);
+    const LineNoise ln = noise_line(d.noise, x, y);
+    float* o = d.nfield + line * nz;
+    for (int z = threadIdx.x; z < nz; z += blockDim.x) o[z] = noise_at(d.noise, ln, z, nz);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // zfwd: real z-lines -> spectral planes
 // ---------------------------------------------------------------------------
 template <class Src, int MODE, int RS>
-__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) void zfwd_kernel(Src src) {
+__global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void zfwd_kernel(Src src) {
   int wg = blockIdx.x;
   const PassDesc& d = src.get(PASS_Z, wg);
   const ConvGeom& g = d.g;
@@ -146,13 +178,9 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) vo
       const int y = y0 + t;
       const bool yv = y < g.ny;
       const int64_t line = ((int64_t)x * g.ny + y) * nzin;
-      LineNoise ln;
       bool win = false;
-      if (MODE == MODE_FLUX && yv) {
-        ln = noise_line(d.noise, x, y);
-        win = xin && y >= d.win[2] && y < d.win[3];
-      }
-      for (int m = tid; m < g.nfft; m += FFT_NTH) {
+      if (MODE == MODE_FLUX && yv) win = xin && y >= d.win[2] && y < d.win[3];
+      for (int m = tid; m < g.nfft; m += Z_NTH) {
         const int z = z0 + m;
         float v = 0.f;
         if (yv && m < L && z < nzin) {
@@ -160,7 +188,7 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) vo
             v = d.in[line + z];
           } else if (MODE == MODE_FLUX) {
             float f = entering_value(pd, d.out[line + z]);
-            float nse = noise_at(d.noise, ln, z, g.nz);
+            const float nse = d.nfield[line + z];
             float p = __fmul_rn(nse, f);
             if (win) {
               acc[0] += (double)f;
@@ -177,10 +205,10 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) vo
       }
     }
     __syncthreads();
-    fft_lines<true, RS, Z_STAGE_CALLS>(smem, g.az, g.Tz, g.logTz, ldn, 1, false, tid);
+    fft_lines<true, RS, Z_STAGE_CALLS, Z_EM>(smem, g.az, g.Tz, g.logTz, ldn, 1, false, tid);
     // r2c split: X2[k] = (Z[k] + conj Z[n2-k]) - i W^k (Z[k] - conj Z[n2-k])
     const int nh = g.n2 / 2 + 1;
-    for (int e = tid; e < g.Tz * nh; e += FFT_NTH) {
+    for (int e = tid; e < g.Tz * nh; e += Z_NTH) {
       const int t = e & (g.Tz - 1);
       const int k = e >> g.logTz;
       float2* Lp = smem + t;
@@ -206,7 +234,7 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) vo
     }
     __syncthreads();
     float2* Sb = d.S + (int64_t)b * g.nzc * g.nx * g.ny;
-    for (int e = tid; e < g.nzc * g.Tz; e += FFT_NTH) {
+    for (int e = tid; e < g.nzc * g.Tz; e += Z_NTH) {
       const int kz = e >> g.logTz;
       const int t = e & (g.Tz - 1);
       const int y = y0 + t;
@@ -215,8 +243,8 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) vo
     __syncthreads();
   }
   if (MODE == MODE_FLUX && !d.apply) {
-    __shared__ double red[3 * FFT_NTH];
-    block_sum<3>(acc, red);
+    __shared__ double red[3 * Z_NTH];
+    block_sum<3, Z_NTH>(acc, red);
     if (tid == 0) {
       d.part_z[wg * 3 + 0] = acc[0];
       d.part_z[wg * 3 + 1] = acc[1];
@@ -377,7 +405,7 @@ __global__ __launch_bounds__(1024) void xpass_reg_kernel(Src src) {
 // zinv: planes -> c2r per block -> overlap-add in LDS -> epilogue
 // ---------------------------------------------------------------------------
 template <class Src, int MODE, int RS>
-__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) void zinv_kernel(Src src) {
+__global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void zinv_kernel(Src src) {
   int wg = blockIdx.x;
   const PassDesc& d = src.get(PASS_ZINV, wg);
   const ConvGeom& g = d.g;
@@ -391,13 +419,13 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) vo
   float* accz = reinterpret_cast<float*>(smem + (g.n2 + 1) * ldn);
   float* comp = accz + g.Tz * nz;
   const bool kahan = d.kahan != 0;
-  for (int e = tid; e < g.Tz * nz; e += FFT_NTH) {
+  for (int e = tid; e < g.Tz * nz; e += Z_NTH) {
     accz[e] = 0.f;
     if (kahan) comp[e] = 0.f;
   }
   for (int b = 0; b < g.nblocks; ++b) {
     const float2* Sb = d.S + (int64_t)b * g.nzc * g.nx * g.ny;
-    for (int e = tid; e < g.nzc * g.Tz; e += FFT_NTH) {
+    for (int e = tid; e < g.nzc * g.Tz; e += Z_NTH) {
       const int kz = e >> g.logTz;
       const int t = e & (g.Tz - 1);
       const int y = y0 + t;
@@ -407,7 +435,7 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) vo
     __syncthreads();
     // c2r merge: Z[k] = (Y[k] + conj Y[n2-k]) + i conj(W^k) (Y[k] - conj Y[n2-k])
     const int nh = g.n2 / 2 + 1;
-    for (int e = tid; e < g.Tz * nh; e += FFT_NTH) {
+    for (int e = tid; e < g.Tz * nh; e += Z_NTH) {
       const int t = e & (g.Tz - 1);
       const int k = e >> g.logTz;
       float2* Lp = smem + t;
@@ -433,10 +461,10 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) vo
       }
     }
     __syncthreads();
-    fft_lines<true, RS, Z_STAGE_CALLS>(smem, g.az, g.Tz, g.logTz, ldn, 1, true, tid);
+    fft_lines<true, RS, Z_STAGE_CALLS, Z_EM>(smem, g.az, g.Tz, g.logTz, ldn, 1, true, tid);
     const int zb = b * g.L - g.cz;
     for (int t = 0; t < g.Tz; ++t) {
-      for (int m = tid; m < g.nfft; m += FFT_NTH) {
+      for (int m = tid; m < g.nfft; m += Z_NTH) {
         const int zo = zb + m;
         if (zo >= 0 && zo < nz) {
           const float v = sr[((m >> 1) * ldn + t) * 2 + (m & 1)];
@@ -464,10 +492,10 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) vo
     if (y >= g.ny) break;
     const int64_t line = ((int64_t)x * g.ny + y) * nz;
     if (MODE == MODE_PLAIN) {
-      for (int z = tid; z < nz; z += FFT_NTH) d.out[line + z] = accz[t * nz + z];
+      for (int z = tid; z < nz; z += Z_NTH) d.out[line + z] = accz[t * nz + z];
     } else {
       const bool win = xin && y >= d.win[2] && y < d.win[3];
-      for (int z = tid; z < nz; z += FFT_NTH) {
+      for (int z = tid; z < nz; z += Z_NTH) {
         float f = entering_value(pd, d.out[line + z]);
         f = __fadd_rn(f, accz[t * nz + z]);
         if (f < 0.f) {
@@ -480,8 +508,8 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(2))) vo
     }
   }
   if (MODE == MODE_FLUX && !d.apply) {
-    __shared__ double red[2 * FFT_NTH];
-    block_sum<2>(acc, red);
+    __shared__ double red[2 * Z_NTH];
+    block_sum<2, Z_NTH>(acc, red);
     if (tid == 0) {
       d.part_o[wg * 2 + 0] = acc[0];
       d.part_o[wg * 2 + 1] = acc[1];
@@ -509,7 +537,7 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
       v[3] += d.part_o[i * 2 + 0];
       v[4] += d.part_o[i * 2 + 1];
     }
-    block_sum<5>(v, red);
+    block_sum<5, FFT_NTH>(v, red);
     if (tid != 0) return;
     const double cnt = (double)(d.win[1] - d.win[0]) * (double)(d.win[3] - d.win[2]) *
                        (double)g.nz;
@@ -622,7 +650,8 @@ struct steam_conv_plan {
   int precision;
   float2* tw_dev;    // twiddle tables
   int64_t nx, ny, nz, kx, ky, kz;
-  int64_t s_bytes, pz_off, po_off, led_off, ws_bytes, spec_bytes;
+  int64_t s_bytes, pz_off, po_off, led_off, nf_off, ws_bytes, spec_bytes;
+  int wg_noise;
   size_t lds_z, lds_zinv, lds_y, lds_x;
   int wg_post;
 };
@@ -785,7 +814,9 @@ extern "C" int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz,
   p->pz_off = p->s_bytes;
   p->po_off = p->pz_off + align256((int64_t)g.wg_z * 3 * 8);
   p->led_off = p->po_off + align256((int64_t)g.wg_z * 2 * 8);
-  p->ws_bytes = p->led_off + align256(sizeof(DevLedger));
+  p->nf_off = p->led_off + align256(sizeof(DevLedger));
+  p->ws_bytes = p->nf_off + align256(nx * ny * nz * 4);
+  p->wg_noise = (int)((nx * ny + 3) / 4);
   p->spec_bytes = align256((int64_t)g.nzc * nx * ny * 8);
   p->wg_post = (int)((nx * ny * nz + 4 * FFT_NTH - 1) / (4 * FFT_NTH));
   *out = p;
@@ -822,6 +853,7 @@ static PassDesc base_desc(const steam_conv_plan* p, void* ws) {
   d.part_z = (double*)(w + p->pz_off);
   d.part_o = (double*)(w + p->po_off);
   d.led = (DevLedger*)(w + p->led_off);
+  d.nfield = (float*)(w + p->nf_off);
   d.kahan = p->precision == STEAM_PREC_KAHAN;
   d.win[0] = 0;
   d.win[1] = (int)p->nx;
@@ -887,7 +919,7 @@ static int run_operator(const steam_conv_plan* p, const PassDesc& d,
   {
     ProfScope ps(STEAM_KF_ZFWD, s);
     STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zfwd_kernel<SingleSrc, MODE, RS_>),
-                                              dim3(g.wg_z), dim3(FFT_NTH), p->lds_z, s, src));
+                                              dim3(g.wg_z), dim3(Z_NTH), p->lds_z, s, src));
   }
   STEAM_LAUNCH_CHECK("zfwd_kernel");
   {
@@ -908,7 +940,7 @@ static int run_operator(const steam_conv_plan* p, const PassDesc& d,
   {
     ProfScope ps(STEAM_KF_ZINV, s);
     STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zinv_kernel<SingleSrc, MODE, RS_>),
-                                              dim3(g.wg_z), dim3(FFT_NTH), p->lds_zinv, s, src));
+                                              dim3(g.wg_z), dim3(Z_NTH), p->lds_zinv, s, src));
   }
   STEAM_LAUNCH_CHECK("zinv_kernel");
   return STEAM_OK;
@@ -935,7 +967,7 @@ extern "C" int steam_conv_set_kernel(const steam_conv_plan* p,
   const ConvGeom& g = d.g;
   const int rsz = radix_set(g.az);
   STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zfwd_kernel<SingleSrc, MODE_KERNEL, RS_>),
-                                            dim3(g.wg_z), dim3(FFT_NTH), p->lds_z, s, src));
+                                            dim3(g.wg_z), dim3(Z_NTH), p->lds_z, s, src));
   STEAM_LAUNCH_CHECK("zfwd_kernel(kernel)");
   launch_ypass(g, p->lds_y, s, src, 0);
   STEAM_LAUNCH_CHECK("ypass_kernel(kernel)");
@@ -1027,14 +1059,38 @@ static void ledger_to_host(const DevLedger& L, steam_flux_ledger* h) {
 
 }  // namespace steam
 
+extern "C" int steam_flux_noise(const steam_conv_plan* p,
+                                const steam_noise_params* noise, float* out_dev,
+                                steam_stream_t stream) {
+  if (!p || !out_dev)
+    return set_error(STEAM_EINVAL, "steam_flux_noise: NULL argument");
+  int rc = check_noise_params(noise);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  PassDesc d;
+  memset(&d, 0, sizeof(d));
+  d.g = p->g;
+  d.nfield = out_dev;
+  d.noise = make_noise_dev(noise);
+  {
+    ProfScope ps(STEAM_KF_NOISE, s);
+    hipLaunchKernelGGL((noise_field_kernel<SingleSrc>), dim3(p->wg_noise), dim3(256), 0,
+                       s, SingleSrc{d});
+  }
+  STEAM_LAUNCH_CHECK("noise_field_kernel");
+  return STEAM_OK;
+}
+
 extern "C" int steam_flux_advance(const steam_conv_plan* p,
                                   const void* spectrum_dev, void* workspace_dev,
                                   float* flux_dev, float sigma_in,
-                                  int chain_pending,
                                   const steam_noise_params* noise,
+                                  const float* noise_dev,
                                   const int64_t* window, int mode,
                                   steam_flux_ledger* ledger, float* sk_dev,
-                                  int defer_rescale, steam_stream_t stream) {
+                                  int flags, steam_stream_t stream) {
+  const int defer_rescale = (flags & STEAM_DEFER_RESCALE) != 0;
+  const int chain_pending = (flags & STEAM_CHAIN_PENDING) != 0;
   if (!p || !spectrum_dev || !workspace_dev || !flux_dev)
     return set_error(STEAM_EINVAL, "steam_flux_advance: NULL argument");
   int rc = check_noise_params(noise);
@@ -1061,6 +1117,14 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
     for (int i = 0; i < 4; ++i) d.win[i] = (int)window[i];
   }
   if (d.apply) d.led_in = ledger_to_dev(ledger);
+  if (noise_dev) {
+    d.nfield = const_cast<float*>(noise_dev);
+  } else {
+    ProfScope ps(STEAM_KF_NOISE, s);
+    hipLaunchKernelGGL((noise_field_kernel<SingleSrc>), dim3(p->wg_noise), dim3(256), 0,
+                       s, SingleSrc{d});
+  }
+  STEAM_LAUNCH_CHECK("noise_field_kernel");
   rc = run_operator<MODE_FLUX>(p, d, s);
   if (rc) return rc;
   SingleSrc src{d};

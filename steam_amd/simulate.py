@@ -285,12 +285,18 @@ def advance_flux_device(flux_t, region, kernel_t, flux_noise_scale,
                         zero_bottom=False, zero_top=False, window=None,
                         ledger=None, alpha=None, sigma_in=1.0, sk_t=None,
                         defer_rescale=False, precision='plain', spec=None,
-                        op=None, sync_ledger=True, chain_pending=False):
-    """Fused class advance of a device-resident flux (float32 tensor, in
-    place).  ``sigma_in`` is a pending float32 rescale applied on read;
-    with ``defer_rescale`` the new rescale is left pending (returned in the
-    ledger) instead of applied.  Returns a FluxAdvanceLedger (None when
-    ``sync_ledger`` is False in measure mode: the ledger stays on the device).
+                        op=None, sync_ledger=True, chain_pending=False,
+                        noise_t=None):
+    """Class advance of a device-resident flux (float32 tensor, in place).
+
+    ``sigma_in`` is a pending float32 rescale applied on read; with
+    ``defer_rescale`` the new rescale is left pending (in the ledger) instead
+    of applied, and ``chain_pending`` takes the pending rescale from the
+    ledger the previous advance on this operator left on the device.
+    ``noise_t`` is the class's noise field from generate_noise_device (None
+    generates it inside the advance).  Returns (FluxAdvanceLedger, native
+    ledger); both None when ``sync_ledger`` is False in measure mode (the
+    ledger then stays on the device).
     """
     lib = _lib.load()
     a, pcs, shift = class_scale_shift(flux_noise_scale, n_scale_classes_per_dyad,
@@ -309,18 +315,36 @@ def advance_flux_device(flux_t, region, kernel_t, flux_noise_scale,
         L = _lib.FluxLedger()
         mode = _lib.MEASURE
     want_host = ledger is None and sync_ledger
+    flags = ((_lib.DEFER_RESCALE if defer_rescale else 0)
+             | (_lib.CHAIN_PENDING if chain_pending else 0))
     _lib.check(lib.steam_flux_advance(
         op.plan, _lib.ptr(spec), _lib.ptr(ws), _lib.ptr(flux_t),
-        float(np.float32(sigma_in)), 1 if chain_pending else 0, ctypes.byref(p),
-        _native_window(window),
-        mode, ctypes.byref(L) if (want_host or ledger is not None) else None,
-        _lib.ptr(sk_t), 1 if defer_rescale else 0, _lib.stream_handle()),
-        "steam_flux_advance")
+        float(np.float32(sigma_in)), ctypes.byref(p), _lib.ptr(noise_t),
+        _native_window(window), mode,
+        ctypes.byref(L) if (want_host or ledger is not None) else None,
+        _lib.ptr(sk_t), flags, _lib.stream_handle()), "steam_flux_advance")
     if ledger is not None:
         return ledger, L
     if not want_host:
         return None, None
     return FluxAdvanceLedger.from_native(L), L
+
+
+def generate_noise_device(op, region, flux_noise_scale, n_scale_classes_per_dyad,
+                          sparsity_factors, out, n_zero=0, zero_bottom=False,
+                          zero_top=False, alpha=None, stream=None):
+    """The multiplier-noise half of an advance into ``out`` (float32 tensor of
+    the field's shape) on ``stream`` (default: current).  It does not read the
+    flux, so it can run ahead of the advance on another stream."""
+    lib = _lib.load()
+    a, pcs, shift = class_scale_shift(flux_noise_scale, n_scale_classes_per_dyad,
+                                      alpha)
+    p = noise_params(region, sparsity_factors, a, pcs, shift, n_zero,
+                     zero_bottom, zero_top)
+    _lib.check(lib.steam_flux_noise(op.plan, ctypes.byref(p), _lib.ptr(out),
+                                    _lib.stream_handle(stream)),
+               "steam_flux_noise")
+    return out
 
 
 def read_device_ledger(flux_shape, kernel_shape, precision='plain'):

@@ -158,3 +158,33 @@ def test_flux_only_validation_errors():
     np.testing.assert_array_equal(f, 1.0)
     f, d = simulate_flux_only(g, seed=3, flux_noise_scale=5.0)
     assert d['n_clipped'] > 0 and np.all(f >= 0)
+
+
+def test_pregenerated_noise_and_chained_rescale_are_bit_identical():
+    """The pipelined form (noise generated ahead into a caller buffer, the
+    pending rescale chained on the device) reproduces the plain form bit for
+    bit over three deferred advances."""
+    from steam_amd import simulate as S
+    from steam_amd.utils import conv_operator
+    shape = (48, 40, 36)
+    kern = S._turbulon_envelope(1, .5, .5, .5, as_tensor=True)
+    op = conv_operator(shape, tuple(kern.shape))
+    regions = [_region(M.class_key(sq), (0, 0, 0), shape)
+               for sq in np.random.SeedSequence(4).spawn(3)]
+    f1 = torch.ones(shape, device='cuda')
+    sigma = 1.0
+    for r in regions:
+        led, L = S.advance_flux_device(f1, r, kern, 0.3, 1, (1, 1, 1), sigma_in=sigma,
+                                       defer_rescale=True, op=op)
+        sigma = np.float32(L.rescale)
+    f2 = torch.ones(shape, device='cuda')
+    buf = torch.empty(shape, device='cuda')
+    for i, r in enumerate(regions):
+        S.generate_noise_device(op, r, 0.3, 1, (1, 1, 1), buf)
+        S.advance_flux_device(f2, r, kern, 0.3, 1, (1, 1, 1), defer_rescale=True,
+                              op=op, chain_pending=i > 0, noise_t=buf,
+                              sync_ledger=False)
+    led2, L2 = S.read_device_ledger(shape, tuple(kern.shape))
+    assert np.float32(L2.rescale) == sigma
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(f1.cpu().numpy(), f2.cpu().numpy())
