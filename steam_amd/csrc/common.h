@@ -69,10 +69,12 @@ __device__ __forceinline__ void philox_fast(uint32_t& c0, uint32_t& c1,
       k0 += PHILOX_W0;
       k1 += PHILOX_W1;
     }
-    uint32_t hi0 = __umulhi(PHILOX_M0, c0);
-    uint32_t lo0 = PHILOX_M0 * c0;
-    uint32_t hi1 = __umulhi(PHILOX_M1, c2);
-    uint32_t lo1 = PHILOX_M1 * c2;
+    // one 32x32->64 multiply per product (v_mad_u64_u32) instead of separate
+    // high and low multiplies
+    const uint64_t p0 = (uint64_t)PHILOX_M0 * c0;
+    const uint64_t p1 = (uint64_t)PHILOX_M1 * c2;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
     uint32_t n0 = hi1 ^ c1 ^ k0;
     uint32_t n2 = hi0 ^ c3 ^ k1;
     c0 = n0;

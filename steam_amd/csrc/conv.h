@@ -21,6 +21,9 @@ struct ConvGeom {
   int wg_z, wg_y, wg_x; // workgroups per pass
   int reg_y, reg_x;     // log2 length of the register FFT path (0: generic)
   int Tr;               // columns per workgroup in the register x pass
+  int tile_z;           // single-block z passes with vector tile loads
+  int zreg;             // single-block register z passes (compiled n2)
+  int wg_zr;            // workgroups of the register z passes
   AxisPlan ax, ay, az;
   const float2* wr2c;   // n2+1 entries exp(-2 pi i k / nfft)
 };
@@ -82,6 +85,7 @@ struct PassDesc {
   DevLedger* led;
   DevLedger led_in;     // apply mode values
   float kscale;         // kernel-spectrum normalisation (MODE_KERNEL)
+  int npart;            // partial-sum slots written by the z passes
   int wg_base[7];       // batched: first workgroup of z, y, x, zinv, post,
                         // finalize, noise passes
 };

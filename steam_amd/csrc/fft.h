@@ -1,6 +1,6 @@
 // LDS-resident multi-line Stockham FFT for gfx950 (64-wide wavefronts).
 //
-// A workgroup of nth threads holds T lines of length N (N*T <= EM*nth complex
+// A workgroup of NTH threads holds T lines of length N (N*T <= EM*nth complex
 // values; EM values per thread) in LDS and transforms all of them together.  Mixed radix
 // 2/3/4/5/7/8 stages (Stockham autosort, natural order in and out); a length
 // with a prime factor above 7 is transformed by one direct-DFT stage.  Each
@@ -125,14 +125,14 @@ __device__ __forceinline__ void line_index(int b, int T, int logT, int NR,
   }
 }
 
-template <int R, bool TFAST, int EM>
+template <int R, bool TFAST, int EM, int NTH>
 __device__ __forceinline__ void fft_stage(float2* buf, int N, int T, int logT, int ldn,
                           int ldt, int Ns, const float2* __restrict__ tw,
                           bool inv, int tid) {
   constexpr int MAXB = (EM + R - 1) / R;
   const int NR = N / R;
   const int nb = NR * T;
-  const int nth = blockDim.x;
+  constexpr int nth = NTH;
   float2 v[MAXB][R];
 #pragma unroll
   for (int i = 0; i < MAXB; ++i) {
@@ -172,22 +172,22 @@ __device__ __forceinline__ void fft_stage(float2* buf, int N, int T, int logT, i
   __syncthreads();
 }
 
-template <int R, bool TFAST, int EM>
+template <int R, bool TFAST, int EM, int NTH>
 __device__ __noinline__ void fft_stage_call(float2* buf, int N, int T, int logT,
                                            int ldn, int ldt, int Ns,
                                            const float2* __restrict__ tw,
                                            bool inv, int tid) {
-  fft_stage<R, TFAST, EM>(buf, N, T, logT, ldn, ldt, Ns, tw, inv, tid);
+  fft_stage<R, TFAST, EM, NTH>(buf, N, T, logT, ldn, ldt, Ns, tw, inv, tid);
 }
 
 // Whole-line direct DFT (lengths with a prime factor > 7).
-template <bool TFAST, int EM>
+template <bool TFAST, int EM, int NTH>
 __device__ __noinline__ void fft_direct(float2* buf, int N, int T, int logT, int ldn,
                            int ldt, const float2* __restrict__ tw, bool inv,
                            int tid) {
   float2 o[EM];
   const int ne = N * T;
-  const int nth = blockDim.x;
+  constexpr int nth = NTH;
 #pragma unroll
   for (int i = 0; i < EM; ++i) {
     int e = tid + i * nth;
@@ -236,7 +236,7 @@ inline int radix_set(const AxisPlan& ap) {
 }
 
 // Transform T lines of length ap.n held in LDS.  Caller synchronises before.
-template <bool TFAST, int RSET, bool CALLS = false, int EM = FFT_EMAX>
+template <bool TFAST, int RSET, bool CALLS = false, int EM = FFT_EMAX, int NTH = FFT_NTH>
 __device__ __forceinline__ void fft_lines(float2* buf, const AxisPlan& ap, int T,
                                           int logT, int ldn, int ldt, bool inv,
                                           int tid) {
@@ -247,22 +247,22 @@ __device__ __forceinline__ void fft_lines(float2* buf, const AxisPlan& ap, int T
     const int R = ap.radix[s];
     if (RSET < 2 && !CALLS) {
       switch (R) {
-        case 2: fft_stage<2, TFAST, EM>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-        case 4: fft_stage<4, TFAST, EM>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-        case 8: fft_stage<8, TFAST, EM>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-        case 3: if (RSET >= 1) fft_stage<3, TFAST, EM>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-        case 5: if (RSET >= 1) fft_stage<5, TFAST, EM>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 2: fft_stage<2, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 4: fft_stage<4, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 8: fft_stage<8, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 3: if (RSET >= 1) fft_stage<3, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 5: if (RSET >= 1) fft_stage<5, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
         default: break;
       }
     } else {
       switch (R) {
-        case 2: fft_stage_call<2, TFAST, EM>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-        case 3: fft_stage_call<3, TFAST, EM>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-        case 4: fft_stage_call<4, TFAST, EM>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-        case 5: fft_stage_call<5, TFAST, EM>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-        case 7: fft_stage_call<7, TFAST, EM>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-        case 8: fft_stage_call<8, TFAST, EM>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-        default: fft_direct<TFAST, EM>(buf, ap.n, T, logT, ldn, ldt, ap.tw, inv, tid); break;
+        case 2: fft_stage_call<2, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 3: fft_stage_call<3, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 4: fft_stage_call<4, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 5: fft_stage_call<5, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 7: fft_stage_call<7, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        case 8: fft_stage_call<8, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
+        default: fft_direct<TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, ap.tw, inv, tid); break;
       }
     }
     Ns *= (R == 0) ? ap.n : R;
