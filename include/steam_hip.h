@@ -140,6 +140,14 @@ int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz, int64_t kx,
                            int precision, steam_conv_plan** plan);
 int steam_conv_plan_destroy(steam_conv_plan* plan);
 
+/* The z transform lengths a plan may use for (nz, kz), best model cost
+ * first (spectral planes, favouring lengths with a register z plan); the
+ * first is what n_fft = 0 selects.  Writes up to max_out values.  The
+ * reference picks its block length from a memory budget instead
+ * (steam/utils.py:35-131); an autotuner times the leading candidates. */
+int steam_conv_nfft_candidates(int64_t nz, int64_t kz, int64_t* out,
+                               int32_t max_out, int32_t* count);
+
 typedef struct steam_conv_info {
   int64_t n_fft;           /* z transform length */
   int64_t n_blocks;        /* overlap-add blocks along z */
@@ -215,6 +223,52 @@ int steam_flux_advance(const steam_conv_plan* plan, const void* spectrum_dev,
 int steam_flux_ledger_read(const steam_conv_plan* plan,
                            const void* workspace_dev, steam_flux_ledger* out,
                            steam_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Batched advance: n independent instances (realisations or tiles of
+ * different sizes, alpha and noise scale), each exactly the measure-mode
+ * steam_flux_advance of its own plan, advanced by one launch per pass and
+ * compiled kernel variant.  The reference advances instances one call at a
+ * time (steam/simulate.py:2066-2228 per class and realisation); the batch
+ * returns, per instance, the same result as that call.
+ * ---------------------------------------------------------------------- */
+typedef struct steam_batch steam_batch;
+
+typedef struct steam_batch_item {
+  const steam_conv_plan* plan;
+  const void* spectrum_dev;   /* kernel spectrum of this instance's plan */
+  void* workspace_dev;        /* this instance's own workspace */
+  float* flux_dev;            /* (nx, ny, nz), advanced in place */
+  const float* noise_dev;     /* precomputed noise field, or NULL */
+  float* sk_dev;              /* optional S_k output, or NULL */
+  steam_noise_params noise;
+  int64_t window[4];          /* x0, x1, y0, y1; all zero = whole field */
+  float sigma_in;             /* pending rescale applied on read */
+  int32_t flags;              /* STEAM_DEFER_RESCALE | STEAM_CHAIN_PENDING */
+} steam_batch_item;
+
+/* Group the instances by kernel variant and stage their descriptors
+ * (pinned host copy + device copy).  Allocates. */
+int steam_batch_create(const steam_batch_item* items, int32_t n,
+                       steam_batch** batch);
+int steam_batch_destroy(steam_batch* batch);
+
+/* Replace instance i's keyed-noise parameters for the next advance (waits
+ * for the previous advance's descriptor upload to finish). */
+int steam_batch_set_noise(steam_batch* batch, int32_t i,
+                          const steam_noise_params* noise);
+
+/* Kernel launches per advance (descriptor upload included) and the bytes of
+ * descriptors uploaded per advance. */
+int steam_batch_info(const steam_batch* batch, int32_t* launches,
+                     int64_t* descriptor_bytes);
+
+/* One measure-mode advance of every instance on stream: a host-to-device
+ * descriptor copy followed by the pass launches; nothing else is enqueued
+ * and nothing is allocated, so the call may be captured into a hipGraph
+ * (replays read the pinned descriptor copy as it is at replay time).  Each
+ * instance's ledger stays in its workspace (steam_flux_ledger_read). */
+int steam_batch_advance(steam_batch* batch, steam_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * (d) regrid: periodic trilinear upsample (steam/utils.py:435-534) and its

@@ -58,6 +58,13 @@ class FluxLedger(ctypes.Structure):
                 ("n_clipped", _I64), ("rescale", _F32), ("flatten", _I32)]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("plan", _P), ("spectrum_dev", _P), ("workspace_dev", _P),
+                ("flux_dev", _P), ("noise_dev", _P), ("sk_dev", _P),
+                ("noise", NoiseParams), ("window", _I64 * 4), ("sigma_in", _F32),
+                ("flags", _I32)]
+
+
 # (name, restype, argtypes)
 _SIGNATURES = [
     ("steam_version", ctypes.c_int, []),
@@ -78,6 +85,8 @@ _SIGNATURES = [
      [_I64, _I64, _I64, _I64, _I64, _I64, _I64, ctypes.c_int,
       ctypes.POINTER(_P)]),
     ("steam_conv_plan_destroy", ctypes.c_int, [_P]),
+    ("steam_conv_nfft_candidates", ctypes.c_int,
+     [_I64, _I64, ctypes.POINTER(_I64), _I32, ctypes.POINTER(_I32)]),
     ("steam_conv_plan_info", ctypes.c_int, [_P, ctypes.POINTER(ConvInfo)]),
     ("steam_conv_set_kernel", ctypes.c_int, [_P, _P, _P, _P, _P]),
     ("steam_conv_execute", ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
@@ -88,6 +97,12 @@ _SIGNATURES = [
       ctypes.c_int, _P]),
     ("steam_flux_ledger_read", ctypes.c_int,
      [_P, _P, ctypes.POINTER(FluxLedger), _P]),
+    ("steam_batch_create", ctypes.c_int,
+     [ctypes.POINTER(BatchItem), _I32, ctypes.POINTER(_P)]),
+    ("steam_batch_destroy", ctypes.c_int, [_P]),
+    ("steam_batch_set_noise", ctypes.c_int, [_P, _I32, ctypes.POINTER(NoiseParams)]),
+    ("steam_batch_info", ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),
+    ("steam_batch_advance", ctypes.c_int, [_P, _P]),
     ("steam_regrid_trilinear", ctypes.c_int,
      [_P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64,
       ctypes.POINTER(_I32), _F32, _P, _P]),

@@ -1,0 +1,234 @@
+"""Batched flux advance and the FFT schedule autotuner.
+
+The reference advances one realisation or tile per call
+(steam/simulate.py:2066-2228).  ``BatchedAdvance`` advances many independent
+instances of different size, stability index and noise scale together: one
+launch per pass and compiled kernel variant over all instances
+(``steam_batch_advance``), optionally captured once into a hipGraph and
+replayed.  Each instance's result is the one the single-instance advance
+(``simulate.advance_flux_device``) gives with the same plan.
+
+``autotune_nfft`` times the planner's leading z transform lengths for a
+(field, kernel) shape pair and keeps the fastest; the choice fixes the radix
+schedule of the z passes (register plan or LDS stages) and the number of
+spectral planes of the x/y passes.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .ledger import FluxAdvanceLedger
+from .simulate import class_scale_shift, noise_params
+from .utils import conv_operator, kernel_key
+
+_TUNED = {}
+
+
+def nfft_candidates(nz, kz, max_out=8):
+    """Usable z transform lengths for (nz, kz), best model cost first."""
+    lib = _lib.load()
+    out = (ctypes.c_int64 * max_out)()
+    n = ctypes.c_int32()
+    _lib.check(lib.steam_conv_nfft_candidates(int(nz), int(kz), out, max_out,
+                                              ctypes.byref(n)),
+               "steam_conv_nfft_candidates")
+    return [int(out[i]) for i in range(n.value)]
+
+
+def autotune_nfft(field_shape, kernel_shape, top=3, reps=5, table=None):
+    """Fastest z transform length among the ``top`` model candidates, timed
+    with HIP events on the convolution operator (cached per shape pair).
+
+    Returns (n_fft, {n_fft: ms}).
+    """
+    table = _TUNED if table is None else table
+    key = (tuple(int(n) for n in field_shape), tuple(int(n) for n in kernel_shape))
+    if key in table:
+        return table[key]
+    _lib.require_gpu('cuda')
+    cands = nfft_candidates(field_shape[2], kernel_shape[2], max_out=top)
+    gen = torch.Generator(device='cuda')
+    gen.manual_seed(0)
+    field = torch.randn(key[0], device='cuda', generator=gen)
+    kern = torch.randn(key[1], device='cuda', generator=gen)
+    out = torch.empty_like(field)
+    times = {}
+    for n in cands:
+        op = conv_operator(key[0], key[1], n_fft=n)
+        spec = op.spectrum(kern)
+        op.execute(field, out, spec)
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(reps):
+            op.execute(field, out, spec)
+        stop.record()
+        stop.synchronize()
+        times[n] = start.elapsed_time(stop) / reps
+    best = min(times, key=times.get)
+    table[key] = (best, times)
+    return best, times
+
+
+class Instance:
+    """One member of a batch: its flux state (device tensor, advanced in
+    place), kernel (device tensor), noise region and class parameters."""
+
+    def __init__(self, flux, kernel, region, flux_noise_scale,
+                 n_scale_classes_per_dyad=1, sparsity_factors=(1, 1, 1),
+                 alpha=None, n_zero=0, zero_bottom=False, zero_top=False,
+                 window=None, defer_rescale=False, chain_pending=False):
+        if not (isinstance(flux, torch.Tensor) and flux.is_cuda
+                and flux.dtype == torch.float32 and flux.is_contiguous()):
+            raise ValueError("batched instances need contiguous float32 device "
+                             "tensors for the flux")
+        self.flux = flux
+        self.kernel = kernel.contiguous()
+        self.region = region
+        self.flux_noise_scale = flux_noise_scale
+        self.n_scale_classes_per_dyad = n_scale_classes_per_dyad
+        self.sparsity_factors = tuple(sparsity_factors)
+        self.alpha = alpha
+        self.n_zero = n_zero
+        self.zero_bottom = zero_bottom
+        self.zero_top = zero_top
+        self.window = window
+        self.defer_rescale = defer_rescale
+        self.chain_pending = chain_pending
+
+    def native_noise(self, region=None):
+        a, pcs, shift = class_scale_shift(self.flux_noise_scale,
+                                          self.n_scale_classes_per_dyad, self.alpha)
+        return noise_params(self.region if region is None else region,
+                            self.sparsity_factors, a, pcs, shift, self.n_zero,
+                            self.zero_bottom, self.zero_top)
+
+
+class BatchedAdvance:
+    """Measure-mode class advance of many instances per call.
+
+    ``autotune`` chooses each distinct shape pair's z transform length by
+    timing (``autotune_nfft``); otherwise the planner's model choice is used.
+    Every instance owns its workspace, so its ledger can be read back after
+    any advance (``ledgers``).
+    """
+
+    def __init__(self, instances, autotune=True):
+        lib = _lib.load()
+        self.instances = list(instances)
+        if not self.instances:
+            raise ValueError("empty batch")
+        self.ops, self.specs, self.workspaces = [], [], []
+        self.n_fft = []
+        items = (_lib.BatchItem * len(self.instances))()
+        for i, inst in enumerate(self.instances):
+            shape, kshape = tuple(inst.flux.shape), tuple(inst.kernel.shape)
+            n_fft = autotune_nfft(shape, kshape)[0] if autotune else 0
+            op = conv_operator(shape, kshape, n_fft=n_fft)
+            spec = op.spectrum(inst.kernel, key=kernel_key(inst.kernel))
+            ws = torch.empty(op.workspace_bytes, dtype=torch.uint8, device='cuda')
+            self.ops.append(op)
+            self.specs.append(spec)
+            self.workspaces.append(ws)
+            self.n_fft.append(op.n_fft)
+            it = items[i]
+            it.plan = op.plan
+            it.spectrum_dev = spec.data_ptr()
+            it.workspace_dev = ws.data_ptr()
+            it.flux_dev = inst.flux.data_ptr()
+            it.noise_dev = None
+            it.sk_dev = None
+            it.noise = inst.native_noise()
+            if inst.window is not None:
+                for a in range(4):
+                    it.window[a] = int(inst.window[a])
+            it.sigma_in = 1.0
+            it.flags = ((_lib.DEFER_RESCALE if inst.defer_rescale else 0)
+                        | (_lib.CHAIN_PENDING if inst.chain_pending else 0))
+        self._items = items
+        handle = ctypes.c_void_p()
+        _lib.check(lib.steam_batch_create(items, len(self.instances),
+                                          ctypes.byref(handle)),
+                   "steam_batch_create")
+        self.handle = handle
+        launches = ctypes.c_int32()
+        desc = ctypes.c_int64()
+        _lib.check(lib.steam_batch_info(handle, ctypes.byref(launches),
+                                        ctypes.byref(desc)), "steam_batch_info")
+        self.launches_per_advance = launches.value
+        self.descriptor_bytes = desc.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.load().steam_batch_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    @property
+    def voxels(self):
+        return int(sum(inst.flux.numel() for inst in self.instances))
+
+    def set_regions(self, regions):
+        """New noise regions (keys) for the next advance, one per instance."""
+        lib = _lib.load()
+        for i, (inst, r) in enumerate(zip(self.instances, regions)):
+            p = inst.native_noise(r)
+            _lib.check(lib.steam_batch_set_noise(self.handle, i, ctypes.byref(p)),
+                       "steam_batch_set_noise")
+
+    def advance(self, stream=None):
+        """Advance every instance by one class on ``stream`` (default:
+        current); asynchronous."""
+        _lib.check(_lib.load().steam_batch_advance(self.handle,
+                                                   _lib.stream_handle(stream)),
+                   "steam_batch_advance")
+
+    def capture(self):
+        """A torch CUDAGraph (hipGraph) holding one advance."""
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.advance()  # warm the launch path outside the capture
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g):
+            self.advance()
+        return g
+
+    def ledgers(self):
+        """Each instance's ledger of the last advance (synchronises)."""
+        lib = _lib.load()
+        out = []
+        for op, ws in zip(self.ops, self.workspaces):
+            L = _lib.FluxLedger()
+            _lib.check(lib.steam_flux_ledger_read(op.plan, ws.data_ptr(),
+                                                  ctypes.byref(L),
+                                                  _lib.stream_handle()),
+                       "steam_flux_ledger_read")
+            out.append(FluxAdvanceLedger.from_native(L))
+        return out
+
+
+def draw_batch_config(n=48, seed=48):
+    """The batched workload's instance list: grid uniformly from
+    {32^3, 64^3, 96x96x48, 128^3, 192x192x96, 256^3}, alpha ~ U[1.5, 1.9],
+    C1 ~ U[0.05, 0.15] (c = C1^(1/alpha)), H ~ U[0.2, 0.5], seeds 0..n-1,
+    drawn in that order from default_rng(seed)."""
+    grids = [(32, 32, 32), (64, 64, 64), (96, 96, 48), (128, 128, 128),
+             (192, 192, 96), (256, 256, 256)]
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        g = grids[int(rng.integers(len(grids)))]
+        alpha = float(rng.uniform(1.5, 1.9))
+        c1 = float(rng.uniform(0.05, 0.15))
+        h = float(rng.uniform(0.2, 0.5))
+        out.append({"seed": i, "grid": g, "alpha": alpha, "C1": c1,
+                    "c": c1 ** (1.0 / alpha), "H": h})
+    return out

@@ -1,0 +1,387 @@
+// Batched flux advance: many independent instances (each with its own plan,
+// kernel spectrum, workspace, flux and keyed noise) advanced together.
+//
+// Every pass runs as one launch per kernel variant: the instances whose
+// geometry selects the same compiled kernel (register z plan, register or
+// generic x/y FFT) share a launch over the concatenation of their
+// workgroups; a workgroup finds its instance by a binary search over the
+// group's first-workgroup offsets.  Descriptors live in device memory and
+// are refreshed from a pinned host copy at the start of every advance, so
+// the whole advance can be captured once into a hipGraph and replayed with
+// new noise keys.
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "conv_plan.h"
+
+using namespace steam;
+
+namespace {
+
+enum BatchPass { BP_NOISE = 0, BP_Z, BP_Y, BP_X, BP_ZINV, BP_FIN, BP_POST, BP_COUNT };
+
+struct Group {
+  int offset;   // first descriptor of the group in the flat array
+  int count;    // instances
+  int wgs;      // total workgroups of the launch
+  int variant;  // kernel selector (pass-specific)
+  int arg;      // second selector (radix set, post rescale flag)
+  size_t lds;   // dynamic LDS of the generic kernels (group maximum)
+  int threads;  // block size of the register x pass
+};
+
+// one instance's choices, fixed at creation
+struct ItemPlan {
+  const steam_conv_plan* p;
+  bool zreg, tile, gen_noise, post, post_rescale;
+};
+
+}  // namespace
+
+struct steam_batch {
+  int n = 0;
+  std::vector<PassDesc> items;   // per-instance descriptors (host master copy)
+  std::vector<ItemPlan> plans;
+  std::vector<int> slot_item;    // flat descriptor slot -> instance
+  std::vector<Group> groups[BP_COUNT];
+  PassDesc* host = nullptr;      // pinned staging of the flat array
+  PassDesc* dev = nullptr;
+  int nslots = 0;
+  hipEvent_t uploaded = nullptr;
+  bool pending_upload = false;
+};
+
+namespace {
+
+int wg_count(const ItemPlan& ip, int pass) {
+  const ConvGeom& g = ip.p->g;
+  switch (pass) {
+    case BP_NOISE: return ip.p->wg_noise;
+    case BP_Z:
+    case BP_ZINV: return ip.zreg ? g.wg_zr : g.wg_z;
+    case BP_Y: return g.wg_y;
+    case BP_X: return g.wg_x;
+    case BP_FIN: return 1;
+    default: return ip.p->wg_post;
+  }
+}
+
+// pass slot used by the kernels' descriptor lookup
+int desc_pass(int pass) {
+  switch (pass) {
+    case BP_NOISE: return PASS_NOISE;
+    case BP_Z: return PASS_Z;
+    case BP_Y: return PASS_Y;
+    case BP_X: return PASS_X;
+    case BP_ZINV: return PASS_ZINV;
+    case BP_FIN: return PASS_FIN;
+    default: return PASS_POST;
+  }
+}
+
+// (variant, arg, threads) of an instance in a pass; variant < 0: not run
+std::tuple<int, int, int> variant_of(const ItemPlan& ip, int pass) {
+  const ConvGeom& g = ip.p->g;
+  switch (pass) {
+    case BP_NOISE: return {ip.gen_noise ? 0 : -1, 0, 256};
+    case BP_Z:
+    case BP_ZINV:
+      if (ip.zreg) return {1, g.n2, ZR_NTH};
+      return {ip.tile ? 2 : 3, radix_set(g.az), Z_NTH};
+    case BP_Y:
+      if (g.reg_y) return {1, g.reg_y, 256};
+      return {2, radix_set(g.ay), FFT_NTH};
+    case BP_X:
+      if (g.reg_x) return {1, g.reg_x, (g.nx / 8) * g.Tr};
+      return {2, radix_set(g.ax), FFT_NTH};
+    case BP_FIN: return {0, 0, FFT_NTH};
+    default: return {ip.post ? 0 : -1, ip.post_rescale ? 1 : 0, FFT_NTH};
+  }
+}
+
+size_t lds_of(const ItemPlan& ip, int pass) {
+  switch (pass) {
+    case BP_Z: return ip.zreg ? 0 : ip.p->lds_z;
+    case BP_ZINV: return ip.zreg ? 0 : ip.p->lds_zinv;
+    case BP_Y: return ip.p->g.reg_y ? 0 : ip.p->lds_y;
+    case BP_X: return ip.p->g.reg_x ? 0 : ip.p->lds_x;
+    default: return 0;
+  }
+}
+
+void build_groups(steam_batch* b) {
+  std::vector<PassDesc> flat;
+  b->slot_item.clear();
+  for (int pass = 0; pass < BP_COUNT; ++pass) {
+    std::map<std::tuple<int, int, int>, std::vector<int>> by;
+    for (int i = 0; i < b->n; ++i) {
+      auto v = variant_of(b->plans[i], pass);
+      if (std::get<0>(v) >= 0) by[v].push_back(i);
+    }
+    for (auto& kv : by) {
+      Group gr;
+      gr.offset = (int)flat.size();
+      gr.count = (int)kv.second.size();
+      gr.variant = std::get<0>(kv.first);
+      gr.arg = std::get<1>(kv.first);
+      gr.threads = std::get<2>(kv.first);
+      gr.lds = 0;
+      int base = 0;
+      for (int i : kv.second) {
+        PassDesc d = b->items[i];
+        d.wg_base[desc_pass(pass)] = base;
+        base += wg_count(b->plans[i], pass);
+        gr.lds = std::max(gr.lds, lds_of(b->plans[i], pass));
+        flat.push_back(d);
+        b->slot_item.push_back(i);
+      }
+      gr.wgs = base;
+      b->groups[pass].push_back(gr);
+    }
+  }
+  b->nslots = (int)flat.size();
+  memcpy(b->host, flat.data(), flat.size() * sizeof(PassDesc));
+}
+
+// refresh the flat array from the per-instance master copies (keeps each
+// slot's workgroup offsets)
+void refresh_slots(steam_batch* b) {
+  for (int s = 0; s < b->nslots; ++s) {
+    int wb[7];
+    memcpy(wb, b->host[s].wg_base, sizeof(wb));
+    b->host[s] = b->items[b->slot_item[s]];
+    memcpy(b->host[s].wg_base, wb, sizeof(wb));
+  }
+}
+
+int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
+  for (const Group& gr : b->groups[pass]) {
+    BatchSrc src{b->dev + gr.offset, gr.count};
+    const dim3 grid(gr.wgs);
+    switch (pass) {
+      case BP_NOISE:
+        hipLaunchKernelGGL((noise_field_kernel<BatchSrc>), grid, dim3(256), 0, s, src);
+        break;
+      case BP_Z:
+        if (gr.variant == 1) {
+          with_zplan(gr.arg, [&](auto zp) {
+            hipLaunchKernelGGL((zfwd_reg_kernel<BatchSrc, MODE_FLUX, decltype(zp)>), grid,
+                               dim3(ZR_NTH), 0, s, src);
+          });
+        } else if (gr.variant == 2) {
+          STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((zfwd_tile_kernel<BatchSrc, MODE_FLUX, RS_>),
+                                                       grid, dim3(Z_NTH), gr.lds, s, src));
+        } else {
+          STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((zfwd_kernel<BatchSrc, MODE_FLUX, RS_>),
+                                                       grid, dim3(Z_NTH), gr.lds, s, src));
+        }
+        break;
+      case BP_Y:
+        if (gr.variant == 1) {
+          STEAM_LOGN_DISPATCH(gr.arg, hipLaunchKernelGGL((ypass_reg_kernel<BatchSrc, LN_>), grid,
+                                                         dim3(256), 0, s, src, inv));
+        } else {
+          STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((ypass_kernel<BatchSrc, RS_>), grid,
+                                                       dim3(FFT_NTH), gr.lds, s, src, inv));
+        }
+        break;
+      case BP_X:
+        if (gr.variant == 1) {
+          STEAM_LOGN_DISPATCH(gr.arg, hipLaunchKernelGGL((xpass_reg_kernel<BatchSrc, 0, LN_>), grid,
+                                                         dim3(gr.threads), 0, s, src));
+        } else {
+          STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((xpass_kernel<BatchSrc, 0, RS_>), grid,
+                                                       dim3(FFT_NTH), gr.lds, s, src));
+        }
+        break;
+      case BP_ZINV:
+        if (gr.variant == 1) {
+          with_zplan(gr.arg, [&](auto zp) {
+            hipLaunchKernelGGL((zinv_reg_kernel<BatchSrc, MODE_FLUX, decltype(zp)>), grid,
+                               dim3(ZR_NTH), 0, s, src);
+          });
+        } else if (gr.variant == 2) {
+          STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((zinv_tile_kernel<BatchSrc, MODE_FLUX, RS_>),
+                                                       grid, dim3(Z_NTH), gr.lds, s, src));
+        } else {
+          STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((zinv_kernel<BatchSrc, MODE_FLUX, RS_>),
+                                                       grid, dim3(Z_NTH), gr.lds, s, src));
+        }
+        break;
+      case BP_FIN:
+        hipLaunchKernelGGL((finalize_kernel<BatchSrc>), grid, dim3(FFT_NTH), 0, s, src);
+        break;
+      default:
+        hipLaunchKernelGGL((post_kernel<BatchSrc>), grid, dim3(FFT_NTH), 0, s, src, gr.arg);
+        break;
+    }
+    STEAM_LAUNCH_CHECK("batched pass");
+  }
+  return STEAM_OK;
+}
+
+int make_item(const steam_batch_item& it, PassDesc& d, ItemPlan& ip) {
+  const steam_conv_plan* p = it.plan;
+  if (!p || !it.spectrum_dev || !it.workspace_dev || !it.flux_dev)
+    return set_error(STEAM_EINVAL, "steam_batch_create: NULL plan, spectrum, workspace or flux");
+  int rc = check_noise_params(&it.noise);
+  if (rc) return rc;
+  d = base_desc(p, it.workspace_dev);
+  d.mode = MODE_FLUX;
+  d.K = (const float2*)it.spectrum_dev;
+  d.out = it.flux_dev;
+  d.sigma_in = it.sigma_in;
+  d.pend = (it.flags & STEAM_CHAIN_PENDING)
+               ? (const DevLedger*)((const char*)it.workspace_dev + p->led_off)
+               : nullptr;
+  d.noise = make_noise_dev(&it.noise);
+  d.sk = it.sk_dev;
+  d.apply = 0;
+  const int64_t* w = it.window;
+  if (w[0] || w[1] || w[2] || w[3]) {
+    if (w[0] < 0 || w[1] > p->nx || w[0] >= w[1] || w[2] < 0 || w[3] > p->ny || w[2] >= w[3])
+      return set_error(STEAM_EINVAL, "steam_batch_create: window out of range");
+    for (int i = 0; i < 4; ++i) d.win[i] = (int)w[i];
+  }
+  if (it.noise_dev) d.nfield = const_cast<float*>(it.noise_dev);
+  auto a8 = [](const void* q) { return ((uintptr_t)q & 7) == 0; };
+  auto a16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
+  ip.p = p;
+  ip.zreg = p->g.zreg && a8(d.out) && a8(d.nfield) && a8(d.sk);
+  ip.tile = !ip.zreg && p->g.tile_z && a16(d.out) && a16(d.nfield) && a16(d.sk);
+  ip.gen_noise = it.noise_dev == nullptr;
+  const bool defer = (it.flags & STEAM_DEFER_RESCALE) != 0;
+  ip.post = !defer || it.sk_dev != nullptr;
+  ip.post_rescale = !defer;
+  d.npart = ip.zreg ? p->g.wg_zr : p->g.wg_z;
+  return STEAM_OK;
+}
+
+}  // namespace
+
+extern "C" int steam_batch_create(const steam_batch_item* items, int n,
+                                  steam_batch** out) {
+  if (!out) return set_error(STEAM_EINVAL, "steam_batch_create: NULL output");
+  *out = nullptr;
+  if (!items || n < 1) return set_error(STEAM_EINVAL, "steam_batch_create: no items");
+  steam_batch* b = new steam_batch();
+  b->n = n;
+  b->items.resize(n);
+  b->plans.resize(n);
+  for (int i = 0; i < n; ++i) {
+    int rc = make_item(items[i], b->items[i], b->plans[i]);
+    if (rc) {
+      delete b;
+      return rc;
+    }
+  }
+  // upper bound on descriptor slots: every pass holds each instance once
+  const size_t bytes = (size_t)BP_COUNT * n * sizeof(PassDesc);
+  hipError_t e = hipHostMalloc((void**)&b->host, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    delete b;
+    return check_hip(e, "hipHostMalloc(batch descriptors)");
+  }
+  e = hipMalloc((void**)&b->dev, bytes);
+  if (e != hipSuccess) {
+    (void)hipHostFree(b->host);
+    delete b;
+    return check_hip(e, "hipMalloc(batch descriptors)");
+  }
+  e = hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    (void)hipFree(b->dev);
+    (void)hipHostFree(b->host);
+    delete b;
+    return check_hip(e, "hipEventCreate(batch)");
+  }
+  build_groups(b);
+  *out = b;
+  return STEAM_OK;
+}
+
+extern "C" int steam_batch_destroy(steam_batch* b) {
+  if (!b) return STEAM_OK;
+  if (b->pending_upload) (void)hipEventSynchronize(b->uploaded);
+  if (b->uploaded) (void)hipEventDestroy(b->uploaded);
+  if (b->dev) (void)hipFree(b->dev);
+  if (b->host) (void)hipHostFree(b->host);
+  delete b;
+  return STEAM_OK;
+}
+
+extern "C" int steam_batch_set_noise(steam_batch* b, int i,
+                                     const steam_noise_params* noise) {
+  if (!b || i < 0 || i >= b->n)
+    return set_error(STEAM_EINVAL, "steam_batch_set_noise: bad batch or index");
+  int rc = check_noise_params(noise);
+  if (rc) return rc;
+  // the previous advance's upload must have read the staging copy
+  if (b->pending_upload) {
+    STEAM_HIP(hipEventSynchronize(b->uploaded));
+    b->pending_upload = false;
+  }
+  b->items[i].noise = make_noise_dev(noise);
+  refresh_slots(b);
+  return STEAM_OK;
+}
+
+extern "C" int steam_batch_info(const steam_batch* b, int* launches,
+                                int64_t* descriptor_bytes) {
+  if (!b) return set_error(STEAM_EINVAL, "steam_batch_info: NULL batch");
+  int n = 1;  // descriptor upload
+  for (int p = 0; p < BP_COUNT; ++p)
+    n += (int)b->groups[p].size() * (p == BP_Y ? 2 : 1);
+  if (launches) *launches = n;
+  if (descriptor_bytes) *descriptor_bytes = (int64_t)b->nslots * (int64_t)sizeof(PassDesc);
+  return STEAM_OK;
+}
+
+extern "C" int steam_batch_advance(steam_batch* b, steam_stream_t stream) {
+  if (!b) return set_error(STEAM_EINVAL, "steam_batch_advance: NULL batch");
+  hipStream_t s = (hipStream_t)stream;
+  STEAM_HIP(hipMemcpyAsync(b->dev, b->host, (size_t)b->nslots * sizeof(PassDesc),
+                           hipMemcpyHostToDevice, s));
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  STEAM_HIP(hipStreamIsCapturing(s, &cs));
+  if (cs == hipStreamCaptureStatusNone) {
+    STEAM_HIP(hipEventRecord(b->uploaded, s));
+    b->pending_upload = true;
+  }
+  int rc;
+  {
+    ProfScope ps(STEAM_KF_NOISE, s);
+    if ((rc = launch_pass(b, BP_NOISE, s, 0))) return rc;
+  }
+  {
+    ProfScope ps(STEAM_KF_ZFWD, s);
+    if ((rc = launch_pass(b, BP_Z, s, 0))) return rc;
+  }
+  {
+    ProfScope ps(STEAM_KF_YPASS, s);
+    if ((rc = launch_pass(b, BP_Y, s, 0))) return rc;
+  }
+  {
+    ProfScope ps(STEAM_KF_XPASS, s);
+    if ((rc = launch_pass(b, BP_X, s, 0))) return rc;
+  }
+  {
+    ProfScope ps(STEAM_KF_YPASS, s);
+    if ((rc = launch_pass(b, BP_Y, s, 1))) return rc;
+  }
+  {
+    ProfScope ps(STEAM_KF_ZINV, s);
+    if ((rc = launch_pass(b, BP_ZINV, s, 0))) return rc;
+  }
+  {
+    ProfScope ps(STEAM_KF_FINALIZE, s);
+    if ((rc = launch_pass(b, BP_FIN, s, 0))) return rc;
+  }
+  {
+    ProfScope ps(STEAM_KF_POST, s);
+    if ((rc = launch_pass(b, BP_POST, s, 0))) return rc;
+  }
+  return STEAM_OK;
+}

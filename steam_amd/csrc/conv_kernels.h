@@ -1,0 +1,1174 @@
+// FFT overlap-add convolution (periodic x/y, zero-padded z) and the fused
+// flux-cascade advance, for gfx950.
+//
+// Operator (steam/utils.py:293-427): out = field (*) kernel with circular
+// x/y over the whole extent, linear z with zero padding, kernel centred at
+// ((kx-1)/2, (ky-1)/2, (kz-1)/2) and folded onto the field when wider than
+// it (steam/utils.py:222-242).  Overlap-add runs along z over blocks of
+// L = nfft - kz + 1 input planes; x and y keep their exact periods.
+//
+// Five passes per operator application, all through LDS:
+//   zfwd  real z-lines -> half-length complex FFT -> r2c split -> planes
+//         S[block][kz][x][y] (the flux advance generates the keyed noise,
+//         multiplies it by the entering flux and takes its partial sums here)
+//   ypass contiguous y-lines, forward
+//   xpass x-lines (T-wide column tiles): forward, * kernel spectrum, inverse
+//   ypass inverse
+//   zinv  c2r per block, overlap-add of all blocks in LDS, one coalesced
+//         write per output line (the flux advance adds the increment, clips
+//         and takes the realized partial sums here)
+// Partial sums are per workgroup and reduced in a fixed order by one
+// workgroup, so results do not depend on launch composition.
+#pragma once
+
+#include <math.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "conv.h"
+#include "fft_reg.h"
+#include "fft_zreg.h"
+
+// z-pass FFT stages inlined (false) or called out of line (true)
+#ifndef Z_STAGE_CALLS
+#define Z_STAGE_CALLS false
+#endif
+
+namespace steam {
+// z passes: 512 threads holding 8 values each (same tile as 256 x 16, half
+// the registers per thread)
+constexpr int Z_NTH = 512;
+constexpr int Z_EM = 8;
+}
+
+namespace steam {
+
+// ---------------------------------------------------------------------------
+// descriptor sources: one instance by value, or a batch in device memory
+// ---------------------------------------------------------------------------
+struct SingleSrc {
+  PassDesc d;
+  __device__ __forceinline__ const PassDesc& get(int, int&) const { return d; }
+};
+
+struct BatchSrc {
+  const PassDesc* d;
+  int n;
+  __device__ __forceinline__ const PassDesc& get(int pass, int& wg) const {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (d[mid].wg_base[pass] <= wg)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    wg -= d[lo].wg_base[pass];
+    return d[lo];
+  }
+};
+
+enum { PASS_Z = 0, PASS_Y = 1, PASS_X = 2, PASS_ZINV = 3, PASS_POST = 4,
+       PASS_FIN = 5, PASS_NOISE = 6 };
+
+// workgroup sum of NV doubles over NT threads, result valid in thread 0
+// (fixed order: independent of launch composition)
+template <int NV, int NT>
+__device__ void block_sum(double (&v)[NV], double* red) {
+  const int tid = threadIdx.x;
+  for (int i = 0; i < NV; ++i) red[i * NT + tid] = v[i];
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s)
+      for (int i = 0; i < NV; ++i) red[i * NT + tid] += red[i * NT + tid + s];
+    __syncthreads();
+  }
+  if (tid == 0)
+    for (int i = 0; i < NV; ++i) v[i] = red[i * NT];
+}
+
+// The entering flux value of a stored cell: the pending rescale of the
+// previous advance (sigma_in, or chained from the ledger it left on the
+// device) applied on read, or the flat entering mean of an all-clipped class.
+struct Pending {
+  float sig, em;
+  int flat;
+};
+__device__ __forceinline__ Pending pending_of(const PassDesc& d) {
+  Pending p{d.sigma_in, 0.f, 0};
+  if (d.pend) {
+    p.flat = d.pend->flatten;
+    p.sig = d.pend->rescale;
+    p.em = d.pend->entering_mean_f;
+  }
+  return p;
+}
+__device__ __forceinline__ float entering_value(const Pending& p, float raw) {
+  return p.flat ? p.em : __fmul_rn(p.sig, raw);
+}
+
+// kp[x, y, z] of the rolled, folded kernel: taps i = x + offx (mod nx),
+// j = y + offy (mod ny), summed x-fold first then y-fold.
+__device__ __forceinline__ float kernel_gather(const PassDesc& d, int x, int y,
+                                               int z) {
+  const ConvGeom& g = d.g;
+  int i0 = (int)pmod((int64_t)x + g.offx, g.nx);
+  int j0 = (int)pmod((int64_t)y + g.offy, g.ny);
+  float acc = 0.f;
+  for (int j = j0; j < g.ky; j += g.ny) {
+    float s = 0.f;
+    for (int i = i0; i < g.kx; i += g.nx) s += d.in[((int64_t)i * g.ky + j) * g.kz + z];
+    acc += s;
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// noise: the unit-mean multiplier noise field of one advance (VALU-bound:
+// Philox + CMS + expm1 per centre).  Independent of the flux, so it may run
+// ahead of the advance on another stream.  4 lines per workgroup, threads
+// along z (coalesced stores).
+// ---------------------------------------------------------------------------
+template <class Src>
+__global__ __launch_bounds__(256) void noise_field_kernel(Src src) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_NOISE, wg);
+  const ConvGeom& g = d.g;
+  const int nz = g.nz;
+  const int64_t lines = (int64_t)g.nx * g.ny;
+  const int lpw = 4;
+  for (int l = 0; l < lpw; ++l) {
+    const int64_t line = (int64_t)wg * lpw + l;
+    if (line >= lines) break;
+    const int x = (int)(line / g.ny);
+    const int y = (int)(line - (int64_t)x * g.ny);
+    const LineNoise ln = noise_line(d.noise, x, y);
+    float* o = d.nfield + line * nz;
+    for (int z = threadIdx.x; z < nz; z += 256) o[z] = noise_at(d.noise, ln, z, nz);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// zfwd: real z-lines -> spectral planes
+// ---------------------------------------------------------------------------
+template <class Src, int MODE, int RS>
+__global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void zfwd_kernel(Src src) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_Z, wg);
+  const ConvGeom& g = d.g;
+  extern __shared__ float2 smem[];
+  const int tid = threadIdx.x;
+  const int x = wg / g.nyt_z;
+  const int y0 = (wg - x * g.nyt_z) * g.Tz;
+  const int ldn = g.Tz + 1;  // element (line t, index n) at n*ldn + t
+  float* sr = reinterpret_cast<float*>(smem);
+  const int nzin = (MODE == MODE_KERNEL) ? g.kz : g.nz;
+  const int nblk = (MODE == MODE_KERNEL) ? 1 : g.nblocks;
+  const int L = (MODE == MODE_KERNEL) ? g.nfft : g.L;
+  double acc[3] = {0.0, 0.0, 0.0};
+  bool xin = false;
+  Pending pd{1.f, 0.f, 0};
+  if (MODE == MODE_FLUX) {
+    xin = x >= d.win[0] && x < d.win[1];
+    pd = pending_of(d);
+  }
+
+  for (int b = 0; b < nblk; ++b) {
+    const int z0 = b * L;
+    for (int t = 0; t < g.Tz; ++t) {
+      const int y = y0 + t;
+      const bool yv = y < g.ny;
+      const int64_t line = ((int64_t)x * g.ny + y) * nzin;
+      bool win = false;
+      if (MODE == MODE_FLUX && yv) win = xin && y >= d.win[2] && y < d.win[3];
+      for (int m = tid; m < g.nfft; m += Z_NTH) {
+        const int z = z0 + m;
+        float v = 0.f;
+        if (yv && m < L && z < nzin) {
+          if (MODE == MODE_PLAIN) {
+            v = d.in[line + z];
+          } else if (MODE == MODE_FLUX) {
+            float f = entering_value(pd, d.out[line + z]);
+            const float nse = d.nfield[line + z];
+            float p = __fmul_rn(nse, f);
+            if (win) {
+              acc[0] += (double)f;
+              acc[1] += (double)fabsf(nse);
+              acc[2] += (nse != 0.f) ? 1.0 : 0.0;
+            }
+            if (d.sk) d.sk[line + z] = p;
+            v = p;
+          } else {
+            v = kernel_gather(d, x, y, z);
+          }
+        }
+        sr[((m >> 1) * ldn + t) * 2 + (m & 1)] = v;
+      }
+    }
+    __syncthreads();
+    fft_lines<true, RS, Z_STAGE_CALLS, Z_EM, Z_NTH>(smem, g.az, g.Tz, g.logTz, ldn, 1, false, tid);
+    // r2c split: X2[k] = (Z[k] + conj Z[n2-k]) - i W^k (Z[k] - conj Z[n2-k])
+    const int nh = g.n2 / 2 + 1;
+    for (int e = tid; e < g.Tz * nh; e += Z_NTH) {
+      const int t = e & (g.Tz - 1);
+      const int k = e >> g.logTz;
+      float2* Lp = smem + t;
+      if (k == 0) {
+        float2 z0v = Lp[0];
+        Lp[0] = make_float2(2.f * (z0v.x + z0v.y), 0.f);
+        Lp[g.n2 * ldn] = make_float2(2.f * (z0v.x - z0v.y), 0.f);
+      } else {
+        const int km = g.n2 - k;
+        float2 zk = Lp[k * ldn], zm = Lp[km * ldn];
+        float2 w = g.wr2c[k];
+        float2 A = make_float2(zk.x + zm.x, zk.y - zm.y);
+        float2 B = make_float2(zk.x - zm.x, zk.y + zm.y);
+        float2 wB = cmul(w, B);
+        Lp[k * ldn] = make_float2(A.x + wB.y, A.y - wB.x);
+        if (km != k) {
+          float2 A2 = make_float2(zm.x + zk.x, zm.y - zk.y);
+          float2 B2 = make_float2(zm.x - zk.x, zm.y + zk.y);
+          float2 wB2 = cmul(make_float2(-w.x, w.y), B2);
+          Lp[km * ldn] = make_float2(A2.x + wB2.y, A2.y - wB2.x);
+        }
+      }
+    }
+    __syncthreads();
+    float2* Sb = d.S + (int64_t)b * g.nzc * g.nx * g.ny;
+    for (int e = tid; e < g.nzc * g.Tz; e += Z_NTH) {
+      const int kz = e >> g.logTz;
+      const int t = e & (g.Tz - 1);
+      const int y = y0 + t;
+      if (y < g.ny) Sb[((int64_t)kz * g.nx + x) * g.ny + y] = smem[kz * ldn + t];
+    }
+    __syncthreads();
+  }
+  if (MODE == MODE_FLUX && !d.apply) {
+    __shared__ double red[3 * Z_NTH];
+    block_sum<3, Z_NTH>(acc, red);
+    if (tid == 0) {
+      d.part_z[wg * 3 + 0] = acc[0];
+      d.part_z[wg * 3 + 1] = acc[1];
+      d.part_z[wg * 3 + 2] = acc[2];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ypass: contiguous y-lines (Ty per workgroup), in place
+// ---------------------------------------------------------------------------
+template <class Src, int RS>
+__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void ypass_kernel(Src src, int inv) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_Y, wg);
+  const ConvGeom& g = d.g;
+  extern __shared__ float2 smem[];
+  const int tid = threadIdx.x;
+  const int P = (d.mode == MODE_KERNEL) ? g.nzc : g.nblocks * g.nzc;
+  const int64_t lines = (int64_t)P * g.nx;
+  const int64_t l0 = (int64_t)wg * g.Ty;
+  const int ny = g.ny;
+  const int ldn = g.Ty + 1;
+  for (int t = 0; t < g.Ty; ++t) {
+    const int64_t l = l0 + t;
+    const float2* srcl = d.S + l * ny;
+    for (int n = tid; n < ny; n += FFT_NTH)
+      smem[n * ldn + t] = (l < lines) ? srcl[n] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lines<true, RS>(smem, g.ay, g.Ty, g.logTy, ldn, 1, inv != 0, tid);
+  for (int t = 0; t < g.Ty; ++t) {
+    const int64_t l = l0 + t;
+    if (l >= lines) break;
+    float2* dstl = d.S + l * ny;
+    for (int n = tid; n < ny; n += FFT_NTH) dstl[n] = smem[n * ldn + t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// xpass: x-lines in column tiles of Tx; XMODE 0: fwd * K inv, 1: fwd * scale
+// ---------------------------------------------------------------------------
+template <class Src, int XMODE, int RS>
+__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void xpass_kernel(Src src) {
+  int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const PassDesc& d = src.get(PASS_X, wg);
+  const ConvGeom& g = d.g;
+  extern __shared__ float2 smem[];
+  const int tid = threadIdx.x;
+  const int p = wg / g.nyt_x;
+  const int y0 = (wg - p * g.nyt_x) * g.Tx;
+  const int nx = g.nx, ny = g.ny, Tx = g.Tx, logTx = g.logTx;
+  float2* Sp = d.S + (int64_t)p * nx * ny;
+  const int ne = nx * Tx;
+  for (int e = tid; e < ne; e += FFT_NTH) {
+    const int t = e & (Tx - 1);
+    const int n = e >> logTx;
+    const int y = y0 + t;
+    smem[e] = (y < ny) ? Sp[(int64_t)n * ny + y] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lines<true, RS>(smem, g.ax, Tx, logTx, Tx, 1, false, tid);
+  if (XMODE == 0) {
+    const int kzp = p % g.nzc;
+    const float2* Kp = d.K + (int64_t)kzp * nx * ny;
+    for (int e = tid; e < ne; e += FFT_NTH) {
+      const int t = e & (Tx - 1);
+      const int n = e >> logTx;
+      const int y = y0 + t;
+      if (y < ny) smem[e] = cmul(smem[e], Kp[(int64_t)n * ny + y]);
+    }
+    __syncthreads();
+    fft_lines<true, RS>(smem, g.ax, Tx, logTx, Tx, 1, true, tid);
+  } else {
+    for (int e = tid; e < ne; e += FFT_NTH) smem[e] = cscale(smem[e], d.kscale);
+    __syncthreads();
+  }
+  for (int e = tid; e < ne; e += FFT_NTH) {
+    const int t = e & (Tx - 1);
+    const int n = e >> logTx;
+    const int y = y0 + t;
+    if (y < ny) Sp[(int64_t)n * ny + y] = smem[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// register-FFT y pass (power-of-two ny): P = ny/8 threads per contiguous
+// line, 256/P lines per workgroup, HBM -> registers -> HBM
+// ---------------------------------------------------------------------------
+template <class Src, int LOGN>
+__global__ __launch_bounds__(256) void ypass_reg_kernel(Src src, int inv) {
+  using F = RegFFT<LOGN>;
+  constexpr int P = F::P, LPW = 256 / P;
+  __shared__ float2 lds[LPW * F::LDSN];
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_Y, wg);
+  const ConvGeom& g = d.g;
+  const int tid = threadIdx.x;
+  const int q = tid % P, lw = tid / P;
+  const int planes = (d.mode == MODE_KERNEL) ? g.nzc : g.nblocks * g.nzc;
+  const int64_t lines = (int64_t)planes * g.nx;
+  const int64_t line = (int64_t)wg * LPW + lw;
+  const bool valid = line < lines;
+  float2* Sl = d.S + (valid ? line : 0) * F::N;
+  float2 v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = valid ? Sl[q + m * P] : make_float2(0.f, 0.f);
+  fft_reg<LOGN>(v, q, lds, 1, lw * F::LDSN, g.ay.tw, inv != 0);
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) Sl[q + m * P] = v[m];
+  }
+}
+
+// register-FFT x pass (power-of-two nx): P = nx/8 threads per x-line, Tr
+// adjacent y columns per workgroup (lane t = column, so each row access is a
+// Tr*8-byte run); XMODE 0: forward, * kernel spectrum, inverse; 1: forward,
+// * scale.
+template <class Src, int XMODE, int LOGN>
+__global__ __launch_bounds__(1024) void xpass_reg_kernel(Src src) {
+  using F = RegFFT<LOGN>;
+  constexpr int P = F::P;
+  constexpr int T = (1024 / P < 8) ? 1024 / P : 8;
+  __shared__ float2 lds[T * F::LDSN];
+  int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const PassDesc& d = src.get(PASS_X, wg);
+  const ConvGeom& g = d.g;
+  const int tid = threadIdx.x;
+  const int t = tid % T, q = tid / T;
+  const int nyt = (g.ny + T - 1) / T;
+  const int p = wg / nyt;
+  const int y = (wg - p * nyt) * T + t;
+  const bool valid = y < g.ny;
+  const int nx = g.nx, ny = g.ny;
+  float2* Sp = d.S + (int64_t)p * nx * ny + (valid ? y : 0);
+  float2 v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m)
+    v[m] = valid ? Sp[(int64_t)(q + m * P) * ny] : make_float2(0.f, 0.f);
+  fft_reg<LOGN>(v, q, lds, T, t, g.ax.tw, false);
+  if (XMODE == 0) {
+    const float2* Kp = d.K + (int64_t)(p % g.nzc) * nx * ny + (valid ? y : 0);
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      v[m] = cmul(v[m], valid ? Kp[(int64_t)(q + m * P) * ny] : make_float2(0.f, 0.f));
+    fft_reg<LOGN>(v, q, lds, T, t, g.ax.tw, true);
+  } else {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = cscale(v[m], d.kscale);
+  }
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) Sp[(int64_t)(q + m * P) * ny] = v[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// zinv: planes -> c2r per block -> overlap-add in LDS -> epilogue
+// ---------------------------------------------------------------------------
+template <class Src, int MODE, int RS>
+__global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void zinv_kernel(Src src) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_ZINV, wg);
+  const ConvGeom& g = d.g;
+  extern __shared__ float2 smem[];
+  const int tid = threadIdx.x;
+  const int x = wg / g.nyt_z;
+  const int y0 = (wg - x * g.nyt_z) * g.Tz;
+  const int ldn = g.Tz + 1;
+  const int nz = g.nz;
+  float* sr = reinterpret_cast<float*>(smem);
+  float* accz = reinterpret_cast<float*>(smem + (g.n2 + 1) * ldn);
+  float* comp = accz + g.Tz * nz;
+  const bool kahan = d.kahan != 0;
+  for (int e = tid; e < g.Tz * nz; e += Z_NTH) {
+    accz[e] = 0.f;
+    if (kahan) comp[e] = 0.f;
+  }
+  for (int b = 0; b < g.nblocks; ++b) {
+    const float2* Sb = d.S + (int64_t)b * g.nzc * g.nx * g.ny;
+    for (int e = tid; e < g.nzc * g.Tz; e += Z_NTH) {
+      const int kz = e >> g.logTz;
+      const int t = e & (g.Tz - 1);
+      const int y = y0 + t;
+      smem[kz * ldn + t] = (y < g.ny) ? Sb[((int64_t)kz * g.nx + x) * g.ny + y]
+                                      : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    // c2r merge: Z[k] = (Y[k] + conj Y[n2-k]) + i conj(W^k) (Y[k] - conj Y[n2-k])
+    const int nh = g.n2 / 2 + 1;
+    for (int e = tid; e < g.Tz * nh; e += Z_NTH) {
+      const int t = e & (g.Tz - 1);
+      const int k = e >> g.logTz;
+      float2* Lp = smem + t;
+      if (k == 0) {
+        float2 Y0 = Lp[0], YN = Lp[g.n2 * ldn];
+        float2 A = make_float2(Y0.x + YN.x, Y0.y - YN.y);
+        float2 B = make_float2(Y0.x - YN.x, Y0.y + YN.y);
+        Lp[0] = make_float2(A.x - B.y, A.y + B.x);
+      } else {
+        const int km = g.n2 - k;
+        float2 Yk = Lp[k * ldn], Ym = Lp[km * ldn];
+        float2 w = g.wr2c[k];
+        float2 A = make_float2(Yk.x + Ym.x, Yk.y - Ym.y);
+        float2 B = make_float2(Yk.x - Ym.x, Yk.y + Ym.y);
+        float2 cB = cmul(make_float2(w.x, -w.y), B);  // conj(W^k) B
+        Lp[k * ldn] = make_float2(A.x - cB.y, A.y + cB.x);  // A + i cB
+        if (km != k) {
+          float2 A2 = make_float2(Ym.x + Yk.x, Ym.y - Yk.y);
+          float2 B2 = make_float2(Ym.x - Yk.x, Ym.y + Yk.y);
+          float2 cB2 = cmul(make_float2(-w.x, -w.y), B2);  // conj(W^{n2-k}) = -W^k
+          Lp[km * ldn] = make_float2(A2.x - cB2.y, A2.y + cB2.x);
+        }
+      }
+    }
+    __syncthreads();
+    fft_lines<true, RS, Z_STAGE_CALLS, Z_EM, Z_NTH>(smem, g.az, g.Tz, g.logTz, ldn, 1, true, tid);
+    const int zb = b * g.L - g.cz;
+    for (int t = 0; t < g.Tz; ++t) {
+      for (int m = tid; m < g.nfft; m += Z_NTH) {
+        const int zo = zb + m;
+        if (zo >= 0 && zo < nz) {
+          const float v = sr[((m >> 1) * ldn + t) * 2 + (m & 1)];
+          const int ia = t * nz + zo;
+          if (!kahan) {
+            accz[ia] += v;
+          } else {
+            float s = accz[ia];
+            float yv = v - comp[ia];
+            float tt = s + yv;
+            comp[ia] = (tt - s) - yv;
+            accz[ia] = tt;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double acc[2] = {0.0, 0.0};
+  const bool xin = (MODE == MODE_FLUX) && x >= d.win[0] && x < d.win[1];
+  Pending pd{1.f, 0.f, 0};
+  if (MODE == MODE_FLUX) pd = pending_of(d);
+  for (int t = 0; t < g.Tz; ++t) {
+    const int y = y0 + t;
+    if (y >= g.ny) break;
+    const int64_t line = ((int64_t)x * g.ny + y) * nz;
+    if (MODE == MODE_PLAIN) {
+      for (int z = tid; z < nz; z += Z_NTH) d.out[line + z] = accz[t * nz + z];
+    } else {
+      const bool win = xin && y >= d.win[2] && y < d.win[3];
+      for (int z = tid; z < nz; z += Z_NTH) {
+        float f = entering_value(pd, d.out[line + z]);
+        f = __fadd_rn(f, accz[t * nz + z]);
+        if (f < 0.f) {
+          acc[1] += 1.0;
+          f = 0.f;
+        }
+        if (win) acc[0] += (double)f;
+        d.out[line + z] = f;
+      }
+    }
+  }
+  if (MODE == MODE_FLUX && !d.apply) {
+    __shared__ double red[2 * Z_NTH];
+    block_sum<2, Z_NTH>(acc, red);
+    if (tid == 0) {
+      d.part_o[wg * 2 + 0] = acc[0];
+      d.part_o[wg * 2 + 1] = acc[1];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Single-block z passes (the whole z extent fits one transform, nz % 4 == 0,
+// the tile's T*nz values fit one batch of ZT_U float4 per thread).  The tile of
+// T adjacent lines is one contiguous run of T*nz floats, read with 16-byte
+// loads all issued before the first use; twiddles are staged in LDS; zinv
+// prefetches the flux tile before its transforms, and reads the trimmed
+// convolution straight out of the transform buffer (one block: no overlap-add).
+// ---------------------------------------------------------------------------
+constexpr int ZT_U = 4;   // float4 groups per thread (tile <= 16 * Z_NTH floats)
+constexpr int ZT_S = 8;   // float2 spectral values per thread per batch
+
+// deterministic workgroup sum (fixed shuffle tree per wave, then the waves in
+// order); result valid in thread 0; red holds NV * waves doubles
+template <int NV, int NT>
+__device__ __forceinline__ void block_sum_w(double (&v)[NV], double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = NT >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = v[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    v[i] = s;
+  }
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[w * NV + i] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < nw; ++k) s += red[k * NV + i];
+      v[i] = s;
+    }
+  }
+}
+
+// LDS layout of the tile kernels: transform buffer [(n2+1) * (T+1)] complex,
+// then z twiddles [n2], then r2c twiddles [n2/2 + 1].
+__device__ __forceinline__ void stage_twiddles(const ConvGeom& g, float2* twz,
+                                               float2* twr, int tid) {
+  for (int i = tid; i < g.n2; i += Z_NTH) twz[i] = g.az.tw[i];
+  for (int i = tid; i <= g.n2 / 2; i += Z_NTH) twr[i] = g.wr2c[i];
+}
+
+template <class Src, int MODE, int RS>
+__global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void zfwd_tile_kernel(Src src) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_Z, wg);
+  const ConvGeom& g = d.g;
+  extern __shared__ float2 smem[];
+  const int tid = threadIdx.x;
+  const int T = g.Tz, logT = g.logTz, ldn = T + 1;
+  const int x = wg / g.nyt_z;
+  const int y0 = (wg - x * g.nyt_z) * T;
+  const int nz = g.nz, n2 = g.n2;
+  float2* twz = smem + (n2 + 1) * ldn;
+  float2* twr = twz + n2;
+  const int tl = min(T, g.ny - y0);
+  const int64_t base = ((int64_t)x * g.ny + y0) * nz;
+  const int n4 = (tl * nz) >> 2;
+  const float* srcp = (MODE == MODE_FLUX) ? d.out : d.in;
+  const float4* in4 = reinterpret_cast<const float4*>(srcp + base);
+  const float4* nf4 = reinterpret_cast<const float4*>(d.nfield + base);
+  float4 fv[ZT_U], nv[ZT_U];
+#pragma unroll
+  for (int i = 0; i < ZT_U; ++i) {
+    const int e = i * Z_NTH + tid;
+    if (e < n4) {
+      fv[i] = in4[e];
+      if (MODE == MODE_FLUX) nv[i] = nf4[e];
+    }
+  }
+  stage_twiddles(g, twz, twr, tid);
+  double acc[3] = {0.0, 0.0, 0.0};
+  Pending pd{1.f, 0.f, 0};
+  bool xin = false;
+  if (MODE == MODE_FLUX) {
+    pd = pending_of(d);
+    xin = x >= d.win[0] && x < d.win[1];
+  }
+  float4* sk4 = (MODE == MODE_FLUX && d.sk) ? reinterpret_cast<float4*>(d.sk + base) : nullptr;
+#pragma unroll
+  for (int i = 0; i < ZT_U; ++i) {
+    const int e = i * Z_NTH + tid;
+    if (e < n4) {
+      const int f = e << 2;
+      const int t = (int)((unsigned)f / (unsigned)nz);
+      const int z = f - t * nz;
+      float4 v = fv[i];
+      if (MODE == MODE_FLUX) {
+        const float4 nn = nv[i];
+        const float f0 = entering_value(pd, v.x), f1 = entering_value(pd, v.y);
+        const float f2 = entering_value(pd, v.z), f3 = entering_value(pd, v.w);
+        v = make_float4(__fmul_rn(nn.x, f0), __fmul_rn(nn.y, f1), __fmul_rn(nn.z, f2),
+                        __fmul_rn(nn.w, f3));
+        const int y = y0 + t;
+        if (xin && y >= d.win[2] && y < d.win[3]) {
+          acc[0] += ((double)f0 + (double)f1) + ((double)f2 + (double)f3);
+          acc[1] += ((double)fabsf(nn.x) + (double)fabsf(nn.y)) +
+                    ((double)fabsf(nn.z) + (double)fabsf(nn.w));
+          acc[2] += (double)((nn.x != 0.f) + (nn.y != 0.f) + (nn.z != 0.f) + (nn.w != 0.f));
+        }
+        if (sk4) sk4[e] = v;
+      }
+      const int c = z >> 1;
+      smem[c * ldn + t] = make_float2(v.x, v.y);
+      smem[(c + 1) * ldn + t] = make_float2(v.z, v.w);
+    }
+  }
+  // zero padding beyond nz (lines past the field edge are never stored)
+  const int c0 = nz >> 1;
+  for (int e = tid; e < (n2 - c0) * T; e += Z_NTH) {
+    const int c = c0 + (e >> logT);
+    smem[c * ldn + (e & (T - 1))] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  AxisPlan az = g.az;
+  az.tw = twz;
+  fft_lines<true, RS, Z_STAGE_CALLS, Z_EM, Z_NTH>(smem, az, T, logT, ldn, 1, false, tid);
+  // r2c split: X2[k] = (Z[k] + conj Z[n2-k]) - i W^k (Z[k] - conj Z[n2-k])
+  const int nh = n2 / 2 + 1;
+  for (int e = tid; e < T * nh; e += Z_NTH) {
+    const int t = e & (T - 1);
+    const int k = e >> logT;
+    float2* Lp = smem + t;
+    if (k == 0) {
+      float2 z0v = Lp[0];
+      Lp[0] = make_float2(2.f * (z0v.x + z0v.y), 0.f);
+      Lp[n2 * ldn] = make_float2(2.f * (z0v.x - z0v.y), 0.f);
+    } else {
+      const int km = n2 - k;
+      float2 zk = Lp[k * ldn], zm = Lp[km * ldn];
+      float2 w = twr[k];
+      float2 A = make_float2(zk.x + zm.x, zk.y - zm.y);
+      float2 B = make_float2(zk.x - zm.x, zk.y + zm.y);
+      float2 wB = cmul(w, B);
+      Lp[k * ldn] = make_float2(A.x + wB.y, A.y - wB.x);
+      if (km != k) {
+        float2 A2 = make_float2(zm.x + zk.x, zm.y - zk.y);
+        float2 B2 = make_float2(zm.x - zk.x, zm.y + zk.y);
+        float2 wB2 = cmul(make_float2(-w.x, w.y), B2);
+        Lp[km * ldn] = make_float2(A2.x + wB2.y, A2.y - wB2.x);
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t plane = (int64_t)g.nx * g.ny;
+  float2* So = d.S + (int64_t)x * g.ny + y0;
+  for (int e = tid; e < g.nzc * T; e += Z_NTH) {
+    const int kz = e >> logT;
+    const int t = e & (T - 1);
+    if (t < tl) So[kz * plane + t] = smem[kz * ldn + t];
+  }
+  if (MODE == MODE_FLUX && !d.apply) {
+    __syncthreads();
+    block_sum_w<3, Z_NTH>(acc, reinterpret_cast<double*>(smem));
+    if (tid == 0) {
+      d.part_z[wg * 3 + 0] = acc[0];
+      d.part_z[wg * 3 + 1] = acc[1];
+      d.part_z[wg * 3 + 2] = acc[2];
+    }
+  }
+}
+
+template <class Src, int MODE, int RS>
+__global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void zinv_tile_kernel(Src src) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_ZINV, wg);
+  const ConvGeom& g = d.g;
+  extern __shared__ float2 smem[];
+  const int tid = threadIdx.x;
+  const int T = g.Tz, logT = g.logTz, ldn = T + 1;
+  const int x = wg / g.nyt_z;
+  const int y0 = (wg - x * g.nyt_z) * T;
+  const int nz = g.nz, n2 = g.n2;
+  float2* twz = smem + (n2 + 1) * ldn;
+  float2* twr = twz + n2;
+  const int tl = min(T, g.ny - y0);
+  const int64_t base = ((int64_t)x * g.ny + y0) * nz;
+  const int n4 = (tl * nz) >> 2;
+  float4* io4 = reinterpret_cast<float4*>(d.out + base);
+  // flux prefetch: in flight while the tile is transformed
+  float4 fv[ZT_U];
+  if (MODE == MODE_FLUX) {
+#pragma unroll
+    for (int i = 0; i < ZT_U; ++i) {
+      const int e = i * Z_NTH + tid;
+      if (e < n4) fv[i] = io4[e];
+    }
+  }
+  const int64_t plane = (int64_t)g.nx * g.ny;
+  const float2* Si = d.S + (int64_t)x * g.ny + y0;
+  const int ns = g.nzc * T;
+  for (int e0 = 0; e0 < ns; e0 += ZT_S * Z_NTH) {
+    float2 sv[ZT_S];
+#pragma unroll
+    for (int i = 0; i < ZT_S; ++i) {
+      const int e = e0 + i * Z_NTH + tid;
+      const int t = e & (T - 1);
+      if (e < ns && t < tl) sv[i] = Si[(e >> logT) * plane + t];
+    }
+#pragma unroll
+    for (int i = 0; i < ZT_S; ++i) {
+      const int e = e0 + i * Z_NTH + tid;
+      const int t = e & (T - 1);
+      if (e < ns) smem[(e >> logT) * ldn + t] = (t < tl) ? sv[i] : make_float2(0.f, 0.f);
+    }
+  }
+  stage_twiddles(g, twz, twr, tid);
+  __syncthreads();
+  // c2r merge: Z[k] = (Y[k] + conj Y[n2-k]) + i conj(W^k) (Y[k] - conj Y[n2-k])
+  const int nh = n2 / 2 + 1;
+  for (int e = tid; e < T * nh; e += Z_NTH) {
+    const int t = e & (T - 1);
+    const int k = e >> logT;
+    float2* Lp = smem + t;
+    if (k == 0) {
+      float2 Y0 = Lp[0], YN = Lp[n2 * ldn];
+      float2 A = make_float2(Y0.x + YN.x, Y0.y - YN.y);
+      float2 B = make_float2(Y0.x - YN.x, Y0.y + YN.y);
+      Lp[0] = make_float2(A.x - B.y, A.y + B.x);
+    } else {
+      const int km = n2 - k;
+      float2 Yk = Lp[k * ldn], Ym = Lp[km * ldn];
+      float2 w = twr[k];
+      float2 A = make_float2(Yk.x + Ym.x, Yk.y - Ym.y);
+      float2 B = make_float2(Yk.x - Ym.x, Yk.y + Ym.y);
+      float2 cB = cmul(make_float2(w.x, -w.y), B);
+      Lp[k * ldn] = make_float2(A.x - cB.y, A.y + cB.x);
+      if (km != k) {
+        float2 A2 = make_float2(Ym.x + Yk.x, Ym.y - Yk.y);
+        float2 B2 = make_float2(Ym.x - Yk.x, Ym.y + Yk.y);
+        float2 cB2 = cmul(make_float2(-w.x, -w.y), B2);
+        Lp[km * ldn] = make_float2(A2.x - cB2.y, A2.y + cB2.x);
+      }
+    }
+  }
+  __syncthreads();
+  AxisPlan az = g.az;
+  az.tw = twz;
+  fft_lines<true, RS, Z_STAGE_CALLS, Z_EM, Z_NTH>(smem, az, T, logT, ldn, 1, true, tid);
+  // output z of line t is real sample m = z + cz of the block transform
+  const float* sr = reinterpret_cast<const float*>(smem);
+  double acc[2] = {0.0, 0.0};
+  Pending pd{1.f, 0.f, 0};
+  bool xin = false;
+  if (MODE == MODE_FLUX) {
+    pd = pending_of(d);
+    xin = x >= d.win[0] && x < d.win[1];
+  }
+#pragma unroll
+  for (int i = 0; i < ZT_U; ++i) {
+    const int e = i * Z_NTH + tid;
+    if (e < n4) {
+      const int f = e << 2;
+      const int t = (int)((unsigned)f / (unsigned)nz);
+      const int m = f - t * nz + g.cz;
+      float c[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[j] = sr[(((m + j) >> 1) * ldn + t) * 2 + ((m + j) & 1)];
+      if (MODE == MODE_PLAIN) {
+        io4[e] = make_float4(c[0], c[1], c[2], c[3]);
+      } else {
+        const float4 r = fv[i];
+        float o[4] = {r.x, r.y, r.z, r.w};
+        const int y = y0 + t;
+        const bool win = xin && y >= d.win[2] && y < d.win[3];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          float v = __fadd_rn(entering_value(pd, o[j]), c[j]);
+          if (v < 0.f) {
+            acc[1] += 1.0;
+            v = 0.f;
+          }
+          o[j] = v;
+        }
+        if (win) acc[0] += ((double)o[0] + (double)o[1]) + ((double)o[2] + (double)o[3]);
+        io4[e] = make_float4(o[0], o[1], o[2], o[3]);
+      }
+    }
+  }
+  if (MODE == MODE_FLUX && !d.apply) {
+    __syncthreads();
+    block_sum_w<2, Z_NTH>(acc, reinterpret_cast<double*>(smem));
+    if (tid == 0) {
+      d.part_o[wg * 2 + 0] = acc[0];
+      d.part_o[wg * 2 + 1] = acc[1];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Register z passes (single block, nz even, n_fft/2 one of the compiled
+// z-line plans).  Lines are numbered line = x*ny + y, so a workgroup takes
+// ZR_WAVES * LINES consecutive lines and S[k][x][y] = S[k*plane + line].
+// Each lane loads the float2 pairs (z = 2n, 2n+1) of its natural indices
+// n = q + m P straight from HBM, which is also the packing of the half-length
+// complex transform; the r2c split / c2r merge pair k with n2 - k through
+// the wave's LDS line.  Partial sums: one slot per workgroup.
+// ---------------------------------------------------------------------------
+#ifndef STEAM_ZR_WAVES
+#define STEAM_ZR_WAVES 4
+#endif
+#ifndef STEAM_ZR_PREFETCH
+#define STEAM_ZR_PREFETCH 0
+#endif
+constexpr int ZR_WAVES = STEAM_ZR_WAVES;  // wavefronts per workgroup
+constexpr int ZR_NTH = ZR_WAVES * 64;
+
+using ZP32 = ZPlan<32, 8, 8, 4>;
+using ZP64 = ZPlan<64, 8, 8, 8>;
+using ZP80 = ZPlan<80, 20, 4, 20>;
+using ZP96 = ZPlan<96, 24, 8, 12>;
+using ZP128 = ZPlan<128, 16, 8, 16>;
+using ZP160 = ZPlan<160, 20, 4, 4, 10>;
+using ZP192 = ZPlan<192, 24, 8, 24>;
+using ZP256 = ZPlan<256, 16, 16, 16>;
+using ZP320 = ZPlan<320, 20, 4, 4, 20>;
+using ZP512 = ZPlan<512, 16, 8, 8, 8>;
+
+// call f(ZPlan{}) for a compiled z-line length; false if there is none
+template <class F>
+inline bool with_zplan(int n2, F&& f) {
+  switch (n2) {
+    case 32: f(ZP32{}); return true;
+    case 64: f(ZP64{}); return true;
+    case 80: f(ZP80{}); return true;
+    case 96: f(ZP96{}); return true;
+    case 128: f(ZP128{}); return true;
+    case 160: f(ZP160{}); return true;
+    case 192: f(ZP192{}); return true;
+    case 256: f(ZP256{}); return true;
+    case 320: f(ZP320{}); return true;
+    case 512: f(ZP512{}); return true;
+    default: return false;
+  }
+}
+inline bool zplan_lines(int n2, int& lines) {
+  return with_zplan(n2, [&](auto zp) { lines = decltype(zp)::LINES * ZR_WAVES; });
+}
+
+// W_2N^k (k < 2N) from the r2c table W_2N^k (k <= N): W_2N^{k+N} = -W_2N^k
+__device__ __forceinline__ void stage_tw2(const ConvGeom& g, float2* tw2, int N) {
+  for (int i = threadIdx.x; i < 2 * N; i += ZR_NTH) {
+    const float2 w = g.wr2c[i < N ? i : i - N];
+    tw2[i] = i < N ? w : make_float2(-w.x, -w.y);
+  }
+}
+
+template <int NV>
+__device__ __forceinline__ void zr_block_sum(double (&v)[NV], double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = v[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    v[i] = s;
+  }
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[w * NV + i] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < ZR_WAVES; ++k) s += red[k * NV + i];
+      v[i] = s;
+    }
+}
+
+template <class Src, int MODE, class ZP>
+__global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
+  constexpr int N = ZP::N, V = ZP::V, P = ZP::P, LPW = ZP::LINES;
+  __shared__ float2 lds_lines[ZR_WAVES * LPW * ZP::PITCH];
+  __shared__ float2 tw2[2 * N];
+  __shared__ double red[ZR_WAVES * 3];
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_Z, wg);
+  const ConvGeom& g = d.g;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int t = lane / P, q = lane % P;
+  const int64_t nlines = (int64_t)g.nx * g.ny;
+  const int64_t line = ((int64_t)wg * ZR_WAVES + w) * LPW + t;
+  const bool valid = line < nlines;
+  const int nz = g.nz;
+  const int64_t zoff = (valid ? line : 0) * nz;
+  const float* srcp = (MODE == MODE_FLUX) ? d.out : d.in;
+  float2 v[V], nv[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int n2i = 2 * (q + m * P);
+    v[m] = make_float2(0.f, 0.f);
+    nv[m] = make_float2(0.f, 0.f);
+    if (valid && n2i < nz) {
+      v[m] = *reinterpret_cast<const float2*>(srcp + zoff + n2i);
+      if (MODE == MODE_FLUX) nv[m] = *reinterpret_cast<const float2*>(d.nfield + zoff + n2i);
+    }
+  }
+  stage_tw2(g, tw2, N);
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (MODE == MODE_FLUX) {
+    const Pending pd = pending_of(d);
+    const int x = (int)(line / g.ny), y = (int)(line - (int64_t)x * g.ny);
+    const bool win = valid && x >= d.win[0] && x < d.win[1] && y >= d.win[2] && y < d.win[3];
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      const int n2i = 2 * (q + m * P);
+      if (valid && n2i < nz) {
+        const float f0 = entering_value(pd, v[m].x), f1 = entering_value(pd, v[m].y);
+        const float2 p = make_float2(__fmul_rn(nv[m].x, f0), __fmul_rn(nv[m].y, f1));
+        if (win) {
+          acc[0] += (double)f0 + (double)f1;
+          acc[1] += (double)fabsf(nv[m].x) + (double)fabsf(nv[m].y);
+          acc[2] += (double)((nv[m].x != 0.f) + (nv[m].y != 0.f));
+        }
+        if (d.sk) *reinterpret_cast<float2*>(d.sk + zoff + n2i) = p;
+        v[m] = p;
+      }
+    }
+  }
+  __syncthreads();
+  float2* L = lds_lines + (w * LPW + t) * ZP::PITCH;
+  ZP::template fft<false>(v, q, L, tw2);
+  // r2c split: X2[k] = (Z[k] + conj Z[N-k]) - i W_2N^k (Z[k] - conj Z[N-k])
+#pragma unroll
+  for (int m = 0; m < V; ++m) L[zpad(q + m * P)] = v[m];
+  wave_lds_sync();
+  const int64_t plane = nlines;
+  float2* So = d.S + line;
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int k = q + m * P;
+    const float2 zk = v[m];
+    float2 X;
+    if (k == 0) {
+      X = make_float2(2.f * (zk.x + zk.y), 0.f);
+      if (valid) So[(int64_t)N * plane] = make_float2(2.f * (zk.x - zk.y), 0.f);
+    } else {
+      const float2 zm = L[zpad(N - k)];
+      const float2 wk = tw2[k];
+      const float2 A = make_float2(zk.x + zm.x, zk.y - zm.y);
+      const float2 B = make_float2(zk.x - zm.x, zk.y + zm.y);
+      const float2 wB = cmul(wk, B);
+      X = make_float2(A.x + wB.y, A.y - wB.x);
+    }
+    if (valid) So[(int64_t)k * plane] = X;
+  }
+  if (MODE == MODE_FLUX && !d.apply) {
+    zr_block_sum<3>(acc, red);
+    if (tid == 0) {
+      d.part_z[wg * 3 + 0] = acc[0];
+      d.part_z[wg * 3 + 1] = acc[1];
+      d.part_z[wg * 3 + 2] = acc[2];
+    }
+  }
+}
+
+template <class Src, int MODE, class ZP>
+__global__ __launch_bounds__(ZR_NTH) void zinv_reg_kernel(Src src) {
+  constexpr int N = ZP::N, V = ZP::V, P = ZP::P, LPW = ZP::LINES;
+  __shared__ float2 lds_lines[ZR_WAVES * LPW * ZP::PITCH];
+  __shared__ float2 tw2[2 * N];
+  __shared__ double red[ZR_WAVES * 2];
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_ZINV, wg);
+  const ConvGeom& g = d.g;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int t = lane / P, q = lane % P;
+  const int64_t nlines = (int64_t)g.nx * g.ny;
+  const int64_t line = ((int64_t)wg * ZR_WAVES + w) * LPW + t;
+  const bool valid = line < nlines;
+  const int nz = g.nz, cz = g.cz;
+  const bool even = (cz & 1) == 0;
+  const int64_t zoff = (valid ? line : 0) * nz;
+  const int64_t plane = nlines;
+  const float2* Si = d.S + (valid ? line : 0);
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) v[m] = valid ? Si[(int64_t)(q + m * P) * plane] : make_float2(0.f, 0.f);
+  float2 yn = make_float2(0.f, 0.f);
+  if (q == 0 && valid) yn = Si[(int64_t)N * plane];
+  // flux prefetch: output z = 2n - cz (+1) of natural index n
+  float2 fl[V];
+  auto load_flux = [&]() {
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      const int z0 = 2 * (q + m * P) - cz;
+      fl[m] = make_float2(0.f, 0.f);
+      if (valid) {
+        if (even) {
+          if (z0 >= 0 && z0 < nz) fl[m] = *reinterpret_cast<const float2*>(d.out + zoff + z0);
+        } else {
+          if (z0 >= 0 && z0 < nz) fl[m].x = d.out[zoff + z0];
+          if (z0 + 1 >= 0 && z0 + 1 < nz) fl[m].y = d.out[zoff + z0 + 1];
+        }
+      }
+    }
+  };
+  if (MODE == MODE_FLUX && STEAM_ZR_PREFETCH) load_flux();
+  stage_tw2(g, tw2, N);
+  __syncthreads();
+  float2* L = lds_lines + (w * LPW + t) * ZP::PITCH;
+#pragma unroll
+  for (int m = 0; m < V; ++m) L[zpad(q + m * P)] = v[m];
+  wave_lds_sync();
+  // c2r merge: Z[k] = (Y[k] + conj Y[N-k]) + i conj(W_2N^k) (Y[k] - conj Y[N-k])
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int k = q + m * P;
+    const float2 yk = v[m];
+    const float2 ym = (k == 0) ? yn : L[zpad(N - k)];
+    const float2 wk = tw2[k];
+    const float2 A = make_float2(yk.x + ym.x, yk.y - ym.y);
+    const float2 B = make_float2(yk.x - ym.x, yk.y + ym.y);
+    const float2 cB = cmul(make_float2(wk.x, -wk.y), B);
+    v[m] = make_float2(A.x - cB.y, A.y + cB.x);
+  }
+  wave_lds_sync();
+  ZP::template fft<true>(v, q, L, tw2);
+  double acc[2] = {0.0, 0.0};
+  if (MODE == MODE_PLAIN) {
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        const int z0 = 2 * (q + m * P) - cz;
+        if (even) {
+          if (z0 >= 0 && z0 < nz) *reinterpret_cast<float2*>(d.out + zoff + z0) = v[m];
+        } else {
+          if (z0 >= 0 && z0 < nz) d.out[zoff + z0] = v[m].x;
+          if (z0 + 1 >= 0 && z0 + 1 < nz) d.out[zoff + z0 + 1] = v[m].y;
+        }
+      }
+    }
+  } else {
+    if (!STEAM_ZR_PREFETCH) load_flux();
+    const Pending pd = pending_of(d);
+    const int x = (int)(line / g.ny), y = (int)(line - (int64_t)x * g.ny);
+    const bool win = valid && x >= d.win[0] && x < d.win[1] && y >= d.win[2] && y < d.win[3];
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      const int z0 = 2 * (q + m * P) - cz;
+      const bool in0 = valid && z0 >= 0 && z0 < nz;
+      const bool in1 = valid && z0 + 1 >= 0 && z0 + 1 < nz;
+      float o0 = __fadd_rn(entering_value(pd, fl[m].x), v[m].x);
+      float o1 = __fadd_rn(entering_value(pd, fl[m].y), v[m].y);
+      if (o0 < 0.f) {
+        if (in0) acc[1] += 1.0;
+        o0 = 0.f;
+      }
+      if (o1 < 0.f) {
+        if (in1) acc[1] += 1.0;
+        o1 = 0.f;
+      }
+      if (win) acc[0] += (in0 ? (double)o0 : 0.0) + (in1 ? (double)o1 : 0.0);
+      if (even) {
+        if (in0) *reinterpret_cast<float2*>(d.out + zoff + z0) = make_float2(o0, o1);
+      } else {
+        if (in0) d.out[zoff + z0] = o0;
+        if (in1) d.out[zoff + z0 + 1] = o1;
+      }
+    }
+    if (!d.apply) {
+      zr_block_sum<2>(acc, red);
+      if (tid == 0) {
+        d.part_o[wg * 2 + 0] = acc[0];
+        d.part_o[wg * 2 + 1] = acc[1];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// finalize: fixed-order reduction of the partials into the ledger
+// ---------------------------------------------------------------------------
+template <class Src>
+__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void finalize_kernel(Src src) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_FIN, wg);
+  const ConvGeom& g = d.g;
+  const int tid = threadIdx.x;
+  __shared__ double red[5 * FFT_NTH];
+  DevLedger L;
+  if (!d.apply) {
+    double v[5] = {0, 0, 0, 0, 0};
+    for (int i = tid; i < d.npart; i += FFT_NTH) {
+      v[0] += d.part_z[i * 3 + 0];
+      v[1] += d.part_z[i * 3 + 1];
+      v[2] += d.part_z[i * 3 + 2];
+      v[3] += d.part_o[i * 2 + 0];
+      v[4] += d.part_o[i * 2 + 1];
+    }
+    block_sum<5, FFT_NTH>(v, red);
+    if (tid != 0) return;
+    const double cnt = (double)(d.win[1] - d.win[0]) * (double)(d.win[3] - d.win[2]) *
+                       (double)g.nz;
+    L.entering_total = v[0];
+    L.entering_count = cnt;
+    L.noise_abs_total = v[1];
+    L.noise_abs_count = v[2];
+    L.realized_total = v[3];
+    L.realized_count = cnt;
+    L.n_clipped = v[4];
+  } else {
+    if (tid != 0) return;
+    L = d.led_in;
+  }
+  const double em = L.entering_total / fmax(L.entering_count, 1.0);
+  const double rm = L.realized_total / fmax(L.realized_count, 1.0);
+  const double ma = L.noise_abs_total / fmax(L.noise_abs_count, 1.0);
+  if (rm > 0.0) {
+    L.rescale = (float)(em / rm);
+    L.flatten = 0;
+  } else {
+    L.rescale = 1.0f;
+    L.flatten = 1;
+  }
+  L.entering_mean_f = (float)em;
+  L.have_mean_abs = ma > 0.0;
+  L.inv_mean_abs = ma > 0.0 ? (float)(1.0 / ma) : 1.0f;
+  L.pad = 0;
+  *d.led = L;
+}
+
+// ---------------------------------------------------------------------------
+// post: apply the rescale to the flux and the 1/<|noise|> scale to S_k
+// ---------------------------------------------------------------------------
+template <class Src>
+__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void post_kernel(Src src, int do_rescale) {
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_POST, wg);
+  const ConvGeom& g = d.g;
+  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+  const DevLedger* L = d.led;
+  const float rs = L->rescale;
+  const int flat = L->flatten;
+  const float em = L->entering_mean_f;
+  const float inv = L->inv_mean_abs;
+  const int have = L->have_mean_abs;
+  const int64_t i0 = ((int64_t)wg * FFT_NTH + threadIdx.x) * 4;
+  for (int q = 0; q < 4; ++q) {
+    const int64_t i = i0 + q;
+    if (i >= n) break;
+    if (do_rescale) d.out[i] = flat ? em : __fmul_rn(d.out[i], rs);
+    if (d.sk && have) d.sk[i] = __fmul_rn(d.sk[i], inv);
+  }
+}
+
+}  // namespace steam

@@ -1,0 +1,135 @@
+"""The batched advance (one launch per pass and kernel variant over many
+instances, optionally replayed from a hipGraph) reproduces the
+single-instance advance of every member bit for bit."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import model as M
+
+pytestmark = pytest.mark.gpu
+
+# instance set: several shapes selecting different kernel variants (register
+# and LDS z plans, register and generic x/y FFTs), alphas and noise scales
+SHAPES = [((32, 32, 24), 1.8, 0.3), ((64, 48, 40), 1.6, 0.25),
+          ((48, 40, 30), 1.7, 0.2), ((64, 64, 32), 1.9, 0.35),
+          ((32, 32, 24), 1.5, 0.15)]
+
+
+def _instances(seed=0, defer=False):
+    from steam_amd import simulate as S
+    from steam_amd.batch import Instance
+    kern = S._turbulon_envelope(1, .5, .5, .5, as_tensor=True)
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, (shape, alpha, c) in enumerate(SHAPES):
+        flux = torch.from_numpy(np.abs(1 + 0.2 * rng.standard_normal(shape))
+                                .astype(np.float32)).cuda()
+        key = M.class_key(np.random.SeedSequence(100 + i))
+        region = S.NoiseRegion(tuple(np.uint64(k) for k in key), (0, 0, 0), shape)
+        out.append(Instance(flux, kern, region, c, alpha=alpha,
+                            defer_rescale=defer))
+    return out
+
+
+def _single(inst, op, spec, region=None):
+    from steam_amd import simulate as S
+    f = inst.flux.clone()
+    S.advance_flux_device(f, inst.region if region is None else region, inst.kernel,
+                          inst.flux_noise_scale, 1, (1, 1, 1), alpha=inst.alpha,
+                          op=op, spec=spec)
+    return f
+
+
+def test_batched_advance_equals_single_instance_advance():
+    from steam_amd.batch import BatchedAdvance
+    insts = _instances()
+    batch = BatchedAdvance(insts, autotune=False)
+    refs = [_single(inst, op, sp) for inst, op, sp in zip(insts, batch.ops, batch.specs)]
+    batch.advance()
+    torch.cuda.synchronize()
+    for inst, ref in zip(insts, refs):
+        np.testing.assert_array_equal(inst.flux.cpu().numpy(), ref.cpu().numpy())
+    assert batch.launches_per_advance < 8 * len(insts)
+
+
+def test_batched_ledgers_match_oracle_sums():
+    from steam_amd.batch import BatchedAdvance
+    insts = _instances(seed=1)
+    start = [inst.flux.cpu().numpy() for inst in insts]
+    batch = BatchedAdvance(insts, autotune=False)
+    batch.advance()
+    for inst, f0, led in zip(insts, start, batch.ledgers()):
+        kern = inst.kernel.cpu().numpy()
+        fo, so, lo, do = M.advance_flux(f0, inst.region.key, (0, 0, 0), f0.shape, kern,
+                                        inst.flux_noise_scale, 1, (1, 1, 1),
+                                        alpha=inst.alpha)
+        assert abs(led.entering.total / lo['entering'][0] - 1) < 1e-12
+        assert abs(led.realized.total / lo['realized'][0] - 1) < 1e-5
+        f = inst.flux.cpu().numpy().astype(np.float64)
+        assert np.linalg.norm(f - fo) / np.linalg.norm(fo) < 1e-5
+
+
+def test_graph_replays_match_eager_and_follow_new_keys():
+    """Replays with eager work and synchronisations between them; every
+    replay is checked against the eager batched result."""
+    from steam_amd import simulate as S
+    from steam_amd.batch import BatchedAdvance
+    insts = _instances(seed=2)
+    twins = _instances(seed=2)
+    g_batch = BatchedAdvance(insts, autotune=False)
+    e_batch = BatchedAdvance(twins, autotune=False)
+    graph = g_batch.capture()  # capture() itself runs one warm advance
+    e_batch.advance()
+    torch.cuda.synchronize()
+    for rep in range(3):
+        regions = [S.NoiseRegion(tuple(np.uint64(k) for k in
+                                       M.class_key(np.random.SeedSequence(7 * rep + i))),
+                                 (0, 0, 0), tuple(inst.flux.shape))
+                   for i, inst in enumerate(insts)]
+        torch.cuda.synchronize()
+        g_batch.set_regions(regions)
+        e_batch.set_regions(regions)
+        graph.replay()
+        e_batch.advance()
+        torch.cuda.synchronize()
+        junk = torch.randn(1 << 20, device='cuda').sum().item()  # eager work between replays
+        assert np.isfinite(junk)
+        for a, b in zip(insts, twins):
+            np.testing.assert_array_equal(a.flux.cpu().numpy(), b.flux.cpu().numpy())
+
+
+def test_deferred_batch_leaves_rescale_pending():
+    from steam_amd.batch import BatchedAdvance
+    insts = _instances(seed=3, defer=True)
+    twins = _instances(seed=3, defer=False)
+    b1 = BatchedAdvance(insts, autotune=False)
+    b2 = BatchedAdvance(twins, autotune=False)
+    b1.advance()
+    b2.advance()
+    for a, b, led in zip(insts, twins, b1.ledgers()):
+        scaled = (a.flux * np.float32(led.rescale)).cpu().numpy()
+        np.testing.assert_array_equal(scaled, b.flux.cpu().numpy())
+
+
+def test_autotune_picks_a_timed_candidate():
+    from steam_amd.batch import autotune_nfft, nfft_candidates
+    cands = nfft_candidates(40, 13, max_out=3)
+    assert len(cands) == 3 and all(n % 2 == 0 and n >= 13 for n in cands)
+    best, times = autotune_nfft((32, 32, 40), (13, 13, 13), top=3, reps=2, table={})
+    assert best in cands and set(times) == set(cands)
+    assert times[best] == min(times.values())
+
+
+def test_batch_rejects_bad_items():
+    from steam_amd.batch import BatchedAdvance, Instance
+    insts = _instances()
+    with pytest.raises(ValueError):
+        BatchedAdvance([])
+    with pytest.raises(ValueError):
+        Instance(insts[0].flux.cpu(), insts[0].kernel, insts[0].region, 0.2)
+    bad = _instances()
+    bad[0].window = (0, 999, 0, 4)
+    with pytest.raises(ValueError):
+        BatchedAdvance(bad, autotune=False)
