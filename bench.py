@@ -48,6 +48,14 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, nargs=3, default=None,
                     help="host sub-box for the CPU baseline (default: the grid)")
     ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched form (48 realisations)")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the served form (tile stream)")
+    ap.add_argument("--serve-rate", type=float, default=1000.0,
+                    help="tile requests per second of the served form")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-world", type=int, nargs=3, default=[2048, 2048, 256])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / "
@@ -65,6 +73,40 @@ def copy_calibration():
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4,
                       "launches": 3}))
+
+
+def read_traffic(path):
+    """Per-family HBM bytes per launch from the committed rocprofv3 summary
+    (FETCH_SIZE / WRITE_SIZE scaled by the copy calibration)."""
+    out = {}
+    if not os.path.exists(path):
+        return out
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("#") or not line.strip():
+                continue
+            f = line.split()
+            out[f[0]] = int(float(f[-3]))
+    return out
+
+
+def measure_copy_gbs(nbytes=1 << 30, reps=5):
+    """Read+write bandwidth of a plain device copy (the practical HBM ceiling
+    the kernels are compared with beside the 8 TB/s specification)."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda')
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    stop.record()
+    stop.synchronize()
+    ms = start.elapsed_time(stop) / reps
+    del src, dst
+    return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
 def family_bytes(info, shape, kernel_elems_planes):
@@ -113,6 +155,120 @@ def cpu_baseline(sample, c, alpha, kernel_np, seed_key, repeats=3):
             "sample": (f"one class advance of a {shape[0]}x{shape[1]}x{shape[2]} "
                        f"float32 grid of the same workload (same kernel, alpha, "
                        f"c), best of {max(1, repeats)}: {dt:.2f} s")}
+
+
+def batched_form(steps, warmup):
+    """48 realisations of uneven size, alpha and noise scale advanced by one
+    class per step: graph-captured batched advance vs the plain drop-in
+    (one host-array _advance_flux call per realisation) and vs the resident
+    single-instance advance called once per realisation."""
+    import numpy as np
+    import torch
+    from steam_amd import simulate as S
+    from steam_amd.batch import BatchedAdvance, Instance, draw_batch_config
+
+    cfg = draw_batch_config(48, seed=48)
+    kern = S._turbulon_envelope(1, 0.5, 0.5, 0.5, as_tensor=True)
+    insts = []
+    for d in cfg:
+        ss = np.random.SeedSequence(d["seed"])
+        region = S.NoiseRegion(S.class_key(ss.spawn(1)[0]), (0, 0, 0), d["grid"])
+        flux = torch.ones(d["grid"], dtype=torch.float32, device='cuda')
+        insts.append(Instance(flux, kern, region, d["c"], alpha=d["alpha"]))
+    t_setup = time.perf_counter()
+    batch = BatchedAdvance(insts, autotune=True)
+    setup_s = time.perf_counter() - t_setup
+    graph = batch.capture()
+    for _ in range(warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    # eager batched advance with fresh keys every step
+    regions = [[S.NoiseRegion(S.class_key(np.random.SeedSequence(1000 * k + i)),
+                              (0, 0, 0), d["grid"]) for i, d in enumerate(cfg)]
+               for k in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        batch.set_regions(regions[k])
+        batch.advance()
+    torch.cuda.synchronize()
+    ms_eager = (time.perf_counter() - t0) * 1e3 / steps
+    vox = batch.voxels
+    # resident single-instance advances, one call per realisation
+    for i, inst in enumerate(insts):
+        S.advance_flux_device(inst.flux, inst.region, kern, inst.flux_noise_scale, 1,
+                              (1, 1, 1), alpha=inst.alpha, op=batch.ops[i],
+                              spec=batch.specs[i], sync_ledger=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, inst in enumerate(insts):
+        S.advance_flux_device(inst.flux, inst.region, kern, inst.flux_noise_scale, 1,
+                              (1, 1, 1), alpha=inst.alpha, op=batch.ops[i],
+                              spec=batch.specs[i], sync_ledger=False)
+    torch.cuda.synchronize()
+    single_s = time.perf_counter() - t0
+    # plain drop-in: NumPy arrays through the reference's call, one per realisation
+    kern_np = kern.cpu().numpy()
+    hosts = [np.ones(d["grid"], np.float32) for d in cfg]
+    S._advance_flux(hosts[0], insts[0].region, kern_np, insts[0].flux_noise_scale, 1,
+                    (1, 1, 1), alpha=insts[0].alpha)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for h, inst in zip(hosts, insts):
+        S._advance_flux(h, inst.region, kern_np, inst.flux_noise_scale, 1, (1, 1, 1),
+                        alpha=inst.alpha)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    value = vox / (ms * 1e-3)
+    return {
+        "value": value, "unit": "voxels/s", "instances": len(insts), "voxels": vox,
+        "ms_per_step": ms, "graph": True,
+        "eager_new_keys_ms_per_step": ms_eager,
+        "launches_per_step": batch.launches_per_advance,
+        "plain_dropin_voxels_per_s": vox / plain_s,
+        "vs_plain_dropin": value / (vox / plain_s),
+        "single_resident_voxels_per_s": vox / single_s,
+        "vs_single_resident": value / (vox / single_s),
+        "n_fft_autotuned": sorted(set(batch.n_fft)), "setup_s": setup_s,
+        "kernel": list(kern.shape),
+        "workload": "48 realisations, grids from {32^3, 64^3, 96x96x48, 128^3, "
+                    "192x192x96, 256^3}, alpha~U[1.5,1.9], C1~U[0.05,0.15], "
+                    "default_rng(48)",
+    }
+
+
+def served_form(world, n_req, rate):
+    """World-keyed tile stream: tiled world measure, then Poisson requests
+    served with continuous batching; voxels/s and p50/p99 latency."""
+    import numpy as np
+    import torch
+    from steam_amd.serve import TileServer, WorldClass, make_trace
+
+    w = WorldClass(tuple(world), seed=0)
+    t0 = time.perf_counter()
+    w.measure(tile=(512, 512))
+    torch.cuda.synchronize()
+    measure_s = time.perf_counter() - t0
+    warm = make_trace(16, rate, w.world_shape, seed=99)
+    TileServer(w, cache_bytes=0).serve(warm)
+    trace = make_trace(n_req, rate, w.world_shape, seed=0)
+    srv = TileServer(w, budget_bytes=16 << 30, streams=4, cache_bytes=4 << 30)
+    lat, wall, vox, _ = srv.serve(trace)
+    return {
+        "value": vox / wall, "unit": "voxels/s", "requests": n_req,
+        "rate_per_s": rate, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "mean_ms": float(lat.mean() * 1e3), "wall_s": wall,
+        "world": list(w.world_shape), "kernel": list(w.kernel.shape),
+        "measure_pass_s": measure_s, "cache_hits": srv.stats["cache_hits"],
+        "max_inflight": srv.stats["max_inflight"],
+        "tiles": "{64^3, 128^3, 256^3, 384x384x192}, 70% periodic neighbours",
+    }
 
 
 def main():
@@ -231,14 +387,18 @@ def main():
               key=lambda k: fam[k]["total_ms"])
     achieved = fbytes[dom] / (fam[dom]["avg_ms"] * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_flux_advance_512x512x256.json")
-    if os.path.exists(tfile):
-        with open(tfile) as fh:
-            traffic = json.load(fh).get(dom)
+    if shape == (512, 512, 256):
+        traffic = read_traffic(os.path.join(ROOT, "profiles",
+                                            "traffic_flux_advance_512x512x256.txt")).get(dom)
+    copy_gbs = measure_copy_gbs()
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "algorithmic_bytes_per_launch": fbytes[dom]}
+                "algorithmic_bytes_per_launch": fbytes[dom],
+                "measured_copy_GBs": round(copy_gbs, 1),
+                "frac_of_copy": round(achieved / copy_gbs, 4),
+                "per_kernel_GBs": {k: round(fbytes[k] / (fam[k]["avg_ms"] * 1e-3) / 1e9, 1)
+                                   for k in fbytes if k in fam and k != "noise"}}
 
     plain = None
     if not args.no_plain:
@@ -252,6 +412,11 @@ def main():
             S._advance_flux(host, region, kern_np, c, 1, (1, 1, 1), alpha=alpha)
         torch.cuda.synchronize()
         plain = voxels * reps / (time.perf_counter() - p0)
+
+    batched = None if args.no_batched else batched_form(10, 2)
+    served = None if args.no_serve else served_form(args.serve_world,
+                                                     args.serve_requests,
+                                                     args.serve_rate)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -282,6 +447,8 @@ def main():
         "serial_resident_voxels_per_s": voxels / (serial_ms * 1e-3),
         "serial_resident_ms_per_step": serial_ms,
         "kernels": fam,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out))
 

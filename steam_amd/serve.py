@@ -1,0 +1,293 @@
+"""World-keyed tile serving with continuous batching.
+
+One class advance of a world field (e.g. 2048 x 2048 x 256), served as tiles.
+The class's global means (entering flux, |noise|, realized flux) are
+measured once per world by a tiled measure pass (the streamed cascade's
+measure/reduce passes, steam/streaming.py:850-1447, restricted to the flux).
+Every tile request then runs the apply-only advance
+(steam/simulate.py:2066-2228 with ``ledger=``) on its tile plus a halo of
+the kernel half-width: the keyed noise is a function of world coordinates
+(steam/streaming.py:822-847), so the tile's interior equals the monolithic
+solve restricted to it, seams included.
+
+The server admits requests as they arrive, against a device-memory budget
+and a pool of streams; admitted requests run concurrently on their own
+streams (their kernels overlap on the device) and new arrivals are admitted
+as soon as a slot and memory free up.  Kernel spectra are cached per padded
+tile shape; finished tiles are kept in a device-resident LRU cache so a
+request for a recently served tile (a neighbour walk coming back) is
+answered without recomputation.  Latency is arrival to completion of the
+tile on the device.
+"""
+
+import time
+from collections import OrderedDict, deque
+
+import numpy as np
+import torch
+
+from . import _lib
+from .ledger import FluxAdvanceLedger, MeanReduction
+from .simulate import NoiseRegion, _turbulon_envelope, advance_flux_device, class_key
+from .utils import conv_operator, kernel_key
+
+
+def fft_length(m):
+    """Smallest transform-friendly length >= m: a power of two when within
+    15% (register FFT path), else the smallest 2^a 3^b 5^c."""
+    p2 = 1
+    while p2 < m:
+        p2 *= 2
+    if p2 <= 1.15 * m:
+        return p2
+    n = m
+    while True:
+        r = n
+        for f in (2, 3, 5):
+            while r % f == 0:
+                r //= f
+        if r == 1:
+            return n
+        n += 1
+
+
+class WorldClass:
+    """One flux class of a world: keyed noise, parameters, kernel, halo."""
+
+    def __init__(self, world_shape, seed=0, flux_noise_scale=0.1 ** (1 / 1.8),
+                 alpha=1.8, k=1.0, spacing=(0.5, 0.5, 0.5), support_factor=3):
+        self.world_shape = tuple(int(n) for n in world_shape)
+        self.key = class_key(np.random.SeedSequence(seed).spawn(1)[0])
+        self.flux_noise_scale = float(flux_noise_scale)
+        self.alpha = float(alpha)
+        self.kernel = _turbulon_envelope(k, *spacing, support_factor=support_factor,
+                                         as_tensor=True)
+        # halo per axis = kernel half-width (+1 as the streamed driver keeps,
+        # steam/streaming.py:672-796)
+        self.halo = tuple(int(n) // 2 + 1 for n in self.kernel.shape)
+        self.ledger = None
+
+    def padded_box(self, origin, shape):
+        """(padded origin, padded shape, interior offset) of a tile: x/y
+        grow by at least the halo on both sides (periodic world) up to an
+        FFT-friendly length, z by the halo inside the world only (z is not
+        periodic), kept even.  A wider margin than the halo leaves the
+        interior exact."""
+        (x0, y0, z0), (tx, ty, tz) = origin, shape
+        hx, hy, hz = self.halo
+        nz = self.world_shape[2]
+        px, py = fft_length(tx + 2 * hx), fft_length(ty + 2 * hy)
+        zlo = max(0, z0 - hz)
+        zhi = min(nz, z0 + tz + hz)
+        if (zhi - zlo) % 2:
+            if zhi < nz:
+                zhi += 1
+            elif zlo > 0:
+                zlo -= 1
+        return ((x0 - hx, y0 - hy, zlo), (px, py, zhi - zlo), (hx, hy, z0 - zlo))
+
+    def region(self, padded_origin):
+        return NoiseRegion(self.key, tuple(int(v) for v in padded_origin),
+                           self.world_shape)
+
+    def advance_tile(self, origin, shape, ledger=None, stream=None):
+        """Flux after the class advance on one tile (device tensor).  With a
+        ledger: apply-only (the world's means); without: measure mode over
+        the tile interior, returning the tile's ledger too."""
+        po, ps, (ix, iy, iz) = self.padded_box(origin, shape)
+        flux = torch.ones(ps, dtype=torch.float32, device='cuda')
+        op = conv_operator(ps, tuple(self.kernel.shape))
+        spec = op.spectrum(self.kernel, key=kernel_key(self.kernel))
+        window = None if ledger is not None else (ix, ix + shape[0], iy, iy + shape[1])
+        if ledger is None and (iz != 0 or ps[2] != shape[2]):
+            raise ValueError("measure tiles must span the whole z column")
+        led, _ = advance_flux_device(flux, self.region(po), self.kernel,
+                                     self.flux_noise_scale, 1, (1, 1, 1),
+                                     alpha=self.alpha, ledger=ledger, op=op,
+                                     spec=spec, window=window)
+        out = flux[ix:ix + shape[0], iy:iy + shape[1], iz:iz + shape[2]]
+        return out, led
+
+    def measure(self, tile=(512, 512)):
+        """World ledger from a tiled measure pass (x/y tiles, whole z)."""
+        nx, ny, nz = self.world_shape
+        tx, ty = min(tile[0], nx), min(tile[1], ny)
+        ent = nab = rea = None
+        clipped = 0
+        for x0 in range(0, nx, tx):
+            for y0 in range(0, ny, ty):
+                shape = (min(tx, nx - x0), min(ty, ny - y0), nz)
+                _, led = self.advance_tile((x0, y0, 0), shape)
+                ent = led.entering if ent is None else ent + led.entering
+                nab = led.noise_abs if nab is None else nab + led.noise_abs
+                rea = led.realized if rea is None else rea + led.realized
+                clipped += led.n_clipped  # per call, halo cells included
+        self.ledger = FluxAdvanceLedger(ent, nab, rea, clipped)
+        return self.ledger
+
+    def monolithic(self):
+        """The whole world advanced in one piece (measure mode)."""
+        flux = torch.ones(self.world_shape, dtype=torch.float32, device='cuda')
+        led, _ = advance_flux_device(flux, NoiseRegion(self.key, (0, 0, 0),
+                                                       self.world_shape),
+                                     self.kernel, self.flux_noise_scale, 1, (1, 1, 1),
+                                     alpha=self.alpha)
+        return flux, led
+
+
+def make_trace(n, rate, world_shape, sizes=((64, 64, 64), (128, 128, 128),
+                                            (256, 256, 256), (384, 384, 192)),
+               p_neighbor=0.7, seed=0):
+    """Poisson arrivals at ``rate`` requests/s; tile size uniform over
+    ``sizes``; with probability ``p_neighbor`` the origin is a periodic
+    neighbour (one tile extent in x or y) of the previous request's."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    prev = None
+    nx, ny, nz = world_shape
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        size = tuple(sizes[int(rng.integers(len(sizes)))])
+        size = (min(size[0], nx), min(size[1], ny), min(size[2], nz))
+        if prev is not None and rng.random() < p_neighbor:
+            (px, py, pz), psz = prev
+            axis = int(rng.integers(2))
+            step = int(rng.choice((-1, 1)))
+            x = (px + (step * psz[0] if axis == 0 else 0)) % nx
+            y = (py + (step * psz[1] if axis == 1 else 0)) % ny
+            z = min(pz, nz - size[2])
+        else:
+            x = int(rng.integers(nx))
+            y = int(rng.integers(ny))
+            z = int(rng.integers(nz - size[2] + 1))
+        origin = (int(x), int(y), int(z))
+        out.append({"id": i, "arrival": t, "origin": origin, "shape": size})
+        prev = (origin, size)
+    return out
+
+
+class _InFlight:
+    __slots__ = ("req", "stream", "event", "out", "flux", "bytes", "start")
+
+
+class TileServer:
+    """Continuous-batching tile server over one WorldClass."""
+
+    def __init__(self, world, budget_bytes=8 << 30, streams=4, cache_bytes=2 << 30):
+        if world.ledger is None:
+            raise ValueError("measure the world ledger first (WorldClass.measure)")
+        _lib.require_gpu('cuda')
+        self.world = world
+        self.budget = int(budget_bytes)
+        self.streams = [torch.cuda.Stream() for _ in range(max(1, streams))]
+        self.cache_bytes = int(cache_bytes)
+        self.cache = OrderedDict()
+        self.cache_used = 0
+        self.stats = {"cache_hits": 0, "spectra": 0, "max_inflight": 0,
+                      "max_bytes": 0}
+
+    @staticmethod
+    def request_bytes(world, shape):
+        _, ps, _ = world.padded_box((0, 0, 0), shape)
+        return 4 * int(np.prod(ps)) + 4 * int(np.prod(shape))
+
+    def _launch(self, req, stream):
+        w = self.world
+        with torch.cuda.stream(stream):
+            po, ps, (ix, iy, iz) = w.padded_box(req["origin"], req["shape"])
+            op = conv_operator(ps, tuple(w.kernel.shape))
+            key = kernel_key(w.kernel)
+            fresh = key not in op._spectra
+            spec = op.spectrum(w.kernel, key=key)
+            if fresh:
+                # other streams will read this cached spectrum
+                stream.synchronize()
+                self.stats["spectra"] += 1
+            flux = torch.ones(ps, dtype=torch.float32, device='cuda')
+            advance_flux_device(flux, w.region(po), w.kernel, w.flux_noise_scale, 1,
+                                (1, 1, 1), alpha=w.alpha, ledger=w.ledger, op=op,
+                                spec=spec)
+            tx, ty, tz = req["shape"]
+            out = flux[ix:ix + tx, iy:iy + ty, iz:iz + tz].contiguous()
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        return flux, out, ev
+
+    def _cache_put(self, key, t):
+        nb = t.numel() * 4
+        if nb > self.cache_bytes:
+            return
+        while self.cache_used + nb > self.cache_bytes and self.cache:
+            _, old = self.cache.popitem(last=False)
+            self.cache_used -= old.numel() * 4
+        self.cache[key] = t
+        self.cache_used += nb
+
+    def serve(self, trace, keep_results=False):
+        """Run a request trace in real time.  Returns (latencies_s,
+        wall_s, voxels, results) with results[id] = device tile when
+        ``keep_results``."""
+        pending = deque(sorted(trace, key=lambda r: r["arrival"]))
+        queue = deque()
+        inflight = []
+        free_streams = list(self.streams)
+        used = 0
+        lat = {}
+        results = {}
+        voxels = 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        while pending or queue or inflight:
+            now = time.perf_counter() - t0
+            while pending and pending[0]["arrival"] <= now:
+                queue.append(pending.popleft())
+            # completions first: they free streams and memory
+            still = []
+            for f in inflight:
+                if f.event.query():
+                    done = time.perf_counter() - t0
+                    lat[f.req["id"]] = done - f.req["arrival"]
+                    voxels += f.out.numel()
+                    self._cache_put((f.req["origin"], f.req["shape"]), f.out)
+                    if keep_results:
+                        results[f.req["id"]] = f.out
+                    used -= f.bytes
+                    f.flux = None
+                    free_streams.append(f.stream)
+                else:
+                    still.append(f)
+            inflight = still
+            # admission: in arrival order, while a stream and memory are free
+            while queue:
+                req = queue[0]
+                key = (req["origin"], req["shape"])
+                hit = self.cache.get(key)
+                if hit is not None:
+                    queue.popleft()
+                    self.cache.move_to_end(key)
+                    self.stats["cache_hits"] += 1
+                    lat[req["id"]] = (time.perf_counter() - t0) - req["arrival"]
+                    voxels += hit.numel()
+                    if keep_results:
+                        results[req["id"]] = hit
+                    continue
+                nb = self.request_bytes(self.world, req["shape"])
+                if not free_streams or (inflight and used + nb > self.budget):
+                    break
+                queue.popleft()
+                s = free_streams.pop()
+                f = _InFlight()
+                f.req, f.stream, f.bytes = req, s, nb
+                f.flux, f.out, f.event = self._launch(req, s)
+                used += nb
+                inflight.append(f)
+                self.stats["max_inflight"] = max(self.stats["max_inflight"], len(inflight))
+                self.stats["max_bytes"] = max(self.stats["max_bytes"], used)
+            if not queue and not inflight and pending:
+                wait = pending[0]["arrival"] - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 1e-3))
+        wall = time.perf_counter() - t0
+        lats = np.array([lat[r["id"]] for r in trace])
+        return lats, wall, voxels, results
