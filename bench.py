@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=1000.0,
                     help="tile requests per second of the served form")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-sweep", type=float, nargs="*", default=None,
+                    help="also serve the trace at these rates (req/s) and report "
+                         "throughput and p50/p99 per rate")
     ap.add_argument("--serve-world", type=int, nargs=3, default=[2048, 2048, 256])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the "
@@ -242,7 +245,7 @@ def batched_form(steps, warmup):
     }
 
 
-def served_form(world, n_req, rate):
+def served_form(world, n_req, rate, sweep=None):
     """World-keyed tile stream: tiled world measure, then Poisson requests
     served with continuous batching; voxels/s and p50/p99 latency."""
     import numpy as np
@@ -259,7 +262,16 @@ def served_form(world, n_req, rate):
     trace = make_trace(n_req, rate, w.world_shape, seed=0)
     srv = TileServer(w, budget_bytes=16 << 30, streams=4, cache_bytes=4 << 30)
     lat, wall, vox, _ = srv.serve(trace)
+    curve = []
+    for r in sweep or []:
+        tr = make_trace(n_req, r, w.world_shape, seed=0)
+        sv = TileServer(w, budget_bytes=16 << 30, streams=4, cache_bytes=4 << 30)
+        la, wa, vo, _ = sv.serve(tr)
+        curve.append({"rate_per_s": r, "voxels_per_s": vo / wa,
+                      "p50_ms": float(np.percentile(la, 50) * 1e3),
+                      "p99_ms": float(np.percentile(la, 99) * 1e3)})
     return {
+        "sweep": curve,
         "value": vox / wall, "unit": "voxels/s", "requests": n_req,
         "rate_per_s": rate, "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
@@ -416,7 +428,8 @@ def main():
     batched = None if args.no_batched else batched_form(10, 2)
     served = None if args.no_serve else served_form(args.serve_world,
                                                      args.serve_requests,
-                                                     args.serve_rate)
+                                                     args.serve_rate,
+                                                     args.serve_sweep)
 
     cpu = None
     if not args.no_cpu_baseline:
