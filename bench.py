@@ -59,6 +59,11 @@ def parse():
                     help="also serve the trace at these rates (req/s) and report "
                          "throughput and p50/p99 per rate")
     ap.add_argument("--serve-world", type=int, nargs=3, default=[2048, 2048, 256])
+    ap.add_argument("--precision-form", action="store_true",
+                    help="only run the precision form: one class advance of a "
+                         "1024x1024x256 float32 field (alpha 1.6, C1 0.12, 129^3 "
+                         "kernel) timed on the GPU and checked against the FP64 "
+                         "oracle; prints one JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / "
@@ -283,10 +288,88 @@ def served_form(world, n_req, rate, sweep=None):
     }
 
 
+def precision_form(steps=10):
+    """The precision workload: float32 FFTs, spectral multiply and cascade
+    arithmetic; float64 only in the kernel-correction sums and the volume
+    sums / rescale (all on the GPU); checked against the FP64 oracle on the
+    same keyed uniforms (flux and S_k: rel-L2 and max-abs)."""
+    import numpy as np
+    import torch
+    from oracle import model as M
+    from steam_amd import simulate as S
+    from steam_amd.utils import conv_operator
+
+    def say(msg):
+        print(f"[precision form] {msg}", file=sys.stderr, flush=True)
+
+    shape = (1024, 1024, 256)
+    alpha, c1 = 1.6, 0.12
+    c = c1 ** (1.0 / alpha)
+    key = S.class_key(np.random.SeedSequence(0).spawn(1)[0])
+    region = S.NoiseRegion(key, (0, 0, 0), shape)
+    kern = S._turbulon_envelope(1, 0.5, 0.5, 0.5, support_factor=32, as_tensor=True)
+    rng = np.random.default_rng(1)
+    flux0 = np.abs(1 + 0.3 * rng.standard_normal(shape, dtype=np.float32))
+    op = conv_operator(shape, tuple(kern.shape), precision='kahan')
+    spec = op.spectrum(kern)
+    say(f"kernel {tuple(kern.shape)}, n_fft {op.n_fft}, blocks {op.n_blocks}")
+    ft = torch.from_numpy(flux0).cuda()
+    sk = torch.empty_like(ft)
+    S.advance_flux_device(ft, region, kern, c, 1, (1, 1, 1), alpha=alpha, op=op,
+                          spec=spec, sk_t=sk, precision='kahan')
+    f32 = ft.cpu().numpy()
+    s32 = sk.cpu().numpy()
+    # timing: deferred, chained advances of the resident field
+    work = torch.from_numpy(flux0).cuda()
+    for i in range(2):
+        S.advance_flux_device(work, region, kern, c, 1, (1, 1, 1), alpha=alpha, op=op,
+                              spec=spec, defer_rescale=True, sync_ledger=False,
+                              chain_pending=i > 0, precision='kahan')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        S.advance_flux_device(work, region, kern, c, 1, (1, 1, 1), alpha=alpha, op=op,
+                              spec=spec, defer_rescale=True, sync_ledger=False,
+                              chain_pending=True, precision='kahan')
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    say(f"GPU {ms:.3f} ms per advance; running the FP64 oracle")
+    kern64 = M.turbulon_envelope(1, 0.5, 0.5, 0.5, support_factor=32, dtype=np.float64)
+    t0 = time.perf_counter()
+    f64, s64, led, _ = M.advance_flux(flux0, key, (0, 0, 0), shape, kern64, c, 1,
+                                      (1, 1, 1), alpha=alpha, dtype=np.float64)
+    oracle_s = time.perf_counter() - t0
+    say(f"oracle {oracle_s:.1f} s")
+
+    def err(a, b):
+        d = a.astype(np.float64) - b
+        return (float(np.sqrt((d * d).sum() / (b * b).sum())), float(np.abs(d).max()))
+
+    fl2, fmax = err(f32, f64)
+    sl2, smax = err(s32, s64)
+    vox = int(np.prod(shape))
+    return {"metric": "precision form: float32 class advance vs FP64 oracle",
+            "workload": "flux_advance_class_1024x1024x256", "grid": list(shape),
+            "kernel": list(kern.shape), "alpha": alpha, "C1": c1, "c": c,
+            "n_fft": op.n_fft, "n_blocks": op.n_blocks,
+            "ms_per_step": ms, "voxels_per_s": vox / (ms * 1e-3),
+            "flux_rel_l2": fl2, "flux_max_abs": fmax,
+            "sk_rel_l2": sl2, "sk_max_abs": smax,
+            "flux_max_abs_relative_to_max": fmax / float(np.abs(f64).max()),
+            "oracle_s": oracle_s,
+            "data": "entering flux |1 + 0.3 N(0,1)| from default_rng(1), key "
+                    "SeedSequence(0).spawn(1)[0]"}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.precision_form:
+        from steam_amd import _lib
+        _lib.require_gpu('cuda')
+        print(json.dumps(precision_form()))
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus "

@@ -263,10 +263,12 @@ def conv_fft(field, kernel):
     kp = np.zeros((nx, ny, kz), dtype=np.float64)
     kp[:kx, :ky, :] = kernel
     kp = np.roll(kp, (-cx, -cy), axis=(0, 1))
-    spec = np.fft.rfftn(np.asarray(field, dtype=np.float64), s=(nx, ny, nzl),
-                        axes=(0, 1, 2))
-    spec *= np.fft.rfftn(kp, s=(nx, ny, nzl), axes=(0, 1, 2))
-    out = np.fft.irfftn(spec, s=(nx, ny, nzl), axes=(0, 1, 2))
+    # scipy's pocketfft (numpy's transform, multi-threaded) in float64
+    from scipy import fft as sfft
+    spec = sfft.rfftn(np.asarray(field, dtype=np.float64), s=(nx, ny, nzl),
+                      axes=(0, 1, 2), workers=-1)
+    spec *= sfft.rfftn(kp, s=(nx, ny, nzl), axes=(0, 1, 2), workers=-1)
+    out = sfft.irfftn(spec, s=(nx, ny, nzl), axes=(0, 1, 2), workers=-1)
     return np.ascontiguousarray(out[:, :, cz:cz + nz])
 
 

@@ -121,7 +121,13 @@ __device__ __forceinline__ LineNoise noise_line(const NoiseDev& nd, int x, int y
 // array with nz levels; 0 off-centre and in the zeroed boundary levels.
 // Follows steam/simulate.py:2146-2161 (zero levels, scale, off-centre mask,
 // shift, expm1) on the keyed draw of steam/simulate.py:554-598 (centres are
-// world sites divisible by s after the x/y wrap; z never wraps).
+// world sites divisible by s after the x/y wrap; z never wraps).  The
+// reference finds off-centre cells as those whose scaled draw is exactly 0;
+// here they are the cells off the centre lattice.  The two differ only where
+// a centre's draw is zero: the measure-zero set where sin(alpha(phi - phi0))
+// vanishes (e.g. u0 = 3/8 at alpha = 1.6), on which any evaluation returns
+// rounding noise and the FP64 transform, like the reference's own float32
+// one, keeps the cell as a centre (noise expm1(-shift)).
 __device__ __forceinline__ float noise_at(const NoiseDev& nd,
                                          const LineNoise& ln, int z, int nz) {
   if (!ln.xy_center) return 0.f;
@@ -134,8 +140,7 @@ __device__ __forceinline__ float noise_at(const NoiseDev& nd,
   uniform_pair(ln.wx, ln.wy, (int64_t)(z + (int)nd.oz), nd.key0, nd.key1,
                nd.key2, u0, u1);
   float gam = cms_f32(u0, u1, nd.cms);
-  float g = __fmul_rn(gam, nd.pcs);
-  if (g == 0.f) return 0.f;
+  const float g = __fmul_rn(gam, nd.pcs);
   return expm1_f32(__fsub_rn(g, nd.shift));
 }
 
