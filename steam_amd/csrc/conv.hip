@@ -96,7 +96,7 @@ static void set_tiles(ConvGeom& g, int nz_acc, int kahan, size_t& lds_z,
   }
   if (g.reg_x) {
     const int pth = g.nx / 8;
-    g.Tr = (1024 / pth < 8) ? 1024 / pth : 8;
+    g.Tr = (1024 / pth < XREG_COLS) ? 1024 / pth : XREG_COLS;
     g.wg_x = (int)(P * ((g.ny + g.Tr - 1) / g.Tr));
   } else {
     g.Tr = 0;

@@ -36,7 +36,22 @@
 #define Z_STAGE_CALLS false
 #endif
 
+// compile-time tuning knobs (overridable with -D for measurement builds)
+#ifndef STEAM_ZR_WAVES
+#define STEAM_ZR_WAVES 4
+#endif
+#ifndef STEAM_ZR_PREFETCH
+#define STEAM_ZR_PREFETCH 0
+#endif
+#ifndef STEAM_XREG_COLS
+#define STEAM_XREG_COLS 8
+#endif
+#ifndef STEAM_ZINV_ATTR
+#define STEAM_ZINV_ATTR
+#endif
+
 namespace steam {
+constexpr int XREG_COLS = STEAM_XREG_COLS;  // y columns per register x-pass workgroup
 // z passes: 512 threads holding 8 values each (same tile as 256 x 16, half
 // the registers per thread)
 constexpr int Z_NTH = 512;
@@ -123,6 +138,17 @@ __device__ __forceinline__ float kernel_gather(const PassDesc& d, int x, int y,
     acc += s;
   }
   return acc;
+}
+
+// Kernel-spectrum factor of z bin kz: the normalisation kscale times the
+// phase exp(+2 pi i kz cz / nfft), which shifts the circular z output by the
+// 'same' trim cz so that output level z is transform sample z.
+__device__ __forceinline__ float2 kernel_bin_factor(const PassDesc& d, int kz) {
+  const ConvGeom& g = d.g;
+  const int64_t r = ((int64_t)kz * g.cz) % g.nfft;
+  double sn, cs;
+  sincospi(2.0 * (double)r / (double)g.nfft, &sn, &cs);
+  return make_float2((float)(cs * d.kscale), (float)(sn * d.kscale));
 }
 
 // ---------------------------------------------------------------------------
@@ -322,7 +348,8 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
     __syncthreads();
     fft_lines<true, RS>(smem, g.ax, Tx, logTx, Tx, 1, true, tid);
   } else {
-    for (int e = tid; e < ne; e += FFT_NTH) smem[e] = cscale(smem[e], d.kscale);
+    const float2 f = kernel_bin_factor(d, p % g.nzc);
+    for (int e = tid; e < ne; e += FFT_NTH) smem[e] = cmul(smem[e], f);
     __syncthreads();
   }
   for (int e = tid; e < ne; e += FFT_NTH) {
@@ -370,7 +397,7 @@ template <class Src, int XMODE, int LOGN>
 __global__ __launch_bounds__(1024) void xpass_reg_kernel(Src src) {
   using F = RegFFT<LOGN>;
   constexpr int P = F::P;
-  constexpr int T = (1024 / P < 8) ? 1024 / P : 8;
+  constexpr int T = (1024 / P < XREG_COLS) ? 1024 / P : XREG_COLS;
   __shared__ float2 lds[T * F::LDSN];
   int wg = xcd_remap(blockIdx.x, gridDim.x);
   const PassDesc& d = src.get(PASS_X, wg);
@@ -395,8 +422,9 @@ __global__ __launch_bounds__(1024) void xpass_reg_kernel(Src src) {
       v[m] = cmul(v[m], valid ? Kp[(int64_t)(q + m * P) * ny] : make_float2(0.f, 0.f));
     fft_reg<LOGN>(v, q, lds, T, t, g.ax.tw, true);
   } else {
+    const float2 f = kernel_bin_factor(d, p % g.nzc);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = cscale(v[m], d.kscale);
+    for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], f);
   }
   if (valid) {
 #pragma unroll
@@ -465,10 +493,12 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
     }
     __syncthreads();
     fft_lines<true, RS, Z_STAGE_CALLS, Z_EM, Z_NTH>(smem, g.az, g.Tz, g.logTz, ldn, 1, true, tid);
-    const int zb = b * g.L - g.cz;
+    // sample m is output level b L + m; the last cz samples are the circular
+    // wrap, levels b L + m - nfft (the previous block's tail)
+    const int wrap = g.nfft - g.cz;
     for (int t = 0; t < g.Tz; ++t) {
       for (int m = tid; m < g.nfft; m += Z_NTH) {
-        const int zo = zb + m;
+        const int zo = b * g.L + (m < wrap ? m : m - g.nfft);
         if (zo >= 0 && zo < nz) {
           const float v = sr[((m >> 1) * ldn + t) * 2 + (m & 1)];
           const int ia = t * nz + zo;
@@ -777,7 +807,7 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
     if (e < n4) {
       const int f = e << 2;
       const int t = (int)((unsigned)f / (unsigned)nz);
-      const int m = f - t * nz + g.cz;
+      const int m = f - t * nz;
       float c[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) c[j] = sr[(((m + j) >> 1) * ldn + t) * 2 + ((m + j) & 1)];
@@ -821,12 +851,6 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
 // complex transform; the r2c split / c2r merge pair k with n2 - k through
 // the wave's LDS line.  Partial sums: one slot per workgroup.
 // ---------------------------------------------------------------------------
-#ifndef STEAM_ZR_WAVES
-#define STEAM_ZR_WAVES 4
-#endif
-#ifndef STEAM_ZR_PREFETCH
-#define STEAM_ZR_PREFETCH 0
-#endif
 constexpr int ZR_WAVES = STEAM_ZR_WAVES;  // wavefronts per workgroup
 constexpr int ZR_NTH = ZR_WAVES * 64;
 
@@ -982,7 +1006,7 @@ __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
 }
 
 template <class Src, int MODE, class ZP>
-__global__ __launch_bounds__(ZR_NTH) void zinv_reg_kernel(Src src) {
+__global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src src) {
   constexpr int N = ZP::N, V = ZP::V, P = ZP::P, LPW = ZP::LINES;
   __shared__ float2 lds_lines[ZR_WAVES * LPW * ZP::PITCH];
   __shared__ float2 tw2[2 * N];
@@ -995,8 +1019,7 @@ __global__ __launch_bounds__(ZR_NTH) void zinv_reg_kernel(Src src) {
   const int64_t nlines = (int64_t)g.nx * g.ny;
   const int64_t line = ((int64_t)wg * ZR_WAVES + w) * LPW + t;
   const bool valid = line < nlines;
-  const int nz = g.nz, cz = g.cz;
-  const bool even = (cz & 1) == 0;
+  const int nz = g.nz;
   const int64_t zoff = (valid ? line : 0) * nz;
   const int64_t plane = nlines;
   const float2* Si = d.S + (valid ? line : 0);
@@ -1005,21 +1028,15 @@ __global__ __launch_bounds__(ZR_NTH) void zinv_reg_kernel(Src src) {
   for (int m = 0; m < V; ++m) v[m] = valid ? Si[(int64_t)(q + m * P) * plane] : make_float2(0.f, 0.f);
   float2 yn = make_float2(0.f, 0.f);
   if (q == 0 && valid) yn = Si[(int64_t)N * plane];
-  // flux prefetch: output z = 2n - cz (+1) of natural index n
+  // output levels z = 2n, 2n+1 of natural index n (the kernel spectrum
+  // carries the 'same' trim shift); nz is even
   float2 fl[V];
   auto load_flux = [&]() {
 #pragma unroll
     for (int m = 0; m < V; ++m) {
-      const int z0 = 2 * (q + m * P) - cz;
-      fl[m] = make_float2(0.f, 0.f);
-      if (valid) {
-        if (even) {
-          if (z0 >= 0 && z0 < nz) fl[m] = *reinterpret_cast<const float2*>(d.out + zoff + z0);
-        } else {
-          if (z0 >= 0 && z0 < nz) fl[m].x = d.out[zoff + z0];
-          if (z0 + 1 >= 0 && z0 + 1 < nz) fl[m].y = d.out[zoff + z0 + 1];
-        }
-      }
+      const int z0 = 2 * (q + m * P);
+      fl[m] = (valid && z0 < nz) ? *reinterpret_cast<const float2*>(d.out + zoff + z0)
+                                 : make_float2(0.f, 0.f);
     }
   };
   if (MODE == MODE_FLUX && STEAM_ZR_PREFETCH) load_flux();
@@ -1045,17 +1062,10 @@ __global__ __launch_bounds__(ZR_NTH) void zinv_reg_kernel(Src src) {
   ZP::template fft<true>(v, q, L, tw2);
   double acc[2] = {0.0, 0.0};
   if (MODE == MODE_PLAIN) {
-    if (valid) {
 #pragma unroll
-      for (int m = 0; m < V; ++m) {
-        const int z0 = 2 * (q + m * P) - cz;
-        if (even) {
-          if (z0 >= 0 && z0 < nz) *reinterpret_cast<float2*>(d.out + zoff + z0) = v[m];
-        } else {
-          if (z0 >= 0 && z0 < nz) d.out[zoff + z0] = v[m].x;
-          if (z0 + 1 >= 0 && z0 + 1 < nz) d.out[zoff + z0 + 1] = v[m].y;
-        }
-      }
+    for (int m = 0; m < V; ++m) {
+      const int z0 = 2 * (q + m * P);
+      if (valid && z0 < nz) *reinterpret_cast<float2*>(d.out + zoff + z0) = v[m];
     }
   } else {
     if (!STEAM_ZR_PREFETCH) load_flux();
@@ -1064,25 +1074,16 @@ __global__ __launch_bounds__(ZR_NTH) void zinv_reg_kernel(Src src) {
     const bool win = valid && x >= d.win[0] && x < d.win[1] && y >= d.win[2] && y < d.win[3];
 #pragma unroll
     for (int m = 0; m < V; ++m) {
-      const int z0 = 2 * (q + m * P) - cz;
-      const bool in0 = valid && z0 >= 0 && z0 < nz;
-      const bool in1 = valid && z0 + 1 >= 0 && z0 + 1 < nz;
+      const int z0 = 2 * (q + m * P);
       float o0 = __fadd_rn(entering_value(pd, fl[m].x), v[m].x);
       float o1 = __fadd_rn(entering_value(pd, fl[m].y), v[m].y);
-      if (o0 < 0.f) {
-        if (in0) acc[1] += 1.0;
-        o0 = 0.f;
-      }
-      if (o1 < 0.f) {
-        if (in1) acc[1] += 1.0;
-        o1 = 0.f;
-      }
-      if (win) acc[0] += (in0 ? (double)o0 : 0.0) + (in1 ? (double)o1 : 0.0);
-      if (even) {
-        if (in0) *reinterpret_cast<float2*>(d.out + zoff + z0) = make_float2(o0, o1);
-      } else {
-        if (in0) d.out[zoff + z0] = o0;
-        if (in1) d.out[zoff + z0 + 1] = o1;
+      const int clipped = (o0 < 0.f) + (o1 < 0.f);
+      o0 = (o0 < 0.f) ? 0.f : o0;
+      o1 = (o1 < 0.f) ? 0.f : o1;
+      if (valid && z0 < nz) {
+        acc[1] += (double)clipped;
+        if (win) acc[0] += (double)o0 + (double)o1;
+        *reinterpret_cast<float2*>(d.out + zoff + z0) = make_float2(o0, o1);
       }
     }
     if (!d.apply) {
