@@ -192,6 +192,11 @@ typedef struct steam_flux_ledger {
 #define STEAM_DEFER_RESCALE 1 /* leave the new rescale pending (in the ledger) */
 #define STEAM_CHAIN_PENDING 2 /* read the pending rescale from the ledger the
                                  previous advance left in this workspace */
+#define STEAM_CLIP_IN_WINDOW 4 /* count clipped cells inside the window only
+                                  (the streamed passes count a tile's interior,
+                                  steam/streaming.py:1233-1234; the in-RAM
+                                  advance counts the whole array,
+                                  steam/simulate.py:2198) */
 
 /* Multiplier-noise field of one advance, (nx, ny, nz) floats into out_dev
  * (the VALU-bound half of the advance).  It does not read the flux, so a
@@ -244,7 +249,8 @@ typedef struct steam_batch_item {
   steam_noise_params noise;
   int64_t window[4];          /* x0, x1, y0, y1; all zero = whole field */
   float sigma_in;             /* pending rescale applied on read */
-  int32_t flags;              /* STEAM_DEFER_RESCALE | STEAM_CHAIN_PENDING */
+  int32_t flags;              /* STEAM_DEFER_RESCALE | STEAM_CHAIN_PENDING |
+                                 STEAM_CLIP_IN_WINDOW */
 } steam_batch_item;
 
 /* Group the instances by kernel variant and stage their descriptors

@@ -118,6 +118,7 @@ KERNEL_FAMILIES = {'zfwd': 0, 'ypass': 1, 'xpass': 2, 'zinv': 3,
                    'finalize': 4, 'post': 5, 'noise': 6}
 DEFER_RESCALE = 1
 CHAIN_PENDING = 2
+CLIP_IN_WINDOW = 4
 
 EXPORTED = tuple(name for name, _, _ in _SIGNATURES)
 

@@ -239,6 +239,7 @@ int make_item(const steam_batch_item& it, PassDesc& d, ItemPlan& ip) {
   d.noise = make_noise_dev(&it.noise);
   d.sk = it.sk_dev;
   d.apply = 0;
+  d.clip_win = (it.flags & STEAM_CLIP_IN_WINDOW) != 0;
   const int64_t* w = it.window;
   if (w[0] || w[1] || w[2] || w[3]) {
     if (w[0] < 0 || w[1] > p->nx || w[0] >= w[1] || w[2] < 0 || w[3] > p->ny || w[2] >= w[3])

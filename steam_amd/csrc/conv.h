@@ -86,6 +86,7 @@ struct PassDesc {
   DevLedger led_in;     // apply mode values
   float kscale;         // kernel-spectrum normalisation (MODE_KERNEL)
   int npart;            // partial-sum slots written by the z passes
+  int clip_win;         // count clipped cells inside the window only
   int wg_base[7];       // batched: first workgroup of z, y, x, zinv, post,
                         // finalize, noise passes
 };

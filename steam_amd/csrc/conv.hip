@@ -425,6 +425,7 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
                                   int flags, steam_stream_t stream) {
   const int defer_rescale = (flags & STEAM_DEFER_RESCALE) != 0;
   const int chain_pending = (flags & STEAM_CHAIN_PENDING) != 0;
+  const int clip_win = (flags & STEAM_CLIP_IN_WINDOW) != 0;
   if (!p || !spectrum_dev || !workspace_dev || !flux_dev)
     return set_error(STEAM_EINVAL, "steam_flux_advance: NULL argument");
   int rc = check_noise_params(noise);
@@ -444,6 +445,7 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
   d.noise = make_noise_dev(noise);
   d.sk = sk_dev;
   d.apply = mode == STEAM_APPLY;
+  d.clip_win = clip_win;
   if (window) {
     if (window[0] < 0 || window[1] > p->nx || window[0] >= window[1] ||
         window[2] < 0 || window[3] > p->ny || window[2] >= window[3])

@@ -532,7 +532,7 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
         float f = entering_value(pd, d.out[line + z]);
         f = __fadd_rn(f, accz[t * nz + z]);
         if (f < 0.f) {
-          acc[1] += 1.0;
+          if (win || !d.clip_win) acc[1] += 1.0;
           f = 0.f;
         }
         if (win) acc[0] += (double)f;
@@ -822,7 +822,7 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
         for (int j = 0; j < 4; ++j) {
           float v = __fadd_rn(entering_value(pd, o[j]), c[j]);
           if (v < 0.f) {
-            acc[1] += 1.0;
+            if (win || !d.clip_win) acc[1] += 1.0;
             v = 0.f;
           }
           o[j] = v;
@@ -1081,7 +1081,7 @@ __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src sr
       o0 = (o0 < 0.f) ? 0.f : o0;
       o1 = (o1 < 0.f) ? 0.f : o1;
       if (valid && z0 < nz) {
-        acc[1] += (double)clipped;
+        if (win || !d.clip_win) acc[1] += (double)clipped;
         if (win) acc[0] += (double)o0 + (double)o1;
         *reinterpret_cast<float2*>(d.out + zoff + z0) = make_float2(o0, o1);
       }

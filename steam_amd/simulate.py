@@ -238,6 +238,45 @@ def _compute_all_grids(k_values, inner_extent_x, inner_extent_y, inner_height,
     return out
 
 
+def nest_world_anchoring(k_values, root_domain_x, root_domain_y,
+                         root_domain_height, sparsity_factors, spheroscale_profile,
+                         z_profile, world_x_min, world_y_min, pad_x_per_class,
+                         pad_y_per_class, nest_z_arrays,
+                         anisotropy='piecewise_isotropic_below_spheroscale'):
+    """Where a nest's padded class grids sit on the world grids of a root run
+    over the root domain (steam/simulate.py:4176-4227): per class the world
+    origin (nearest world cell in x/y; in z the world level nearest the
+    nest's first level) and the world shape, so the nest's keyed noise draws
+    at the same world lattice sites the root does.
+
+    Returns (world_origins, world_shapes), lists of int triples.
+    """
+    world = _compute_all_grids(k_values, root_domain_x, root_domain_y,
+                               root_domain_height, sparsity_factors,
+                               spheroscale_profile, z_profile, anisotropy=anisotropy)
+    origins, shapes = [], []
+    for i in range(len(k_values)):
+        dx_world = float(world['dx'][i])
+        dy_world = float(world['dy'][i])
+        ox = int(round((world_x_min - pad_x_per_class[i]) / dx_world))
+        oy = int(round((world_y_min - pad_y_per_class[i]) / dy_world))
+        z_world = np.asarray(world['z_arrays'][i])
+        oz = int(np.argmin(np.abs(z_world - np.asarray(nest_z_arrays[i])[0])))
+        origins.append((ox, oy, oz))
+        shapes.append((int(world['nx'][i]), int(world['ny'][i]), int(world['nz'][i])))
+    return origins, shapes
+
+
+def nest_class_seeds(root_seed, n_classes_consumed, n_classes, seed=None):
+    """Per-class seed children of a nest (steam/simulate.py:4261-4270): the
+    root's stream continued past the classes it already spent, or an
+    independent stream when ``seed`` is given."""
+    if seed is None:
+        return np.random.SeedSequence(root_seed).spawn(
+            n_classes_consumed + n_classes)[n_classes_consumed:]
+    return np.random.SeedSequence(seed).spawn(n_classes)
+
+
 def _turbulon_envelope(k, dx, dy, dz, support_factor=SUPPORT_FACTOR,
                        shape='mexican_hat', device='cuda', as_tensor=False):
     """Centred turbulon kernel with the Gaussian-weighted zero-sum correction
@@ -286,7 +325,7 @@ def advance_flux_device(flux_t, region, kernel_t, flux_noise_scale,
                         ledger=None, alpha=None, sigma_in=1.0, sk_t=None,
                         defer_rescale=False, precision='plain', spec=None,
                         op=None, sync_ledger=True, chain_pending=False,
-                        noise_t=None):
+                        noise_t=None, clip_in_window=False):
     """Class advance of a device-resident flux (float32 tensor, in place).
 
     ``sigma_in`` is a pending float32 rescale applied on read; with
@@ -294,7 +333,8 @@ def advance_flux_device(flux_t, region, kernel_t, flux_noise_scale,
     of applied, and ``chain_pending`` takes the pending rescale from the
     ledger the previous advance on this operator left on the device.
     ``noise_t`` is the class's noise field from generate_noise_device (None
-    generates it inside the advance).  Returns (FluxAdvanceLedger, native
+    generates it inside the advance).  ``clip_in_window`` counts clipped
+    cells inside ``window`` only (the streamed passes' count).  Returns (FluxAdvanceLedger, native
     ledger); both None when ``sync_ledger`` is False in measure mode (the
     ledger then stays on the device).
     """
@@ -316,7 +356,8 @@ def advance_flux_device(flux_t, region, kernel_t, flux_noise_scale,
         mode = _lib.MEASURE
     want_host = ledger is None and sync_ledger
     flags = ((_lib.DEFER_RESCALE if defer_rescale else 0)
-             | (_lib.CHAIN_PENDING if chain_pending else 0))
+             | (_lib.CHAIN_PENDING if chain_pending else 0)
+             | (_lib.CLIP_IN_WINDOW if clip_in_window else 0))
     _lib.check(lib.steam_flux_advance(
         op.plan, _lib.ptr(spec), _lib.ptr(ws), _lib.ptr(flux_t),
         float(np.float32(sigma_in)), ctypes.byref(p), _lib.ptr(noise_t),

@@ -225,6 +225,12 @@ def regrid(src_t, target_shape, periodic=(True, True, False), scale=1.0,
     if out is None:
         out = torch.empty((x1 - x0, y1 - y0, tz), dtype=torch.float32,
                           device=src_t.device)
+    elif (tuple(out.shape) != (x1 - x0, y1 - y0, tz) or not out.is_contiguous()
+          or out.dtype != torch.float32):
+        raise ValueError("regrid: out must be a contiguous float32 tensor of the "
+                         "window's shape")
+    if not src_t.is_contiguous() or src_t.dtype != torch.float32:
+        raise ValueError("regrid: the source must be a contiguous float32 tensor")
     per = (ctypes.c_int32 * 3)(*(1 if p else 0 for p in periodic))
     _lib.check(lib.steam_regrid_trilinear(
         _lib.ptr(src_t), sx, sy, sz, tx, ty, tz, x0, x1 - x0, y0, y1 - y0, per,

@@ -1,0 +1,98 @@
+"""Streamed (tiled) flux cascade against the resident one: equal to the
+rounding floor at every tiling, no seam structure, deterministic."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# the reference's streamed-vs-resident bound (tests/test_streaming.py:293-324)
+STREAM_REL = 1e-5
+
+
+def _grids():
+    """16, 32, 64 cells across: every class wider than the 13-cell kernel.
+    (Where the kernel is wider than an even field the in-RAM operator folds
+    it with a one-cell centring offset, steam/utils.py:222-242 with the roll
+    at :366-372; tiles are always wider than the kernel and never fold, so
+    the reference's own streamed test also stays above the kernel width.)"""
+    from steam_amd.simulate import _compute_all_grids
+    k = 250.0 / 2.0 ** np.arange(3)
+    return _compute_all_grids(k, 2000., 2000., 800., (1, 1, 1),
+                              np.full(28, 100.), np.arange(28) * 30.)
+
+
+def _max_rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.fixture(scope="module")
+def resident():
+    from steam_amd.simulate import simulate_flux_only
+    g = _grids()
+    f, d = simulate_flux_only(g, seed=42, flux_noise_scale=0.3,
+                              min_distance_to_ground=1, zero_bottom=True,
+                              zero_top=True)
+    return g, f, d
+
+
+@pytest.mark.parametrize("tiles", [(1, 1), (2, 2), (4, 2), (3, 3), (8, 8)])
+def test_streamed_equals_resident(resident, tiles):
+    from steam_amd.streaming import streamed_flux_cascade
+    g, f_res, d_res = resident
+    f, d = streamed_flux_cascade(g, tiles, seed=42, flux_noise_scale=0.3,
+                                 min_distance_to_ground=1, zero_bottom=True,
+                                 zero_top=True)
+    assert f.shape == f_res.shape
+    assert _max_rel(f, f_res) < STREAM_REL
+    for s, r in zip(d['steps'], d_res['steps']):
+        assert abs(s['entering_mean'] / r['entering_mean'] - 1) < 1e-6
+        assert abs(s['realized_mean'] / r['realized_mean'] - 1) < 1e-5
+
+
+def test_no_seam_structure(resident):
+    from steam_amd.streaming import streamed_flux_cascade, tile_bounds
+    g, f_res, _ = resident
+    f, _ = streamed_flux_cascade(g, (4, 4), seed=42, flux_noise_scale=0.3,
+                                 min_distance_to_ground=1, zero_bottom=True,
+                                 zero_top=True)
+    nx, ny = f_res.shape[:2]
+    diff = np.abs(f.astype(np.float64) - f_res).mean(axis=2)
+    if diff.max() == 0:
+        return
+    sx = [lo for lo, _ in tile_bounds(nx, 4)]
+    sy = [lo for lo, _ in tile_bounds(ny, 4)]
+
+    def dist(n, seams):
+        i = np.arange(n)
+        return np.min(np.stack([np.minimum((i - s) % n, (s - i) % n) for s in seams]), 0)
+
+    d = np.minimum(dist(nx, sx)[:, None], dist(ny, sy)[None, :])
+    bins = [float(diff[d == k].mean()) for k in range(int(d.max()) + 1) if (d == k).sum() >= 8]
+    interior = np.median(bins[1:])
+    assert interior == 0 or bins[0] / interior < 4.0
+
+
+def test_streamed_run_is_deterministic(resident):
+    from steam_amd.streaming import streamed_flux_cascade
+    g, _, _ = resident
+    a, _ = streamed_flux_cascade(g, (3, 2), seed=42, flux_noise_scale=0.3)
+    b, _ = streamed_flux_cascade(g, (3, 2), seed=42, flux_noise_scale=0.3)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_regrid_window_writes_are_the_full_regrid():
+    import torch
+    from steam_amd.streaming import regrid_window, tile_windows
+    from steam_amd.utils import regrid
+    rng = np.random.default_rng(4)
+    src = torch.from_numpy(rng.standard_normal((12, 10, 7)).astype(np.float32)).cuda()
+    full = regrid(src, (24, 30, 9), scale=0.75).cpu().numpy()
+    out = np.empty_like(full)
+    for w in tile_windows(24, 30, 3, 4):
+        out[w[0]:w[1], w[2]:w[3]] = regrid_window(src, (24, 30, 9), w, scale=0.75).cpu().numpy()
+    np.testing.assert_array_equal(out, full)
+    with pytest.raises(ValueError):
+        regrid(src, (24, 30, 9), out=torch.empty((24, 30, 9), device='cuda')[:, :15])

@@ -302,3 +302,22 @@ def test_tile_geometry():
     assert M.tile_windows(4, 4, 2, 2)[1] == (0, 2, 2, 4)
     # dyadic: one extra source block each side, crop of one target block
     assert M.axis_regrid_plan(8, 16, 4, 12) == (1, 7, 2, 12)
+
+
+def test_package_tile_geometry_matches_oracle():
+    """The package's tile helpers restate the same reference functions
+    (steam/streaming.py:132-167, 249-269) as the oracle's."""
+    from steam_amd import streaming as ST
+    for n in range(1, 40):
+        for t in range(1, n + 1):
+            assert ST.tile_bounds(n, t) == M.tile_bounds(n, t)
+    assert ST.tile_windows(10, 7, 3, 2) == M.tile_windows(10, 7, 3, 2)
+    for n_in, n_out in [(4, 8), (5, 15), (6, 9), (8, 8), (7, 21)]:
+        for lo in range(0, n_out):
+            for hi in range(lo + 1, n_out + 1):
+                assert ST.axis_regrid_plan(n_in, n_out, lo, hi) == \
+                    M.axis_regrid_plan(n_in, n_out, lo, hi)
+    with pytest.raises(ValueError):
+        ST.tile_bounds(3, 4)
+    with pytest.raises(NotImplementedError):
+        ST.axis_regrid_plan(8, 4, 0, 2)
