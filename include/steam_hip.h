@@ -277,6 +277,94 @@ int steam_batch_info(const steam_batch* batch, int32_t* launches,
 int steam_batch_advance(steam_batch* batch, steam_stream_t stream);
 
 /* ------------------------------------------------------------------------
+ * Scalar increments of the cascade (steam/simulate.py:1924-2029): the joint
+ * pattern, per-level window reductions and the bounded amplitude-preserving
+ * add.  Fields (nx, ny, nz) float32; a level is the (x, y) plane at one z.
+ * ---------------------------------------------------------------------- */
+
+/* np.gradient's z differences in float32 (steam/simulate.py:2709-2741):
+ * uniform spacing uses (f[z+1]-f[z-1])/two_dz, otherwise the three-term
+ * a f[z-1] + b f[z] + c f[z+1] (device arrays of nz-2); one-sided ends. */
+typedef struct steam_zdiff {
+  int32_t uniform;
+  float two_dz;
+  float dz_first;
+  float dz_last;
+  const float* a_dev;
+  const float* b_dev;
+  const float* c_dev;
+} steam_zdiff;
+
+/* out = W * S_k * g per cell: W the advective weight |grad_h phi| +
+ * aspect(z) |d phi/dz| of the running field phi = pert + mean(z)
+ * (_advective_weight, steam/simulate.py:2787-2868), g the bound taper
+ * clip(min(phi - min, max - phi) / b(z), 0, 1) (_bound_taper, :2905-2930),
+ * in the reference's float32 operation order.  sk_dev = NULL gives the
+ * advective weight W alone (b_z_dev unused). */
+int steam_scalar_pattern(const float* pert_dev, const float* mean_z_dev,
+                         const float* sk_dev, const float* b_z_dev,
+                         const float* aspect_z_dev, const steam_zdiff* zdiff,
+                         int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                         double phi_min, double phi_max, float* out_dev,
+                         steam_stream_t stream);
+
+/* w = w / (level_mean > 0 ? level_mean : 1) * c(z), the product-norm apply
+ * (steam/simulate.py:1986-1990); level_mean and c are float32 per level. */
+int steam_pattern_normalize(float* w_dev, const float* level_mean_dev,
+                            const float* c_z_dev, int64_t n, int64_t nz,
+                            steam_stream_t stream);
+
+#define STEAM_LV_SUM 0     /* sum f */
+#define STEAM_LV_ABS 1     /* sum |f| */
+#define STEAM_LV_NONZERO 2 /* count f != 0 */
+#define STEAM_LV_CLAMP 3   /* sum clamp(f - float(mu[z]), lo, hi) */
+#define STEAM_LV_PHI 4     /* sum (f + mean[z]) */
+#define STEAM_LV_MIN 5     /* min (f + mean[z]) over the whole level */
+#define STEAM_LV_MAX 6     /* max (f + mean[z]) over the whole level */
+
+/* Device scratch bytes steam_level_reduce needs for this field shape. */
+int64_t steam_level_scratch_bytes(int64_t nx, int64_t ny, int64_t nz);
+
+/* Per-level reduction over window {x0, x1, y0, y1} (NULL = whole level) into
+ * out_dev[nz] (float64, device), in a fixed order: the level sums of the
+ * product norm (PatternLedger, steam/ledger.py:83-108) and of the bounded
+ * add. */
+int steam_level_reduce(const float* f_dev, const float* mean_z_dev,
+                       const double* mu_dev, int64_t nx, int64_t ny, int64_t nz,
+                       const int64_t* window, int op, double lo, double hi,
+                       double* out_dev, void* scratch_dev, int64_t scratch_bytes,
+                       steam_stream_t stream);
+
+/* The per-level scalars one bounded-add solve realized
+ * (BoundedAddLedger, steam/ledger.py:145-190); host arrays, demean and
+ * scale row-major (nz, n_rescale). */
+typedef struct steam_bounded_add_ledger {
+  double* a0;
+  double* demean;
+  double* scale;
+  int32_t* n_loop;
+  int32_t* n_scale;
+  double* final_demean;
+  double* mu;
+} steam_bounded_add_ledger;
+
+int64_t steam_bounded_add_workspace_bytes(int64_t nx, int64_t ny, int64_t nz);
+
+/* Add increment inc to pert under the three deposit conditions per level
+ * (zero window mean, window mean |increment| preserved, bounds respected):
+ * demean / clip / rescale loop, then a 60-step bisection of the zero-mean
+ * shift, as _bounded_amplitude_add's GPU path (steam/simulate.py:2985-3111).
+ * replay != 0 applies the recorded ledger with no reductions.  With
+ * record_applied, inc is left holding the increment actually added.
+ * Synchronises (once per rescale step and at the end). */
+int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float* inc_dev,
+                      int64_t nx, int64_t ny, int64_t nz, const int64_t* window,
+                      double phi_min, double phi_max, int n_rescale,
+                      int record_applied, steam_bounded_add_ledger* ledger,
+                      int replay, void* workspace_dev, int64_t workspace_bytes,
+                      steam_stream_t stream);
+
+/* ------------------------------------------------------------------------
  * (d) regrid: periodic trilinear upsample (steam/utils.py:435-534) and its
  *     tiled window form (steam/streaming.py:132-242).
  * ---------------------------------------------------------------------- */

@@ -58,6 +58,16 @@ class FluxLedger(ctypes.Structure):
                 ("n_clipped", _I64), ("rescale", _F32), ("flatten", _I32)]
 
 
+class ZDiff(ctypes.Structure):
+    _fields_ = [("uniform", _I32), ("two_dz", _F32), ("dz_first", _F32),
+                ("dz_last", _F32), ("a_dev", _P), ("b_dev", _P), ("c_dev", _P)]
+
+
+class BoundedAddLedgerC(ctypes.Structure):
+    _fields_ = [("a0", _P), ("demean", _P), ("scale", _P), ("n_loop", _P),
+                ("n_scale", _P), ("final_demean", _P), ("mu", _P)]
+
+
 class BatchItem(ctypes.Structure):
     _fields_ = [("plan", _P), ("spectrum_dev", _P), ("workspace_dev", _P),
                 ("flux_dev", _P), ("noise_dev", _P), ("sk_dev", _P),
@@ -103,6 +113,18 @@ _SIGNATURES = [
     ("steam_batch_set_noise", ctypes.c_int, [_P, _I32, ctypes.POINTER(NoiseParams)]),
     ("steam_batch_info", ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),
     ("steam_batch_advance", ctypes.c_int, [_P, _P]),
+    ("steam_scalar_pattern", ctypes.c_int,
+     [_P, _P, _P, _P, _P, ctypes.POINTER(ZDiff), _I64, _I64, _I64, _F64, _F64,
+      _F64, _F64, _P, _P]),
+    ("steam_pattern_normalize", ctypes.c_int, [_P, _P, _P, _I64, _I64, _P]),
+    ("steam_level_scratch_bytes", _I64, [_I64, _I64, _I64]),
+    ("steam_level_reduce", ctypes.c_int,
+     [_P, _P, _P, _I64, _I64, _I64, ctypes.POINTER(_I64), ctypes.c_int, _F64, _F64,
+      _P, _P, _I64, _P]),
+    ("steam_bounded_add_workspace_bytes", _I64, [_I64, _I64, _I64]),
+    ("steam_bounded_add", ctypes.c_int,
+     [_P, _P, _P, _I64, _I64, _I64, ctypes.POINTER(_I64), _F64, _F64, ctypes.c_int,
+      ctypes.c_int, ctypes.POINTER(BoundedAddLedgerC), ctypes.c_int, _P, _I64, _P]),
     ("steam_regrid_trilinear", ctypes.c_int,
      [_P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64,
       ctypes.POINTER(_I32), _F32, _P, _P]),

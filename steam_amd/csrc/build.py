@@ -14,7 +14,7 @@ PKG = os.path.dirname(HERE)
 OBJ = os.path.join(HERE, "_build")
 TARGET = os.path.join(PKG, "libsteam_hip.so")
 SOURCES = ["util.hip", "noise.hip", "envelope.hip", "regrid.hip", "conv.hip",
-           "batch.hip"]
+           "batch.hip", "scalar.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [

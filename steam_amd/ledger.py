@@ -79,8 +79,88 @@ class FluxAdvanceLedger:
                 f"n_clipped={self.n_clipped})")
 
 
+N_RESCALE_DEFAULT = 10
+
+
+class PatternLedger:
+    """The joint product norm of one scalar at one class
+    (steam/ledger.py:83-108): per-level float64 sum of |g W S| over the
+    window and the count of turbulon centers (nonzero cells)."""
+
+    __slots__ = ('level_total', 'level_count')
+
+    def __init__(self, level_total, level_count):
+        self.level_total = np.asarray(level_total, dtype=np.float64)
+        self.level_count = np.asarray(level_count, dtype=np.int64)
+
+    @property
+    def level_mean(self):
+        return (self.level_total / np.maximum(self.level_count, 1)).astype(np.float32)
+
+    def __add__(self, other):
+        return PatternLedger(self.level_total + other.level_total,
+                             self.level_count + other.level_count)
+
+
+class BoundedAddLedger:
+    """The per-level scalars one bounded-add solve realized
+    (steam/ledger.py:145-190): enough to replay it with no reductions."""
+
+    __slots__ = ('a0', 'demean', 'scale', 'n_loop', 'n_scale', 'final_demean', 'mu')
+
+    def __init__(self, nz, n_rescale=N_RESCALE_DEFAULT):
+        self.a0 = np.zeros(nz, dtype=np.float64)
+        self.demean = np.zeros((nz, n_rescale), dtype=np.float64)
+        self.scale = np.ones((nz, n_rescale), dtype=np.float64)
+        self.n_loop = np.zeros(nz, dtype=np.int32)
+        self.n_scale = np.zeros(nz, dtype=np.int32)
+        self.final_demean = np.zeros(nz, dtype=np.float64)
+        self.mu = np.full(nz, np.nan, dtype=np.float64)
+
+    @property
+    def clip_only(self):
+        return self.a0 <= 0.0
+
+    @property
+    def bisected(self):
+        return ~np.isnan(self.mu)
+
+
+class ProjectionLedger:
+    """Per-level mean-preserving shift of one projection onto the bounds
+    (steam/ledger.py:193-209); NaN where a level was left untouched."""
+
+    __slots__ = ('mu',)
+
+    def __init__(self, mu):
+        self.mu = np.asarray(mu, dtype=np.float64)
+
+    @property
+    def violating(self):
+        return ~np.isnan(self.mu)
+
+
+class FluxOutputLedger:
+    """The flux output composition's clip-and-restore means
+    (steam/ledger.py:212-225)."""
+
+    __slots__ = ('entering', 'realized')
+
+    def __init__(self, entering, realized):
+        self.entering = entering
+        self.realized = realized
+
+    @property
+    def rescale(self):
+        realized = self.realized.mean
+        if realized > 0:
+            return np.float32(self.entering.mean / realized)
+        return None
+
+
 class ClassLedger:
-    """One size class; the flux advance is the part on this path."""
+    """One size class: the flux advance plus, per scalar, the product norm
+    and the bounded add (steam/ledger.py:228-237)."""
 
     __slots__ = ('flux', 'pattern', 'bounded_add')
 

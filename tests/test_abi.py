@@ -23,6 +23,10 @@ STRUCTS = {
                                          "noise_abs_total", "noise_abs_count",
                                          "realized_total", "realized_count",
                                          "n_clipped", "rescale", "flatten"]),
+    "steam_zdiff": ("ZDiff", ["uniform", "two_dz", "dz_first", "dz_last", "a_dev",
+                              "b_dev", "c_dev"]),
+    "steam_bounded_add_ledger": ("BoundedAddLedgerC", ["a0", "demean", "scale", "n_loop",
+                                                       "n_scale", "final_demean", "mu"]),
     "steam_batch_item": ("BatchItem", ["plan", "spectrum_dev", "workspace_dev",
                                        "flux_dev", "noise_dev", "sk_dev", "noise",
                                        "window", "sigma_in", "flags"]),
