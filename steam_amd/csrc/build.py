@@ -15,6 +15,10 @@ OBJ = os.path.join(HERE, "_build")
 TARGET = os.path.join(PKG, "libsteam_hip.so")
 SOURCES = ["util.hip", "noise.hip", "envelope.hip", "regrid.hip", "conv.hip",
            "batch.hip", "scalar.hip"]
+# The scalar kernels restate float32 numpy arithmetic operation by operation:
+# no a*b+c may be fused into one fma there (the __f*_rn intrinsics are header
+# functions, outside the reach of a pragma in the source file).
+SOURCE_FLAGS = {"scalar.hip": ["-ffp-contract=off"]}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [
@@ -26,7 +30,7 @@ FLAGS = [
 
 def _compile(src, extra):
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(HERE, src), "-o", obj]
+    cmd = [HIPCC, *FLAGS, *SOURCE_FLAGS.get(src, []), *extra, "-c", os.path.join(HERE, src), "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     return src, obj, res
 

@@ -15,6 +15,12 @@
 
 #include "common.h"
 
+// Every operation below rounds separately, as the reference's float32 numpy
+// arithmetic does (HIP would otherwise contract a*b+c into one fma).  The
+// pragma governs the operators written here; the file is also built with
+// -ffp-contract=off, which reaches the __f*_rn header functions.
+#pragma clang fp contract(off)
+
 namespace steam {
 
 // z differences of np.gradient (steam/simulate.py:2709-2741), float32
@@ -50,7 +56,7 @@ __global__ __launch_bounds__(256) void pattern_kernel(
     };
     const float gx = __fdiv_rn(__fsub_rn(phi(xp, y, z, mz), phi(xm, y, z, mz)), two_dx);
     const float gy = __fdiv_rn(__fsub_rn(phi(x, yp, z, mz), phi(x, ym, z, mz)), two_dy);
-    float w = __fsqrt_rn(__fadd_rn(__fmul_rn(gx, gx), __fmul_rn(gy, gy)));
+    float w = sqrtf(__fadd_rn(__fmul_rn(gx, gx), __fmul_rn(gy, gy)));
     const float fc = phi(x, y, z, mz);
     float gz;
     if (nz == 1) {
