@@ -321,6 +321,10 @@ int steam_pattern_normalize(float* w_dev, const float* level_mean_dev,
 #define STEAM_LV_PHI 4     /* sum (f + mean[z]) */
 #define STEAM_LV_MIN 5     /* min (f + mean[z]) over the whole level */
 #define STEAM_LV_MAX 6     /* max (f + mean[z]) over the whole level */
+/* the field in float64, e = (double)f + (double)mean[z] */
+#define STEAM_LV_E_SUM 7   /* sum e */
+#define STEAM_LV_E_MIN 8   /* min e over the whole level */
+#define STEAM_LV_E_MAX 9   /* max e over the whole level */
 
 /* Device scratch bytes steam_level_reduce needs for this field shape. */
 int64_t steam_level_scratch_bytes(int64_t nx, int64_t ny, int64_t nz);
@@ -363,6 +367,56 @@ int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float* inc_dev,
                       int record_applied, steam_bounded_add_ledger* ledger,
                       int replay, void* workspace_dev, int64_t workspace_bytes,
                       steam_stream_t stream);
+
+/* Compensation deficit update, deficit += float(coef[z] * inc) with
+ * inc = after - before (before may be NULL: inc = after).  Replaces the
+ * deficit accumulations of steam/simulate.py:1893-1919 (flux, with the
+ * state before and after the advance) and 2015-2016 (scalars, with the
+ * recorded applied increment). */
+int steam_deficit_accumulate(float* deficit_dev, const float* after_dev,
+                             const float* before_dev, const float* coef_z_dev,
+                             int64_t n, int64_t nz, steam_stream_t stream);
+
+/* Device workspace bytes steam_project_onto_bounds and
+ * steam_compose_output need for this shape. */
+int64_t steam_project_workspace_bytes(int64_t nx, int64_t ny, int64_t nz);
+
+/* Projection onto [phi_min, phi_max] preserving each level's window mean
+ * (_project_onto_bounds, steam/simulate.py:3417-3500, in the float64 form
+ * of its GPU dispatch _project_onto_bounds_cuda, 3286-3414): levels of e = pert + mean (float64) with any cell out of
+ * bounds become float(clamp(e - mu, lo, hi) - mean) with mu from a 60-step
+ * float64 bisection; other levels are left bit-identical.  mu[nz] (host)
+ * receives the solved constants, NaN for untouched levels; with replay it
+ * supplies them (apply-only, no reduction).  Synchronizes the stream. */
+int steam_project_onto_bounds(float* pert_dev, const float* mean_z_dev, int64_t nx,
+                              int64_t ny, int64_t nz, const int64_t* window,
+                              double phi_min, double phi_max, double* mu, int replay,
+                              void* workspace_dev, int64_t workspace_bytes,
+                              steam_stream_t stream);
+
+/* The written scalar field (_compose_output, steam/simulate.py:3558-3599):
+ * with a deficit, deficit += state, projected as above, out =
+ * clamp(deficit + mean, float(lo), float(hi)) (out may be deficit); with
+ * deficit NULL, out = clamp(state + mean) and no projection is taken. */
+int steam_compose_output(const float* state_dev, float* deficit_dev,
+                         const float* mean_z_dev, int64_t nx, int64_t ny, int64_t nz,
+                         const int64_t* window, double phi_min, double phi_max,
+                         double* mu, int replay, float* out_dev, void* workspace_dev,
+                         int64_t workspace_bytes, steam_stream_t stream);
+
+/* Device workspace bytes steam_compose_flux_output needs for this shape. */
+int64_t steam_compose_flux_workspace_bytes(int64_t nx, int64_t ny, int64_t nz);
+
+/* The written flux (_compose_flux_output, steam/simulate.py:3602-3649):
+ * deficit = max(deficit + state, 0), rescaled by float(entering mean /
+ * realized mean) over the window (or filled with the entering mean when
+ * nothing is left).  sums[4] = {entering total, entering count, realized
+ * total, realized count} (host; out, or in with replay).  Synchronizes the
+ * stream when measuring. */
+int steam_compose_flux_output(const float* state_dev, float* deficit_dev, int64_t nx,
+                              int64_t ny, int64_t nz, const int64_t* window,
+                              double* sums, int replay, void* workspace_dev,
+                              int64_t workspace_bytes, steam_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * (d) regrid: periodic trilinear upsample (steam/utils.py:435-534) and its

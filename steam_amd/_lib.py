@@ -125,6 +125,18 @@ _SIGNATURES = [
     ("steam_bounded_add", ctypes.c_int,
      [_P, _P, _P, _I64, _I64, _I64, ctypes.POINTER(_I64), _F64, _F64, ctypes.c_int,
       ctypes.c_int, ctypes.POINTER(BoundedAddLedgerC), ctypes.c_int, _P, _I64, _P]),
+    ("steam_deficit_accumulate", ctypes.c_int, [_P, _P, _P, _P, _I64, _I64, _P]),
+    ("steam_project_workspace_bytes", _I64, [_I64, _I64, _I64]),
+    ("steam_project_onto_bounds", ctypes.c_int,
+     [_P, _P, _I64, _I64, _I64, ctypes.POINTER(_I64), _F64, _F64, ctypes.POINTER(_F64),
+      ctypes.c_int, _P, _I64, _P]),
+    ("steam_compose_output", ctypes.c_int,
+     [_P, _P, _P, _I64, _I64, _I64, ctypes.POINTER(_I64), _F64, _F64, ctypes.POINTER(_F64),
+      ctypes.c_int, _P, _P, _I64, _P]),
+    ("steam_compose_flux_workspace_bytes", _I64, [_I64, _I64, _I64]),
+    ("steam_compose_flux_output", ctypes.c_int,
+     [_P, _P, _I64, _I64, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_F64), ctypes.c_int, _P,
+      _I64, _P]),
     ("steam_regrid_trilinear", ctypes.c_int,
      [_P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64,
       ctypes.POINTER(_I32), _F32, _P, _P]),

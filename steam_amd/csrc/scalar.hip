@@ -114,26 +114,34 @@ enum LevelOp {
   LV_PHI = 4,        // sum (f + mean[z]) (the running field)
   LV_MIN = 5,        // min (f + mean[z]) over the whole level (not windowed)
   LV_MAX = 6,        // max (f + mean[z]) over the whole level
+  // the same field in float64, e = (double)f + (double)mean[z] (exact)
+  LV_E_SUM = 7,      // sum e
+  LV_E_MIN = 8,      // min e over the whole level
+  LV_E_MAX = 9,      // max e over the whole level
+  LV_E_CLAMP = 10,   // sum clamp(e - mu[z], lo, hi) on levels with active[z]
 };
 
 struct LevelArgs {
   const float* f;
   const float* mean_z;
   const double* mu;
+  const double* active;  // LV_E_CLAMP: levels to reduce (others give 0)
   int nx, ny, nz;
   int win[4];
   int op;
-  float lo, hi;
+  double lo, hi;
 };
 
 constexpr int LV_LINES = 32;   // lines per first-stage workgroup
 constexpr int LV_ROWS = 64;    // partial rows per combine workgroup
 
+__device__ __forceinline__ bool lv_is_min(int op) { return op == LV_MIN || op == LV_E_MIN; }
+__device__ __forceinline__ bool lv_is_max(int op) { return op == LV_MAX || op == LV_E_MAX; }
 __device__ __forceinline__ double lv_init(int op) {
-  return op == LV_MIN ? INFINITY : (op == LV_MAX ? -INFINITY : 0.0);
+  return lv_is_min(op) ? INFINITY : (lv_is_max(op) ? -INFINITY : 0.0);
 }
 __device__ __forceinline__ double lv_join(int op, double a, double b) {
-  return op == LV_MIN ? fmin(a, b) : (op == LV_MAX ? fmax(a, b) : a + b);
+  return lv_is_min(op) ? fmin(a, b) : (lv_is_max(op) ? fmax(a, b) : a + b);
 }
 
 // first stage: workgroup w covers lines [w*LV_LINES, ...); thread t handles
@@ -141,11 +149,17 @@ __device__ __forceinline__ double lv_join(int op, double a, double b) {
 __global__ __launch_bounds__(256) void level_partial_kernel(LevelArgs a, double* part) {
   const int64_t nlines = (int64_t)a.nx * a.ny;
   const int64_t l0 = (int64_t)blockIdx.x * LV_LINES;
-  const bool whole = a.op == LV_MIN || a.op == LV_MAX;
+  const bool whole = lv_is_min(a.op) || lv_is_max(a.op);
+  const float flo = (float)a.lo, fhi = (float)a.hi;
   for (int z = threadIdx.x; z < a.nz; z += blockDim.x) {
     double acc = lv_init(a.op);
     const float mz = a.mean_z ? a.mean_z[z] : 0.f;
     const float muz = a.mu ? (float)a.mu[z] : 0.f;
+    const double mu64 = a.mu ? a.mu[z] : 0.0;
+    if (a.op == LV_E_CLAMP && a.active[z] == 0.0) {
+      part[(int64_t)blockIdx.x * a.nz + z] = 0.0;
+      continue;
+    }
     for (int j = 0; j < LV_LINES; ++j) {
       const int64_t line = l0 + j;
       if (line >= nlines) break;
@@ -158,7 +172,11 @@ __global__ __launch_bounds__(256) void level_partial_kernel(LevelArgs a, double*
         case LV_SUM: t = (double)v; break;
         case LV_ABS: t = (double)fabsf(v); break;
         case LV_NONZERO: t = v != 0.f ? 1.0 : 0.0; break;
-        case LV_CLAMP: t = (double)fminf(fmaxf(__fsub_rn(v, muz), a.lo), a.hi); break;
+        case LV_CLAMP: t = (double)fminf(fmaxf(__fsub_rn(v, muz), flo), fhi); break;
+        case LV_E_SUM:
+        case LV_E_MIN:
+        case LV_E_MAX: t = (double)v + (double)mz; break;
+        case LV_E_CLAMP: t = fmin(fmax(((double)v + (double)mz) - mu64, a.lo), a.hi); break;
         default: t = (double)__fadd_rn(v, mz); break;
       }
       acc = lv_join(a.op, acc, t);
@@ -312,6 +330,117 @@ __global__ void ba_init_done_kernel(BaVec v, int nz) {
   if (z < nz) v.done[z] = v.a0[z] <= 0.0 ? 1.0 : 0.0;
 }
 
+// ---------------------------------------------------------------------------
+// compensation deficits and output composition (steam/simulate.py:1893-1919,
+// 1999-2003, 2015-2016, 3558-3649)
+// ---------------------------------------------------------------------------
+// deficit += f32(coef[z] * inc), inc = after - before (or after alone)
+__global__ __launch_bounds__(256) void deficit_kernel(float* __restrict__ deficit,
+                                                      const float* __restrict__ after,
+                                                      const float* __restrict__ before,
+                                                      const float* __restrict__ coef_z,
+                                                      int64_t n, int nz) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i % nz);
+    float inc = after[i];
+    if (before) inc = __fsub_rn(inc, before[i]);
+    deficit[i] = __fadd_rn(deficit[i], __fmul_rn(coef_z[z], inc));
+  }
+}
+
+// d += s
+__global__ __launch_bounds__(256) void add_state_kernel(float* __restrict__ d,
+                                                        const float* __restrict__ state,
+                                                        int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = __fadd_rn(d[i], state[i]);
+}
+
+// out = clamp(src + mean[z], lo, hi) (float32, the written scalar field)
+// (out may be src)
+__global__ __launch_bounds__(256) void compose_kernel(const float* src,
+                                                      const float* __restrict__ mean_z,
+                                                      float* out, int64_t n,
+                                                      int nz, float lo, float hi) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i % nz);
+    out[i] = fminf(fmaxf(__fadd_rn(src[i], mean_z[z]), lo), hi);
+  }
+}
+
+// projection onto the bounds: active[z] = level violates; bracket set up
+__global__ void proj_init_kernel(BaVec v, const double* vmin, const double* vmax, double* active,
+                                 int nz, double lo, double hi, double width) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z < nz) {
+    active[z] = (vmin[z] < lo || vmax[z] > hi) ? 1.0 : 0.0;
+    v.mu_lo[z] = -width;
+    v.mu_hi[z] = width;
+    v.mu[z] = 0.5 * (v.mu_lo[z] + v.mu_hi[z]);
+  }
+}
+// recorded mu: NaN on levels left alone
+__global__ void proj_record_kernel(const double* mu, const double* active, double* out, int nz) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z < nz) out[z] = active[z] != 0.0 ? mu[z] : __builtin_nan("");
+}
+// replay: active where the recorded mu is a number
+__global__ void proj_active_kernel(const double* mu, double* active, int nz) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z < nz) active[z] = (mu[z] == mu[z]) ? 1.0 : 0.0;
+}
+// f = f32(clamp(e - mu, lo, hi) - mean) on active levels; e = f + mean in float64
+__global__ __launch_bounds__(256) void proj_apply_kernel(float* __restrict__ f,
+                                                         const float* __restrict__ mean_z,
+                                                         const double* __restrict__ mu,
+                                                         const double* __restrict__ active,
+                                                         int64_t n, int nz, double lo,
+                                                         double hi) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i % nz);
+    if (active[z] == 0.0) continue;
+    const double m = (double)mean_z[z];
+    const double e = (double)f[i] + m;
+    f[i] = (float)(fmin(fmax(e - mu[z], lo), hi) - m);
+  }
+}
+
+// flux composition: d = max(d + state, 0)
+__global__ __launch_bounds__(256) void flux_compose_kernel(float* __restrict__ d,
+                                                           const float* __restrict__ state,
+                                                           int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = fmaxf(__fadd_rn(d[i], state[i]), 0.f);
+}
+// total of per-level sums, in level order, and its mean over count cells
+// (one thread)
+__global__ void sum_levels_kernel(const double* lv, int nz, double count, double* total,
+                                  double* mean) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double s = 0.0;
+    for (int z = 0; z < nz; ++z) s += lv[z];
+    *total = s;
+    *mean = s / count;
+  }
+}
+// rescale to the entering volume mean: d *= f32(em / rm), or d = f32(em)
+// when nothing is left (means[0] entering, means[1] realized)
+__global__ __launch_bounds__(256) void flux_restore_kernel(float* __restrict__ d,
+                                                           const double* __restrict__ means,
+                                                           int64_t n) {
+  const double em = means[0], rm = means[1];
+  const bool fill = !(rm > 0.0);
+  const float r = fill ? (float)em : (float)(em / rm);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = fill ? r : __fmul_rn(d[i], r);
+}
+
 }  // namespace steam
 
 using namespace steam;
@@ -422,8 +551,8 @@ extern "C" int steam_level_reduce(const float* f_dev, const float* mean_z_dev,
                                   steam_stream_t stream) {
   if (!f_dev || !out_dev || !scratch_dev || nx < 1 || ny < 1 || nz < 1)
     return set_error(STEAM_EINVAL, "steam_level_reduce: bad arguments");
-  if (op < LV_SUM || op > LV_MAX) return set_error(STEAM_EINVAL, "steam_level_reduce: unknown op");
-  if ((op == LV_PHI || op == LV_MIN || op == LV_MAX) && !mean_z_dev)
+  if (op < LV_SUM || op > LV_E_MAX) return set_error(STEAM_EINVAL, "steam_level_reduce: unknown op");
+  if ((op == LV_PHI || op == LV_MIN || op == LV_MAX || op >= LV_E_SUM) && !mean_z_dev)
     return set_error(STEAM_EINVAL, "steam_level_reduce: op needs mean_z");
   if (op == LV_CLAMP && !mu_dev) return set_error(STEAM_EINVAL, "steam_level_reduce: op needs mu");
   if (scratch_bytes < level_scratch_bytes(nx * ny, (int)nz))
@@ -437,8 +566,8 @@ extern "C" int steam_level_reduce(const float* f_dev, const float* mean_z_dev,
   a.ny = (int)ny;
   a.nz = (int)nz;
   a.op = op;
-  a.lo = (float)lo;
-  a.hi = (float)hi;
+  a.lo = (double)(float)lo;
+  a.hi = (double)(float)hi;
   int rc = check_window(window, (int)nx, (int)ny, a.win);
   if (rc) return rc;
   return level_reduce(a, out_dev, (double*)scratch_dev, (hipStream_t)stream);
@@ -446,7 +575,8 @@ extern "C" int steam_level_reduce(const float* f_dev, const float* mean_z_dev,
 
 extern "C" int64_t steam_bounded_add_workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
   const int64_t n = nx * ny * nz;
-  return ((n * 4 + 255) & ~(int64_t)255) + 16 * nz * 8 + level_scratch_bytes(nx * ny, (int)nz) + 256;
+  (void)n;
+  return 16 * nz * 8 + level_scratch_bytes(nx * ny, (int)nz) + 256;
 }
 
 extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float* inc_dev,
@@ -472,10 +602,9 @@ extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float
   const double npts = (double)(win[1] - win[0]) * (double)(win[3] - win[2]);
   const float lo = (float)phi_min, hi = (float)phi_max;
   const double width = phi_max - phi_min;
-  char* ws = (char*)workspace_dev;
-  float* scratch = (float*)ws;  // not needed as a field: the solve runs in place on inc
-  (void)scratch;
-  double* vec = (double*)(ws + ((n * 4 + 255) & ~(int64_t)255));
+  // the solve runs in place on inc; the workspace holds per-level vectors
+  // and the reduction scratch only
+  double* vec = (double*)workspace_dev;
   BaVec v;
   v.a0 = vec;
   v.done = vec + nz;
@@ -496,8 +625,8 @@ extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float
   a.nz = (int)nz;
   memcpy(a.win, win, sizeof(win));
   a.mean_z = mean_z_dev;
-  a.lo = lo;
-  a.hi = hi;
+  a.lo = (double)lo;
+  a.hi = (double)hi;
   auto reduce = [&](const float* f, int op, double* out, const double* mu) {
     LevelArgs b = a;
     b.f = f;
@@ -596,5 +725,205 @@ extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float
                      mean_z_dev, v.mu, n, (int)nz, lo, hi, record_applied);
   STEAM_LAUNCH_CHECK("bounded add");
   STEAM_HIP(hipStreamSynchronize(s));
+  return STEAM_OK;
+}
+
+extern "C" int steam_deficit_accumulate(float* deficit_dev, const float* after_dev,
+                                        const float* before_dev, const float* coef_z_dev,
+                                        int64_t n, int64_t nz, steam_stream_t stream) {
+  if (!deficit_dev || !after_dev || !coef_z_dev || nz < 1 || n % nz)
+    return set_error(STEAM_EINVAL, "steam_deficit_accumulate: bad arguments");
+  hipLaunchKernelGGL(deficit_kernel, dim3(grid_1d(n)), dim3(256), 0, (hipStream_t)stream,
+                     deficit_dev, after_dev, before_dev, coef_z_dev, n, (int)nz);
+  STEAM_LAUNCH_CHECK("deficit_kernel");
+  return STEAM_OK;
+}
+
+extern "C" int64_t steam_project_workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
+  return 16 * nz * 8 + level_scratch_bytes(nx * ny, (int)nz) + 256;
+}
+
+namespace {
+
+// projection of f (+ mean) onto [lo, hi] preserving every level's window
+// mean, levels with no violation untouched; mu_host[nz] in (replay) or out
+int project(float* f, const float* mean_z, int64_t nx, int64_t ny, int64_t nz, const int win[4],
+            double lo, double hi, double* mu_host, int replay, void* ws, hipStream_t s) {
+  const int64_t n = nx * ny * nz;
+  const double npts = (double)(win[1] - win[0]) * (double)(win[3] - win[2]);
+  const double width = hi - lo;
+  double* vec = (double*)ws;
+  BaVec v;
+  memset(&v, 0, sizeof(v));
+  v.sum = vec;
+  v.mu = vec + nz;
+  v.mu_lo = vec + 2 * nz;
+  v.mu_hi = vec + 3 * nz;
+  v.mean_phi = vec + 4 * nz;
+  double* active = vec + 5 * nz;
+  double* vmin = vec + 6 * nz;
+  double* vmax = vec + 7 * nz;
+  double* rec = vec + 8 * nz;
+  double* red = vec + 16 * nz;
+  const int gz = (int)((nz + 255) / 256);
+  const int gn = grid_1d(n);
+  LevelArgs a;
+  memset(&a, 0, sizeof(a));
+  a.f = f;
+  a.mean_z = mean_z;
+  a.nx = (int)nx;
+  a.ny = (int)ny;
+  a.nz = (int)nz;
+  memcpy(a.win, win, sizeof(a.win));
+  a.lo = lo;
+  a.hi = hi;
+  int rc;
+  auto reduce = [&](int op, double* out, const double* mu) {
+    LevelArgs b = a;
+    b.op = op;
+    b.mu = mu;
+    b.active = active;
+    return level_reduce(b, out, red, s);
+  };
+  if (replay) {
+    STEAM_HIP(hipMemcpyAsync(v.mu, mu_host, (size_t)nz * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(proj_active_kernel, dim3(gz), dim3(256), 0, s, v.mu, active, (int)nz);
+  } else {
+    if ((rc = reduce(LV_E_MIN, vmin, nullptr))) return rc;
+    if ((rc = reduce(LV_E_MAX, vmax, nullptr))) return rc;
+    hipLaunchKernelGGL(proj_init_kernel, dim3(gz), dim3(256), 0, s, v, vmin, vmax, active,
+                       (int)nz, lo, hi, width);
+    if ((rc = reduce(LV_E_SUM, v.sum, nullptr))) return rc;
+    hipLaunchKernelGGL(ba_mean_kernel, dim3(gz), dim3(256), 0, s, v.sum, v.mean_phi, (int)nz,
+                       npts);
+    for (int it = 0; it < 60; ++it) {
+      if ((rc = reduce(LV_E_CLAMP, v.sum, v.mu))) return rc;
+      hipLaunchKernelGGL(ba_bisect_step_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz, npts);
+    }
+    hipLaunchKernelGGL(proj_record_kernel, dim3(gz), dim3(256), 0, s, v.mu, active, rec,
+                       (int)nz);
+    STEAM_HIP(hipMemcpyAsync(mu_host, rec, (size_t)nz * 8, hipMemcpyDeviceToHost, s));
+  }
+  hipLaunchKernelGGL(proj_apply_kernel, dim3(gn), dim3(256), 0, s, f, mean_z, v.mu, active, n,
+                     (int)nz, lo, hi);
+  STEAM_LAUNCH_CHECK("projection");
+  STEAM_HIP(hipStreamSynchronize(s));
+  return STEAM_OK;
+}
+
+}  // namespace
+
+extern "C" int steam_project_onto_bounds(float* pert_dev, const float* mean_z_dev, int64_t nx,
+                                         int64_t ny, int64_t nz, const int64_t* window,
+                                         double phi_min, double phi_max, double* mu, int replay,
+                                         void* workspace_dev, int64_t workspace_bytes,
+                                         steam_stream_t stream) {
+  if (!pert_dev || !mean_z_dev || !mu || !workspace_dev || nx < 1 || ny < 1 || nz < 1)
+    return set_error(STEAM_EINVAL, "steam_project_onto_bounds: bad arguments");
+  if (!(phi_max > phi_min)) return set_error(STEAM_EINVAL, "steam_project_onto_bounds: empty bounds");
+  if (workspace_bytes < steam_project_workspace_bytes(nx, ny, nz))
+    return set_error(STEAM_EINVAL, "steam_project_onto_bounds: workspace too small");
+  int win[4];
+  int rc = check_window(window, (int)nx, (int)ny, win);
+  if (rc) return rc;
+  return project(pert_dev, mean_z_dev, nx, ny, nz, win, phi_min, phi_max, mu, replay,
+                 workspace_dev, (hipStream_t)stream);
+}
+
+extern "C" int steam_compose_output(const float* state_dev, float* deficit_dev,
+                                    const float* mean_z_dev, int64_t nx, int64_t ny, int64_t nz,
+                                    const int64_t* window, double phi_min, double phi_max,
+                                    double* mu, int replay, float* out_dev, void* workspace_dev,
+                                    int64_t workspace_bytes, steam_stream_t stream) {
+  if (!state_dev || !mean_z_dev || !out_dev || nx < 1 || ny < 1 || nz < 1)
+    return set_error(STEAM_EINVAL, "steam_compose_output: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = nx * ny * nz;
+  const float lo = (float)phi_min, hi = (float)phi_max;
+  if (!deficit_dev) {
+    hipLaunchKernelGGL(compose_kernel, dim3(grid_1d(n)), dim3(256), 0, s, state_dev, mean_z_dev,
+                       out_dev, n, (int)nz, lo, hi);
+    STEAM_LAUNCH_CHECK("compose_kernel");
+    return STEAM_OK;
+  }
+  if (!mu || !workspace_dev || workspace_bytes < steam_project_workspace_bytes(nx, ny, nz))
+    return set_error(STEAM_EINVAL, "steam_compose_output: projection needs mu and a workspace");
+  if (!(phi_max > phi_min)) return set_error(STEAM_EINVAL, "steam_compose_output: empty bounds");
+  int win[4];
+  int rc = check_window(window, (int)nx, (int)ny, win);
+  if (rc) return rc;
+  hipLaunchKernelGGL(add_state_kernel, dim3(grid_1d(n)), dim3(256), 0, s, deficit_dev, state_dev,
+                     n);
+  STEAM_LAUNCH_CHECK("add_state_kernel");
+  if ((rc = project(deficit_dev, mean_z_dev, nx, ny, nz, win, phi_min, phi_max, mu, replay,
+                    workspace_dev, s)))
+    return rc;
+  hipLaunchKernelGGL(compose_kernel, dim3(grid_1d(n)), dim3(256), 0, s, deficit_dev, mean_z_dev,
+                     out_dev, n, (int)nz, lo, hi);
+  STEAM_LAUNCH_CHECK("compose_kernel");
+  return STEAM_OK;
+}
+
+extern "C" int64_t steam_compose_flux_workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
+  return (nz + 8) * 8 + level_scratch_bytes(nx * ny, (int)nz) + 256;
+}
+
+extern "C" int steam_compose_flux_output(const float* state_dev, float* deficit_dev, int64_t nx,
+                                         int64_t ny, int64_t nz, const int64_t* window,
+                                         double* sums, int replay, void* workspace_dev,
+                                         int64_t workspace_bytes, steam_stream_t stream) {
+  if (!state_dev || !deficit_dev || !sums || !workspace_dev || nx < 1 || ny < 1 || nz < 1)
+    return set_error(STEAM_EINVAL, "steam_compose_flux_output: bad arguments");
+  if (workspace_bytes < steam_compose_flux_workspace_bytes(nx, ny, nz))
+    return set_error(STEAM_EINVAL, "steam_compose_flux_output: workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  int win[4];
+  int rc = check_window(window, (int)nx, (int)ny, win);
+  if (rc) return rc;
+  const int64_t n = nx * ny * nz;
+  const double count = (double)(win[1] - win[0]) * (double)(win[3] - win[2]) * (double)nz;
+  double* vec = (double*)workspace_dev;
+  double* lv = vec;            // per-level sums
+  double* tot = vec + nz;      // [0] entering total, [1] realized total
+  double* means = tot + 2;     // [0] entering mean, [1] realized mean
+  double* red = vec + nz + 8;
+  LevelArgs a;
+  memset(&a, 0, sizeof(a));
+  a.nx = (int)nx;
+  a.ny = (int)ny;
+  a.nz = (int)nz;
+  a.op = LV_SUM;
+  memcpy(a.win, win, sizeof(a.win));
+  if (replay) {
+    if (!(sums[1] > 0.0) || !(sums[3] > 0.0))
+      return set_error(STEAM_EINVAL, "steam_compose_flux_output: recorded counts must be positive");
+    double m[2] = {sums[0] / sums[1], sums[2] / sums[3]};
+    STEAM_HIP(hipMemcpyAsync(means, m, 16, hipMemcpyHostToDevice, s));
+  } else {
+    a.f = state_dev;
+    if ((rc = level_reduce(a, lv, red, s))) return rc;
+    hipLaunchKernelGGL(sum_levels_kernel, dim3(1), dim3(64), 0, s, lv, (int)nz, count, tot,
+                       means);
+  }
+  hipLaunchKernelGGL(flux_compose_kernel, dim3(grid_1d(n)), dim3(256), 0, s, deficit_dev,
+                     state_dev, n);
+  if (!replay) {
+    a.f = deficit_dev;
+    if ((rc = level_reduce(a, lv, red, s))) return rc;
+    hipLaunchKernelGGL(sum_levels_kernel, dim3(1), dim3(64), 0, s, lv, (int)nz, count,
+                       tot + 1, means + 1);
+  }
+  hipLaunchKernelGGL(flux_restore_kernel, dim3(grid_1d(n)), dim3(256), 0, s, deficit_dev, means,
+                     n);
+  STEAM_LAUNCH_CHECK("flux composition");
+  if (!replay) {
+    double t[2];
+    STEAM_HIP(hipMemcpyAsync(t, tot, 16, hipMemcpyDeviceToHost, s));
+    STEAM_HIP(hipStreamSynchronize(s));
+    sums[0] = t[0];
+    sums[1] = count;
+    sums[2] = t[1];
+    sums[3] = count;
+  }
   return STEAM_OK;
 }

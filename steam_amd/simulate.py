@@ -441,6 +441,10 @@ def _advance_flux(flux, noise_region, kernel, flux_noise_scale,
     return to_caller(sk, was_np), led, diagnostics
 
 
+# The default advance, kept so a driver can tell whether the seam was rebound.
+_advance_flux_default = _advance_flux
+
+
 def _advance_flux_composed(ft, noise_region, kernel, flux_noise_scale,
                            n_scale_classes_per_dyad, sparsity_factors, n_zero,
                            zero_bottom, zero_top, device, window, ledger, alpha):
@@ -605,3 +609,35 @@ def simulate_flux_only(grids, sparsity_factors=(1, 1, 1), seed=None,
         'ledger': ledger,
     }
     return (flux if return_tensor else flux.cpu().numpy()), diagnostics
+
+
+# The cascade driver, its composition and the parameter set-up live in their
+# own modules (steam_amd.cascade, steam_amd.params); they are reachable here
+# under the reference's names.
+_LAZY = {
+    'cascade_loop': ('cascade', 'cascade_loop'),
+    'simulate_fields': ('cascade', 'simulate_fields'),
+    '_compose_output': ('cascade', 'compose_output'),
+    '_compose_flux_output': ('cascade', 'compose_flux_output'),
+    '_project_onto_bounds': ('cascade', 'project_onto_bounds'),
+    '_compute_normalization': ('params', '_compute_normalization'),
+    '_bound_buffers': ('params', '_bound_buffers'),
+    '_compensation_profiles': ('params', '_compensation_profiles'),
+    '_interpolation_compensation': ('params', '_interpolation_compensation'),
+    '_hop_retention': ('params', '_hop_retention'),
+    '_canonical_delivery_reference': ('params', '_canonical_delivery_reference'),
+    'HOP_RETENTION': ('params', 'HOP_RETENTION'),
+    'INTERPOLATION_COMPENSATION': ('params', 'INTERPOLATION_COMPENSATION'),
+    'BOUND_BUFFER_MULTIPLE': ('params', 'BOUND_BUFFER_MULTIPLE'),
+    'HAAR_TO_MHAT': ('params', 'HAAR_TO_MHAT'),
+    '_advective_weight': ('scalars', 'advective_weight'),
+    '_bounded_amplitude_add': ('scalars', 'bounded_amplitude_add'),
+}
+
+
+def __getattr__(name):
+    if name in _LAZY:
+        import importlib
+        mod, attr = _LAZY[name]
+        return getattr(importlib.import_module(f'{__package__}.{mod}'), attr)
+    raise AttributeError(f"module {__name__!r} has no attribute {name!r}")

@@ -1,0 +1,183 @@
+"""The GPU cascade driver and output composition: the whole root run against
+the reference's committed fixtures (its CUDA-path fixture root_cuda and the
+host-path ones) and against the oracle's cascade; apply-only replay; the
+projection, composition and deficit kernels against the oracle; the
+reference's realized-norm and error-behaviour tests."""
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import GOLDEN
+from oracle import cascade as OC
+from test_oracle_cascade import CASES, TOL, fixture_setup, rel, run_oracle, H_PROFILE, \
+    QT_PROFILE
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ('h', 'qt', 'flux', 'h_perturbation', 'qt_perturbation', 'flux_state')
+
+
+def gpu_fields(s, n, ledger=None):
+    from steam_amd import cascade as C
+    return C.simulate_fields(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0,
+                             800.0, 30.0, sparsity_factors=s, n_scale_classes_per_dyad=n,
+                             seed=42, ledger=ledger)
+
+
+@pytest.mark.parametrize("name,s,n", CASES)
+def test_gpu_root_run_matches_reference_fixtures(name, s, n):
+    out = gpu_fields(s, n)
+    ref = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    for field, tol in TOL.items():
+        assert out[field].shape == ref[field].shape, field
+        assert rel(out[field], ref[field]) < tol, (name, field, rel(out[field], ref[field]))
+    hmin, hmax, qmin, qmax = out['setup']['bounds']
+    assert out['h'].min() >= np.float32(hmin) and out['h'].max() <= np.float32(hmax)
+    assert out['qt'].min() >= np.float32(qmin) and out['qt'].max() <= np.float32(qmax)
+    assert out['flux'].min() >= 0
+
+
+@pytest.mark.parametrize("s,n", [((1, 1, 1), 1), ((2, 2, 2), 1)])
+def test_gpu_root_run_matches_oracle(s, n):
+    out = gpu_fields(s, n)
+    ora = run_oracle(fixture_setup(s, n))
+    for field in FIELDS:
+        assert rel(out[field], ora[field]) < TOL[field], (field, rel(out[field], ora[field]))
+
+
+def test_gpu_root_run_replays_bit_for_bit():
+    first = gpu_fields((1, 1, 1), 1)
+    again = gpu_fields((1, 1, 1), 1, ledger=first['ledger'])
+    for field in FIELDS:
+        np.testing.assert_array_equal(again[field], first[field], err_msg=field)
+    fresh = gpu_fields((1, 1, 1), 1)
+    for field in FIELDS:
+        np.testing.assert_array_equal(fresh[field], first[field], err_msg=field)
+
+
+def _projection_case(seed=3, shape=(96, 96, 24)):
+    rng = np.random.default_rng(seed)
+    lo, hi = 0.0, 0.03
+    mean = np.linspace(0.018, 0.0004, shape[2]).astype(np.float32)
+    amp = np.linspace(0.05, 1.2, shape[2]).astype(np.float32) * (hi - lo) * 0.25
+    pert = (rng.standard_normal(shape, dtype=np.float32) * amp).astype(np.float32)
+    pert[:, :, :4] = 0.0     # four levels that cannot violate
+    return pert, mean, lo, hi
+
+
+@pytest.mark.parametrize("window", [None, (5, 90, 7, 80)])
+def test_projection_matches_oracle(window):
+    from steam_amd import cascade as C
+    pert, mean, lo, hi = _projection_case()
+    host = pert.copy()
+    mu_ref = OC.project_onto_bounds(host, mean, lo, hi, window)
+    dev = torch.from_numpy(pert).cuda()
+    led = C.project_onto_bounds(dev, mean, lo, hi, window)
+    got = dev.cpu().numpy()
+    np.testing.assert_array_equal(np.isnan(led.mu), np.isnan(mu_ref))
+    ok = ~np.isnan(mu_ref)
+    np.testing.assert_allclose(led.mu[ok], mu_ref[ok], rtol=0, atol=1e-12 * (hi - lo))
+    ulp = float(np.spacing(np.float32(hi)))
+    assert np.abs(got - host).max() <= 4 * ulp
+    np.testing.assert_array_equal(got[:, :, :4], pert[:, :, :4])
+    # apply-only replay reproduces the solve bit for bit
+    dev2 = torch.from_numpy(pert).cuda()
+    C.project_onto_bounds(dev2, mean, lo, hi, window, ledger=led)
+    np.testing.assert_array_equal(dev2.cpu().numpy(), got)
+
+
+def test_compose_and_deficit_kernels_match_oracle():
+    from steam_amd import cascade as C
+    rng = np.random.default_rng(11)
+    shape = (20, 18, 9)
+    state = rng.uniform(-0.012, 0.012, shape).astype(np.float32)
+    deficit = rng.uniform(-0.006, 0.006, shape).astype(np.float32)
+    mean = np.linspace(0.018, 0.0004, 9).astype(np.float32)
+    ref, mu = OC.compose_output(state, deficit, mean, 0.0, 0.03)
+    got, led = C.compose_output(torch.from_numpy(state).cuda(),
+                                torch.from_numpy(deficit).cuda(), mean, 0.0, 0.03)
+    ulp = float(np.spacing(np.float32(0.03)))
+    assert np.abs(got.cpu().numpy() - ref).max() <= 4 * ulp
+    np.testing.assert_array_equal(np.isnan(led.mu), np.isnan(mu))
+    plain, none = C.compose_output(state, None, mean, 0.0, 0.03)
+    assert none is None
+    np.testing.assert_array_equal(plain, np.clip(state + mean, np.float32(0), np.float32(0.03)))
+    # flux composition: clip at zero and restore the volume mean
+    flux = rng.uniform(0.0, 2.0, shape).astype(np.float32)
+    fd = rng.uniform(-1.0, 1.0, shape).astype(np.float32)
+    for window in (None, (2, 17, 3, 15)):
+        fref, sums = OC.compose_flux_output(flux, fd, window)
+        fgot, fled = C.compose_flux_output(torch.from_numpy(flux).cuda(),
+                                           torch.from_numpy(fd).cuda(), window)
+        assert fled.entering.count == sums[0][1] and fled.realized.count == sums[1][1]
+        assert fled.entering.total == pytest.approx(sums[0][0], rel=1e-12)
+        assert fled.realized.total == pytest.approx(sums[1][0], rel=1e-12)
+        np.testing.assert_allclose(fgot.cpu().numpy(), fref, rtol=2e-7, atol=0)
+        again, _ = C.compose_flux_output(torch.from_numpy(flux).cuda(),
+                                         torch.from_numpy(fd).cuda(), window, ledger=fled)
+        np.testing.assert_array_equal(again.cpu().numpy(), fgot.cpu().numpy())
+    # deficit updates, bit for bit against float32 numpy
+    coef = np.linspace(-0.6, 0.0, 9).astype(np.float32)
+    after = rng.standard_normal(shape).astype(np.float32)
+    before = rng.standard_normal(shape).astype(np.float32)
+    d = torch.from_numpy(deficit).cuda()
+    C.deficit_accumulate(d, torch.from_numpy(after).cuda(), coef,
+                         before=torch.from_numpy(before).cuda())
+    np.testing.assert_array_equal(d.cpu().numpy(), deficit + coef * (after - before))
+    d = torch.from_numpy(deficit).cuda()
+    C.deficit_accumulate(d, torch.from_numpy(after).cuda(), coef)
+    np.testing.assert_array_equal(d.cpu().numpy(), deficit + coef * after)
+
+
+def test_realized_norm_unit_level_mean_with_structure_zero_without(monkeypatch):
+    """The reference's test (tests/test_bounds.py:211-253) through the seams:
+    with C = 1, S = 1 and g = 1 the convolved pattern is the normalised
+    product itself, mean |W| = 1 per level; a flat profile gives W = 0."""
+    from steam_amd import simulate as sm
+    k_values = np.array([8000.0, 4000.0, 2000.0])
+    z_profile = np.arange(17, dtype=np.float64) * 200.0
+    grids = sm._compute_all_grids(k_values, 16000.0, 16000.0, 3200.0, (1, 1, 1),
+                                  np.full_like(z_profile, 100.0), z_profile)
+    h_profile = 330_000.0 + 100.0 * z_profile
+    qt_profile = np.zeros_like(z_profile)
+    captured = []
+
+    def fake_advance(flux, noise_region, kernel, flux_noise_scale, n_scale_classes_per_dyad,
+                     sparsity_factors, n_zero, zero_bottom, zero_top, device="cuda",
+                     window=None, ledger=None):
+        return np.ones_like(flux), ledger, {"n_clipped": 0}
+
+    def capture(field, kernel, device="cuda"):
+        captured.append(field.copy())
+        return np.zeros_like(field)
+
+    monkeypatch.setattr(sm, "_advance_flux", fake_advance)
+    monkeypatch.setattr(sm, "CONVOLVE", capture)
+    ones = [np.ones(int(nz), dtype=np.float32) for nz in grids["nz"]]
+    tiny = [np.full(int(nz), 1e-30, dtype=np.float32) for nz in grids["nz"]]
+    sm.cascade_loop(h_profile, qt_profile, z_profile, grids, ones, ones, tiny, tiny, ones,
+                    0.0, 1e9, -1.0, 1.0, 0, (1, 1, 1),
+                    np.random.SeedSequence(4).spawn(len(grids["k"])))
+    np.testing.assert_allclose(np.abs(captured[0]).mean(axis=(0, 1)), 1.0, rtol=1e-5)
+    np.testing.assert_array_equal(captured[1], 0.0)
+
+
+def test_cascade_rejects_what_the_reference_rejects():
+    from steam_amd import simulate as sm
+    st = fixture_setup((1, 1, 1), 1)
+    hmin, hmax, qmin, qmax = st['bounds']
+    args = (st['h_profile'], st['qt_profile'], st['z_profile'], st['grids'], st['C_h_k'],
+            st['C_qt_k'], st['b_h_k'], st['b_qt_k'], st['aspect_k'], hmin, hmax, qmin, qmax, 1,
+            (1, 1, 1))
+    with pytest.raises(TypeError):
+        sm.cascade_loop(*args, np.random.default_rng(0))
+    with pytest.raises(ValueError):
+        sm.cascade_loop(*args, st['class_seeds'][:-1])
+    with pytest.raises(ValueError):
+        sm.cascade_loop(*args, st['class_seeds'], n_scale_classes_per_dyad=2)
+    from steam_amd.ledger import RunLedger
+    with pytest.raises(ValueError):
+        sm.cascade_loop(*args, st['class_seeds'], ledger=RunLedger(2))

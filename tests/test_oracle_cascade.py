@@ -1,0 +1,235 @@
+"""The cascade's parameter set-up (steam_amd.params, host code) against the
+reference's closed forms, and the oracle's whole cascade + composition
+against the reference's committed fixtures.  No GPU needed."""
+
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from oracle import cascade as OC
+from oracle import scalars as OS
+from steam_amd import params as P
+
+Z = np.arange(28) * 30.0
+H_PROFILE = 340e3 - 20e3 * (Z / Z.max())
+QT_PROFILE = 0.018 - 0.016 * (Z / Z.max())
+# the reference fixtures' configuration (tests/heavy/make_refactor_references.py:50-83)
+CASES = [("root_dyadic", (1, 1, 1), 1), ("root_s2", (2, 2, 2), 1),
+         ("root_half", (1, 1, 1), 2), ("root_cuda", (1, 1, 1), 1)]
+
+
+def fixture_setup(s, n):
+    return P.root_setup(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0, 800.0,
+                        30.0, sparsity_factors=s, n_scale_classes_per_dyad=n, seed=42)
+
+
+def run_oracle(st, comp_k=None):
+    hmin, hmax, qmin, qmax = st['bounds']
+    h, qt, fl, dh, dqt, dfl, fin, rec = OC.cascade_loop(
+        st['h_profile'], st['qt_profile'], st['z_profile'], st['grids'], st['C_h_k'],
+        st['C_qt_k'], st['b_h_k'], st['b_qt_k'], st['aspect_k'], hmin, hmax, qmin, qmax,
+        st['min_distance_to_ground'], st['sparsity_factors'], st['class_seeds'],
+        n_scale_classes_per_dyad=st['n_scale_classes_per_dyad'],
+        comp_k=st['comp_k'] if comp_k is None else comp_k)
+    zf = fin['z'].astype(np.float32)
+    hm = np.interp(zf, st['z_profile'], st['h_profile']).astype(np.float32)
+    qm = np.interp(zf, st['z_profile'], st['qt_profile']).astype(np.float32)
+    H, _ = OC.compose_output(h, dh, hm, hmin, hmax)
+    Q, _ = OC.compose_output(qt, dqt, qm, qmin, qmax)
+    F, _ = OC.compose_flux_output(fl, dfl)
+    return {'h_perturbation': h, 'qt_perturbation': qt, 'flux_state': fl, 'h': H,
+            'qt': Q, 'flux': F}
+
+
+def rel(a, b):
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b)
+
+
+# Tolerances against the reference's float32 fixtures: the states to 5e-5
+# rel-L2 (the reference's own host and CUDA fixtures differ by 7.9e-6 on
+# h_perturbation), the written fields and flux state to 2e-6.
+TOL = {'h_perturbation': 5e-5, 'qt_perturbation': 5e-5, 'flux_state': 2e-6,
+       'h': 2e-6, 'qt': 2e-6, 'flux': 2e-6}
+
+
+@pytest.mark.parametrize("name,s,n", CASES)
+def test_oracle_cascade_matches_reference_fixtures(name, s, n):
+    out = run_oracle(fixture_setup(s, n))
+    ref = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    for field, tol in TOL.items():
+        assert out[field].shape == ref[field].shape, field
+        assert rel(out[field], ref[field]) < tol, (field, rel(out[field], ref[field]))
+
+
+def test_fixture_pin_detects_planted_mistakes(monkeypatch):
+    """Each plausible slip moves a pinned field far outside its tolerance."""
+    st = fixture_setup((1, 1, 1), 1)
+    ref = np.load(os.path.join(GOLDEN, "root_dyadic.npz"))
+    # interpolation compensation dropped: written h and flux move
+    out = run_oracle(st, comp_k=[np.ones_like(c) for c in st['comp_k']])
+    assert rel(out['h'], ref['h']) > 100 * TOL['h']
+    assert rel(out['flux'], ref['flux']) > 100 * TOL['flux']
+    # aspect ratio taken as 1: the advective weight's z term is wrong
+    bad = dict(st, aspect_k=[np.ones_like(a) for a in st['aspect_k']])
+    assert rel(run_oracle(bad)['h_perturbation'], ref['h_perturbation']) > 100 * 5e-5
+    # bound taper dropped
+    monkeypatch.setattr(OS, "bound_taper",
+                        lambda r, b, lo, hi: np.ones_like(np.asarray(r, np.float32)))
+    assert rel(run_oracle(st)['h_perturbation'], ref['h_perturbation']) > 50 * 5e-5
+
+
+# ---------------------------------------------------------------------------
+# parameter set-up: the reference's closed forms (tests/test_normalization.py,
+# tests/test_bounds.py)
+# ---------------------------------------------------------------------------
+
+def _norm(profile, k_z_L, z, k_values=(1000.0,), outer=1000.0):
+    return P._compute_normalization(profile, z, np.full(z.size, k_z_L), np.asarray(k_values),
+                                    outer, {'z_arrays': [z] * len(k_values)})
+
+
+def test_normalization_linear_profile_is_gradient_times_half_window():
+    z = np.arange(2001) * 10.0
+    C = _norm(100.0 + 2.0 * z, 400.0, z)
+    assert float(C[0][1000]) == pytest.approx(P.HAAR_TO_MHAT * 2.0 * 400.0 / 2, rel=1e-6)
+
+
+def test_normalization_constant_profile_is_zero():
+    z = np.arange(501) * 10.0
+    assert np.all(_norm(np.full(501, 5.0), 400.0, z)[0] == 0.0)
+
+
+def test_normalization_resolution_independent_and_hurst_ladder():
+    vals = []
+    for n, dz in [(2001, 10.0), (4001, 5.0)]:
+        z = np.arange(n) * dz
+        vals.append(float(_norm(100.0 + 2.0 * z, 400.0, z)[0][n // 2]))
+    assert vals[1] == pytest.approx(vals[0], rel=1e-6)
+    z = np.arange(2001) * 10.0
+    C = _norm(100.0 + 2.0 * z, 400.0, z, k_values=(1000.0, 500.0))
+    assert float(C[1][1000] / C[0][1000]) == pytest.approx(0.5 ** P.H_h, rel=1e-5)
+
+
+def test_bound_buffers_close_the_remaining_ladder():
+    C = [np.full(4, c, dtype=np.float32) for c in (4.0, 2.0, 1.0)]
+    b = P._bound_buffers(C, 1)
+    tail = 1.0 / (1.0 - 2.0 ** -P.H_h)
+    for i, c in enumerate((4.0, 2.0, 1.0)):
+        np.testing.assert_allclose(b[i], P.BOUND_BUFFER_MULTIPLE * tail * c, rtol=1e-6)
+    assert P._bound_buffers(C, 2)[0][0] > b[0][0]
+
+
+def test_compensation_table_and_profiles():
+    # the canonical table: D_REF over the retention of the hops below k/dx
+    r = P.HOP_RETENTION['canonical']
+    d_ref = np.prod([r[y] for y in sorted(r)])
+    assert P.INTERPOLATION_COMPENSATION[2.0] == pytest.approx(d_ref)
+    assert P.INTERPOLATION_COMPENSATION[16.0] == pytest.approx(d_ref / (r[2] * r[4] * r[8]))
+    assert P.INTERPOLATION_COMPENSATION[512.0] == 1.0
+    k = np.array([8000.0, 4000.0, 2000.0, 1000.0])
+    zp = np.arange(10, dtype=np.float64) * 100.0
+    zs = [zp.copy() for _ in k]
+    prof = P._compensation_profiles(k, zs, np.full_like(zp, 100.0), zp,
+                                    'piecewise_isotropic_below_spheroscale')
+    for i, ki in enumerate(k):
+        np.testing.assert_allclose(prof[i], P._interpolation_compensation(2.0 * ki / k[-1]),
+                                   rtol=1e-6)
+    iso = P._compensation_profiles(k, zs, np.full_like(zp, 1e6), zp,
+                                   'piecewise_isotropic_below_spheroscale')
+    ri = P.HOP_RETENTION['isotropic']
+    np.testing.assert_allclose(iso[0], d_ref / (ri[2] * ri[4] * ri[8]), rtol=1e-6)
+
+
+def test_root_setup_rejects_what_the_reference_rejects():
+    with pytest.raises(ValueError):
+        P.root_setup(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0, 800.0, 30.0,
+                     h_min=400e3)
+    with pytest.raises(ValueError):
+        P.root_setup(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 200.0, 100.0, 800.0, 30.0)
+    with pytest.raises(ValueError):
+        P.root_setup(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0, 900.0, 30.0)
+    with pytest.raises(ValueError):
+        P.root_setup(H_PROFILE, QT_PROFILE, 15, 16, 125.0, 125.0, 1000.0, 100.0, 800.0, 30.0)
+
+
+# ---------------------------------------------------------------------------
+# projection and composition
+# ---------------------------------------------------------------------------
+
+def _projection_case(seed=3, shape=(24, 20, 8)):
+    rng = np.random.default_rng(seed)
+    lo, hi = 0.0, 0.03
+    mean = np.linspace(0.018, 0.0004, shape[2]).astype(np.float32)
+    amp = np.linspace(0.05, 1.2, shape[2]).astype(np.float32) * (hi - lo) * 0.25
+    pert = (rng.standard_normal(shape, dtype=np.float32) * amp).astype(np.float32)
+    pert[:, :, 0] = 0.0   # a level that cannot violate
+    return pert, mean, lo, hi
+
+
+@pytest.mark.parametrize("window", [None, (2, 20, 3, 17)])
+def test_projection_properties(window):
+    pert, mean, lo, hi = _projection_case()
+    e0 = pert.astype(np.float64) + mean
+    inner0 = e0 if window is None else e0[window[0]:window[1], window[2]:window[3]]
+    before = inner0.mean(axis=(0, 1))
+    f = pert.copy()
+    mu = OC.project_onto_bounds(f, mean, lo, hi, window)
+    assert np.isnan(mu[0]) and np.array_equal(f[:, :, 0], pert[:, :, 0])
+    assert (~np.isnan(mu[1:])).sum() >= 4
+    e = f.astype(np.float64) + mean
+    ulp = float(np.spacing(np.float32(hi)))
+    assert e.min() >= lo - ulp and e.max() <= hi + ulp
+    inner = e if window is None else e[window[0]:window[1], window[2]:window[3]]
+    # the mean is kept wherever it lies inside the bounds (elsewhere no
+    # projection can keep it)
+    feasible = (before > lo) & (before < hi)
+    assert feasible.sum() >= 6
+    drift = np.abs(inner.mean(axis=(0, 1)) - before)[feasible] / (hi - lo)
+    assert drift.max() < 1e-7
+    # replay reproduces the solve bit for bit
+    g = pert.copy()
+    OC.project_onto_bounds(g, mean, lo, hi, window, mu=mu)
+    np.testing.assert_array_equal(f, g)
+
+
+def test_projection_mu_is_the_exact_root_on_a_tiny_level():
+    """The level mean of clip(e - mu, lo, hi) is piecewise linear in mu; its
+    root is found exactly by walking the breakpoints."""
+    e = np.array([[-0.4, 0.2, 0.9], [1.7, 0.1, -0.2], [0.3, 2.5, 0.6]], dtype=np.float32)
+    lo, hi = 0.0, 1.0
+    f = e[:, :, None].copy()
+    mu = OC.project_onto_bounds(f, np.zeros(1, np.float32), lo, hi)[0]
+    ev = e.astype(np.float64).ravel()
+    target = ev.mean()
+
+    def g(m):
+        return np.clip(ev - m, lo, hi).mean() - target
+
+    knots = np.sort(np.concatenate([ev - lo, ev - hi]))
+    for a, b in zip(knots[:-1], knots[1:]):
+        if g(a) >= 0 >= g(b) and g(a) != g(b):
+            exact = a + (b - a) * g(a) / (g(a) - g(b))
+            break
+    assert mu == pytest.approx(exact, abs=1e-12)
+
+
+def test_compose_closed_forms():
+    rng = np.random.default_rng(5)
+    state = rng.uniform(-0.2, 0.2, (6, 5, 4)).astype(np.float32)
+    mean = np.array([0.1, 0.2, 0.3, 0.4], np.float32)
+    out, mu = OC.compose_output(state, None, mean, -1.0, 1.0)
+    np.testing.assert_array_equal(out, np.clip(state + mean, -1, 1))
+    assert mu is None
+    deficit = (0.5 * state).astype(np.float32)
+    out, mu = OC.compose_output(state, deficit, mean, -1.0, 1.0)
+    assert np.all(np.isnan(mu))
+    np.testing.assert_array_equal(out, (deficit + state) + mean)
+    flux = rng.uniform(0.0, 2.0, (6, 5, 4)).astype(np.float32)
+    fd = rng.uniform(-1.0, 1.0, (6, 5, 4)).astype(np.float32)
+    fo, (ent, real) = OC.compose_flux_output(flux, fd)
+    assert fo.min() >= 0
+    assert fo.astype(np.float64).mean() == pytest.approx(flux.astype(np.float64).mean(),
+                                                         rel=1e-6)
