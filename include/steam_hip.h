@@ -325,14 +325,18 @@ int steam_pattern_normalize(float* w_dev, const float* level_mean_dev,
 #define STEAM_LV_E_SUM 7   /* sum e */
 #define STEAM_LV_E_MIN 8   /* min e over the whole level */
 #define STEAM_LV_E_MAX 9   /* max e over the whole level */
+#define STEAM_LV_E_CLAMP 10 /* sum clamp(e - mu[z], lo, hi) in float64 */
 
 /* Device scratch bytes steam_level_reduce needs for this field shape. */
 int64_t steam_level_scratch_bytes(int64_t nx, int64_t ny, int64_t nz);
 
 /* Per-level reduction over window {x0, x1, y0, y1} (NULL = whole level) into
- * out_dev[nz] (float64, device), in a fixed order: the level sums of the
- * product norm (PatternLedger, steam/ledger.py:83-108) and of the bounded
- * add. */
+ * out_dev[nz] (float64, device), in a fixed order (runs of consecutive window
+ * lines, then the runs in index order; the order depends on the window's
+ * size only): the level sums of the product norm (PatternLedger,
+ * steam/ledger.py:83-108), of the bounded add and of the projection.  The
+ * bounded add's and the projection's bisections evaluate exactly these sums
+ * at each of their midpoints. */
 int steam_level_reduce(const float* f_dev, const float* mean_z_dev,
                        const double* mu_dev, int64_t nx, int64_t ny, int64_t nz,
                        const int64_t* window, int op, double lo, double hi,

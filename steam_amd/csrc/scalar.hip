@@ -103,6 +103,7 @@ __global__ __launch_bounds__(256) void pattern_normalize_kernel(
   }
 }
 
+
 // ---------------------------------------------------------------------------
 // per-level window reductions
 // ---------------------------------------------------------------------------
@@ -121,19 +122,31 @@ enum LevelOp {
   LV_E_CLAMP = 10,   // sum clamp(e - mu[z], lo, hi) on levels with active[z]
 };
 
+// The window's lines (x, y) are split into G contiguous runs of
+// lines_per_group, G a function of the window alone; workgroup g reduces run
+// g into one partial per (candidate, level), and one combine pass joins the
+// G partials of each (candidate, level) in a fixed order.  A level's result therefore depends
+// only on the field, the window and the shift, never on how many candidate
+// shifts share the launch.
+constexpr int RD_MAX_GROUPS = 512;
+constexpr int RD_MIN_LINES = 32;
+// candidate shifts of one bisection pass: the 15 midpoints of a 4-deep
+// bisection tree, so one pass over the field takes 4 bisection steps
+constexpr int KCAND = 15;
+constexpr int KSTEPS = 4;
+
 struct LevelArgs {
   const float* f;
   const float* mean_z;
-  const double* mu;
+  const double* mu;      // [ncand][nz] shifts (clamp ops)
   const double* active;  // LV_E_CLAMP: levels to reduce (others give 0)
+  const int* stop;       // when set and nonzero the launch does nothing
   int nx, ny, nz;
   int win[4];
   int op;
+  int ncand;
   double lo, hi;
 };
-
-constexpr int LV_LINES = 32;   // lines per first-stage workgroup
-constexpr int LV_ROWS = 64;    // partial rows per combine workgroup
 
 __device__ __forceinline__ bool lv_is_min(int op) { return op == LV_MIN || op == LV_E_MIN; }
 __device__ __forceinline__ bool lv_is_max(int op) { return op == LV_MAX || op == LV_E_MAX; }
@@ -144,98 +157,226 @@ __device__ __forceinline__ double lv_join(int op, double a, double b) {
   return lv_is_min(op) ? fmin(a, b) : (lv_is_max(op) ? fmax(a, b) : a + b);
 }
 
-// first stage: workgroup w covers lines [w*LV_LINES, ...); thread t handles
-// levels t, t+256, ...; partial[w][z]
-__global__ __launch_bounds__(256) void level_partial_kernel(LevelArgs a, double* part) {
-  const int64_t nlines = (int64_t)a.nx * a.ny;
-  const int64_t l0 = (int64_t)blockIdx.x * LV_LINES;
-  const bool whole = lv_is_min(a.op) || lv_is_max(a.op);
+// Workgroup g reduces window lines [g*lpg, (g+1)*lpg).  Levels are taken in
+// chunks of up to 256; within a chunk of `sub` levels, lane t owns level
+// zc + t % sub and walks every (256/sub)-th line of the run, so a wave reads
+// consecutive addresses of consecutive lines.  Lanes sharing a level are then
+// summed through LDS in lane order.  OP is the LevelOp; NC > 1 only for the
+// two clamp ops (candidate shifts).
+template <int NC, int OP>
+__global__ __launch_bounds__(256) void level_partial_kernel(LevelArgs a, double* __restrict__ part,
+                                                            int64_t lpg) {
+  constexpr bool kMin = OP == LV_MIN || OP == LV_E_MIN;
+  constexpr bool kMax = OP == LV_MAX || OP == LV_E_MAX;
+  constexpr double kInit = kMin ? INFINITY : (kMax ? -INFINITY : 0.0);
+  if (a.stop && *a.stop) return;
+  __shared__ double red[NC][256];
+  const int t = threadIdx.x;
+  const int wx0 = a.win[0], wy0 = a.win[2];
+  const int wy = a.win[3] - a.win[2];
+  const int64_t wlines = (int64_t)(a.win[1] - a.win[0]) * wy;
+  const int64_t l0 = (int64_t)blockIdx.x * lpg;
+  const int64_t l1 = l0 + lpg < wlines ? l0 + lpg : wlines;
+  const int nc = NC == 1 ? 1 : a.ncand;
   const float flo = (float)a.lo, fhi = (float)a.hi;
-  for (int z = threadIdx.x; z < a.nz; z += blockDim.x) {
-    double acc = lv_init(a.op);
-    const float mz = a.mean_z ? a.mean_z[z] : 0.f;
-    const float muz = a.mu ? (float)a.mu[z] : 0.f;
-    const double mu64 = a.mu ? a.mu[z] : 0.0;
-    if (a.op == LV_E_CLAMP && a.active[z] == 0.0) {
-      part[(int64_t)blockIdx.x * a.nz + z] = 0.0;
-      continue;
-    }
-    for (int j = 0; j < LV_LINES; ++j) {
-      const int64_t line = l0 + j;
-      if (line >= nlines) break;
-      const int x = (int)(line / a.ny), y = (int)(line - (int64_t)x * a.ny);
-      if (!whole && (x < a.win[0] || x >= a.win[1] || y < a.win[2] || y >= a.win[3]))
-        continue;
-      const float v = a.f[line * a.nz + z];
-      double t;
-      switch (a.op) {
-        case LV_SUM: t = (double)v; break;
-        case LV_ABS: t = (double)fabsf(v); break;
-        case LV_NONZERO: t = v != 0.f ? 1.0 : 0.0; break;
-        case LV_CLAMP: t = (double)fminf(fmaxf(__fsub_rn(v, muz), flo), fhi); break;
-        case LV_E_SUM:
-        case LV_E_MIN:
-        case LV_E_MAX: t = (double)v + (double)mz; break;
-        case LV_E_CLAMP: t = fmin(fmax(((double)v + (double)mz) - mu64, a.lo), a.hi); break;
-        default: t = (double)__fadd_rn(v, mz); break;
+  for (int zc = 0; zc < a.nz; zc += 256) {
+    const int sub = a.nz - zc < 256 ? a.nz - zc : 256;
+    const int lpi = 256 / sub;
+    const bool on = t < lpi * sub;
+    const int z = zc + (on ? t % sub : 0);
+    const int first = on ? t / sub : 0;
+    double acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = kInit;
+    bool skip = false;
+    if constexpr (OP == LV_E_CLAMP) skip = a.active && a.active[z] == 0.0;
+    if (on && !skip && l0 + first < l1) {
+      const float mz = a.mean_z ? a.mean_z[z] : 0.f;
+      float muf[NC];
+      double mu64[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        mu64[c] = (a.mu && c < nc) ? a.mu[(int64_t)c * a.nz + z] : 0.0;
+        muf[c] = (float)mu64[c];
       }
-      acc = lv_join(a.op, acc, t);
+      int64_t wl = l0 + first;
+      int x = wx0 + (int)(wl / wy);
+      int y = wy0 + (int)(wl - (int64_t)(x - wx0) * wy);
+      for (; wl < l1; wl += lpi) {
+        const float v = a.f[((int64_t)x * a.ny + y) * a.nz + z];
+        if constexpr (OP == LV_SUM) acc[0] += (double)v;
+        if constexpr (OP == LV_ABS) acc[0] += (double)fabsf(v);
+        if constexpr (OP == LV_NONZERO) acc[0] += v != 0.f ? 1.0 : 0.0;
+        if constexpr (OP == LV_PHI) acc[0] += (double)__fadd_rn(v, mz);
+        if constexpr (OP == LV_MIN) acc[0] = fmin(acc[0], (double)__fadd_rn(v, mz));
+        if constexpr (OP == LV_MAX) acc[0] = fmax(acc[0], (double)__fadd_rn(v, mz));
+        if constexpr (OP == LV_E_SUM) acc[0] += (double)v + (double)mz;
+        if constexpr (OP == LV_E_MIN) acc[0] = fmin(acc[0], (double)v + (double)mz);
+        if constexpr (OP == LV_E_MAX) acc[0] = fmax(acc[0], (double)v + (double)mz);
+        if constexpr (OP == LV_CLAMP) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            if (c < nc) acc[c] += (double)fminf(fmaxf(__fsub_rn(v, muf[c]), flo), fhi);
+        }
+        if constexpr (OP == LV_E_CLAMP) {
+          const double e = (double)v + (double)mz;
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            if (c < nc) acc[c] += fmin(fmax(e - mu64[c], a.lo), a.hi);
+        }
+        y += lpi;
+        if (y >= wy0 + wy) {
+          const int k = (y - wy0) / wy;
+          x += k;
+          y -= k * wy;
+        }
+      }
     }
-    part[(int64_t)blockIdx.x * a.nz + z] = acc;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) red[c][t] = acc[c];
+    __syncthreads();
+    // one thread per (candidate, level) joins that level's lpi lanes in lane order
+    for (int p = t; p < nc * sub; p += 256) {
+      const int c = p / sub, zl = p - c * sub;
+      double s = red[c][zl];
+      for (int j = 1; j < lpi; ++j) {
+        const double o = red[c][zl + j * sub];
+        s = kMin ? fmin(s, o) : (kMax ? fmax(s, o) : s + o);
+      }
+      part[((int64_t)c * a.nz + zc + zl) * gridDim.x + blockIdx.x] = s;
+    }
+    __syncthreads();
   }
 }
 
-// combine stage: rows [r0, r0 + LV_ROWS) of a partial table into one row;
-// block (64 levels x 4 row groups), fixed order
-__global__ __launch_bounds__(256) void level_combine_kernel(const double* in, int64_t rows,
-                                                            int nz, int op, double* out) {
-  __shared__ double red[4][64];
-  const int zl = threadIdx.x & 63, grp = threadIdx.x >> 6;
-  const int z = blockIdx.x * 64 + zl;
-  const int64_t r0 = (int64_t)blockIdx.y * LV_ROWS;
+// out[c][z] = (the G partials of (c, z), part[(c*nz + z)*G + g]) joined in
+// a fixed order, divided by div when div > 0: one wave per (c, z), lane l
+// takes runs l, l+64, ... in order, then the lanes are joined by a fixed
+// butterfly
+__global__ __launch_bounds__(256) void level_combine_kernel(const double* __restrict__ part,
+                                                            int groups, int nc, int nz, int op,
+                                                            double div, double* __restrict__ out,
+                                                            const int* stop) {
+  if (stop && *stop) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t pair = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (pair >= (int64_t)nc * nz) return;
+  const double* row = part + pair * groups;
   double acc = lv_init(op);
-  if (z < nz) {
-    for (int j = grp; j < LV_ROWS; j += 4) {
-      const int64_t r = r0 + j;
-      if (r >= rows) break;
-      acc = lv_join(op, acc, in[r * nz + z]);
+#pragma unroll 8
+  for (int g = lane; g < groups; g += 64) acc = lv_join(op, acc, row[g]);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) acc = lv_join(op, acc, __shfl_xor(acc, off, 64));
+  if (lane == 0) out[pair] = div > 0.0 ? acc / div : acc;
+}
+
+// ---------------------------------------------------------------------------
+// bisection on the level means, KSTEPS steps per pass over the field
+//
+// A bisection step halves the bracket [lo, hi] at mid = 0.5 * (lo + hi),
+// moving lo up to mid when the level mean at mid is "positive" and hi down
+// otherwise.  The KCAND = 15 candidates of a pass are the midpoints of the
+// complete 4-deep tree of brackets below the current one (heap order), each
+// computed by that same formula; after the pass the 4 steps are taken along
+// the tree with the measured sums.  Every step therefore sees exactly the
+// midpoint and the level sum a one-step-per-pass bisection would, and 60
+// steps end on the same bracket.
+// ---------------------------------------------------------------------------
+struct BisectVec {
+  double* lo;
+  double* hi;
+  double* mu;      // 0.5 * (lo + hi), the result after the last pass
+  double* cand;    // [KCAND][nz] the next pass's candidates
+  const double* target;
+  const double* sums;  // [KCAND][nz] level sums at the candidates
+};
+
+__device__ __forceinline__ void bisect_candidates(BisectVec v, int nz, int z, double lo, double hi) {
+  double a[KCAND + 1], b[KCAND + 1];
+  a[1] = lo;
+  b[1] = hi;
+#pragma unroll
+  for (int n = 1; n <= KCAND; ++n) {
+    const double m = 0.5 * (a[n] + b[n]);
+    v.cand[(int64_t)(n - 1) * nz + z] = m;
+    if (2 * n + 1 <= KCAND) {
+      a[2 * n] = a[n];
+      b[2 * n] = m;
+      a[2 * n + 1] = m;
+      b[2 * n + 1] = b[n];
     }
   }
-  red[grp][zl] = acc;
-  __syncthreads();
-  if (grp == 0 && z < nz) {
-    double s = red[0][zl];
-    for (int g = 1; g < 4; ++g) s = lv_join(op, s, red[g][zl]);
-    out[(int64_t)blockIdx.y * nz + z] = s;
+}
+
+__global__ void bisect_init_kernel(BisectVec v, int nz, double width) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z < nz) {
+    v.lo[z] = -width;
+    v.hi[z] = width;
+    v.mu[z] = 0.5 * (v.lo[z] + v.hi[z]);
+    bisect_candidates(v, nz, z, -width, width);
   }
+}
+
+// mode 0: positive = sum / npts - target > 0 (the bounded add);
+// mode 1: positive = sum / npts > target (the projection)
+__global__ void bisect_pass_kernel(BisectVec v, int nz, double npts, int mode) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= nz) return;
+  double lo = v.lo[z], hi = v.hi[z];
+  const double target = v.target[z];
+  int n = 1;
+  for (int s = 0; s < KSTEPS; ++s) {
+    const double mid = 0.5 * (lo + hi);
+    const double mean = v.sums[(int64_t)(n - 1) * nz + z] / npts;
+    const bool positive = mode == 0 ? (mean - target > 0.0) : (mean > target);
+    if (positive) {
+      lo = mid;
+      n = 2 * n + 1;
+    } else {
+      hi = mid;
+      n = 2 * n;
+    }
+  }
+  v.lo[z] = lo;
+  v.hi[z] = hi;
+  v.mu[z] = 0.5 * (lo + hi);
+  bisect_candidates(v, nz, z, lo, hi);
 }
 
 // ---------------------------------------------------------------------------
 // bounded amplitude-preserving add, device pieces (the algorithm of
-// steam/simulate.py:2985-3111, the reference's GPU path)
+// steam/simulate.py:2985-3111, the reference's GPU path).  The rescale loop
+// runs without host round trips: a device flag `stop` is raised by the step
+// whose levels are all done, and every later launch of the loop returns at
+// once, so the loop's launches are the same whatever the data.
 // ---------------------------------------------------------------------------
-// d = clamp(d - f32(shift[z]) + phi, lo, hi) - phi  (shift may be NULL)
+// d = clamp(d - f32(shift[z*stride]) + phi, lo, hi) - phi  (shift may be NULL)
 __global__ __launch_bounds__(256) void ba_shift_clip_kernel(
     float* __restrict__ d, const float* __restrict__ pert, const float* __restrict__ mean_z,
-    const double* __restrict__ shift, int64_t n, int nz, float lo, float hi) {
+    const double* __restrict__ shift, int stride, int64_t n, int nz, float lo, float hi,
+    const int* stop) {
+  if (stop && *stop) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int z = (int)(i % nz);
     const float phi = __fadd_rn(pert[i], mean_z[z]);
     float v = d[i];
-    if (shift) v = __fsub_rn(v, (float)shift[z]);
+    if (shift) v = __fsub_rn(v, (float)shift[(int64_t)z * stride]);
     const float e = fminf(fmaxf(__fadd_rn(v, phi), lo), hi);
     d[i] = __fsub_rn(e, phi);
   }
 }
 
-// d *= f32(scale[z])
+// d *= f32(scale[z*stride])
 __global__ __launch_bounds__(256) void ba_scale_kernel(float* __restrict__ d,
                                                        const double* __restrict__ scale,
-                                                       int64_t n, int nz) {
+                                                       int stride, int64_t n, int nz,
+                                                       const int* stop) {
+  if (stop && *stop) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    d[i] = __fmul_rn(d[i], (float)scale[i % nz]);
+    d[i] = __fmul_rn(d[i], (float)scale[(int64_t)(i % nz) * stride]);
 }
 
 // e = d + phi (in place on d)
@@ -252,9 +393,8 @@ __global__ __launch_bounds__(256) void ba_to_field_kernel(float* __restrict__ d,
 
 // final write: v = clamp(e - f32(mu[z]), lo, hi) - mean[z]; with record,
 // inc = v - pert; pert = v
-__global__ __launch_bounds__(256) void ba_finish_kernel(const float* __restrict__ e,
-                                                        float* __restrict__ pert,
-                                                        float* __restrict__ inc,
+__global__ __launch_bounds__(256) void ba_finish_kernel(const float* e, float* __restrict__ pert,
+                                                        float* inc,
                                                         const float* __restrict__ mean_z,
                                                         const double* __restrict__ mu,
                                                         int64_t n, int nz, float lo, float hi,
@@ -269,65 +409,76 @@ __global__ __launch_bounds__(256) void ba_finish_kernel(const float* __restrict_
   }
 }
 
-// per-level scalar steps (one thread per level)
+// per-level state of the rescale loop
 struct BaVec {
   double* a0;
-  double* done;       // 1.0 / 0.0
-  double* sum;        // last reduction (per level)
-  double* demean;     // step vectors
-  double* scale;
-  double* mu;
-  double* mu_lo;
-  double* mu_hi;
-  double* mean_phi;
+  double* done;          // 1.0 / 0.0
+  double* mean;          // last level mean (of d or |d|)
+  double* shift;         // this step's demean
+  double* scale;         // this step's applied scale
+  double* hist_demean;   // [nz][n_rescale]
+  double* hist_scale;    // [nz][n_rescale]
+  int* flags;            // [0] stop, [1] n_loop
 };
 
-// demean = done ? 0 : sum / n
-__global__ void ba_demean_kernel(BaVec v, int nz, double npts) {
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  if (z < nz) v.demean[z] = v.done[z] != 0.0 ? 0.0 : v.sum[z] / npts;
-}
-// m_abs = sum / n; scale = min(a0 / max(m_abs, 1e-300), 2); done |= ...;
-// applied = done ? 1 : scale  (written to scale)
-__global__ void ba_rescale_kernel(BaVec v, int nz, double npts) {
+// done = a0 <= 0; history cleared (demean 0, scale 1); stop = 0, n_loop = 0
+__global__ void ba_init_kernel(BaVec v, int nz, int n_rescale) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   if (z < nz) {
-    const double m_abs = v.sum[z] / npts;
+    v.done[z] = v.a0[z] <= 0.0 ? 1.0 : 0.0;
+    for (int s = 0; s < n_rescale; ++s) {
+      v.hist_demean[(int64_t)z * n_rescale + s] = 0.0;
+      v.hist_scale[(int64_t)z * n_rescale + s] = 1.0;
+    }
+  }
+  if (z == 0) {
+    v.flags[0] = 0;
+    v.flags[1] = 0;
+  }
+}
+// demean = done ? 0 : level mean; recorded for this step
+__global__ void ba_demean_kernel(BaVec v, int nz, int step, int n_rescale) {
+  if (v.flags[0]) return;
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z < nz) {
+    const double dm = v.done[z] != 0.0 ? 0.0 : v.mean[z];
+    v.shift[z] = dm;
+    v.hist_demean[(int64_t)z * n_rescale + step] = dm;
+  }
+}
+// m_abs = level mean |d|; scale = min(a0 / max(m_abs, 1e-300), 2);
+// done |= m_abs <= 0 | |scale - 1| < 1e-4; applied = done ? 1 : scale
+__global__ void ba_rescale_kernel(BaVec v, int nz) {
+  if (v.flags[0]) return;
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z < nz) {
+    const double m_abs = v.mean[z];
     const double s = fmin(v.a0[z] / fmax(m_abs, 1e-300), 2.0);
     const bool d = v.done[z] != 0.0 || m_abs <= 0.0 || fabs(s - 1.0) < 1e-4;
     v.done[z] = d ? 1.0 : 0.0;
     v.scale[z] = d ? 1.0 : s;
   }
 }
-__global__ void ba_bisect_init_kernel(BaVec v, int nz, double width) {
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  if (z < nz) {
-    v.mu_lo[z] = -width;
-    v.mu_hi[z] = width;
-    v.mu[z] = 0.5 * (v.mu_lo[z] + v.mu_hi[z]);
+// one workgroup: n_loop = step + 1; when every level is done raise stop
+// (the step's scale is then neither applied nor recorded), else record it
+__global__ __launch_bounds__(256) void ba_check_kernel(BaVec v, int nz, int step, int n_rescale) {
+  if (v.flags[0]) return;
+  __shared__ int all_done;
+  if (threadIdx.x == 0) all_done = 1;
+  __syncthreads();
+  for (int z = threadIdx.x; z < nz; z += blockDim.x)
+    if (v.done[z] == 0.0) all_done = 0;
+  __syncthreads();
+  if (all_done) {
+    if (threadIdx.x == 0) {
+      v.flags[1] = step + 1;
+      v.flags[0] = 1;
+    }
+    return;
   }
-}
-// positive = sum / n - mean_phi > 0; narrow the bracket; next midpoint
-__global__ void ba_bisect_step_kernel(BaVec v, int nz, double npts) {
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  if (z < nz) {
-    const bool positive = v.sum[z] / npts - v.mean_phi[z] > 0.0;
-    if (positive)
-      v.mu_lo[z] = v.mu[z];
-    else
-      v.mu_hi[z] = v.mu[z];
-    v.mu[z] = 0.5 * (v.mu_lo[z] + v.mu_hi[z]);
-  }
-}
-// copy sum / n into dst (a level mean)
-__global__ void ba_mean_kernel(const double* sum, double* dst, int nz, double npts) {
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  if (z < nz) dst[z] = sum[z] / npts;
-}
-// done = a0 <= 0
-__global__ void ba_init_done_kernel(BaVec v, int nz) {
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  if (z < nz) v.done[z] = v.a0[z] <= 0.0 ? 1.0 : 0.0;
+  for (int z = threadIdx.x; z < nz; z += blockDim.x)
+    v.hist_scale[(int64_t)z * n_rescale + step] = v.scale[z];
+  if (threadIdx.x == 0) v.flags[1] = step + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -371,16 +522,11 @@ __global__ __launch_bounds__(256) void compose_kernel(const float* src,
   }
 }
 
-// projection onto the bounds: active[z] = level violates; bracket set up
-__global__ void proj_init_kernel(BaVec v, const double* vmin, const double* vmax, double* active,
-                                 int nz, double lo, double hi, double width) {
+// projection onto the bounds: active[z] = the level has a cell out of bounds
+__global__ void proj_active_kernel(const double* vmin, const double* vmax, double* active, int nz,
+                                   double lo, double hi) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  if (z < nz) {
-    active[z] = (vmin[z] < lo || vmax[z] > hi) ? 1.0 : 0.0;
-    v.mu_lo[z] = -width;
-    v.mu_hi[z] = width;
-    v.mu[z] = 0.5 * (v.mu_lo[z] + v.mu_hi[z]);
-  }
+  if (z < nz) active[z] = (vmin[z] < lo || vmax[z] > hi) ? 1.0 : 0.0;
 }
 // recorded mu: NaN on levels left alone
 __global__ void proj_record_kernel(const double* mu, const double* active, double* out, int nz) {
@@ -388,7 +534,7 @@ __global__ void proj_record_kernel(const double* mu, const double* active, doubl
   if (z < nz) out[z] = active[z] != 0.0 ? mu[z] : __builtin_nan("");
 }
 // replay: active where the recorded mu is a number
-__global__ void proj_active_kernel(const double* mu, double* active, int nz) {
+__global__ void proj_replay_active_kernel(const double* mu, double* active, int nz) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   if (z < nz) active[z] = (mu[z] == mu[z]) ? 1.0 : 0.0;
 }
@@ -441,6 +587,7 @@ __global__ __launch_bounds__(256) void flux_restore_kernel(float* __restrict__ d
     d[i] = fill ? r : __fmul_rn(d[i], r);
 }
 
+
 }  // namespace steam
 
 using namespace steam;
@@ -452,36 +599,68 @@ int grid_1d(int64_t n) {
   return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 1 << 16);
 }
 
-// bytes of reduction scratch for a field of nlines lines and nz levels
-int64_t level_scratch_bytes(int64_t nlines, int nz) {
-  const int64_t rows = (nlines + LV_LINES - 1) / LV_LINES;
-  const int64_t rows2 = (rows + LV_ROWS - 1) / LV_ROWS;
-  return (rows + rows2 + 1) * (int64_t)nz * 8;
+// lines per workgroup run for a window of wlines lines, and the run count
+int64_t rd_lines_per_group(int64_t wlines) {
+  int64_t g = (wlines + RD_MIN_LINES - 1) / RD_MIN_LINES;
+  g = std::min<int64_t>(std::max<int64_t>(g, 1), RD_MAX_GROUPS);
+  return (wlines + g - 1) / g;
+}
+int64_t rd_groups(int64_t wlines) {
+  const int64_t lpg = rd_lines_per_group(wlines);
+  return (wlines + lpg - 1) / lpg;
 }
 
-// per-level reduction of a into out[nz] (device); scratch from the caller
-int level_reduce(const LevelArgs& a, double* out, double* scratch, hipStream_t s) {
-  const int64_t nlines = (int64_t)a.nx * a.ny;
-  int64_t rows = (nlines + LV_LINES - 1) / LV_LINES;
-  double* p0 = scratch;
-  double* p1 = scratch + rows * a.nz;
-  hipLaunchKernelGGL(level_partial_kernel, dim3((unsigned)rows), dim3(256), 0, s, a, p0);
-  STEAM_LAUNCH_CHECK("level_partial_kernel");
-  double* src = p0;
-  double* dst = p1;
-  while (rows > 1) {
-    const int64_t next = (rows + LV_ROWS - 1) / LV_ROWS;
-    double* target = (next == 1) ? out : dst;
-    hipLaunchKernelGGL(level_combine_kernel, dim3((a.nz + 63) / 64, (unsigned)next), dim3(256),
-                       0, s, src, rows, a.nz, a.op, target);
-    STEAM_LAUNCH_CHECK("level_combine_kernel");
-    rows = next;
-    std::swap(src, dst);
-    src = target;
-    dst = (target == p0) ? p1 : p0;
+// bytes of reduction scratch for windows of up to nlines lines, nz levels
+// and nc candidate shifts per launch (the run count of any window of at
+// most nlines lines is at most this bound)
+int64_t level_scratch_bytes(int64_t nlines, int64_t nz, int nc) {
+  const int64_t g = std::min<int64_t>(
+      std::max<int64_t>((nlines + RD_MIN_LINES - 1) / RD_MIN_LINES, 1), RD_MAX_GROUPS);
+  return g * nc * nz * 8;
+}
+
+// out[c][z] = per-level reduction (divided by div when div > 0) of a's op
+// over its window, for each of a.ncand shifts
+int level_reduce(const LevelArgs& a, double* out, double div, double* scratch, hipStream_t s) {
+  const int64_t wlines = (int64_t)(a.win[1] - a.win[0]) * (a.win[3] - a.win[2]);
+  const int64_t lpg = rd_lines_per_group(wlines);
+  const int64_t groups = rd_groups(wlines);
+  const dim3 grid((unsigned)groups), block(256);
+  if (a.ncand > 1) {
+    if (a.op == LV_CLAMP)
+      hipLaunchKernelGGL((level_partial_kernel<KCAND, LV_CLAMP>), grid, block, 0, s, a, scratch, lpg);
+    else if (a.op == LV_E_CLAMP)
+      hipLaunchKernelGGL((level_partial_kernel<KCAND, LV_E_CLAMP>), grid, block, 0, s, a, scratch,
+                         lpg);
+    else
+      return set_error(STEAM_EINVAL, "level reduction: candidates need a clamp op");
+  } else {
+    switch (a.op) {
+#define STEAM_LV_CASE(OPV)                                                                  \
+  case OPV:                                                                                \
+    hipLaunchKernelGGL((level_partial_kernel<1, OPV>), grid, block, 0, s, a, scratch, lpg); \
+    break;
+      STEAM_LV_CASE(LV_SUM)
+      STEAM_LV_CASE(LV_ABS)
+      STEAM_LV_CASE(LV_NONZERO)
+      STEAM_LV_CASE(LV_CLAMP)
+      STEAM_LV_CASE(LV_PHI)
+      STEAM_LV_CASE(LV_MIN)
+      STEAM_LV_CASE(LV_MAX)
+      STEAM_LV_CASE(LV_E_SUM)
+      STEAM_LV_CASE(LV_E_MIN)
+      STEAM_LV_CASE(LV_E_MAX)
+      STEAM_LV_CASE(LV_E_CLAMP)
+#undef STEAM_LV_CASE
+      default:
+        return set_error(STEAM_EINVAL, "level reduction: unknown op");
+    }
   }
-  if (src != out)
-    STEAM_HIP(hipMemcpyAsync(out, src, (size_t)a.nz * 8, hipMemcpyDeviceToDevice, s));
+  STEAM_LAUNCH_CHECK("level_partial_kernel");
+  const int64_t pairs = (int64_t)a.ncand * a.nz;
+  hipLaunchKernelGGL(level_combine_kernel, dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, s,
+                     scratch, (int)groups, a.ncand, a.nz, a.op, div, out, a.stop);
+  STEAM_LAUNCH_CHECK("level_combine_kernel");
   return STEAM_OK;
 }
 
@@ -493,6 +672,28 @@ int check_window(const int64_t* w, int nx, int ny, int out[4]) {
   if (w[0] < 0 || w[1] > nx || w[0] >= w[1] || w[2] < 0 || w[3] > ny || w[2] >= w[3])
     return set_error(STEAM_EINVAL, "window out of range");
   for (int i = 0; i < 4; ++i) out[i] = (int)w[i];
+  return STEAM_OK;
+}
+
+bool shape_ok(int64_t nx, int64_t ny, int64_t nz) {
+  return nx >= 1 && ny >= 1 && nz >= 1 && nx < (1 << 30) && ny < (1 << 30) && nz < (1 << 20) &&
+         nx * ny < (1ll << 31) && nx * ny * nz < (1ll << 40);
+}
+
+// the 60-step bisection of the level means of a's clamp op against target,
+// KSTEPS steps per pass; leaves the result in b.mu
+int bisect(LevelArgs a, BisectVec b, int nz, double width, double npts, int mode, double* red,
+           hipStream_t s) {
+  const int gz = (nz + 255) / 256;
+  hipLaunchKernelGGL(bisect_init_kernel, dim3(gz), dim3(256), 0, s, b, nz, width);
+  a.mu = b.cand;
+  a.ncand = KCAND;
+  for (int pass = 0; pass < 60 / KSTEPS; ++pass) {
+    int rc = level_reduce(a, (double*)b.sums, 0.0, red, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(bisect_pass_kernel, dim3(gz), dim3(256), 0, s, b, nz, npts, mode);
+  }
+  STEAM_LAUNCH_CHECK("bisection");
   return STEAM_OK;
 }
 
@@ -541,7 +742,7 @@ extern "C" int steam_pattern_normalize(float* w_dev, const float* level_mean_dev
 }
 
 extern "C" int64_t steam_level_scratch_bytes(int64_t nx, int64_t ny, int64_t nz) {
-  return level_scratch_bytes(nx * ny, (int)nz);
+  return level_scratch_bytes(nx * ny, nz, 1);
 }
 
 extern "C" int steam_level_reduce(const float* f_dev, const float* mean_z_dev,
@@ -549,13 +750,14 @@ extern "C" int steam_level_reduce(const float* f_dev, const float* mean_z_dev,
                                   const int64_t* window, int op, double lo, double hi,
                                   double* out_dev, void* scratch_dev, int64_t scratch_bytes,
                                   steam_stream_t stream) {
-  if (!f_dev || !out_dev || !scratch_dev || nx < 1 || ny < 1 || nz < 1)
+  if (!f_dev || !out_dev || !scratch_dev || !shape_ok(nx, ny, nz))
     return set_error(STEAM_EINVAL, "steam_level_reduce: bad arguments");
-  if (op < LV_SUM || op > LV_E_MAX) return set_error(STEAM_EINVAL, "steam_level_reduce: unknown op");
+  if (op < LV_SUM || op > LV_E_CLAMP) return set_error(STEAM_EINVAL, "steam_level_reduce: unknown op");
   if ((op == LV_PHI || op == LV_MIN || op == LV_MAX || op >= LV_E_SUM) && !mean_z_dev)
     return set_error(STEAM_EINVAL, "steam_level_reduce: op needs mean_z");
-  if (op == LV_CLAMP && !mu_dev) return set_error(STEAM_EINVAL, "steam_level_reduce: op needs mu");
-  if (scratch_bytes < level_scratch_bytes(nx * ny, (int)nz))
+  if ((op == LV_CLAMP || op == LV_E_CLAMP) && !mu_dev)
+    return set_error(STEAM_EINVAL, "steam_level_reduce: op needs mu");
+  if (scratch_bytes < level_scratch_bytes(nx * ny, nz, 1))
     return set_error(STEAM_EINVAL, "steam_level_reduce: scratch too small");
   LevelArgs a;
   memset(&a, 0, sizeof(a));
@@ -566,17 +768,23 @@ extern "C" int steam_level_reduce(const float* f_dev, const float* mean_z_dev,
   a.ny = (int)ny;
   a.nz = (int)nz;
   a.op = op;
-  a.lo = (double)(float)lo;
-  a.hi = (double)(float)hi;
-  int rc = check_window(window, (int)nx, (int)ny, a.win);
+  a.ncand = 1;
+  // the float32 clamp takes float32 bounds; the float64 one takes them as given
+  a.lo = op == LV_E_CLAMP ? lo : (double)(float)lo;
+  a.hi = op == LV_E_CLAMP ? hi : (double)(float)hi;
+  const bool whole = op == LV_MIN || op == LV_MAX || op == LV_E_MIN || op == LV_E_MAX;
+  int rc = check_window(whole ? nullptr : window, (int)nx, (int)ny, a.win);
   if (rc) return rc;
-  return level_reduce(a, out_dev, (double*)scratch_dev, (hipStream_t)stream);
+  return level_reduce(a, out_dev, 0.0, (double*)scratch_dev, (hipStream_t)stream);
 }
 
+// bounded-add workspace: per-level vectors (5), the two step histories
+// (up to 64 steps), the bisection state (4 + 2 * KCAND), the loop flags,
+// then the reduction scratch
+constexpr int BA_VEC = 5 + 2 * 64 + 4 + 2 * KCAND;
+
 extern "C" int64_t steam_bounded_add_workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
-  const int64_t n = nx * ny * nz;
-  (void)n;
-  return 16 * nz * 8 + level_scratch_bytes(nx * ny, (int)nz) + 256;
+  return (BA_VEC * nz + 32) * 8 + level_scratch_bytes(nx * ny, nz, KCAND) + 256;
 }
 
 extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float* inc_dev,
@@ -587,7 +795,7 @@ extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float
                                  steam_stream_t stream) {
   if (!pert_dev || !mean_z_dev || !inc_dev || !ledger || !workspace_dev)
     return set_error(STEAM_EINVAL, "steam_bounded_add: NULL argument");
-  if (nx < 1 || ny < 1 || nz < 1 || n_rescale < 0 || n_rescale > 64)
+  if (!shape_ok(nx, ny, nz) || n_rescale < 0 || n_rescale > 64)
     return set_error(STEAM_EINVAL, "steam_bounded_add: bad shape or n_rescale");
   if (!ledger->a0 || !ledger->demean || !ledger->scale || !ledger->n_loop ||
       !ledger->n_scale || !ledger->final_demean || !ledger->mu)
@@ -602,20 +810,26 @@ extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float
   const double npts = (double)(win[1] - win[0]) * (double)(win[3] - win[2]);
   const float lo = (float)phi_min, hi = (float)phi_max;
   const double width = phi_max - phi_min;
-  // the solve runs in place on inc; the workspace holds per-level vectors
-  // and the reduction scratch only
   double* vec = (double*)workspace_dev;
   BaVec v;
   v.a0 = vec;
   v.done = vec + nz;
-  v.sum = vec + 2 * nz;
-  v.demean = vec + 3 * nz;
+  v.mean = vec + 2 * nz;
+  v.shift = vec + 3 * nz;
   v.scale = vec + 4 * nz;
-  v.mu = vec + 5 * nz;
-  v.mu_lo = vec + 6 * nz;
-  v.mu_hi = vec + 7 * nz;
-  v.mean_phi = vec + 8 * nz;
-  double* red = vec + 16 * nz;
+  v.hist_demean = vec + 5 * nz;
+  v.hist_scale = vec + 69 * nz;
+  double* mean_phi = vec + 133 * nz;
+  BisectVec b;
+  b.lo = vec + 134 * nz;
+  b.hi = vec + 135 * nz;
+  b.mu = vec + 136 * nz;
+  b.cand = vec + 137 * nz;
+  double* sums = vec + (137 + KCAND) * nz;
+  b.sums = sums;
+  b.target = mean_phi;
+  v.flags = (int*)(vec + BA_VEC * nz);
+  double* red = vec + BA_VEC * nz + 32;
   const int gz = (int)((nz + 255) / 256);
   const int gn = grid_1d(n);
   LevelArgs a;
@@ -627,102 +841,83 @@ extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float
   a.mean_z = mean_z_dev;
   a.lo = (double)lo;
   a.hi = (double)hi;
-  auto reduce = [&](const float* f, int op, double* out, const double* mu) {
-    LevelArgs b = a;
-    b.f = f;
-    b.op = op;
-    b.mu = mu;
-    return level_reduce(b, out, red, s);
+  a.ncand = 1;
+  auto reduce_mean = [&](const float* f, int op, double* out, const int* stop) {
+    LevelArgs c = a;
+    c.f = f;
+    c.op = op;
+    c.stop = stop;
+    return level_reduce(c, out, npts, red, s);
   };
-  std::vector<double> host(nz);
-  auto down = [&](const double* src, double* dst) -> int {
-    STEAM_HIP(hipMemcpyAsync(dst, src, (size_t)nz * 8, hipMemcpyDeviceToHost, s));
-    return STEAM_OK;
-  };
-  auto up = [&](const double* src, double* dst) -> int {
-    STEAM_HIP(hipMemcpyAsync(dst, src, (size_t)nz * 8, hipMemcpyHostToDevice, s));
-    return STEAM_OK;
-  };
+  const size_t vbytes = (size_t)nz * 8;
 
   if (replay) {
     // apply-only: the recorded per-level scalars, no reductions
     int nloop = 0;
     for (int z = 0; z < nz; ++z) nloop = std::max(nloop, (int)ledger->n_loop[z]);
-    std::vector<double> col(nz);
+    if (nloop > n_rescale) return set_error(STEAM_EINVAL, "steam_bounded_add: ledger n_loop too large");
+    STEAM_HIP(hipMemcpyAsync(v.hist_demean, ledger->demean, vbytes * n_rescale,
+                             hipMemcpyHostToDevice, s));
+    STEAM_HIP(hipMemcpyAsync(v.hist_scale, ledger->scale, vbytes * n_rescale,
+                             hipMemcpyHostToDevice, s));
+    STEAM_HIP(hipMemcpyAsync(v.shift, ledger->final_demean, vbytes, hipMemcpyHostToDevice, s));
+    STEAM_HIP(hipMemcpyAsync(b.mu, ledger->mu, vbytes, hipMemcpyHostToDevice, s));
     for (int step = 0; step < nloop; ++step) {
-      for (int z = 0; z < nz; ++z) col[z] = ledger->demean[(int64_t)z * n_rescale + step];
-      if ((rc = up(col.data(), v.demean))) return rc;
       hipLaunchKernelGGL(ba_shift_clip_kernel, dim3(gn), dim3(256), 0, s, inc_dev, pert_dev,
-                         mean_z_dev, v.demean, n, (int)nz, lo, hi);
-      for (int z = 0; z < nz; ++z) col[z] = ledger->scale[(int64_t)z * n_rescale + step];
-      if ((rc = up(col.data(), v.scale))) return rc;
-      hipLaunchKernelGGL(ba_scale_kernel, dim3(gn), dim3(256), 0, s, inc_dev, v.scale, n,
-                         (int)nz);
-      STEAM_HIP(hipStreamSynchronize(s));
+                         mean_z_dev, v.hist_demean + step, n_rescale, n, (int)nz, lo, hi,
+                         nullptr);
+      hipLaunchKernelGGL(ba_scale_kernel, dim3(gn), dim3(256), 0, s, inc_dev,
+                         v.hist_scale + step, n_rescale, n, (int)nz, nullptr);
     }
-    if ((rc = up(ledger->final_demean, v.demean))) return rc;
     hipLaunchKernelGGL(ba_shift_clip_kernel, dim3(gn), dim3(256), 0, s, inc_dev, pert_dev,
-                       mean_z_dev, v.demean, n, (int)nz, lo, hi);
-    if ((rc = up(ledger->mu, v.mu))) return rc;
+                       mean_z_dev, v.shift, 1, n, (int)nz, lo, hi, nullptr);
+    hipLaunchKernelGGL(ba_to_field_kernel, dim3(gn), dim3(256), 0, s, inc_dev, pert_dev,
+                       mean_z_dev, n, (int)nz);
   } else {
     // mean of the running field over the window, and the target amplitude
-    if ((rc = reduce(pert_dev, LV_PHI, v.sum, nullptr))) return rc;
-    hipLaunchKernelGGL(ba_mean_kernel, dim3(gz), dim3(256), 0, s, v.sum, v.mean_phi, (int)nz, npts);
-    if ((rc = reduce(inc_dev, LV_ABS, v.sum, nullptr))) return rc;
-    hipLaunchKernelGGL(ba_mean_kernel, dim3(gz), dim3(256), 0, s, v.sum, v.a0, (int)nz, npts);
-    hipLaunchKernelGGL(ba_init_done_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz);
-    if ((rc = down(v.a0, ledger->a0))) return rc;
-    for (int64_t i = 0; i < nz * (int64_t)n_rescale; ++i) {
-      ledger->demean[i] = 0.0;
-      ledger->scale[i] = 1.0;
-    }
-    int n_loop = 0;
-    std::vector<double> dm(nz), sc(nz), dn(nz);
+    if ((rc = reduce_mean(pert_dev, LV_PHI, mean_phi, nullptr))) return rc;
+    if ((rc = reduce_mean(inc_dev, LV_ABS, v.a0, nullptr))) return rc;
+    hipLaunchKernelGGL(ba_init_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz, n_rescale);
     for (int step = 0; step < n_rescale; ++step) {
-      if ((rc = reduce(inc_dev, LV_SUM, v.sum, nullptr))) return rc;
-      hipLaunchKernelGGL(ba_demean_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz, npts);
+      if ((rc = reduce_mean(inc_dev, LV_SUM, v.mean, v.flags))) return rc;
+      hipLaunchKernelGGL(ba_demean_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz, step,
+                         n_rescale);
       hipLaunchKernelGGL(ba_shift_clip_kernel, dim3(gn), dim3(256), 0, s, inc_dev, pert_dev,
-                         mean_z_dev, v.demean, n, (int)nz, lo, hi);
-      n_loop = step + 1;
-      if ((rc = reduce(inc_dev, LV_ABS, v.sum, nullptr))) return rc;
-      hipLaunchKernelGGL(ba_rescale_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz, npts);
-      if ((rc = down(v.demean, dm.data()))) return rc;
-      if ((rc = down(v.done, dn.data()))) return rc;
-      if ((rc = down(v.scale, sc.data()))) return rc;
-      STEAM_HIP(hipStreamSynchronize(s));
-      bool all = true;
-      for (int z = 0; z < nz; ++z) {
-        ledger->demean[(int64_t)z * n_rescale + step] = dm[z];
-        all = all && dn[z] != 0.0;
-      }
-      if (all) break;
-      for (int z = 0; z < nz; ++z) ledger->scale[(int64_t)z * n_rescale + step] = sc[z];
-      hipLaunchKernelGGL(ba_scale_kernel, dim3(gn), dim3(256), 0, s, inc_dev, v.scale, n,
-                         (int)nz);
+                         mean_z_dev, v.shift, 1, n, (int)nz, lo, hi, v.flags);
+      if ((rc = reduce_mean(inc_dev, LV_ABS, v.mean, v.flags))) return rc;
+      hipLaunchKernelGGL(ba_rescale_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz);
+      hipLaunchKernelGGL(ba_check_kernel, dim3(1), dim3(256), 0, s, v, (int)nz, step, n_rescale);
+      hipLaunchKernelGGL(ba_scale_kernel, dim3(gn), dim3(256), 0, s, inc_dev, v.scale, 1, n,
+                         (int)nz, v.flags);
     }
-    for (int z = 0; z < nz; ++z) {
-      ledger->n_loop[z] = n_loop;
-      ledger->n_scale[z] = n_loop;
-    }
-    if ((rc = reduce(inc_dev, LV_SUM, v.sum, nullptr))) return rc;
-    hipLaunchKernelGGL(ba_mean_kernel, dim3(gz), dim3(256), 0, s, v.sum, v.demean, (int)nz, npts);
-    if ((rc = down(v.demean, ledger->final_demean))) return rc;
+    // the final demean and clip run whatever the loop did
+    if ((rc = reduce_mean(inc_dev, LV_SUM, v.shift, nullptr))) return rc;
     hipLaunchKernelGGL(ba_shift_clip_kernel, dim3(gn), dim3(256), 0, s, inc_dev, pert_dev,
-                       mean_z_dev, v.demean, n, (int)nz, lo, hi);
-  }
-  // e = d + phi, in place; bisect mu in the shifted space
-  hipLaunchKernelGGL(ba_to_field_kernel, dim3(gn), dim3(256), 0, s, inc_dev, pert_dev,
-                     mean_z_dev, n, (int)nz);
-  if (!replay) {
-    hipLaunchKernelGGL(ba_bisect_init_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz, width);
-    for (int it = 0; it < 60; ++it) {
-      if ((rc = reduce(inc_dev, LV_CLAMP, v.sum, v.mu))) return rc;
-      hipLaunchKernelGGL(ba_bisect_step_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz, npts);
+                       mean_z_dev, v.shift, 1, n, (int)nz, lo, hi, nullptr);
+    // e = d + phi, in place; bisect mu in the shifted space
+    hipLaunchKernelGGL(ba_to_field_kernel, dim3(gn), dim3(256), 0, s, inc_dev, pert_dev,
+                       mean_z_dev, n, (int)nz);
+    LevelArgs c = a;
+    c.f = inc_dev;
+    c.op = LV_CLAMP;
+    if ((rc = bisect(c, b, (int)nz, width, npts, 0, red, s))) return rc;
+    int flags[2];
+    STEAM_HIP(hipMemcpyAsync(ledger->a0, v.a0, vbytes, hipMemcpyDeviceToHost, s));
+    STEAM_HIP(hipMemcpyAsync(ledger->demean, v.hist_demean, vbytes * n_rescale,
+                             hipMemcpyDeviceToHost, s));
+    STEAM_HIP(hipMemcpyAsync(ledger->scale, v.hist_scale, vbytes * n_rescale,
+                             hipMemcpyDeviceToHost, s));
+    STEAM_HIP(hipMemcpyAsync(ledger->final_demean, v.shift, vbytes, hipMemcpyDeviceToHost, s));
+    STEAM_HIP(hipMemcpyAsync(ledger->mu, b.mu, vbytes, hipMemcpyDeviceToHost, s));
+    STEAM_HIP(hipMemcpyAsync(flags, v.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    STEAM_HIP(hipStreamSynchronize(s));
+    for (int z = 0; z < nz; ++z) {
+      ledger->n_loop[z] = flags[1];
+      ledger->n_scale[z] = flags[1];
     }
-    if ((rc = down(v.mu, ledger->mu))) return rc;
   }
   hipLaunchKernelGGL(ba_finish_kernel, dim3(gn), dim3(256), 0, s, inc_dev, pert_dev, inc_dev,
-                     mean_z_dev, v.mu, n, (int)nz, lo, hi, record_applied);
+                     mean_z_dev, b.mu, n, (int)nz, lo, hi, record_applied);
   STEAM_LAUNCH_CHECK("bounded add");
   STEAM_HIP(hipStreamSynchronize(s));
   return STEAM_OK;
@@ -739,8 +934,12 @@ extern "C" int steam_deficit_accumulate(float* deficit_dev, const float* after_d
   return STEAM_OK;
 }
 
+// projection workspace: active, vmin, vmax, record, target, the bisection
+// state (3 + 2 * KCAND), then the reduction scratch
+constexpr int PJ_VEC = 8 + 2 * KCAND;
+
 extern "C" int64_t steam_project_workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
-  return 16 * nz * 8 + level_scratch_bytes(nx * ny, (int)nz) + 256;
+  return (PJ_VEC * nz + 32) * 8 + level_scratch_bytes(nx * ny, nz, KCAND) + 256;
 }
 
 namespace {
@@ -753,18 +952,20 @@ int project(float* f, const float* mean_z, int64_t nx, int64_t ny, int64_t nz, c
   const double npts = (double)(win[1] - win[0]) * (double)(win[3] - win[2]);
   const double width = hi - lo;
   double* vec = (double*)ws;
-  BaVec v;
-  memset(&v, 0, sizeof(v));
-  v.sum = vec;
-  v.mu = vec + nz;
-  v.mu_lo = vec + 2 * nz;
-  v.mu_hi = vec + 3 * nz;
-  v.mean_phi = vec + 4 * nz;
-  double* active = vec + 5 * nz;
-  double* vmin = vec + 6 * nz;
-  double* vmax = vec + 7 * nz;
-  double* rec = vec + 8 * nz;
-  double* red = vec + 16 * nz;
+  double* active = vec;
+  double* vmin = vec + nz;
+  double* vmax = vec + 2 * nz;
+  double* rec = vec + 3 * nz;
+  double* target = vec + 4 * nz;
+  BisectVec b;
+  b.lo = vec + 5 * nz;
+  b.hi = vec + 6 * nz;
+  b.mu = vec + 7 * nz;
+  b.cand = vec + 8 * nz;
+  double* sums = vec + (8 + KCAND) * nz;
+  b.sums = sums;
+  b.target = target;
+  double* red = vec + PJ_VEC * nz + 32;
   const int gz = (int)((nz + 255) / 256);
   const int gn = grid_1d(n);
   LevelArgs a;
@@ -774,37 +975,38 @@ int project(float* f, const float* mean_z, int64_t nx, int64_t ny, int64_t nz, c
   a.nx = (int)nx;
   a.ny = (int)ny;
   a.nz = (int)nz;
-  memcpy(a.win, win, sizeof(a.win));
   a.lo = lo;
   a.hi = hi;
+  a.ncand = 1;
+  a.active = active;
   int rc;
-  auto reduce = [&](int op, double* out, const double* mu) {
-    LevelArgs b = a;
-    b.op = op;
-    b.mu = mu;
-    b.active = active;
-    return level_reduce(b, out, red, s);
-  };
   if (replay) {
-    STEAM_HIP(hipMemcpyAsync(v.mu, mu_host, (size_t)nz * 8, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(proj_active_kernel, dim3(gz), dim3(256), 0, s, v.mu, active, (int)nz);
+    STEAM_HIP(hipMemcpyAsync(b.mu, mu_host, (size_t)nz * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(proj_replay_active_kernel, dim3(gz), dim3(256), 0, s, b.mu, active,
+                       (int)nz);
   } else {
-    if ((rc = reduce(LV_E_MIN, vmin, nullptr))) return rc;
-    if ((rc = reduce(LV_E_MAX, vmax, nullptr))) return rc;
-    hipLaunchKernelGGL(proj_init_kernel, dim3(gz), dim3(256), 0, s, v, vmin, vmax, active,
-                       (int)nz, lo, hi, width);
-    if ((rc = reduce(LV_E_SUM, v.sum, nullptr))) return rc;
-    hipLaunchKernelGGL(ba_mean_kernel, dim3(gz), dim3(256), 0, s, v.sum, v.mean_phi, (int)nz,
-                       npts);
-    for (int it = 0; it < 60; ++it) {
-      if ((rc = reduce(LV_E_CLAMP, v.sum, v.mu))) return rc;
-      hipLaunchKernelGGL(ba_bisect_step_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz, npts);
-    }
-    hipLaunchKernelGGL(proj_record_kernel, dim3(gz), dim3(256), 0, s, v.mu, active, rec,
+    LevelArgs whole = a;
+    whole.win[0] = 0;
+    whole.win[1] = (int)nx;
+    whole.win[2] = 0;
+    whole.win[3] = (int)ny;
+    whole.op = LV_E_MIN;
+    if ((rc = level_reduce(whole, vmin, 0.0, red, s))) return rc;
+    whole.op = LV_E_MAX;
+    if ((rc = level_reduce(whole, vmax, 0.0, red, s))) return rc;
+    hipLaunchKernelGGL(proj_active_kernel, dim3(gz), dim3(256), 0, s, vmin, vmax, active,
+                       (int)nz, lo, hi);
+    LevelArgs c = a;
+    memcpy(c.win, win, sizeof(c.win));
+    c.op = LV_E_SUM;
+    if ((rc = level_reduce(c, target, npts, red, s))) return rc;
+    c.op = LV_E_CLAMP;
+    if ((rc = bisect(c, b, (int)nz, width, npts, 1, red, s))) return rc;
+    hipLaunchKernelGGL(proj_record_kernel, dim3(gz), dim3(256), 0, s, b.mu, active, rec,
                        (int)nz);
     STEAM_HIP(hipMemcpyAsync(mu_host, rec, (size_t)nz * 8, hipMemcpyDeviceToHost, s));
   }
-  hipLaunchKernelGGL(proj_apply_kernel, dim3(gn), dim3(256), 0, s, f, mean_z, v.mu, active, n,
+  hipLaunchKernelGGL(proj_apply_kernel, dim3(gn), dim3(256), 0, s, f, mean_z, b.mu, active, n,
                      (int)nz, lo, hi);
   STEAM_LAUNCH_CHECK("projection");
   STEAM_HIP(hipStreamSynchronize(s));
@@ -818,7 +1020,7 @@ extern "C" int steam_project_onto_bounds(float* pert_dev, const float* mean_z_de
                                          double phi_min, double phi_max, double* mu, int replay,
                                          void* workspace_dev, int64_t workspace_bytes,
                                          steam_stream_t stream) {
-  if (!pert_dev || !mean_z_dev || !mu || !workspace_dev || nx < 1 || ny < 1 || nz < 1)
+  if (!pert_dev || !mean_z_dev || !mu || !workspace_dev || !shape_ok(nx, ny, nz))
     return set_error(STEAM_EINVAL, "steam_project_onto_bounds: bad arguments");
   if (!(phi_max > phi_min)) return set_error(STEAM_EINVAL, "steam_project_onto_bounds: empty bounds");
   if (workspace_bytes < steam_project_workspace_bytes(nx, ny, nz))
@@ -835,7 +1037,7 @@ extern "C" int steam_compose_output(const float* state_dev, float* deficit_dev,
                                     const int64_t* window, double phi_min, double phi_max,
                                     double* mu, int replay, float* out_dev, void* workspace_dev,
                                     int64_t workspace_bytes, steam_stream_t stream) {
-  if (!state_dev || !mean_z_dev || !out_dev || nx < 1 || ny < 1 || nz < 1)
+  if (!state_dev || !mean_z_dev || !out_dev || !shape_ok(nx, ny, nz))
     return set_error(STEAM_EINVAL, "steam_compose_output: bad arguments");
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = nx * ny * nz;
@@ -865,14 +1067,14 @@ extern "C" int steam_compose_output(const float* state_dev, float* deficit_dev,
 }
 
 extern "C" int64_t steam_compose_flux_workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
-  return (nz + 8) * 8 + level_scratch_bytes(nx * ny, (int)nz) + 256;
+  return (nz + 8) * 8 + level_scratch_bytes(nx * ny, nz, 1) + 256;
 }
 
 extern "C" int steam_compose_flux_output(const float* state_dev, float* deficit_dev, int64_t nx,
                                          int64_t ny, int64_t nz, const int64_t* window,
                                          double* sums, int replay, void* workspace_dev,
                                          int64_t workspace_bytes, steam_stream_t stream) {
-  if (!state_dev || !deficit_dev || !sums || !workspace_dev || nx < 1 || ny < 1 || nz < 1)
+  if (!state_dev || !deficit_dev || !sums || !workspace_dev || !shape_ok(nx, ny, nz))
     return set_error(STEAM_EINVAL, "steam_compose_flux_output: bad arguments");
   if (workspace_bytes < steam_compose_flux_workspace_bytes(nx, ny, nz))
     return set_error(STEAM_EINVAL, "steam_compose_flux_output: workspace too small");
@@ -893,6 +1095,7 @@ extern "C" int steam_compose_flux_output(const float* state_dev, float* deficit_
   a.ny = (int)ny;
   a.nz = (int)nz;
   a.op = LV_SUM;
+  a.ncand = 1;
   memcpy(a.win, win, sizeof(a.win));
   if (replay) {
     if (!(sums[1] > 0.0) || !(sums[3] > 0.0))
@@ -901,7 +1104,7 @@ extern "C" int steam_compose_flux_output(const float* state_dev, float* deficit_
     STEAM_HIP(hipMemcpyAsync(means, m, 16, hipMemcpyHostToDevice, s));
   } else {
     a.f = state_dev;
-    if ((rc = level_reduce(a, lv, red, s))) return rc;
+    if ((rc = level_reduce(a, lv, 0.0, red, s))) return rc;
     hipLaunchKernelGGL(sum_levels_kernel, dim3(1), dim3(64), 0, s, lv, (int)nz, count, tot,
                        means);
   }
@@ -909,7 +1112,7 @@ extern "C" int steam_compose_flux_output(const float* state_dev, float* deficit_
                      state_dev, n);
   if (!replay) {
     a.f = deficit_dev;
-    if ((rc = level_reduce(a, lv, red, s))) return rc;
+    if ((rc = level_reduce(a, lv, 0.0, red, s))) return rc;
     hipLaunchKernelGGL(sum_levels_kernel, dim3(1), dim3(64), 0, s, lv, (int)nz, count,
                        tot + 1, means + 1);
   }

@@ -22,7 +22,8 @@ import torch
 from . import _lib
 from .ledger import N_RESCALE_DEFAULT, BoundedAddLedger, PatternLedger
 
-LV_SUM, LV_ABS, LV_NONZERO, LV_CLAMP, LV_PHI, LV_MIN, LV_MAX = range(7)
+(LV_SUM, LV_ABS, LV_NONZERO, LV_CLAMP, LV_PHI, LV_MIN, LV_MAX, LV_E_SUM, LV_E_MIN, LV_E_MAX,
+ LV_E_CLAMP) = range(11)
 
 
 def z_difference_coefficients(z_coords):

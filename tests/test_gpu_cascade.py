@@ -181,3 +181,33 @@ def test_cascade_rejects_what_the_reference_rejects():
     from steam_amd.ledger import RunLedger
     with pytest.raises(ValueError):
         sm.cascade_loop(*args, st['class_seeds'], ledger=RunLedger(2))
+
+
+def test_projection_bisection_equals_one_step_per_pass_bisection():
+    """The projection takes 4 bisection steps per pass over the field (15
+    candidate midpoints at once); driving a one-step-per-pass bisection from
+    the host with the same per-level sums ends on the same mu bit for bit."""
+    from steam_amd import cascade as C
+    from steam_amd import scalars as S
+    pert, mean, lo, hi = _projection_case(seed=7, shape=(40, 36, 12))
+    window = (3, 37, 2, 30)
+    dev = torch.from_numpy(pert).cuda()
+    led = C.project_onto_bounds(dev.clone(), mean, lo, hi, window)
+    mean_t = torch.from_numpy(mean).cuda()
+    red = S.LevelReducer(pert.shape)
+    npts = float((window[1] - window[0]) * (window[3] - window[2]))
+    target = red(dev, S.LV_E_SUM, window, mean_1d=mean_t).cpu().numpy() / npts
+    width = hi - lo
+    mu_lo = np.full(12, -width)
+    mu_hi = np.full(12, width)
+    for _ in range(60):
+        mid = 0.5 * (mu_lo + mu_hi)
+        sums = red(dev, S.LV_E_CLAMP, window, mean_1d=mean_t,
+                   mu=torch.from_numpy(mid).cuda(), lo=lo, hi=hi).cpu().numpy()
+        positive = sums / npts > target
+        mu_lo = np.where(positive, mid, mu_lo)
+        mu_hi = np.where(positive, mu_hi, mid)
+    mu = 0.5 * (mu_lo + mu_hi)
+    ok = ~np.isnan(led.mu)
+    assert ok.sum() >= 4
+    np.testing.assert_array_equal(led.mu[ok], mu[ok])
