@@ -59,6 +59,8 @@ def parse():
                     help="also serve the trace at these rates (req/s) and report "
                          "throughput and p50/p99 per rate")
     ap.add_argument("--serve-world", type=int, nargs=3, default=[2048, 2048, 256])
+    ap.add_argument("--no-full-simulate", action="store_true",
+                    help="skip the full-simulate form (flux + h + qt, all classes)")
     ap.add_argument("--precision-form", action="store_true",
                     help="only run the precision form: one class advance of a "
                          "1024x1024x256 float32 field (alpha 1.6, C1 0.12, 129^3 "
@@ -361,6 +363,56 @@ def precision_form(steps=10):
                     "SeedSequence(0).spawn(1)[0]"}
 
 
+# The reference's published cost of a full simulate() (flux + h + qt, all
+# classes) on a 16-core AMD Ryzen 9950X, CPU only (paper/supplement.tex:713-716).
+PUBLISHED_FULL_SIMULATE_S = {(1024, 64, 78): 1.7, (2048, 128, 115): 3.4}
+
+
+def full_simulate_form(reps=3):
+    """The whole root run on the GPU (set-up, cascade over every class
+    carrying h, qt and flux, composition of the written fields) at the two
+    grids of the reference's cost comparison: a 384 km x 6144 km channel at
+    6 km (1024x64x78) and 3 km (2048x128x115) spacing, outer scale 384 km,
+    spheroscale 10 m, 20 km deep.  The mean profiles are synthetic (the
+    reference's were taken from CM1 output), so the grid ladder and the work
+    match and the fields do not."""
+    import numpy as np
+    import torch
+    from steam_amd import cascade as C
+
+    z = np.arange(201) * 100.0
+    hp = 340e3 - 20e3 * z / z.max()
+    qp = 0.018 - 0.016 * z / z.max()
+    out = []
+    for nx, ny, dx in ((1024, 64, 6000.0), (2048, 128, 3000.0)):
+        def run(seed):
+            return C.simulate_fields(hp, qp, nx, ny, dx, dx, 384000.0, 10.0, 20000.0, 100.0,
+                                     seed=seed, as_tensors=True)
+        res = run(0)
+        torch.cuda.synchronize()
+        times = []
+        for r in range(reps):
+            t0 = time.perf_counter()
+            res = run(r + 1)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        grids = res['grids']
+        finest = tuple(int(v) for v in res['h'].shape)
+        class_voxels = int(sum(int(a) * int(b) * int(c) for a, b, c in
+                               zip(grids['nx'], grids['ny'], grids['nz'])))
+        best = min(times)
+        pub = PUBLISHED_FULL_SIMULATE_S.get(finest)
+        out.append({"finest": list(finest), "classes": len(grids['k']),
+                    "class_voxels": class_voxels, "best_ms": best * 1e3,
+                    "median_ms": sorted(times)[len(times) // 2] * 1e3,
+                    "finest_voxels_per_s": float(np.prod(finest)) / best,
+                    "published_cpu_s": pub,
+                    "speedup_vs_published_cpu": (pub / best) if pub else None})
+    return {"workload": "full simulate (flux + h + qt), reference cost-comparison grids",
+            "published_cpu": "16-core AMD Ryzen 9950X (paper/supplement.tex:713-716)",
+            "data": "synthetic linear h/qt profiles, seeds 0..reps", "runs": out}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -514,6 +566,8 @@ def main():
                                                      args.serve_rate,
                                                      args.serve_sweep)
 
+    full = None if args.no_full_simulate else full_simulate_form()
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_sample or list(shape), c, alpha,
@@ -545,6 +599,7 @@ def main():
         "kernels": fam,
         "batched": batched,
         "served": served,
+        "full_simulate": full,
     }
     print(json.dumps(out))
 

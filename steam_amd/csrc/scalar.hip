@@ -120,6 +120,10 @@ enum LevelOp {
   LV_E_MIN = 8,      // min e over the whole level
   LV_E_MAX = 9,      // max e over the whole level
   LV_E_CLAMP = 10,   // sum clamp(e - mu[z], lo, hi) on levels with active[z]
+  // the bounded add's rescale step, fused (the field is the increment d)
+  LV_SCALED_SUM = 11,  // sum f32(d * f32(scale[z]))
+  LV_APPLY_ABS = 12,   // d = clamp(f32(d * f32(scale[z])) - f32(shift[z]) + phi, lo, hi) - phi
+                       // over the whole field, written back; sum |d| over the window
 };
 
 // The window's lines (x, y) are split into G contiguous runs of
@@ -141,8 +145,13 @@ struct LevelArgs {
   const double* mu;      // [ncand][nz] shifts (clamp ops)
   const double* active;  // LV_E_CLAMP: levels to reduce (others give 0)
   const int* stop;       // when set and nonzero the launch does nothing
+  const float* pert;     // LV_APPLY_ABS: the perturbation (phi = pert + mean)
+  float* out;            // LV_APPLY_ABS: where d is written (may be f)
+  const double* scale;   // LV_SCALED_SUM, LV_APPLY_ABS: pending per-level scale
+  const double* shift;   // LV_APPLY_ABS: per-level demean
   int nx, ny, nz;
-  int win[4];
+  int win[4];            // lines visited
+  int awin[4];           // LV_APPLY_ABS: lines whose |d| is summed
   int op;
   int ncand;
   double lo, hi;
@@ -199,11 +208,25 @@ __global__ __launch_bounds__(256) void level_partial_kernel(LevelArgs a, double*
         mu64[c] = (a.mu && c < nc) ? a.mu[(int64_t)c * a.nz + z] : 0.0;
         muf[c] = (float)mu64[c];
       }
+      float sc = 1.f, sh = 0.f;
+      if constexpr (OP == LV_SCALED_SUM || OP == LV_APPLY_ABS) sc = (float)a.scale[z];
+      if constexpr (OP == LV_APPLY_ABS) sh = (float)a.shift[z];
       int64_t wl = l0 + first;
       int x = wx0 + (int)(wl / wy);
       int y = wy0 + (int)(wl - (int64_t)(x - wx0) * wy);
       for (; wl < l1; wl += lpi) {
-        const float v = a.f[((int64_t)x * a.ny + y) * a.nz + z];
+        const int64_t idx = ((int64_t)x * a.ny + y) * a.nz + z;
+        const float v = a.f[idx];
+        if constexpr (OP == LV_SCALED_SUM) acc[0] += (double)__fmul_rn(v, sc);
+        if constexpr (OP == LV_APPLY_ABS) {
+          const float phi = __fadd_rn(a.pert[idx], mz);
+          const float u = __fsub_rn(__fmul_rn(v, sc), sh);
+          const float e = fminf(fmaxf(__fadd_rn(u, phi), flo), fhi);
+          const float d = __fsub_rn(e, phi);
+          a.out[idx] = d;
+          if (x >= a.awin[0] && x < a.awin[1] && y >= a.awin[2] && y < a.awin[3])
+            acc[0] += (double)fabsf(d);
+        }
         if constexpr (OP == LV_SUM) acc[0] += (double)v;
         if constexpr (OP == LV_ABS) acc[0] += (double)fabsf(v);
         if constexpr (OP == LV_NONZERO) acc[0] += v != 0.f ? 1.0 : 0.0;
@@ -213,16 +236,17 @@ __global__ __launch_bounds__(256) void level_partial_kernel(LevelArgs a, double*
         if constexpr (OP == LV_E_SUM) acc[0] += (double)v + (double)mz;
         if constexpr (OP == LV_E_MIN) acc[0] = fmin(acc[0], (double)v + (double)mz);
         if constexpr (OP == LV_E_MAX) acc[0] = fmax(acc[0], (double)v + (double)mz);
+        // every candidate slot is evaluated (slots past nc hold shift 0 and
+        // are never stored), so the loop body is straight-line code
         if constexpr (OP == LV_CLAMP) {
 #pragma unroll
           for (int c = 0; c < NC; ++c)
-            if (c < nc) acc[c] += (double)fminf(fmaxf(__fsub_rn(v, muf[c]), flo), fhi);
+            acc[c] += (double)fminf(fmaxf(__fsub_rn(v, muf[c]), flo), fhi);
         }
         if constexpr (OP == LV_E_CLAMP) {
           const double e = (double)v + (double)mz;
 #pragma unroll
-          for (int c = 0; c < NC; ++c)
-            if (c < nc) acc[c] += fmin(fmax(e - mu64[c], a.lo), a.hi);
+          for (int c = 0; c < NC; ++c) acc[c] += fmin(fmax(e - mu64[c], a.lo), a.hi);
         }
         y += lpi;
         if (y >= wy0 + wy) {
@@ -418,6 +442,7 @@ struct BaVec {
   double* scale;         // this step's applied scale
   double* hist_demean;   // [nz][n_rescale]
   double* hist_scale;    // [nz][n_rescale]
+  double* pend;          // scale still to be applied to d (1 when none)
   int* flags;            // [0] stop, [1] n_loop
 };
 
@@ -426,6 +451,7 @@ __global__ void ba_init_kernel(BaVec v, int nz, int n_rescale) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   if (z < nz) {
     v.done[z] = v.a0[z] <= 0.0 ? 1.0 : 0.0;
+    v.pend[z] = 1.0;
     for (int s = 0; s < n_rescale; ++s) {
       v.hist_demean[(int64_t)z * n_rescale + s] = 0.0;
       v.hist_scale[(int64_t)z * n_rescale + s] = 1.0;
@@ -461,6 +487,7 @@ __global__ void ba_rescale_kernel(BaVec v, int nz) {
 }
 // one workgroup: n_loop = step + 1; when every level is done raise stop
 // (the step's scale is then neither applied nor recorded), else record it
+// and leave it pending, to be applied by the next pass over d
 __global__ __launch_bounds__(256) void ba_check_kernel(BaVec v, int nz, int step, int n_rescale) {
   if (v.flags[0]) return;
   __shared__ int all_done;
@@ -470,15 +497,36 @@ __global__ __launch_bounds__(256) void ba_check_kernel(BaVec v, int nz, int step
     if (v.done[z] == 0.0) all_done = 0;
   __syncthreads();
   if (all_done) {
+    for (int z = threadIdx.x; z < nz; z += blockDim.x) v.pend[z] = 1.0;
     if (threadIdx.x == 0) {
       v.flags[1] = step + 1;
       v.flags[0] = 1;
     }
     return;
   }
-  for (int z = threadIdx.x; z < nz; z += blockDim.x)
+  for (int z = threadIdx.x; z < nz; z += blockDim.x) {
     v.hist_scale[(int64_t)z * n_rescale + step] = v.scale[z];
+    v.pend[z] = v.scale[z];
+  }
   if (threadIdx.x == 0) v.flags[1] = step + 1;
+}
+
+// the final demean and clip, then e = d + phi, in one pass:
+// d' = clamp(f32(d * f32(pend)) - f32(shift) + phi, lo, hi) - phi; d = d' + phi
+__global__ __launch_bounds__(256) void ba_final_kernel(float* __restrict__ d,
+                                                       const float* __restrict__ pert,
+                                                       const float* __restrict__ mean_z,
+                                                       const double* __restrict__ pend,
+                                                       const double* __restrict__ shift,
+                                                       int64_t n, int nz, float lo, float hi) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i % nz);
+    const float phi = __fadd_rn(pert[i], mean_z[z]);
+    const float u = __fsub_rn(__fmul_rn(d[i], (float)pend[z]), (float)shift[z]);
+    const float e = fminf(fmaxf(__fadd_rn(u, phi), lo), hi);
+    d[i] = __fadd_rn(__fsub_rn(e, phi), phi);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -651,6 +699,8 @@ int level_reduce(const LevelArgs& a, double* out, double div, double* scratch, h
       STEAM_LV_CASE(LV_E_MIN)
       STEAM_LV_CASE(LV_E_MAX)
       STEAM_LV_CASE(LV_E_CLAMP)
+      STEAM_LV_CASE(LV_SCALED_SUM)
+      STEAM_LV_CASE(LV_APPLY_ABS)
 #undef STEAM_LV_CASE
       default:
         return set_error(STEAM_EINVAL, "level reduction: unknown op");
@@ -781,7 +831,7 @@ extern "C" int steam_level_reduce(const float* f_dev, const float* mean_z_dev,
 // bounded-add workspace: per-level vectors (5), the two step histories
 // (up to 64 steps), the bisection state (4 + 2 * KCAND), the loop flags,
 // then the reduction scratch
-constexpr int BA_VEC = 5 + 2 * 64 + 4 + 2 * KCAND;
+constexpr int BA_VEC = 5 + 2 * 64 + 4 + 2 * KCAND + 1;
 
 extern "C" int64_t steam_bounded_add_workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
   return (BA_VEC * nz + 32) * 8 + level_scratch_bytes(nx * ny, nz, KCAND) + 256;
@@ -828,6 +878,7 @@ extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float
   double* sums = vec + (137 + KCAND) * nz;
   b.sums = sums;
   b.target = mean_phi;
+  v.pend = vec + (137 + 2 * KCAND) * nz;
   v.flags = (int*)(vec + BA_VEC * nz);
   double* red = vec + BA_VEC * nz + 32;
   const int gz = (int)((nz + 255) / 256);
@@ -878,25 +929,38 @@ extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float
     if ((rc = reduce_mean(pert_dev, LV_PHI, mean_phi, nullptr))) return rc;
     if ((rc = reduce_mean(inc_dev, LV_ABS, v.a0, nullptr))) return rc;
     hipLaunchKernelGGL(ba_init_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz, n_rescale);
+    // each step is two passes over d: the mean of the (pending-scaled) d,
+    // then scale, demean and clip written back with the mean |d| taken on
+    // the way; the step's own scale stays pending for the next pass
+    LevelArgs scaled = a;
+    scaled.f = inc_dev;
+    scaled.op = LV_SCALED_SUM;
+    scaled.scale = v.pend;
+    scaled.stop = v.flags;
+    LevelArgs apply = scaled;
+    apply.op = LV_APPLY_ABS;
+    apply.out = inc_dev;
+    apply.pert = pert_dev;
+    apply.shift = v.shift;
+    memcpy(apply.awin, win, sizeof(win));
+    apply.win[0] = 0;
+    apply.win[1] = (int)nx;
+    apply.win[2] = 0;
+    apply.win[3] = (int)ny;
     for (int step = 0; step < n_rescale; ++step) {
-      if ((rc = reduce_mean(inc_dev, LV_SUM, v.mean, v.flags))) return rc;
+      if ((rc = level_reduce(scaled, v.mean, npts, red, s))) return rc;
       hipLaunchKernelGGL(ba_demean_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz, step,
                          n_rescale);
-      hipLaunchKernelGGL(ba_shift_clip_kernel, dim3(gn), dim3(256), 0, s, inc_dev, pert_dev,
-                         mean_z_dev, v.shift, 1, n, (int)nz, lo, hi, v.flags);
-      if ((rc = reduce_mean(inc_dev, LV_ABS, v.mean, v.flags))) return rc;
+      if ((rc = level_reduce(apply, v.mean, npts, red, s))) return rc;
       hipLaunchKernelGGL(ba_rescale_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz);
       hipLaunchKernelGGL(ba_check_kernel, dim3(1), dim3(256), 0, s, v, (int)nz, step, n_rescale);
-      hipLaunchKernelGGL(ba_scale_kernel, dim3(gn), dim3(256), 0, s, inc_dev, v.scale, 1, n,
-                         (int)nz, v.flags);
     }
-    // the final demean and clip run whatever the loop did
-    if ((rc = reduce_mean(inc_dev, LV_SUM, v.shift, nullptr))) return rc;
-    hipLaunchKernelGGL(ba_shift_clip_kernel, dim3(gn), dim3(256), 0, s, inc_dev, pert_dev,
-                       mean_z_dev, v.shift, 1, n, (int)nz, lo, hi, nullptr);
-    // e = d + phi, in place; bisect mu in the shifted space
-    hipLaunchKernelGGL(ba_to_field_kernel, dim3(gn), dim3(256), 0, s, inc_dev, pert_dev,
-                       mean_z_dev, n, (int)nz);
+    // the final demean and clip run whatever the loop did; then e = d + phi
+    // in place, and mu is bisected in that shifted space
+    scaled.stop = nullptr;
+    if ((rc = level_reduce(scaled, v.shift, npts, red, s))) return rc;
+    hipLaunchKernelGGL(ba_final_kernel, dim3(gn), dim3(256), 0, s, inc_dev, pert_dev,
+                       mean_z_dev, v.pend, v.shift, n, (int)nz, lo, hi);
     LevelArgs c = a;
     c.f = inc_dev;
     c.op = LV_CLAMP;
