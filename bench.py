@@ -9,7 +9,7 @@ zero-padded-z FFT convolution, clip, and the volume-mean restore (deferred and
 chained on the device, as the resident cascade runs it).  The flux is resident
 in HBM when the timed region starts.
 
-Prints one JSON line (rank 0 only; this build runs on one GPU):
+Prints one JSON line (one GPU):
   value         voxels/s of the resident advance
   roofline      dominant kernel's algorithmic bytes / its average duration
                 (HIP events on its stream, live over the timed region)
@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--serve-world", type=int, nargs=3, default=[2048, 2048, 256])
     ap.add_argument("--no-full-simulate", action="store_true",
                     help="skip the full-simulate form (flux + h + qt, all classes)")
+    ap.add_argument("--full-simulate-only", action="store_true",
+                    help="only run the full-simulate form; prints one JSON line")
     ap.add_argument("--precision-form", action="store_true",
                     help="only run the precision form: one class advance of a "
                          "1024x1024x256 float32 field (alpha 1.6, C1 0.12, 129^3 "
@@ -423,10 +425,15 @@ def main():
         _lib.require_gpu('cuda')
         print(json.dumps(precision_form()))
         return
+    if args.full_simulate_only:
+        from steam_amd import _lib
+        _lib.require_gpu('cuda')
+        print(json.dumps(full_simulate_form()))
+        return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus "
-              f"{args.gpus}); the distributed decomposition is designed, not "
-              f"built. Run with --gpus 1.", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (got --gpus {args.gpus}); the "
+              f"distributed decomposition is designed, not built. Run with "
+              f"--gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -1,9 +1,10 @@
 """steam_amd — the STEAM field-generation hot path on AMD MI355X (gfx950).
 
 World-keyed Philox4x32-10 noise with CMS extremal-Levy variates, turbulon
-kernel synthesis, the periodic-x/y zero-padded-z FFT overlap-add convolution
-and the multiplicative flux-cascade advance, as hand-written HIP kernels
-(libsteam_hip.so) behind the reference's Python API:
+kernel synthesis, the periodic-x/y zero-padded-z FFT overlap-add convolution,
+the multiplicative flux-cascade advance and the scalar increments of the
+cascade, as hand-written HIP kernels (libsteam_hip.so) behind the reference's
+Python API:
 
     steam_amd.noise       keyed_uniforms, class_key, center_indices
     steam_amd.turbulons   SHAPES, mexican_hat, morlet_omega0_6
@@ -13,7 +14,14 @@ and the multiplicative flux-cascade advance, as hand-written HIP kernels
                           _levy_from_uniforms, _turbulon_envelope,
                           _compute_all_grids, _advance_flux,
                           simulate_flux_only
-    steam_amd.ledger      MeanReduction, FluxAdvanceLedger, RunLedger
+    steam_amd.scalars     advective_weight, joint_pattern, product_norm,
+                          bounded_amplitude_add
+    steam_amd.cascade     cascade_loop, project_onto_bounds, compose_output,
+                          compose_flux_output, simulate_fields
+    steam_amd.params      _compute_normalization, _bound_buffers,
+                          _compensation_profiles, root_setup
+    steam_amd.ledger      MeanReduction, FluxAdvanceLedger, PatternLedger,
+                          BoundedAddLedger, ProjectionLedger, RunLedger
     steam_amd.streaming   tile geometry, tiled regrid, tiled flux cascade
     steam_amd.batch       many uneven realisations per launch
     steam_amd.serve       continuous-batching tile server

@@ -4,12 +4,13 @@ steam/simulate.py).
 Module-level names are looked up at call time, as in the reference, so a
 caller may rebind ``CONVOLVE`` or ``_keyed_sparse_levy`` (the reference's test
 seams).  With both at their defaults ``_advance_flux`` runs the fused HIP
-advance (noise regenerated from counters inside the FFT load pass, clip and
-realized sums inside the overlap-add pass); with either rebound it composes
-the seams instead, still on the device.
+advance (the class noise drawn from counters by its own kernel, or taken
+pregenerated, and read by the first FFT pass; clip and realized sums inside
+the overlap-add pass); with either rebound it composes the seams instead,
+still on the device.
 
 Every hot-path call takes ``device``; only 'cuda' is accepted (there is no CPU
-path in this build).
+path).
 """
 
 import ctypes

@@ -166,8 +166,8 @@ def convolve_fft_xy_oa_z(field, kernel, device='cuda', precision='plain'):
     """Circular x/y, linear zero-padded z convolution (steam/utils.py:293-427).
 
     ``field`` (nx, ny, nz) float32 is not modified; returns a new array of the
-    caller's type.  ``device`` other than 'cuda' is an error (this build runs
-    the operator on the MI355X only); 'cuda' without a GPU raises
+    caller's type.  ``device`` other than 'cuda' is an error (the operator
+    runs on the MI355X only); 'cuda' without a GPU raises
     RuntimeError, as in the reference.
     """
     check_device(device)
