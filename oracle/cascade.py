@@ -12,6 +12,8 @@ against.  Only ``tests/`` may import it.
                         unconditional 60-step bisection on violating levels)
   compose_output        steam/simulate.py:3558-3599
   compose_flux_output   steam/simulate.py:3602-3649
+  refine_with_plan      steam/simulate.py:4092-4360 (the nest's cascade from
+                        its parent's state and per-class increments)
 
 Arithmetic follows the reference's float32 operation order on the fields and
 float64 for every reduction and solved scalar; CONVOLVE is the float64
@@ -119,10 +121,11 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
                  deficit_qt=None, deficit_flux=None, inner_windows=None, world_origins=None,
                  world_shapes=None, flux_noise_scale=None, turbulon_shape='mexican_hat',
                  zero_bottom=True, zero_top=True, comp_k=None, kernel=None,
-                 flux_ledgers=None):
+                 flux_ledgers=None, increments=None):
     """Returns (h_pert, qt_pert, flux, deficit_h, deficit_qt, deficit_flux,
     final_grid_info, records) with records a list of per-class dicts (flux
-    ledger, product norms, bounded-add ledgers)."""
+    ledger, product norms, bounded-add ledgers).  ``increments`` (a list)
+    receives each class's applied h, qt and flux increments."""
     s_x, s_y, s_z = sparsity_factors
     n_classes = len(grids['k'])
     n_zero = int(round(2 * s_z * min_distance_to_ground))
@@ -193,6 +196,8 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
             sparsity_factors, n_zero, zero_bottom, zero_top, window=window,
             dtype=np.float32, ledger=led_in)
         rec['flux'] = led
+        if increments is not None:
+            increments.append({'flux': fl - before})
         if has_deficit:
             dfl = dfl + deficit_i * (fl - before)
         fields = {'h': h, 'qt': qt}
@@ -208,6 +213,8 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
             new, applied, bled = S.bounded_amplitude_add(pert, mean, inc, lo, hi, window)
             rec['bounded_add'][name] = bled
             fields[name] = new
+            if increments is not None:
+                increments[-1][name] = applied
             if deficits[name] is not None:
                 deficits[name] = deficits[name] + deficit_i * applied
         h, qt = fields['h'], fields['qt']
@@ -221,3 +228,59 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
     final = {'nx': shape[0], 'ny': shape[1], 'nz': shape[2], 'dx': dx_k, 'dy': dy_k,
              'dz': grids['dz_arrays'][n_classes - 1], 'z': grids['z_arrays'][n_classes - 1]}
     return h, qt, fl, dh, dqt, dfl, final, records
+
+
+def refine_with_plan(parent, plan):
+    """steam/simulate.py:4092-4360 given the host plan of the nest (its
+    grids, windows, anchoring, ladders and parent slab indices): the parent
+    slab, the inherited deficits carried down the parent's regrid chain, the
+    cascade over the nest's classes, the trim to the inner window and z
+    range, and the composition.  Returns the six fields."""
+    ix = np.ix_(plan['x_indices'], plan['y_indices'], plan['z_indices'])
+    state = {k: np.ascontiguousarray(np.asarray(parent[k], dtype=F32)[ix])
+             for k in ('h_perturbation', 'qt_perturbation', 'flux_state')}
+    inherited = {}
+    for name in ('h', 'qt', 'flux'):
+        acc = None
+        for j in range(int(parent['n_classes_consumed'])):
+            dj = np.asarray(plan['comp_inherited'][j], dtype=F32) - F32(1.0)
+            if acc is None and not np.any(dj):
+                continue
+            inc = np.asarray(parent['class_increments'][j][name], dtype=F32)
+            if acc is None:
+                acc = np.zeros_like(inc)
+            elif acc.shape != inc.shape:
+                acc = _regrid32(acc, inc.shape)
+            acc = acc + dj * inc
+        inherited[name] = None if acc is None else np.ascontiguousarray(acc[ix])
+    h_min, h_max, qt_min, qt_max = parent['bounds']
+    h, qt, fl, dh, dqt, dfl, final, _ = cascade_loop(
+        parent['h_profile'], parent['qt_profile'], parent['z_profile'], plan['grids'],
+        plan['C_h_k'], plan['C_qt_k'], plan['b_h_k'], plan['b_qt_k'], plan['aspect_k'],
+        h_min, h_max, qt_min, qt_max, int(parent['min_distance_to_ground']),
+        plan['sparsity_factors'], plan['class_seeds'],
+        n_scale_classes_per_dyad=int(parent['n_scale_classes_per_dyad']),
+        h_perturbation=state['h_perturbation'], qt_perturbation=state['qt_perturbation'],
+        flux=state['flux_state'], deficit_h=inherited['h'], deficit_qt=inherited['qt'],
+        deficit_flux=inherited['flux'], inner_windows=plan['inner_windows'],
+        world_origins=plan['world_origins'], world_shapes=plan['world_shapes'],
+        flux_noise_scale=float(parent['flux_noise_scale']),
+        turbulon_shape=plan['turbulon_shape'], zero_bottom=plan['at_ground'],
+        zero_top=plan['at_top'], comp_k=plan['comp_k'])
+    wx0, wx1, wy0, wy1 = plan['inner_windows'][-1]
+    z_full = np.asarray(final['z'])
+    zi = np.where((z_full >= plan['z_min'] - 1e-6) & (z_full < plan['z_max'] - 1e-6))[0]
+    sl = (slice(wx0, wx1), slice(wy0, wy1), slice(int(zi[0]), int(zi[-1]) + 1))
+
+    def inner(a):
+        return None if a is None else np.ascontiguousarray(a[sl])
+
+    z_final = z_full[sl[2]].astype(F32)
+    zp = np.asarray(parent['z_profile'], dtype=np.float64)
+    hm = np.interp(z_final, zp, parent['h_profile']).astype(F32)
+    qm = np.interp(z_final, zp, parent['qt_profile']).astype(F32)
+    H, _ = compose_output(inner(h), inner(dh), hm, h_min, h_max)
+    Q, _ = compose_output(inner(qt), inner(dqt), qm, qt_min, qt_max)
+    F, _ = compose_flux_output(inner(fl), inner(dfl))
+    return {'h': H, 'qt': Q, 'flux': F, 'h_perturbation': inner(h),
+            'qt_perturbation': inner(qt), 'flux_state': inner(fl)}

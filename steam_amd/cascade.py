@@ -166,6 +166,13 @@ def _crop(carried, axis, keep, start):
     return out
 
 
+def _record(increment_dir, increments, i, name, arr):
+    if increment_dir is not None:
+        np.save(os.path.join(increment_dir, f"c{i:02d}_{name}.npy"), arr)
+    if increments is not None:
+        increments[-1][name] = arr
+
+
 def _convolve(W, kernel_t, kernel_np, device):
     conv = _sim.CONVOLVE
     if conv is convolve_fft_xy_oa_z:
@@ -194,13 +201,16 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
                  deficit_qt=None, deficit_flux=None, inner_windows=None, world_origins=None,
                  world_shapes=None, ledger=None, flux_noise_scale=None,
                  turbulon_shape='mexican_hat', zero_bottom=True, zero_top=True, device='cuda',
-                 increment_dir=None, comp_k=None, as_tensors=None):
+                 increment_dir=None, comp_k=None, as_tensors=None, increments=None):
     """Run the turbulon cascade over every size class on the GPU
     (steam/simulate.py:1566-2049); same arguments, errors and 8-tuple result
     (h_perturbation, qt_perturbation, flux, deficit_h, deficit_qt,
     deficit_flux, final_grid_info, ledger).  Fields come back as NumPy arrays
     unless the caller passed device tensors (or ``as_tensors`` is True).
-    A recorded ``ledger`` replays the run apply-only."""
+    A recorded ``ledger`` replays the run apply-only.  ``increment_dir``
+    receives each class's applied increments as c{i:02d}_{h,qt,flux}.npy,
+    as in the reference; ``increments`` (a list) receives them in memory,
+    one dict of NumPy arrays per class."""
     check_device(device)
     s_x, s_y, s_z = sparsity_factors
     n_classes = len(grids['k'])
@@ -293,7 +303,10 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
                 dqt = torch.zeros_like(qt)
             if dfl is None:
                 dfl = torch.zeros_like(fl)
-        need_increment = has_deficit or increment_dir is not None
+        record = increment_dir is not None or increments is not None
+        if increments is not None:
+            increments.append({})
+        need_increment = has_deficit or record
         before = fl.clone() if need_increment else None
         S_k, class_ledger.flux, _ = _advance(
             fl, region, kernel_t, kernel_np, c, n_scale_classes_per_dyad, sparsity_factors,
@@ -301,9 +314,8 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
         if need_increment:
             if has_deficit:
                 deficit_accumulate(dfl, fl, deficit_i, before=before)
-            if increment_dir is not None:
-                np.save(os.path.join(increment_dir, f"c{i:02d}_flux.npy"),
-                        (fl - before).cpu().numpy())
+            if record:
+                _record(increment_dir, increments, i, 'flux', (fl - before).cpu().numpy())
             del before
         zdiff = ZDifferences(z_k)
         reducer = LevelReducer(shape)
@@ -323,9 +335,8 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
                 ledger=class_ledger.bounded_add.get(name))
             if dfield is not None:
                 deficit_accumulate(dfield, increment, deficit_i)
-            if increment_dir is not None:
-                np.save(os.path.join(increment_dir, f"c{i:02d}_{name}.npy"),
-                        increment.cpu().numpy())
+            if record:
+                _record(increment_dir, increments, i, name, increment.cpu().numpy())
             del increment
         del S_k
         prev = {'dx': dx_k, 'dy': dy_k, 'dz': float(grids['dz'][i]),
@@ -349,26 +360,30 @@ def simulate_fields(h_profile, qt_profile, nx, ny, dx, dy, outer_scale, spherosc
                     turbulon_shape='mexican_hat',
                     anisotropy='piecewise_isotropic_below_spheroscale', device='cuda',
                     hurst_horizontal=None, haar_to_mhat=None, bound_buffer_multiple=None,
-                    ledger=None, as_tensors=False):
+                    ledger=None, as_tensors=False, save_for_refinement=False):
     """A root run's fields, as ``simulate`` computes them before writing its
     file (steam/simulate.py:946-1444): set-up, the cascade on the GPU, and
     the composition of the written h, qt and flux.  Returns a dict with
     'h', 'qt', 'flux' (written fields), 'h_perturbation', 'qt_perturbation',
-    'flux_state' (the cascade state), 'ledger', 'grids', 'setup'.  The
-    NetCDF file, its attributes and the streamed/resumable drivers are not
-    part of this path."""
+    'flux_state' (the cascade state), 'ledger', 'grids', 'setup', and with
+    ``save_for_refinement`` a 'parent' record (what the reference's file
+    holds for refine: the state, the per-class increments, the run's
+    attributes).  The NetCDF file itself and the streamed/resumable drivers
+    are not part of this path."""
     from .params import root_setup
     check_device(device)
     st = root_setup(h_profile, qt_profile, nx, ny, dx, dy, outer_scale, spheroscale,
                     domain_height, profile_dz, sparsity_factors, n_scale_classes_per_dyad,
                     seed, h_min, h_max, qt_min, qt_max, min_distance_to_ground, anisotropy,
                     hurst_horizontal, haar_to_mhat, bound_buffer_multiple)
+    increments = [] if save_for_refinement else None
     (h_pert, qt_pert, flux_state, dh, dqt, dfl, final, run_ledger) = cascade_loop(
         st['h_profile'], st['qt_profile'], st['z_profile'], st['grids'], st['C_h_k'],
         st['C_qt_k'], st['b_h_k'], st['b_qt_k'], st['aspect_k'], h_min, h_max, qt_min,
         qt_max, min_distance_to_ground, sparsity_factors, st['class_seeds'],
         n_scale_classes_per_dyad=n_scale_classes_per_dyad, turbulon_shape=turbulon_shape,
-        device=device, comp_k=st['comp_k'], ledger=ledger, as_tensors=True)
+        device=device, comp_k=st['comp_k'], ledger=ledger, as_tensors=True,
+        increments=increments)
     z_final = final['z'].astype(np.float32)
     h_mean = np.interp(z_final, st['z_profile'], st['h_profile']).astype(np.float32)
     qt_mean = np.interp(z_final, st['z_profile'], st['qt_profile']).astype(np.float32)
@@ -384,7 +399,39 @@ def simulate_fields(h_profile, qt_profile, nx, ny, dx, dy, outer_scale, spherosc
         run_ledger.projection['qt'] = proj_qt
     out = {'h': h_out, 'qt': qt_out, 'flux': flux_out, 'h_perturbation': h_pert,
            'qt_perturbation': qt_pert, 'flux_state': flux_state}
+    parent = None
+    if save_for_refinement:
+        g = st['grids']
+        nxf = int(h_pert.shape[0])
+        nyf = int(h_pert.shape[1])
+        parent = {
+            'h_perturbation': h_pert.cpu().numpy(),
+            'qt_perturbation': qt_pert.cpu().numpy(),
+            'flux_state': flux_state.cpu().numpy(),
+            'x0': 0.0, 'y0': 0.0, 'nx': nxf, 'ny': nyf,
+            'dx': (nx * dx) / nxf, 'dy': (ny * dy) / nyf,
+            # the file stores z in float32
+            'z': z_final.astype(np.float64),
+            'h_profile': st['h_profile'], 'qt_profile': st['qt_profile'],
+            'z_profile': st['z_profile'], 'spheroscale_profile': st['spheroscale_profile'],
+            'domain_height': float(domain_height), 'domain_z_min': 0.0,
+            'profile_dz': float(profile_dz), 'bounds': (h_min, h_max, qt_min, qt_max),
+            'min_distance_to_ground': min_distance_to_ground,
+            'n_scale_classes_per_dyad': n_scale_classes_per_dyad,
+            'root_domain_x': nx * dx, 'root_domain_y': ny * dy,
+            'root_domain_height': float(domain_height), 'root_outer_scale': float(outer_scale),
+            'root_seed': st['seed'], 'n_classes_consumed': len(g['k']),
+            'H_h': st['hurst_horizontal'], 'haar_to_mhat': st['haar_to_mhat'],
+            'flux_noise_scale': _sim.FLUX_SCALE,
+            'bound_buffer_multiple': st['bound_buffer_multiple'],
+            'sparsity_factors': tuple(sparsity_factors), 'turbulon_shape': turbulon_shape,
+            'anisotropy': anisotropy, 'periodic_x': True, 'periodic_y': True,
+            'class_increments': [dict(inc, z=np.asarray(g['z_arrays'][j], dtype=np.float64))
+                                 for j, inc in enumerate(increments)],
+        }
     if not as_tensors:
         out = {k: v.cpu().numpy() for k, v in out.items()}
     out.update(ledger=run_ledger, grids=st['grids'], setup=st)
+    if parent is not None:
+        out['parent'] = parent
     return out

@@ -290,4 +290,8 @@ def root_setup(h_profile, qt_profile, nx, ny, dx, dy, outer_scale, spheroscale,
         'n_scale_classes_per_dyad': n_scale_classes_per_dyad,
         'min_distance_to_ground': min_distance_to_ground,
         'support_factor': SUPPORT_FACTOR,
+        'hurst_horizontal': H_h if hurst_horizontal is None else float(hurst_horizontal),
+        'haar_to_mhat': HAAR_TO_MHAT if haar_to_mhat is None else float(haar_to_mhat),
+        'bound_buffer_multiple': (BOUND_BUFFER_MULTIPLE if bound_buffer_multiple is None
+                                  else float(bound_buffer_multiple)),
     }

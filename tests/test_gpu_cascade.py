@@ -211,3 +211,22 @@ def test_projection_bisection_equals_one_step_per_pass_bisection():
     ok = ~np.isnan(led.mu)
     assert ok.sum() >= 4
     np.testing.assert_array_equal(led.mu[ok], mu[ok])
+
+
+def test_gpu_nest_matches_reference_fixture_and_replays():
+    """refine of root_dyadic over x, y in [4, 12) at dx = 62.5 (the
+    reference's pinned nest): the root run and the nest both on the GPU."""
+    from steam_amd import cascade as C
+    from steam_amd.refine import refine_fields
+    root = C.simulate_fields(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0,
+                             800.0, 30.0, seed=42, save_for_refinement=True)
+    parent = root['parent']
+    assert len(parent['class_increments']) == 4
+    out = refine_fields(parent, 4, 12, 4, 12, 62.5, 62.5)
+    ref = np.load(os.path.join(GOLDEN, "nest.npz"))
+    for field, tol in TOL.items():
+        assert out[field].shape == ref[field].shape, field
+        assert rel(out[field], ref[field]) < tol, (field, rel(out[field], ref[field]))
+    again = refine_fields(parent, 4, 12, 4, 12, 62.5, 62.5, ledger=out['ledger'])
+    for field in FIELDS:
+        np.testing.assert_array_equal(again[field], out[field], err_msg=field)

@@ -233,3 +233,82 @@ def test_compose_closed_forms():
     assert fo.min() >= 0
     assert fo.astype(np.float64).mean() == pytest.approx(flux.astype(np.float64).mean(),
                                                          rel=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# refine: the nest of root_dyadic the reference pinned
+# (tests/heavy/make_refactor_references.py:96-103: x, y in [4, 12), dx = k_finest/4)
+# ---------------------------------------------------------------------------
+
+def oracle_parent(st):
+    """The parent record (what the reference's file holds for refine) of
+    the oracle's root run."""
+    hmin, hmax, qmin, qmax = st['bounds']
+    g = st['grids']
+    incs = []
+    h, qt, fl, _, _, _, fin, _ = OC.cascade_loop(
+        st['h_profile'], st['qt_profile'], st['z_profile'], g, st['C_h_k'], st['C_qt_k'],
+        st['b_h_k'], st['b_qt_k'], st['aspect_k'], hmin, hmax, qmin, qmax,
+        st['min_distance_to_ground'], st['sparsity_factors'], st['class_seeds'],
+        n_scale_classes_per_dyad=st['n_scale_classes_per_dyad'], comp_k=st['comp_k'],
+        increments=incs)
+    nx, ny = h.shape[0], h.shape[1]
+    return {'h_perturbation': h, 'qt_perturbation': qt, 'flux_state': fl,
+            'x0': 0.0, 'y0': 0.0, 'nx': nx, 'ny': ny, 'dx': 2000.0 / nx, 'dy': 2000.0 / ny,
+            'z': fin['z'].astype(np.float32).astype(np.float64),
+            'h_profile': st['h_profile'], 'qt_profile': st['qt_profile'],
+            'z_profile': st['z_profile'], 'spheroscale_profile': st['spheroscale_profile'],
+            'domain_height': 800.0, 'domain_z_min': 0.0, 'profile_dz': 30.0,
+            'bounds': st['bounds'], 'min_distance_to_ground': st['min_distance_to_ground'],
+            'n_scale_classes_per_dyad': st['n_scale_classes_per_dyad'],
+            'root_domain_x': 2000.0, 'root_domain_y': 2000.0, 'root_domain_height': 800.0,
+            'root_outer_scale': 2000.0, 'root_seed': st['seed'],
+            'n_classes_consumed': len(g['k']), 'H_h': st['hurst_horizontal'],
+            'haar_to_mhat': st['haar_to_mhat'], 'flux_noise_scale': 0.2085,
+            'bound_buffer_multiple': st['bound_buffer_multiple'],
+            'sparsity_factors': st['sparsity_factors'], 'turbulon_shape': 'mexican_hat',
+            'anisotropy': 'piecewise_isotropic_below_spheroscale',
+            'periodic_x': True, 'periodic_y': True,
+            'class_increments': [dict(inc, z=np.asarray(g['z_arrays'][j], np.float64))
+                                 for j, inc in enumerate(incs)]}
+
+
+def test_oracle_nest_matches_reference_fixture(monkeypatch):
+    from steam_amd.refine import plan_nest
+    parent = oracle_parent(fixture_setup((1, 1, 1), 1))
+    plan = plan_nest(parent, 4, 12, 4, 12, 62.5, 62.5)
+    # one nest class (k = 125) on a 28x28x15 padded grid, inner window 16x16
+    assert plan['n_classes'] == 1 and plan['inner_windows'] == [(6, 22, 6, 22)]
+    out = OC.refine_with_plan(parent, plan)
+    ref = np.load(os.path.join(GOLDEN, "nest.npz"))
+    for field, tol in TOL.items():
+        assert out[field].shape == ref[field].shape, field
+        assert rel(out[field], ref[field]) < tol, (field, rel(out[field], ref[field]))
+    # planted mistakes: the nest's noise laid at the world origin instead of
+    # the restriction of the root's; the inherited deficits dropped
+    bad = dict(plan, world_origins=[(0, 0, 0)] * plan['n_classes'])
+    assert rel(OC.refine_with_plan(parent, bad)['h_perturbation'],
+               ref['h_perturbation']) > 100 * TOL['h_perturbation']
+    lean = dict(parent, class_increments=[{k: np.zeros_like(v) if k != 'z' else v
+                                           for k, v in c.items()}
+                                          for c in parent['class_increments']])
+    assert rel(OC.refine_with_plan(lean, plan)['flux'], ref['flux']) > 100 * TOL['flux']
+
+
+def test_plan_nest_rejects_what_the_reference_rejects():
+    from steam_amd.refine import plan_nest
+    st = fixture_setup((1, 1, 1), 1)
+    parent = {'n_scale_classes_per_dyad': 1, 'sparsity_factors': (1, 1, 1),
+              'turbulon_shape': 'mexican_hat',
+              'anisotropy': 'piecewise_isotropic_below_spheroscale', 'nx': 16, 'ny': 16,
+              'dx': 125.0, 'dy': 125.0, 'z': np.arange(10) * 80.0,
+              'z_profile': st['z_profile'], 'spheroscale_profile': st['spheroscale_profile'],
+              'domain_height': 800.0, 'root_outer_scale': 2000.0, 'n_classes_consumed': 4}
+    with pytest.raises(ValueError):          # no class below the parent's finest
+        plan_nest(parent, 4, 12, 4, 12, 125.0, 125.0)
+    with pytest.raises(ValueError):          # empty nest
+        plan_nest(parent, 4, 4, 4, 12, 62.5, 62.5)
+    with pytest.raises(ValueError):          # above the parent's top
+        plan_nest(parent, 4, 12, 4, 12, 62.5, 62.5, z_max=900.0)
+    with pytest.raises(ValueError):
+        plan_nest(dict(parent, anisotropy='bogus'), 4, 12, 4, 12, 62.5, 62.5)
