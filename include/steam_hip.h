@@ -197,6 +197,10 @@ typedef struct steam_flux_ledger {
                                   steam/streaming.py:1233-1234; the in-RAM
                                   advance counts the whole array,
                                   steam/simulate.py:2198) */
+#define STEAM_SK_RAW 8 /* leave noise * entering flux in sk_dev, without the
+                          1/<|noise|> scale: the streamed passes measure the
+                          pattern on the raw product before the class's mean
+                          |noise| is known (steam/streaming.py:1166-1196) */
 
 /* Multiplier-noise field of one advance, (nx, ny, nz) floats into out_dev
  * (the VALU-bound half of the advance).  It does not read the flux, so a

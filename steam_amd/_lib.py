@@ -153,6 +153,7 @@ KERNEL_FAMILIES = {'zfwd': 0, 'ypass': 1, 'xpass': 2, 'zinv': 3,
 DEFER_RESCALE = 1
 CHAIN_PENDING = 2
 CLIP_IN_WINDOW = 4
+SK_RAW = 8
 
 EXPORTED = tuple(name for name, _, _ in _SIGNATURES)
 

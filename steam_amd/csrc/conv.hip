@@ -426,6 +426,7 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
   const int defer_rescale = (flags & STEAM_DEFER_RESCALE) != 0;
   const int chain_pending = (flags & STEAM_CHAIN_PENDING) != 0;
   const int clip_win = (flags & STEAM_CLIP_IN_WINDOW) != 0;
+  const bool sk_scale = sk_dev && !(flags & STEAM_SK_RAW);
   if (!p || !spectrum_dev || !workspace_dev || !flux_dev)
     return set_error(STEAM_EINVAL, "steam_flux_advance: NULL argument");
   int rc = check_noise_params(noise);
@@ -469,10 +470,12 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
     hipLaunchKernelGGL((finalize_kernel<SingleSrc>), dim3(1), dim3(FFT_NTH), 0, s, src);
   }
   STEAM_LAUNCH_CHECK("finalize_kernel");
-  if (!defer_rescale || sk_dev) {
+  if (!defer_rescale || sk_scale) {
+    PassDesc dp = d;
+    if (!sk_scale) dp.sk = nullptr;
     ProfScope ps(STEAM_KF_POST, s);
     hipLaunchKernelGGL((post_kernel<SingleSrc>), dim3(p->wg_post), dim3(FFT_NTH), 0,
-                       s, src, defer_rescale ? 0 : 1);
+                       s, SingleSrc{dp}, defer_rescale ? 0 : 1);
     STEAM_LAUNCH_CHECK("post_kernel");
   }
   if (ledger && mode == STEAM_MEASURE) {

@@ -184,3 +184,177 @@ def streamed_flux_cascade(grids, tiles, sparsity_factors=(1, 1, 1), seed=None,
     diagnostics = {'flux_noise_scale': float(c), 'steps': steps, 'ledger': ledger,
                    'halo': (hx, hy)}
     return (state if return_tensor else state.cpu().numpy()), diagnostics
+
+
+def streamed_cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k,
+                          b_qt_k, aspect_k, h_min, h_max, qt_min, qt_max,
+                          min_distance_to_ground, sparsity_factors, class_seeds, tiles,
+                          n_scale_classes_per_dyad=1, flux_noise_scale=None,
+                          turbulon_shape='mexican_hat', zero_bottom=True, zero_top=True,
+                          device='cuda', comp_k=None, as_tensors=False):
+    """The whole root cascade (flux, h, qt and the compensation deficits)
+    tile by tile: the passes of steam/streaming.py:850-1447 over a state
+    held in HBM.
+
+    Per class: the carried fields are regridded window by window; a first
+    pass over the tiles (tile plus halo, in world raster order) advances the
+    flux with its window set to the interior and measures, on the raw
+    product noise * entering flux, the joint pattern's level sums; a second
+    pass forms S_k with the class's mean |noise|, normalises the pattern by
+    the measured level means and amplitude C_k, convolves it per tile and
+    writes the interiors into the class increments; the per-level passes
+    (the bounded add of each scalar, the flux rescale, the deficits) then
+    run on the whole fields, which a level at a time is what the reference's
+    plane pass does.  ``tiles`` is (tiles_x, tiles_y) for every class or one
+    pair per class.  Returns the 8-tuple of ``cascade_loop``.  The disk
+    store, resume protocol and tiling planner are not part of this path.
+    """
+    from .cascade import _own, deficit_accumulate
+    from .params import _interpolation_compensation
+    from .scalars import (LevelReducer, ZDifferences, bounded_amplitude_add, joint_pattern,
+                          normalize_pattern, product_norm)
+    from .utils import conv_operator, convolve_fft_xy_oa_z, kernel_key
+    from .ledger import PatternLedger
+    from .simulate import FLUX_SCALE, SUPPORT_FACTOR, generate_noise_device
+
+    check_device(device)
+    s_x, s_y, s_z = sparsity_factors
+    n_classes = len(grids['k'])
+    n_zero = int(round(2 * s_z * min_distance_to_ground))
+    c = FLUX_SCALE if flux_noise_scale is None else flux_noise_scale
+    if comp_k is None:
+        k_fin = float(grids['k'][n_classes - 1])
+        comp_k = [np.full(int(grids['nz'][i]), np.float32(
+            _interpolation_compensation(2.0 * grids['k'][i] / k_fin)))
+            for i in range(n_classes)]
+    if isinstance(tiles, (list, tuple)) and len(tiles) and isinstance(tiles[0], (list, tuple)):
+        per_class_tiles = list(tiles)
+        if len(per_class_tiles) != n_classes:
+            raise ValueError("one (tiles_x, tiles_y) pair per class expected")
+    else:
+        per_class_tiles = [tuple(tiles)] * n_classes
+    keys = [class_key(sq) for sq in class_seeds]
+    kt = _turbulon_envelope(1, 1 / (2 * s_x), 1 / (2 * s_y), 1 / (2 * s_z),
+                            support_factor=SUPPORT_FACTOR, shape=turbulon_shape,
+                            as_tensor=True)
+    hx, hy = class_halo(kt.shape)
+    ledger = RunLedger(n_classes)
+    z_profile = np.asarray(z_profile, dtype=np.float64)
+    fields = None
+    for i in range(n_classes):
+        shape = (int(grids['nx'][i]), int(grids['ny'][i]), int(grids['nz'][i]))
+        dx_k, dy_k = float(grids['dx'][i]), float(grids['dy'][i])
+        z_k = grids['z_arrays'][i]
+        tx = min(int(per_class_tiles[i][0]), shape[0])
+        ty = min(int(per_class_tiles[i][1]), shape[1])
+        windows = tile_windows(shape[0], shape[1], tx, ty)
+        class_ledger = ledger.classes[i]
+        # regrid of every carried field, window by window
+        if fields is None:
+            fields = {'h': torch.zeros(shape, dtype=torch.float32, device='cuda'),
+                      'qt': torch.zeros(shape, dtype=torch.float32, device='cuda'),
+                      'flux': torch.ones(shape, dtype=torch.float32, device='cuda')}
+        elif tuple(fields['h'].shape) != shape:
+            new = {}
+            for name, src in fields.items():
+                dst = torch.empty(shape, dtype=torch.float32, device='cuda')
+                for w in windows:
+                    dst[w[0]:w[1], w[2]:w[3]] = regrid_window(src, shape, w)
+                new[name] = dst
+            fields = new
+        deficit_i = np.asarray(comp_k[i], dtype=np.float32) - np.float32(1.0)
+        if bool(np.any(deficit_i)):
+            for name in ('dh', 'dqt', 'dflux'):
+                if name not in fields:
+                    fields[name] = torch.zeros(shape, dtype=torch.float32, device='cuda')
+        means = {'h': np.interp(z_k, z_profile, h_profile).astype(np.float32),
+                 'qt': np.interp(z_k, z_profile, qt_profile).astype(np.float32)}
+        scal = {'h': (C_h_k[i], b_h_k[i], h_min, h_max),
+                'qt': (C_qt_k[i], b_qt_k[i], qt_min, qt_max)}
+        entering_state = fields['flux']
+        flux_next = torch.empty_like(entering_state)
+        entering = MeanReduction(np.float64(0.0), 0)
+        noise_abs = MeanReduction(np.float64(0.0), 0)
+        realized = MeanReduction(np.float64(0.0), 0)
+        n_clipped = 0
+        raw_pattern = {}
+        raws = []
+        zdiff = ZDifferences(z_k)
+        # pass 1: advance the flux per tile, measure the pattern on the raw product
+        for (x_lo, x_hi, y_lo, y_hi) in windows:
+            tile = read_wrapped(entering_state, x_lo - hx, x_hi + hx, y_lo - hy, y_hi + hy)
+            region = NoiseRegion(keys[i], (x_lo - hx, y_lo - hy, 0), shape)
+            inner = (hx, hx + (x_hi - x_lo), hy, hy + (y_hi - y_lo))
+            raw = torch.empty_like(tile)
+            led, _ = advance_flux_device(
+                tile, region, kt, c, n_scale_classes_per_dyad, sparsity_factors, n_zero,
+                zero_bottom, zero_top, window=inner, defer_rescale=True,
+                clip_in_window=True, sk_t=raw, sk_raw=True)
+            entering = entering + led.entering
+            noise_abs = noise_abs + led.noise_abs
+            realized = realized + led.realized
+            n_clipped += led.n_clipped
+            flux_next[x_lo:x_hi, y_lo:y_hi] = tile[inner[0]:inner[1], inner[2]:inner[3]]
+            reducer = LevelReducer(tile.shape)
+            for name in ('h', 'qt'):
+                _, b_i, lo, hi = scal[name]
+                pert = read_wrapped(fields[name], x_lo - hx, x_hi + hx, y_lo - hy, y_hi + hy)
+                W = joint_pattern(pert, means[name], raw, b_i, aspect_k[i], z_k, dx_k, dy_k,
+                                  lo, hi, zdiff=zdiff)
+                part = product_norm(W, inner, reducer)
+                raw_pattern[name] = part if name not in raw_pattern else raw_pattern[name] + part
+            raws.append(raw)
+        class_ledger.flux = FluxAdvanceLedger(entering, noise_abs, realized, n_clipped)
+        if not realized.mean > 0:
+            raise RuntimeError(
+                f"class {i}: the whole domain clipped to zero; the resident cascade "
+                f"flattens the flux to its entering mean, the streamed one does not "
+                f"(as in the reference)")
+        mean_abs = noise_abs.mean
+        divisor = mean_abs if mean_abs > 0 else 1.0
+        for name in ('h', 'qt'):
+            class_ledger.pattern[name] = PatternLedger(raw_pattern[name].level_total / divisor,
+                                                       raw_pattern[name].level_count)
+        # pass 2: S_k, the normalised pattern, its convolution per tile
+        lib = _lib.load()
+        incs = {name: torch.empty(shape, dtype=torch.float32, device='cuda')
+                for name in ('h', 'qt')}
+        inv = float(np.float32(1.0 / mean_abs)) if mean_abs > 0 else 1.0
+        for t, (x_lo, x_hi, y_lo, y_hi) in enumerate(windows):
+            sk = raws[t]
+            if mean_abs > 0:
+                _lib.check(lib.steam_scale_inplace(_lib.ptr(sk), sk.numel(), inv,
+                                                   _lib.stream_handle()), "steam_scale_inplace")
+            for name in ('h', 'qt'):
+                C_i, b_i, lo, hi = scal[name]
+                pert = read_wrapped(fields[name], x_lo - hx, x_hi + hx, y_lo - hy, y_hi + hy)
+                W = joint_pattern(pert, means[name], sk, b_i, aspect_k[i], z_k, dx_k, dy_k,
+                                  lo, hi, zdiff=zdiff)
+                normalize_pattern(W, class_ledger.pattern[name].level_mean, C_i)
+                cand = convolve_fft_xy_oa_z(W, kt, device=device)
+                incs[name][x_lo:x_hi, y_lo:y_hi] = cand[hx:hx + (x_hi - x_lo),
+                                                        hy:hy + (y_hi - y_lo)]
+            raws[t] = None
+        del raws
+        # the per-level passes: flux rescale, bounded adds, deficits
+        sigma = class_ledger.flux.rescale
+        _lib.check(lib.steam_scale_inplace(_lib.ptr(flux_next), flux_next.numel(),
+                                           float(sigma), _lib.stream_handle()),
+                   "steam_scale_inplace")
+        if 'dflux' in fields:
+            deficit_accumulate(fields['dflux'], flux_next, deficit_i, before=entering_state)
+        fields['flux'] = flux_next
+        for name, dname in (('h', 'dh'), ('qt', 'dqt')):
+            _, _, lo, hi = scal[name]
+            class_ledger.bounded_add[name] = bounded_amplitude_add(
+                fields[name], means[name], incs[name], lo, hi, record_applied=True)
+            if dname in fields:
+                deficit_accumulate(fields[dname], incs[name], deficit_i)
+        del incs
+    final = {'nx': shape[0], 'ny': shape[1], 'nz': shape[2], 'dx': dx_k, 'dy': dy_k,
+             'dz': grids['dz_arrays'][n_classes - 1], 'z': grids['z_arrays'][n_classes - 1]}
+    out = [fields['h'], fields['qt'], fields['flux'], fields.get('dh'), fields.get('dqt'),
+           fields.get('dflux')]
+    if not as_tensors:
+        out = [None if a is None else a.cpu().numpy() for a in out]
+    return (*out, final, ledger)

@@ -96,3 +96,56 @@ def test_regrid_window_writes_are_the_full_regrid():
     np.testing.assert_array_equal(out, full)
     with pytest.raises(ValueError):
         regrid(src, (24, 30, 9), out=torch.empty((24, 30, 9), device='cuda')[:, :15])
+
+
+# ---------------------------------------------------------------------------
+# the whole cascade (flux, h, qt, deficits) streamed over tiles
+# ---------------------------------------------------------------------------
+
+def _full_setup():
+    from steam_amd import params as P
+    z = np.arange(28) * 30.0
+    hp = 340e3 - 20e3 * (z / z.max())
+    qp = 0.018 - 0.016 * (z / z.max())
+    # classes 250 / 125 / 62.5 m on 16^2, 32^2, 64^2 columns: every class at
+    # least as wide as the 13-tap kernel
+    return P.root_setup(hp, qp, 64, 64, 31.25, 31.25, 250.0, 100.0, 800.0, 30.0, seed=7)
+
+
+def _cascade_args(st):
+    hmin, hmax, qmin, qmax = st['bounds']
+    return ((st['h_profile'], st['qt_profile'], st['z_profile'], st['grids'], st['C_h_k'],
+             st['C_qt_k'], st['b_h_k'], st['b_qt_k'], st['aspect_k'], hmin, hmax, qmin, qmax,
+             st['min_distance_to_ground'], st['sparsity_factors'], st['class_seeds']),
+            dict(comp_k=st['comp_k']))
+
+
+@pytest.fixture(scope="module")
+def full_resident():
+    from steam_amd.cascade import cascade_loop
+    st = _full_setup()
+    args, kw = _cascade_args(st)
+    return st, cascade_loop(*args, **kw)
+
+
+@pytest.mark.parametrize("tiles", [(1, 1), (2, 2), (3, 2), [(1, 1), (2, 3), (4, 4)]])
+def test_streamed_full_cascade_equals_resident(full_resident, tiles):
+    """Streamed vs resident within 1e-5 (the reference's own bound,
+    tests/test_streaming.py:293-324): h, qt, flux and the three deficits."""
+    from steam_amd.streaming import streamed_cascade_loop
+    st, res = full_resident
+    args, kw = _cascade_args(st)
+    out = streamed_cascade_loop(*args, tiles, **kw)
+    for k, (a, b) in enumerate(zip(out[:6], res[:6])):
+        assert (a is None) == (b is None), k
+        if a is None:
+            continue
+        rel = np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64))
+        assert rel < 1e-5, (tiles, k, rel)
+    for i in range(len(st['grids']['k'])):
+        for name in ('h', 'qt'):
+            np.testing.assert_array_equal(out[7].classes[i].pattern[name].level_count,
+                                          res[7].classes[i].pattern[name].level_count)
+            np.testing.assert_allclose(out[7].classes[i].pattern[name].level_total,
+                                       res[7].classes[i].pattern[name].level_total,
+                                       rtol=1e-6)
