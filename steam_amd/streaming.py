@@ -88,6 +88,117 @@ def class_halo(kernel_shape):
     return (int(kernel_shape[0]) // 2 + 1, int(kernel_shape[1]) // 2 + 1)
 
 
+# Largest x/y extent one convolution plan takes (one workgroup's LDS line).
+MAX_PLAN_AXIS = 4096
+# Tile-sized device buffers a streamed class body holds besides the
+# convolution workspace: the flux tile, the raw product, two scalar tiles,
+# the pattern and its convolution.
+TILE_FIELDS = 6
+
+
+class TilePlan:
+    """Which classes stream, over how many tiles (steam/streaming.py:635-660).
+    ``horizon`` is the first streamed class (earlier classes run as one tile);
+    ``tiles`` maps each streamed class to (tiles_x, tiles_y); ``halo`` is the
+    per-axis halo in cells; ``peak_bytes`` is the device memory the run holds
+    at most (resident state plus one tile's working set)."""
+
+    def __init__(self, horizon, tiles, halo, peak_bytes):
+        self.horizon = horizon
+        self.tiles = tiles
+        self.halo = halo
+        self.peak_bytes = peak_bytes
+
+    def for_class(self, index):
+        return self.tiles.get(index, (1, 1))
+
+
+def _working_bytes(nx, ny, nz, kz):
+    """Device bytes one class body needs over an (nx, ny, nz) tile beyond
+    the resident state: the convolution workspace and kernel spectrum at the
+    planner's transform length, and the tile buffers."""
+    from .batch import nfft_candidates
+    n_fft = nfft_candidates(nz, kz, 1)[0]
+    nzc = n_fft // 2 + 1
+    blocks = -(-nz // (n_fft - kz + 1))
+    plane = nx * ny
+    return int(blocks * nzc * plane * 8 + plane * nz * 4 + nzc * plane * 8
+               + TILE_FIELDS * plane * nz * 4)
+
+
+def _field_bytes(grids, index):
+    return int(grids['nx'][index]) * int(grids['ny'][index]) * int(grids['nz'][index]) * 4
+
+
+def plan_tiling(grids, kernel_shape, budget_bytes=None, force=None):
+    """Choose the first streamed class and each streamed class's tiles
+    (steam/streaming.py:672-796).  The halo is per axis: the kernel's
+    half-width on that axis plus one cell for the gradient stencil.  With
+    ``force`` = (horizon, tiles_x, tiles_y) the search is skipped.  Without
+    it a class streams when its working set exceeds ``budget_bytes`` or an
+    axis exceeds the largest plan extent; its tiles double along the longer
+    tile axis until the padded tile fits.  The resident state (the carried
+    fields of two consecutive classes and the class increments) is counted
+    in ``peak_bytes`` but never tiled: it stays in HBM."""
+    try:
+        kernel_shape = tuple(int(n) for n in kernel_shape)
+    except TypeError:
+        kernel_shape = ()
+    if len(kernel_shape) != 3:
+        raise ValueError(f"plan_tiling needs the kernel's full 3D shape to size the halo "
+                         f"per axis, got {kernel_shape!r}")
+    n_classes = len(grids['k'])
+    kz = kernel_shape[2]
+    halo = (kernel_shape[0] // 2 + 1, kernel_shape[1] // 2 + 1)
+    if force is not None:
+        horizon, tx, ty = (int(v) for v in force)
+        tiles = {index: (tx, ty) for index in range(horizon, n_classes)}
+    else:
+        if budget_bytes is None:
+            raise RuntimeError("plan_tiling was given no budget_bytes; pass one rather "
+                               "than guessing")
+        horizon = n_classes
+        for index in range(n_classes):
+            nx, ny, nz = (int(grids['nx'][index]), int(grids['ny'][index]),
+                          int(grids['nz'][index]))
+            if (nx > MAX_PLAN_AXIS or ny > MAX_PLAN_AXIS
+                    or _working_bytes(nx, ny, nz, kz) > budget_bytes):
+                horizon = index
+                break
+        tiles = {}
+        for index in range(horizon, n_classes):
+            nx, ny, nz = (int(grids['nx'][index]), int(grids['ny'][index]),
+                          int(grids['nz'][index]))
+            tx = ty = 1
+            while True:
+                tnx = -(-nx // tx) + 2 * halo[0]
+                tny = -(-ny // ty) + 2 * halo[1]
+                if (tnx <= MAX_PLAN_AXIS and tny <= MAX_PLAN_AXIS
+                        and _working_bytes(tnx, tny, nz, kz) <= budget_bytes):
+                    break
+                if nx // tx <= 2 * halo[0] and ny // ty <= 2 * halo[1]:
+                    raise MemoryError(
+                        f"class {index} ({nx}x{ny}x{nz}) cannot be tiled small enough to "
+                        f"fit {budget_bytes / 1024**3:.2f} GiB: the halo ({halo[0]}, "
+                        f"{halo[1]} cells per side) would dominate the tile")
+                if nx // tx >= ny // ty:
+                    tx *= 2
+                else:
+                    ty *= 2
+            tiles[index] = (min(tx, nx), min(ty, ny))
+    peak = 0
+    for index in range(n_classes):
+        tx, ty = tiles.get(index, (1, 1))
+        nx, ny, nz = (int(grids['nx'][index]), int(grids['ny'][index]),
+                      int(grids['nz'][index]))
+        tile = (-(-nx // tx) + 2 * halo[0] * (tx > 1 or ty > 1),
+                -(-ny // ty) + 2 * halo[1] * (tx > 1 or ty > 1))
+        previous = 6 * _field_bytes(grids, index - 1) if index > 0 else 0
+        resident = previous + 9 * _field_bytes(grids, index)
+        peak = max(peak, resident + _working_bytes(tile[0], tile[1], nz, kz))
+    return TilePlan(horizon, tiles, halo, peak)
+
+
 def streamed_flux_cascade(grids, tiles, sparsity_factors=(1, 1, 1), seed=None,
                           flux_noise_scale=None, n_scale_classes_per_dyad=1,
                           min_distance_to_ground=0, turbulon_shape='mexican_hat',
@@ -205,8 +316,8 @@ def streamed_cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k
     writes the interiors into the class increments; the per-level passes
     (the bounded add of each scalar, the flux rescale, the deficits) then
     run on the whole fields, which a level at a time is what the reference's
-    plane pass does.  ``tiles`` is (tiles_x, tiles_y) for every class or one
-    pair per class.  Returns the 8-tuple of ``cascade_loop``.  The disk
+    plane pass does.  ``tiles`` is a TilePlan (plan_tiling), (tiles_x,
+    tiles_y) for every class, or one pair per class.  Returns the 8-tuple of ``cascade_loop``.  The disk
     store, resume protocol and tiling planner are not part of this path.
     """
     from .cascade import _own, deficit_accumulate
@@ -227,7 +338,9 @@ def streamed_cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k
         comp_k = [np.full(int(grids['nz'][i]), np.float32(
             _interpolation_compensation(2.0 * grids['k'][i] / k_fin)))
             for i in range(n_classes)]
-    if isinstance(tiles, (list, tuple)) and len(tiles) and isinstance(tiles[0], (list, tuple)):
+    if isinstance(tiles, TilePlan):
+        per_class_tiles = [tiles.for_class(i) for i in range(n_classes)]
+    elif isinstance(tiles, (list, tuple)) and len(tiles) and isinstance(tiles[0], (list, tuple)):
         per_class_tiles = list(tiles)
         if len(per_class_tiles) != n_classes:
             raise ValueError("one (tiles_x, tiles_y) pair per class expected")

@@ -149,3 +149,15 @@ def test_streamed_full_cascade_equals_resident(full_resident, tiles):
             np.testing.assert_allclose(out[7].classes[i].pattern[name].level_total,
                                        res[7].classes[i].pattern[name].level_total,
                                        rtol=1e-6)
+
+
+def test_streamed_full_cascade_from_a_tile_plan(full_resident):
+    """A forced plan streams from class 1 on (class 0 as one tile)."""
+    from steam_amd.streaming import plan_tiling, streamed_cascade_loop
+    st, res = full_resident
+    plan = plan_tiling(st['grids'], (13, 13, 13), force=(1, 2, 2))
+    args, kw = _cascade_args(st)
+    out = streamed_cascade_loop(*args, plan, **kw)
+    for a, b in zip(out[:3], res[:3]):
+        rel = np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64))
+        assert rel < 1e-5

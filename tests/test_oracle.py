@@ -321,3 +321,36 @@ def test_package_tile_geometry_matches_oracle():
         ST.tile_bounds(3, 4)
     with pytest.raises(NotImplementedError):
         ST.axis_regrid_plan(8, 4, 0, 2)
+
+
+def test_plan_tiling():
+    """The planner (steam/streaming.py:672-796): forced plans, a budget that
+    fits all but the finest class, per-axis halos, the largest plan extent,
+    and the refusal when the halo would dominate."""
+    from steam_amd import streaming as ST
+    g = M.compute_all_grids([2000, 1000, 500, 250], 2000., 2000., 800., (1, 1, 1),
+                            np.full(28, 100.), np.arange(28) * 30.)
+    plan = ST.plan_tiling(g, (13, 13, 13), force=(2, 2, 2))
+    assert plan.horizon == 2 and set(plan.tiles) == {2, 3}
+    assert all(t == (2, 2) for t in plan.tiles.values()) and plan.for_class(0) == (1, 1)
+    assert plan.halo == (7, 7)
+    big = M.compute_all_grids([2000, 1000, 500, 250], 8000., 8000., 800., (1, 1, 1),
+                              np.full(28, 100.), np.arange(28) * 30.)
+    budget = ST._working_bytes(int(big['nx'][2]), int(big['ny'][2]), int(big['nz'][2]), 13)
+    plan = ST.plan_tiling(big, (13, 13, 13), budget_bytes=budget)
+    assert plan.horizon == 3 and plan.tiles[3][0] * plan.tiles[3][1] >= 2
+    tx, ty = plan.tiles[3]
+    assert ST._working_bytes(-(-64 // tx) + 14, -(-64 // ty) + 14, int(big['nz'][3]), 13) \
+        <= budget
+    assert plan.peak_bytes > 0
+    # anisotropic sparsity: the halo follows each axis's own kernel width
+    assert ST.plan_tiling(g, (37, 13, 13), force=(0, 1, 1)).halo == (19, 7)
+    with pytest.raises(MemoryError):
+        ST.plan_tiling(g, (13, 13, 13), budget_bytes=1000)
+    with pytest.raises(ValueError):
+        ST.plan_tiling(g, 13, force=(0, 1, 1))
+    # an axis beyond one plan's extent streams whatever the budget
+    wide = {'k': np.array([1.0]), 'nx': np.array([8192]), 'ny': np.array([64]),
+            'nz': np.array([16])}
+    plan = ST.plan_tiling(wide, (13, 13, 13), budget_bytes=1 << 50)
+    assert plan.horizon == 0 and -(-8192 // plan.tiles[0][0]) + 14 <= ST.MAX_PLAN_AXIS
