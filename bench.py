@@ -296,7 +296,8 @@ def precision_form(steps=10):
     """The precision workload: float32 FFTs, spectral multiply and cascade
     arithmetic; float64 only in the kernel-correction sums and the volume
     sums / rescale (all on the GPU); checked against the FP64 oracle on the
-    same keyed uniforms (flux and S_k: rel-L2 and max-abs)."""
+    same keyed uniforms (flux and S_k: rel-L2 and max-abs), and the
+    integrated field (one class's scalar increment) likewise."""
     import numpy as np
     import torch
     from oracle import model as M
@@ -352,6 +353,47 @@ def precision_form(steps=10):
     fl2, fmax = err(f32, f64)
     sl2, smax = err(s32, s64)
     vox = int(np.prod(shape))
+
+    # the integrated field: one class's scalar increment T * (C_k P / <|P|>)
+    # with P the joint pattern of a running field on this class's S_k; the
+    # GPU in float32 from its own S_k, the oracle in float64 from its own
+    from oracle import scalars as OS
+    from steam_amd import scalars as SC
+    nz = shape[2]
+    zc = np.arange(nz) * 0.5
+    pert = (50.0 * np.random.default_rng(2).standard_normal(shape, dtype=np.float32))
+    pert = pert.astype(np.float32)
+    mean = np.linspace(340e3, 320e3, nz).astype(np.float32)
+    c_k = np.full(nz, 500.0, dtype=np.float32)
+    b_z = np.full(nz, 3000.0, dtype=np.float32)
+    aspect = np.full(nz, 0.3, dtype=np.float32)
+    lo, hi = 315 * 1004, 355 * 1004
+    pt = torch.from_numpy(pert).cuda()
+    inc = torch.empty_like(pt)
+
+    def scalar_increment():
+        W = SC.joint_pattern(pt, mean, sk, b_z, aspect, zc, 0.5, 0.5, lo, hi, precise=True)
+        led = SC.product_norm(W)
+        SC.normalize_pattern(W, led.level_mean, c_k)
+        op.execute(W, inc, spec)
+
+    scalar_increment()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        scalar_increment()
+    torch.cuda.synchronize()
+    inc_ms = (time.perf_counter() - t0) * 1e3 / 3
+    i32 = inc.cpu().numpy()
+    say(f"GPU scalar increment {inc_ms:.3f} ms; running the FP64 oracle for it")
+    W64 = OS.joint_pattern(pert, mean, s64, b_z, aspect, zc, 0.5, 0.5, lo, hi,
+                           dtype=np.float64)
+    _, _, lm64 = OS.product_norm(W64, dtype=np.float64)
+    W64 = OS.normalize(W64, lm64, c_k, dtype=np.float64)
+    say("oracle pattern done; convolving it in float64")
+    i64 = M.conv_fft(W64, kern64)
+    del W64
+    il2, imax = err(i32, i64)
     return {"metric": "precision form: float32 class advance vs FP64 oracle",
             "workload": "flux_advance_class_1024x1024x256", "grid": list(shape),
             "kernel": list(kern.shape), "alpha": alpha, "C1": c1, "c": c,
@@ -360,6 +402,12 @@ def precision_form(steps=10):
             "flux_rel_l2": fl2, "flux_max_abs": fmax,
             "sk_rel_l2": sl2, "sk_max_abs": smax,
             "flux_max_abs_relative_to_max": fmax / float(np.abs(f64).max()),
+            "integrated_rel_l2": il2, "integrated_max_abs": imax,
+            "integrated_max_abs_relative_to_max": imax / float(np.abs(i64).max()),
+            "integrated_ms": inc_ms,
+            "integrated_field": "one class's scalar increment T * (C_k P / <|P|>_level), "
+                                "P = W S_k g of a running field 320-340 kJ/kg + 50 N(0,1) "
+                                "(default_rng(2)), C_k 500, b 3000, aspect 0.3",
             "oracle_s": oracle_s,
             "data": "entering flux |1 + 0.3 N(0,1)| from default_rng(1), key "
                     "SeedSequence(0).spawn(1)[0]"}

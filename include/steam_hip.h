@@ -312,6 +312,22 @@ int steam_scalar_pattern(const float* pert_dev, const float* mean_z_dev,
                          double phi_min, double phi_max, float* out_dev,
                          steam_stream_t stream);
 
+/* The joint pattern without forming the float32 running field (the
+ * precision workload): horizontal differences of pert alone, the vertical
+ * one of pert plus dmean_z[z] (the mean profile's z-derivative, formed in
+ * float64 by the caller), the taper from pert + mlo_z[z] and mhi_z[z] - pert
+ * with mlo = mean - phi_min and mhi = phi_max - mean formed in float64.  The
+ * float64 transform of _advective_weight / _bound_taper
+ * (steam/simulate.py:2787-2930) to within the rounding of each difference;
+ * no counterpart in the reference, whose float32 form is
+ * steam_scalar_pattern. */
+int steam_scalar_pattern_precise(const float* pert_dev, const float* dmean_z_dev,
+                                 const float* mlo_z_dev, const float* mhi_z_dev,
+                                 const float* sk_dev, const float* b_z_dev,
+                                 const float* aspect_z_dev, const steam_zdiff* zdiff,
+                                 int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                                 float* out_dev, steam_stream_t stream);
+
 /* w = w / (level_mean > 0 ? level_mean : 1) * c(z), the product-norm apply
  * (steam/simulate.py:1986-1990); level_mean and c are float32 per level. */
 int steam_pattern_normalize(float* w_dev, const float* level_mean_dev,

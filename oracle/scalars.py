@@ -12,6 +12,10 @@ against.  Only ``tests/`` may import it.
                          path: float32 field, float64 per-level scalars,
                          unconditional 60-step bisection)
 
+The pattern functions take ``dtype``: float32 (default) is the reference's
+operation order; float64 is the same operations in double, the FP64 oracle of
+the precision workload.
+
 Pinning (tests/test_oracle_scalars.py): the weight against closed forms of
 the periodic central difference and against np.gradient itself; the taper
 against the reference's own closed-form tests (tests/test_bounds.py:148-179);
@@ -23,25 +27,26 @@ import numpy as np
 F32 = np.float32
 
 
-def z_difference_coefficients(z_coords):
+def z_difference_coefficients(z_coords, dtype=F32):
     """np.gradient's z-difference coefficients in float32
     (steam/simulate.py:2709-2741)."""
-    z = np.asarray(z_coords, dtype=F32)
+    z = np.asarray(z_coords, dtype=dtype)
     if z.ndim == 0:
-        return True, F32(2.0) * z[()], None, None, None, z[()], z[()]
+        return True, dtype(2.0) * z[()], None, None, None, z[()], z[()]
     dz = np.diff(z)
     if bool((dz == dz[0]).all()):
-        return True, F32(2.0) * dz[0], None, None, None, dz[0], dz[-1]
+        return True, dtype(2.0) * dz[0], None, None, None, dz[0], dz[-1]
     dz1, dz2 = dz[:-1], dz[1:]
-    return (False, F32(0.0), -(dz2) / (dz1 * (dz1 + dz2)), (dz2 - dz1) / (dz1 * dz2),
+    return (False, dtype(0.0), -(dz2) / (dz1 * (dz1 + dz2)), (dz2 - dz1) / (dz1 * dz2),
             dz1 / (dz2 * (dz1 + dz2)), dz[0], dz[-1])
 
 
-def advective_weight(field, dx, dy, z_coords, aspect_z):
-    """|grad_h f| + aspect(z) |df/dz| in float32, operation for operation."""
-    f = np.asarray(field, dtype=F32)
-    two_dx, two_dy = F32(2 * dx), F32(2 * dy)
-    uniform, two_dz, a, b, c, first, last = z_difference_coefficients(z_coords)
+def advective_weight(field, dx, dy, z_coords, aspect_z, dtype=F32):
+    """|grad_h f| + aspect(z) |df/dz| in float32, operation for operation
+    (``dtype`` float64: the same operations in double, the FP64 oracle)."""
+    f = np.asarray(field, dtype=dtype)
+    two_dx, two_dy = dtype(2 * dx), dtype(2 * dy)
+    uniform, two_dz, a, b, c, first, last = z_difference_coefficients(z_coords, dtype)
     gx = (np.roll(f, -1, axis=0) - np.roll(f, 1, axis=0)) / two_dx
     gy = (np.roll(f, -1, axis=1) - np.roll(f, 1, axis=1)) / two_dy
     w = np.sqrt(gx * gx + gy * gy)
@@ -56,25 +61,25 @@ def advective_weight(field, dx, dy, z_coords, aspect_z):
             gz[:, :, 1:-1] = np.abs(a * f[:, :, :-2] + b * f[:, :, 1:-1] + c * f[:, :, 2:])
     gz[:, :, 0] = np.abs((f[:, :, 1] - f[:, :, 0]) / first)
     gz[:, :, -1] = np.abs((f[:, :, -1] - f[:, :, -2]) / last)
-    return gz * np.asarray(aspect_z, dtype=F32) + w
+    return gz * np.asarray(aspect_z, dtype=dtype) + w
 
 
-def bound_taper(running_sum, b, phi_min, phi_max):
+def bound_taper(running_sum, b, phi_min, phi_max, dtype=F32):
     """g = clip(min(phi - min, max - phi) / b, 0, 1) in float32."""
-    g = np.asarray(running_sum, dtype=F32) - F32(phi_min)
-    span = F32(phi_max - phi_min)
+    g = np.asarray(running_sum, dtype=dtype) - dtype(phi_min)
+    span = dtype(phi_max - phi_min)
     g = np.minimum(g, span - g)
-    b = np.asarray(b, dtype=F32)
-    g = g / np.where(b > 0, b, F32(1e-30))
-    return np.clip(g, F32(0.0), F32(1.0))
+    b = np.asarray(b, dtype=dtype)
+    g = g / np.where(b > 0, b, dtype(1e-30))
+    return np.clip(g, dtype(0.0), dtype(1.0))
 
 
 def joint_pattern(perturbation, mean_1d, sk, b, aspect_z, z_coords, dx, dy,
-                  phi_min, phi_max):
-    running = np.asarray(perturbation, dtype=F32) + np.asarray(mean_1d, dtype=F32)
-    w = advective_weight(running, dx, dy, z_coords, aspect_z)
-    w = w * np.asarray(sk, dtype=F32)
-    return w * bound_taper(running, b, phi_min, phi_max)
+                  phi_min, phi_max, dtype=F32):
+    running = np.asarray(perturbation, dtype=dtype) + np.asarray(mean_1d, dtype=dtype)
+    w = advective_weight(running, dx, dy, z_coords, aspect_z, dtype)
+    w = w * np.asarray(sk, dtype=dtype)
+    return w * bound_taper(running, b, phi_min, phi_max, dtype)
 
 
 def _inner(a, window):
@@ -84,17 +89,17 @@ def _inner(a, window):
     return a[x0:x1, y0:y1]
 
 
-def product_norm(pattern, window=None):
+def product_norm(pattern, window=None, dtype=F32):
     inner = _inner(pattern, window)
     total = np.abs(inner).sum(axis=(0, 1), dtype=np.float64)
     count = np.count_nonzero(inner, axis=(0, 1))
-    mean = (total / np.maximum(count, 1)).astype(F32)
+    mean = (total / np.maximum(count, 1)).astype(dtype)
     return total, count, mean
 
 
-def normalize(pattern, level_mean, c_k):
-    lm = np.asarray(level_mean, dtype=F32)
-    return (pattern / np.where(lm > 0, lm, F32(1.0))) * np.asarray(c_k, dtype=F32)
+def normalize(pattern, level_mean, c_k, dtype=F32):
+    lm = np.asarray(level_mean, dtype=dtype)
+    return (pattern / np.where(lm > 0, lm, dtype(1.0))) * np.asarray(c_k, dtype=dtype)
 
 
 def bounded_amplitude_add(perturbation, mean_1d, increment, phi_min, phi_max,

@@ -117,6 +117,9 @@ _SIGNATURES = [
      [_P, _P, _P, _P, _P, ctypes.POINTER(ZDiff), _I64, _I64, _I64, _F64, _F64,
       _F64, _F64, _P, _P]),
     ("steam_pattern_normalize", ctypes.c_int, [_P, _P, _P, _I64, _I64, _P]),
+    ("steam_scalar_pattern_precise", ctypes.c_int,
+     [_P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(ZDiff), _I64, _I64, _I64, _F64, _F64, _P,
+      _P]),
     ("steam_level_scratch_bytes", _I64, [_I64, _I64, _I64]),
     ("steam_level_reduce", ctypes.c_int,
      [_P, _P, _P, _I64, _I64, _I64, ctypes.POINTER(_I64), ctypes.c_int, _F64, _F64,

@@ -91,6 +91,57 @@ __global__ __launch_bounds__(256) void pattern_kernel(
   }
 }
 
+// The same joint pattern without the cancellation of the float32 running
+// field: phi = pert + mean(z) is never formed.  Horizontal differences are
+// differences of pert (the mean is horizontally uniform); the vertical one is
+// the difference of pert plus the mean's own z-derivative dmean[z] (formed in
+// float64 by the caller); the taper's distances to the bounds are
+// pert + (mean - lo) and (hi - mean) - pert with both level terms formed in
+// float64.  Used by the precision workload; agrees with the float64
+// transform to the rounding of each difference rather than of the field.
+__global__ __launch_bounds__(256) void pattern_precise_kernel(
+    const float* __restrict__ pert, const float* __restrict__ dmean_z,
+    const float* __restrict__ mlo_z, const float* __restrict__ mhi_z,
+    const float* __restrict__ sk, const float* __restrict__ b_z,
+    const float* __restrict__ aspect_z, ZDiff zd, int nx, int ny, int nz, float two_dx,
+    float two_dy, float* __restrict__ out) {
+  const int64_t n = (int64_t)nx * ny * nz;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i % nz);
+    const int64_t line = i / nz;
+    const int x = (int)(line / ny), y = (int)(line - (int64_t)x * ny);
+    const int xp = x + 1 < nx ? x + 1 : 0, xm = x > 0 ? x - 1 : nx - 1;
+    const int yp = y + 1 < ny ? y + 1 : 0, ym = y > 0 ? y - 1 : ny - 1;
+    auto p = [&](int xx, int yy, int zz) { return pert[((int64_t)xx * ny + yy) * nz + zz]; };
+    const float gx = __fdiv_rn(__fsub_rn(p(xp, y, z), p(xm, y, z)), two_dx);
+    const float gy = __fdiv_rn(__fsub_rn(p(x, yp, z), p(x, ym, z)), two_dy);
+    float w = sqrtf(__fadd_rn(__fmul_rn(gx, gx), __fmul_rn(gy, gy)));
+    const float pc = p(x, y, z);
+    if (nz > 1) {
+      float gz;
+      if (z == 0) {
+        gz = __fdiv_rn(__fsub_rn(p(x, y, 1), pc), zd.dz_first);
+      } else if (z == nz - 1) {
+        gz = __fdiv_rn(__fsub_rn(pc, p(x, y, nz - 2)), zd.dz_last);
+      } else if (zd.uniform) {
+        gz = __fdiv_rn(__fsub_rn(p(x, y, z + 1), p(x, y, z - 1)), zd.two_dz);
+      } else {
+        gz = __fadd_rn(__fadd_rn(__fmul_rn(zd.a[z - 1], p(x, y, z - 1)),
+                                 __fmul_rn(zd.b[z - 1], pc)),
+                       __fmul_rn(zd.c[z - 1], p(x, y, z + 1)));
+      }
+      w = __fadd_rn(__fmul_rn(fabsf(__fadd_rn(gz, dmean_z[z])), aspect_z[z]), w);
+    }
+    w = __fmul_rn(w, sk[i]);
+    float g = fminf(__fadd_rn(pc, mlo_z[z]), __fsub_rn(mhi_z[z], pc));
+    const float bz = b_z[z];
+    g = __fdiv_rn(g, bz > 0.f ? bz : 1e-30f);
+    g = fminf(fmaxf(g, 0.f), 1.f);
+    out[i] = __fmul_rn(w, g);
+  }
+}
+
 // W = W / (level_mean > 0 ? level_mean : 1) * C_k[z]  (steam/simulate.py:1986-1990)
 __global__ __launch_bounds__(256) void pattern_normalize_kernel(
     float* __restrict__ w, const float* __restrict__ level_mean,
@@ -776,6 +827,36 @@ extern "C" int steam_scalar_pattern(const float* pert_dev, const float* mean_z_d
                      mean_z_dev, sk_dev, b_z_dev, aspect_z_dev, zd, (int)nx, (int)ny, (int)nz,
                      (float)(2.0 * dx), (float)(2.0 * dy), lo, span, out_dev);
   STEAM_LAUNCH_CHECK("pattern_kernel");
+  return STEAM_OK;
+}
+
+extern "C" int steam_scalar_pattern_precise(const float* pert_dev, const float* dmean_z_dev,
+                                            const float* mlo_z_dev, const float* mhi_z_dev,
+                                            const float* sk_dev, const float* b_z_dev,
+                                            const float* aspect_z_dev, const steam_zdiff* zdiff,
+                                            int64_t nx, int64_t ny, int64_t nz, double dx,
+                                            double dy, float* out_dev, steam_stream_t stream) {
+  if (!pert_dev || !dmean_z_dev || !mlo_z_dev || !mhi_z_dev || !sk_dev || !b_z_dev ||
+      !aspect_z_dev || !zdiff || !out_dev)
+    return set_error(STEAM_EINVAL, "steam_scalar_pattern_precise: NULL argument");
+  if (nx < 1 || ny < 1 || nz < 1 || nx * ny * nz >= (1ll << 40))
+    return set_error(STEAM_EINVAL, "steam_scalar_pattern_precise: bad shape");
+  if (!zdiff->uniform && nz > 2 && (!zdiff->a_dev || !zdiff->b_dev || !zdiff->c_dev))
+    return set_error(STEAM_EINVAL,
+                     "steam_scalar_pattern_precise: non-uniform z needs coefficients");
+  ZDiff zd;
+  zd.uniform = zdiff->uniform;
+  zd.two_dz = zdiff->two_dz;
+  zd.dz_first = zdiff->dz_first;
+  zd.dz_last = zdiff->dz_last;
+  zd.a = zdiff->a_dev;
+  zd.b = zdiff->b_dev;
+  zd.c = zdiff->c_dev;
+  hipLaunchKernelGGL(pattern_precise_kernel, dim3(grid_1d(nx * ny * nz)), dim3(256), 0,
+                     (hipStream_t)stream, pert_dev, dmean_z_dev, mlo_z_dev, mhi_z_dev, sk_dev,
+                     b_z_dev, aspect_z_dev, zd, (int)nx, (int)ny, (int)nz, (float)(2.0 * dx),
+                     (float)(2.0 * dy), out_dev);
+  STEAM_LAUNCH_CHECK("pattern_precise_kernel");
   return STEAM_OK;
 }
 

@@ -80,11 +80,35 @@ def _check_field(t, name):
         raise ValueError(f"{name} must be a contiguous 3-D float32 device tensor")
 
 
+def _mean_z_derivative(mean_1d, z_coords):
+    """d mean / dz per level in float64, by the same differences as the
+    field's (np.gradient's interior and one-sided end formulas)."""
+    m = np.asarray(mean_1d, dtype=np.float64)
+    z = np.asarray(z_coords, dtype=np.float64)
+    nz = m.size
+    if nz == 1:
+        return np.zeros(1)
+    if z.ndim == 0:
+        z = np.arange(nz) * float(z)
+    d = np.empty(nz)
+    dz = np.diff(z)
+    d[0] = (m[1] - m[0]) / dz[0]
+    d[-1] = (m[-1] - m[-2]) / dz[-1]
+    if nz > 2:
+        dz1, dz2 = dz[:-1], dz[1:]
+        d[1:-1] = (-(dz2) / (dz1 * (dz1 + dz2)) * m[:-2] + (dz2 - dz1) / (dz1 * dz2) * m[1:-1]
+                   + dz1 / (dz2 * (dz1 + dz2)) * m[2:])
+    return d
+
+
 def joint_pattern(perturbation, mean_1d, sk, b_z, aspect_z, z_coords, dx, dy,
-                  phi_min, phi_max, out=None, zdiff=None):
+                  phi_min, phi_max, out=None, zdiff=None, precise=False):
     """P = W * S_k * g (steam/simulate.py:1935-1963): W the advective weight
     of phi = perturbation + mean_1d, g the bound taper.  With ``sk`` None,
-    the advective weight alone (_advective_weight)."""
+    the advective weight alone (_advective_weight).  ``precise`` never
+    forms the float32 running field (steam_scalar_pattern_precise): the
+    differences are of the perturbation, the mean's part is formed in
+    float64, so the result follows the float64 transform."""
     lib = _lib.load()
     _check_field(perturbation, "perturbation")
     nx, ny, nz = perturbation.shape
@@ -98,6 +122,20 @@ def joint_pattern(perturbation, mean_1d, sk, b_z, aspect_z, z_coords, dx, dy,
     zd = ZDifferences(z_coords) if zdiff is None else zdiff
     if out is None:
         out = torch.empty_like(perturbation)
+    if precise:
+        if sk is None:
+            raise ValueError("the precise pattern needs S_k")
+        m64 = (mean_1d.double().cpu().numpy() if isinstance(mean_1d, torch.Tensor)
+               else np.asarray(mean_1d, dtype=np.float32).astype(np.float64))
+        dmean = _f32_dev(_mean_z_derivative(m64, z_coords))
+        mlo = _f32_dev(m64 - float(phi_min))
+        mhi = _f32_dev(float(phi_max) - m64)
+        _lib.check(lib.steam_scalar_pattern_precise(
+            _lib.ptr(perturbation), _lib.ptr(dmean), _lib.ptr(mlo), _lib.ptr(mhi),
+            _lib.ptr(sk), _lib.ptr(b_t), _lib.ptr(aspect_t), ctypes.byref(zd.native), nx, ny,
+            nz, float(dx), float(dy), _lib.ptr(out), _lib.stream_handle()),
+            "steam_scalar_pattern_precise")
+        return out
     _lib.check(lib.steam_scalar_pattern(
         _lib.ptr(perturbation), _lib.ptr(mean_t), _lib.ptr(sk), _lib.ptr(b_t),
         _lib.ptr(aspect_t), ctypes.byref(zd.native), nx, ny, nz, float(dx), float(dy),

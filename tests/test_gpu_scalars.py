@@ -90,3 +90,30 @@ def test_bounded_add_rejects_aliasing_and_bad_shapes():
         S.bounded_amplitude_add(p, np.zeros(3), p, 0.0, 1.0)
     with pytest.raises(ValueError):
         S.bounded_amplitude_add(p, np.zeros(3), torch.zeros((4, 4, 2), device='cuda'), 0.0, 1.0)
+
+
+@pytest.mark.parametrize("zi", [0, 1])
+def test_precise_pattern_follows_the_float64_transform(zi):
+    """With a large mean and a small perturbation the float32 running field
+    loses the gradients to cancellation; the precise pattern does not form
+    it and agrees with the FP64 oracle to ~1e-7 (the plain one only to the
+    float32 rounding of the field, 1e-4 here)."""
+    from steam_amd import scalars as S
+    rng = np.random.default_rng(3)
+    shape = (24, 20, 7)
+    z = Z_GRIDS[zi]
+    pert = (50.0 * rng.standard_normal(shape)).astype(np.float32)
+    mean = np.linspace(340e3, 320e3, 7).astype(np.float32)
+    sk = rng.standard_normal(shape).astype(np.float32)
+    b = np.full(7, 3000.0, np.float32)
+    aspect = np.full(7, 0.3, np.float32)
+    lo, hi = 315 * 1004, 355 * 1004
+    ref = O.joint_pattern(pert, mean, sk, b, aspect, z, 50.0, 50.0, lo, hi, dtype=np.float64)
+    args = (torch.from_numpy(pert).cuda(), mean, torch.from_numpy(sk).cuda(), b, aspect, z,
+            50.0, 50.0, lo, hi)
+    precise = S.joint_pattern(*args, precise=True).cpu().numpy().astype(np.float64)
+    plain = S.joint_pattern(*args).cpu().numpy().astype(np.float64)
+    err_precise = np.linalg.norm(precise - ref) / np.linalg.norm(ref)
+    err_plain = np.linalg.norm(plain - ref) / np.linalg.norm(ref)
+    assert err_precise < 1e-6, err_precise
+    assert err_plain > 10 * err_precise

@@ -77,7 +77,8 @@ def test_fixture_pin_detects_planted_mistakes(monkeypatch):
     assert rel(run_oracle(bad)['h_perturbation'], ref['h_perturbation']) > 100 * 5e-5
     # bound taper dropped
     monkeypatch.setattr(OS, "bound_taper",
-                        lambda r, b, lo, hi: np.ones_like(np.asarray(r, np.float32)))
+                        lambda r, b, lo, hi, dtype=np.float32: np.ones_like(
+                            np.asarray(r, dtype)))
     assert rel(run_oracle(st)['h_perturbation'], ref['h_perturbation']) > 50 * 5e-5
 
 
