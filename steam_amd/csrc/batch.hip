@@ -88,7 +88,7 @@ std::tuple<int, int, int> variant_of(const ItemPlan& ip, int pass) {
     case BP_NOISE: return {ip.gen_noise ? 0 : -1, 0, 256};
     case BP_Z:
     case BP_ZINV:
-      if (ip.zreg) return {1, g.n2, ZR_NTH};
+      if (ip.zreg) return {1, 2 * g.n2 + (g.nz & 1), ZR_NTH};  // plan length, odd nz
       return {ip.tile ? 2 : 3, radix_set(g.az), Z_NTH};
     case BP_Y:
       if (g.reg_y) return {1, g.reg_y, 256};
@@ -166,9 +166,13 @@ int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
         break;
       case BP_Z:
         if (gr.variant == 1) {
-          with_zplan(gr.arg, [&](auto zp) {
-            hipLaunchKernelGGL((zfwd_reg_kernel<BatchSrc, MODE_FLUX, decltype(zp)>), grid,
-                               dim3(ZR_NTH), 0, s, src);
+          with_zplan(gr.arg >> 1, [&](auto zp) {
+            if (gr.arg & 1)
+              hipLaunchKernelGGL((zfwd_reg_kernel<BatchSrc, MODE_FLUX, decltype(zp), true>), grid,
+                                 dim3(ZR_NTH), 0, s, src);
+            else
+              hipLaunchKernelGGL((zfwd_reg_kernel<BatchSrc, MODE_FLUX, decltype(zp), false>),
+                                 grid, dim3(ZR_NTH), 0, s, src);
           });
         } else if (gr.variant == 2) {
           STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((zfwd_tile_kernel<BatchSrc, MODE_FLUX, RS_>),
@@ -198,9 +202,13 @@ int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
         break;
       case BP_ZINV:
         if (gr.variant == 1) {
-          with_zplan(gr.arg, [&](auto zp) {
-            hipLaunchKernelGGL((zinv_reg_kernel<BatchSrc, MODE_FLUX, decltype(zp)>), grid,
-                               dim3(ZR_NTH), 0, s, src);
+          with_zplan(gr.arg >> 1, [&](auto zp) {
+            if (gr.arg & 1)
+              hipLaunchKernelGGL((zinv_reg_kernel<BatchSrc, MODE_FLUX, decltype(zp), true>), grid,
+                                 dim3(ZR_NTH), 0, s, src);
+            else
+              hipLaunchKernelGGL((zinv_reg_kernel<BatchSrc, MODE_FLUX, decltype(zp), false>),
+                                 grid, dim3(ZR_NTH), 0, s, src);
           });
         } else if (gr.variant == 2) {
           STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((zinv_tile_kernel<BatchSrc, MODE_FLUX, RS_>),
@@ -250,7 +258,7 @@ int make_item(const steam_batch_item& it, PassDesc& d, ItemPlan& ip) {
   auto a8 = [](const void* q) { return ((uintptr_t)q & 7) == 0; };
   auto a16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
   ip.p = p;
-  ip.zreg = p->g.zreg && a8(d.out) && a8(d.nfield) && a8(d.sk);
+  ip.zreg = p->g.zreg && ((p->g.nz & 1) || (a8(d.out) && a8(d.nfield) && a8(d.sk)));
   ip.tile = !ip.zreg && p->g.tile_z && a16(d.out) && a16(d.nfield) && a16(d.sk);
   ip.gen_noise = it.noise_dev == nullptr;
   const bool defer = (it.flags & STEAM_DEFER_RESCALE) != 0;

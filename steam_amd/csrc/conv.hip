@@ -106,7 +106,7 @@ static void set_tiles(ConvGeom& g, int nz_acc, int kahan, size_t& lds_z,
   lds_zinv = lds_z + (size_t)g.Tz * nz_acc * 4 * (1 + kahan);
   g.tile_z = g.nblocks == 1 && g.nz % 4 == 0 && g.Tz * g.nz <= 4 * ZT_U * Z_NTH;
   int zr_lines = 0;
-  g.zreg = g.nblocks == 1 && g.nz % 2 == 0 && zplan_lines(g.n2, zr_lines);
+  g.zreg = g.nblocks == 1 && zplan_lines(g.n2, zr_lines);
   g.wg_zr = g.zreg ? (int)(((int64_t)g.nx * g.ny + zr_lines - 1) / zr_lines) : 0;
   if (g.tile_z) {
     lds_z = std::max(lds_z + (size_t)(g.n2 + g.n2 / 2 + 1) * 8, (size_t)3 * 8 * (Z_NTH / 64));
@@ -116,16 +116,19 @@ static void set_tiles(ConvGeom& g, int nz_acc, int kahan, size_t& lds_z,
   lds_x = (size_t)g.nx * g.Tx * 8;
 }
 
-// choose the z transform length: minimise spectral planes nblocks*(n/2+1)
-// model cost of a z transform length: spectral planes nblocks*(n/2+1),
-// weighted up when the z lines have no register plan; -1 if n is not usable
+// model cost of a z transform length: spectral planes nblocks*(n/2+1), each
+// costing its x/y passes plus its z passes; the z passes of a length with a
+// register plan cost about a quarter of the LDS-staged ones (measured per
+// plane at 2048x2048x115: x/y 80 us, register z 38 us, LDS-staged z 135 us),
+// so a plane weighs 5 with a register plan and 9 without; -1 if n is not
+// usable
 static int64_t nfft_cost(int64_t nz, int64_t kz, int64_t n) {
   if (n < 2 || n % 2 || n < kz || n / 2 + 1 > FFT_LDSE || !is_smooth7(n / 2)) return -1;
   const int64_t L = n - kz + 1;
   const int64_t nb = (nz + L - 1) / L;
   int lines = 0;
   const bool reg = nb == 1 && zplan_lines((int)(n / 2), lines);
-  return nb * (n / 2 + 1) * (reg ? 4 : 5);
+  return nb * (n / 2 + 1) * (reg ? 5 : 9);
 }
 
 // candidate lengths in increasing model cost (ties: shorter first)

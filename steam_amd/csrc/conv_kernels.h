@@ -918,7 +918,29 @@ __device__ __forceinline__ void zr_block_sum(double (&v)[NV], double* red) {
     }
 }
 
-template <class Src, int MODE, class ZP>
+// z pairs (2n, 2n+1) of a line starting at off: one 8-byte access when the
+// field has an even number of levels (every line start is then 8-byte
+// aligned), two 4-byte ones when it is odd (the pair past the top level
+// reads 0 and is not written)
+template <bool ODD>
+__device__ __forceinline__ float2 ld_zpair(const float* p, int64_t off, int z0, int nz) {
+  if constexpr (!ODD) {
+    return *reinterpret_cast<const float2*>(p + off + z0);
+  } else {
+    return make_float2(p[off + z0], z0 + 1 < nz ? p[off + z0 + 1] : 0.f);
+  }
+}
+template <bool ODD>
+__device__ __forceinline__ void st_zpair(float* p, int64_t off, int z0, int nz, float2 v) {
+  if constexpr (!ODD) {
+    *reinterpret_cast<float2*>(p + off + z0) = v;
+  } else {
+    p[off + z0] = v.x;
+    if (z0 + 1 < nz) p[off + z0 + 1] = v.y;
+  }
+}
+
+template <class Src, int MODE, class ZP, bool ODD>
 __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
   constexpr int N = ZP::N, V = ZP::V, P = ZP::P, LPW = ZP::LINES;
   __shared__ float2 lds_lines[ZR_WAVES * LPW * ZP::PITCH];
@@ -942,8 +964,8 @@ __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
     v[m] = make_float2(0.f, 0.f);
     nv[m] = make_float2(0.f, 0.f);
     if (valid && n2i < nz) {
-      v[m] = *reinterpret_cast<const float2*>(srcp + zoff + n2i);
-      if (MODE == MODE_FLUX) nv[m] = *reinterpret_cast<const float2*>(d.nfield + zoff + n2i);
+      v[m] = ld_zpair<ODD>(srcp, zoff, n2i, nz);
+      if (MODE == MODE_FLUX) nv[m] = ld_zpair<ODD>(d.nfield, zoff, n2i, nz);
     }
   }
   stage_tw2(g, tw2, N);
@@ -963,7 +985,7 @@ __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
           acc[1] += (double)fabsf(nv[m].x) + (double)fabsf(nv[m].y);
           acc[2] += (double)((nv[m].x != 0.f) + (nv[m].y != 0.f));
         }
-        if (d.sk) *reinterpret_cast<float2*>(d.sk + zoff + n2i) = p;
+        if (d.sk) st_zpair<ODD>(d.sk, zoff, n2i, nz, p);
         v[m] = p;
       }
     }
@@ -1005,7 +1027,7 @@ __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
   }
 }
 
-template <class Src, int MODE, class ZP>
+template <class Src, int MODE, class ZP, bool ODD>
 __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src src) {
   constexpr int N = ZP::N, V = ZP::V, P = ZP::P, LPW = ZP::LINES;
   __shared__ float2 lds_lines[ZR_WAVES * LPW * ZP::PITCH];
@@ -1029,14 +1051,14 @@ __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src sr
   float2 yn = make_float2(0.f, 0.f);
   if (q == 0 && valid) yn = Si[(int64_t)N * plane];
   // output levels z = 2n, 2n+1 of natural index n (the kernel spectrum
-  // carries the 'same' trim shift); nz is even
+  // carries the 'same' trim shift); with an odd nz the last pair's second
+  // level is past the top and is neither written nor counted
   float2 fl[V];
   auto load_flux = [&]() {
 #pragma unroll
     for (int m = 0; m < V; ++m) {
       const int z0 = 2 * (q + m * P);
-      fl[m] = (valid && z0 < nz) ? *reinterpret_cast<const float2*>(d.out + zoff + z0)
-                                 : make_float2(0.f, 0.f);
+      fl[m] = (valid && z0 < nz) ? ld_zpair<ODD>(d.out, zoff, z0, nz) : make_float2(0.f, 0.f);
     }
   };
   if (MODE == MODE_FLUX && STEAM_ZR_PREFETCH) load_flux();
@@ -1065,7 +1087,7 @@ __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src sr
 #pragma unroll
     for (int m = 0; m < V; ++m) {
       const int z0 = 2 * (q + m * P);
-      if (valid && z0 < nz) *reinterpret_cast<float2*>(d.out + zoff + z0) = v[m];
+      if (valid && z0 < nz) st_zpair<ODD>(d.out, zoff, z0, nz, v[m]);
     }
   } else {
     if (!STEAM_ZR_PREFETCH) load_flux();
@@ -1077,13 +1099,14 @@ __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src sr
       const int z0 = 2 * (q + m * P);
       float o0 = __fadd_rn(entering_value(pd, fl[m].x), v[m].x);
       float o1 = __fadd_rn(entering_value(pd, fl[m].y), v[m].y);
+      if (ODD && z0 + 1 >= nz) o1 = 0.f;
       const int clipped = (o0 < 0.f) + (o1 < 0.f);
       o0 = (o0 < 0.f) ? 0.f : o0;
       o1 = (o1 < 0.f) ? 0.f : o1;
       if (valid && z0 < nz) {
         if (win || !d.clip_win) acc[1] += (double)clipped;
         if (win) acc[0] += (double)o0 + (double)o1;
-        *reinterpret_cast<float2*>(d.out + zoff + z0) = make_float2(o0, o1);
+        st_zpair<ODD>(d.out, zoff, z0, nz, make_float2(o0, o1));
       }
     }
     if (!d.apply) {

@@ -102,7 +102,9 @@ inline int run_operator(const steam_conv_plan* p, PassDesc& d, hipStream_t s) {
   // the tile kernels move 16-byte vectors: every field pointer must allow it
   auto a16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
   auto a8 = [](const void* q) { return ((uintptr_t)q & 7) == 0; };
-  const bool zreg = g.zreg && a8(d.in) && a8(d.out) && a8(d.nfield) && a8(d.sk);
+  // the even-level register path moves 8-byte pairs; the odd one 4-byte values
+  const bool odd = g.nz & 1;
+  const bool zreg = g.zreg && (odd || (a8(d.in) && a8(d.out) && a8(d.nfield) && a8(d.sk)));
   const bool tile = !zreg && g.tile_z && a16(d.in) && a16(d.out) && a16(d.nfield) &&
                     a16(d.sk);
   d.npart = zreg ? g.wg_zr : g.wg_z;
@@ -111,8 +113,12 @@ inline int run_operator(const steam_conv_plan* p, PassDesc& d, hipStream_t s) {
     ProfScope ps(STEAM_KF_ZFWD, s);
     if (zreg) {
       with_zplan(g.n2, [&](auto zp) {
-        hipLaunchKernelGGL((zfwd_reg_kernel<SingleSrc, MODE, decltype(zp)>), dim3(g.wg_zr),
-                           dim3(ZR_NTH), 0, s, src);
+        if (odd)
+          hipLaunchKernelGGL((zfwd_reg_kernel<SingleSrc, MODE, decltype(zp), true>),
+                             dim3(g.wg_zr), dim3(ZR_NTH), 0, s, src);
+        else
+          hipLaunchKernelGGL((zfwd_reg_kernel<SingleSrc, MODE, decltype(zp), false>),
+                             dim3(g.wg_zr), dim3(ZR_NTH), 0, s, src);
       });
     } else if (tile) {
       STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zfwd_tile_kernel<SingleSrc, MODE, RS_>),
@@ -142,8 +148,12 @@ inline int run_operator(const steam_conv_plan* p, PassDesc& d, hipStream_t s) {
     ProfScope ps(STEAM_KF_ZINV, s);
     if (zreg) {
       with_zplan(g.n2, [&](auto zp) {
-        hipLaunchKernelGGL((zinv_reg_kernel<SingleSrc, MODE, decltype(zp)>), dim3(g.wg_zr),
-                           dim3(ZR_NTH), 0, s, src);
+        if (odd)
+          hipLaunchKernelGGL((zinv_reg_kernel<SingleSrc, MODE, decltype(zp), true>),
+                             dim3(g.wg_zr), dim3(ZR_NTH), 0, s, src);
+        else
+          hipLaunchKernelGGL((zinv_reg_kernel<SingleSrc, MODE, decltype(zp), false>),
+                             dim3(g.wg_zr), dim3(ZR_NTH), 0, s, src);
       });
     } else if (tile) {
       STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zinv_tile_kernel<SingleSrc, MODE, RS_>),

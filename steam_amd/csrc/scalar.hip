@@ -183,12 +183,18 @@ enum LevelOp {
 // G partials of each (candidate, level) in a fixed order.  A level's result therefore depends
 // only on the field, the window and the shift, never on how many candidate
 // shifts share the launch.
-constexpr int RD_MAX_GROUPS = 512;
+constexpr int RD_MAX_GROUPS = 4096;
 constexpr int RD_MIN_LINES = 32;
-// candidate shifts of one bisection pass: the 15 midpoints of a 4-deep
-// bisection tree, so one pass over the field takes 4 bisection steps
+// lines whose loads a lane issues before it works on them
+constexpr int RD_U = 4;
+// candidate shifts of one bisection pass: the 2^D - 1 midpoints of a D-deep
+// bisection tree, so one pass over the field takes D bisection steps; D = 4
+// (15 candidates) where passes are latency-bound, D = 3 (7 candidates) on
+// large fields, where the candidates' arithmetic would bound the pass
 constexpr int KCAND = 15;
-constexpr int KSTEPS = 4;
+constexpr int KCAND_WIDE = 7;
+// fields above this many cells take the 3-deep tree
+constexpr int64_t KWIDE_CELLS = int64_t(1) << 25;
 
 struct LevelArgs {
   const float* f;
@@ -262,15 +268,11 @@ __global__ __launch_bounds__(256) void level_partial_kernel(LevelArgs a, double*
       float sc = 1.f, sh = 0.f;
       if constexpr (OP == LV_SCALED_SUM || OP == LV_APPLY_ABS) sc = (float)a.scale[z];
       if constexpr (OP == LV_APPLY_ABS) sh = (float)a.shift[z];
-      int64_t wl = l0 + first;
-      int x = wx0 + (int)(wl / wy);
-      int y = wy0 + (int)(wl - (int64_t)(x - wx0) * wy);
-      for (; wl < l1; wl += lpi) {
-        const int64_t idx = ((int64_t)x * a.ny + y) * a.nz + z;
-        const float v = a.f[idx];
+      // the element work, in line order (loads are issued RD_U lines ahead)
+      auto step = [&](int64_t idx, float v, float pv, int x, int y) {
         if constexpr (OP == LV_SCALED_SUM) acc[0] += (double)__fmul_rn(v, sc);
         if constexpr (OP == LV_APPLY_ABS) {
-          const float phi = __fadd_rn(a.pert[idx], mz);
+          const float phi = __fadd_rn(pv, mz);
           const float u = __fsub_rn(__fmul_rn(v, sc), sh);
           const float e = fminf(fmaxf(__fadd_rn(u, phi), flo), fhi);
           const float d = __fsub_rn(e, phi);
@@ -290,21 +292,45 @@ __global__ __launch_bounds__(256) void level_partial_kernel(LevelArgs a, double*
         // every candidate slot is evaluated (slots past nc hold shift 0 and
         // are never stored), so the loop body is straight-line code
         if constexpr (OP == LV_CLAMP) {
+          // med3 is the clamp for finite values and lo <= hi
 #pragma unroll
           for (int c = 0; c < NC; ++c)
-            acc[c] += (double)fminf(fmaxf(__fsub_rn(v, muf[c]), flo), fhi);
+            acc[c] += (double)__builtin_amdgcn_fmed3f(__fsub_rn(v, muf[c]), flo, fhi);
         }
         if constexpr (OP == LV_E_CLAMP) {
           const double e = (double)v + (double)mz;
 #pragma unroll
           for (int c = 0; c < NC; ++c) acc[c] += fmin(fmax(e - mu64[c], a.lo), a.hi);
         }
-        y += lpi;
-        if (y >= wy0 + wy) {
-          const int k = (y - wy0) / wy;
-          x += k;
-          y -= k * wy;
+      };
+      int64_t wl = l0 + first;
+      int x = wx0 + (int)(wl / wy);
+      int y = wy0 + (int)(wl - (int64_t)(x - wx0) * wy);
+      const int wy1 = wy0 + wy;
+      while (wl < l1) {
+        int64_t idx[RD_U];
+        int xs[RD_U], ys[RD_U];
+        float v[RD_U], pv[RD_U];
+#pragma unroll
+        for (int u = 0; u < RD_U; ++u) {
+          const bool ok = wl + (int64_t)u * lpi < l1;
+          xs[u] = x;
+          ys[u] = y;
+          idx[u] = ((int64_t)x * a.ny + y) * a.nz + z;
+          v[u] = ok ? a.f[idx[u]] : 0.f;
+          pv[u] = 0.f;
+          if constexpr (OP == LV_APPLY_ABS) pv[u] = ok ? a.pert[idx[u]] : 0.f;
+          y += lpi;
+          if (y >= wy1) {
+            const int k = (y - wy0) / wy;
+            x += k;
+            y -= k * wy;
+          }
         }
+#pragma unroll
+        for (int u = 0; u < RD_U; ++u)
+          if (wl + (int64_t)u * lpi < l1) step(idx[u], v[u], pv[u], xs[u], ys[u]);
+        wl += (int64_t)RD_U * lpi;
       }
     }
 #pragma unroll
@@ -346,35 +372,37 @@ __global__ __launch_bounds__(256) void level_combine_kernel(const double* __rest
 }
 
 // ---------------------------------------------------------------------------
-// bisection on the level means, KSTEPS steps per pass over the field
+// bisection on the level means, D steps per pass over the field
 //
 // A bisection step halves the bracket [lo, hi] at mid = 0.5 * (lo + hi),
 // moving lo up to mid when the level mean at mid is "positive" and hi down
-// otherwise.  The KCAND = 15 candidates of a pass are the midpoints of the
-// complete 4-deep tree of brackets below the current one (heap order), each
-// computed by that same formula; after the pass the 4 steps are taken along
+// otherwise.  The 2^D - 1 candidates of a pass are the midpoints of the
+// complete D-deep tree of brackets below the current one (heap order), each
+// computed by that same formula; after the pass the D steps are taken along
 // the tree with the measured sums.  Every step therefore sees exactly the
 // midpoint and the level sum a one-step-per-pass bisection would, and 60
-// steps end on the same bracket.
+// steps end on the same bracket whatever D is.
 // ---------------------------------------------------------------------------
 struct BisectVec {
   double* lo;
   double* hi;
   double* mu;      // 0.5 * (lo + hi), the result after the last pass
-  double* cand;    // [KCAND][nz] the next pass's candidates
+  double* cand;    // [2^D - 1][nz] the next pass's candidates
   const double* target;
-  const double* sums;  // [KCAND][nz] level sums at the candidates
+  const double* sums;  // [2^D - 1][nz] level sums at the candidates
 };
 
+template <int D>
 __device__ __forceinline__ void bisect_candidates(BisectVec v, int nz, int z, double lo, double hi) {
-  double a[KCAND + 1], b[KCAND + 1];
+  constexpr int NC = (1 << D) - 1;
+  double a[NC + 1], b[NC + 1];
   a[1] = lo;
   b[1] = hi;
 #pragma unroll
-  for (int n = 1; n <= KCAND; ++n) {
+  for (int n = 1; n <= NC; ++n) {
     const double m = 0.5 * (a[n] + b[n]);
     v.cand[(int64_t)(n - 1) * nz + z] = m;
-    if (2 * n + 1 <= KCAND) {
+    if (2 * n + 1 <= NC) {
       a[2 * n] = a[n];
       b[2 * n] = m;
       a[2 * n + 1] = m;
@@ -383,25 +411,27 @@ __device__ __forceinline__ void bisect_candidates(BisectVec v, int nz, int z, do
   }
 }
 
+template <int D>
 __global__ void bisect_init_kernel(BisectVec v, int nz, double width) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   if (z < nz) {
     v.lo[z] = -width;
     v.hi[z] = width;
     v.mu[z] = 0.5 * (v.lo[z] + v.hi[z]);
-    bisect_candidates(v, nz, z, -width, width);
+    bisect_candidates<D>(v, nz, z, -width, width);
   }
 }
 
 // mode 0: positive = sum / npts - target > 0 (the bounded add);
 // mode 1: positive = sum / npts > target (the projection)
+template <int D>
 __global__ void bisect_pass_kernel(BisectVec v, int nz, double npts, int mode) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= nz) return;
   double lo = v.lo[z], hi = v.hi[z];
   const double target = v.target[z];
   int n = 1;
-  for (int s = 0; s < KSTEPS; ++s) {
+  for (int s = 0; s < D; ++s) {
     const double mid = 0.5 * (lo + hi);
     const double mean = v.sums[(int64_t)(n - 1) * nz + z] / npts;
     const bool positive = mode == 0 ? (mean - target > 0.0) : (mean > target);
@@ -416,7 +446,7 @@ __global__ void bisect_pass_kernel(BisectVec v, int nz, double npts, int mode) {
   v.lo[z] = lo;
   v.hi[z] = hi;
   v.mu[z] = 0.5 * (lo + hi);
-  bisect_candidates(v, nz, z, lo, hi);
+  bisect_candidates<D>(v, nz, z, lo, hi);
 }
 
 // ---------------------------------------------------------------------------
@@ -726,11 +756,18 @@ int level_reduce(const LevelArgs& a, double* out, double div, double* scratch, h
   const int64_t groups = rd_groups(wlines);
   const dim3 grid((unsigned)groups), block(256);
   if (a.ncand > 1) {
-    if (a.op == LV_CLAMP)
-      hipLaunchKernelGGL((level_partial_kernel<KCAND, LV_CLAMP>), grid, block, 0, s, a, scratch, lpg);
-    else if (a.op == LV_E_CLAMP)
+    if (a.ncand == KCAND && a.op == LV_CLAMP)
+      hipLaunchKernelGGL((level_partial_kernel<KCAND, LV_CLAMP>), grid, block, 0, s, a, scratch,
+                         lpg);
+    else if (a.ncand == KCAND && a.op == LV_E_CLAMP)
       hipLaunchKernelGGL((level_partial_kernel<KCAND, LV_E_CLAMP>), grid, block, 0, s, a, scratch,
                          lpg);
+    else if (a.ncand == KCAND_WIDE && a.op == LV_CLAMP)
+      hipLaunchKernelGGL((level_partial_kernel<KCAND_WIDE, LV_CLAMP>), grid, block, 0, s, a,
+                         scratch, lpg);
+    else if (a.ncand == KCAND_WIDE && a.op == LV_E_CLAMP)
+      hipLaunchKernelGGL((level_partial_kernel<KCAND_WIDE, LV_E_CLAMP>), grid, block, 0, s, a,
+                         scratch, lpg);
     else
       return set_error(STEAM_EINVAL, "level reduction: candidates need a clamp op");
   } else {
@@ -782,20 +819,28 @@ bool shape_ok(int64_t nx, int64_t ny, int64_t nz) {
 }
 
 // the 60-step bisection of the level means of a's clamp op against target,
-// KSTEPS steps per pass; leaves the result in b.mu
-int bisect(LevelArgs a, BisectVec b, int nz, double width, double npts, int mode, double* red,
-           hipStream_t s) {
+// D steps per pass; leaves the result in b.mu
+template <int D>
+int bisect_depth(LevelArgs a, BisectVec b, int nz, double width, double npts, int mode,
+                 double* red, hipStream_t s) {
   const int gz = (nz + 255) / 256;
-  hipLaunchKernelGGL(bisect_init_kernel, dim3(gz), dim3(256), 0, s, b, nz, width);
+  hipLaunchKernelGGL((bisect_init_kernel<D>), dim3(gz), dim3(256), 0, s, b, nz, width);
   a.mu = b.cand;
-  a.ncand = KCAND;
-  for (int pass = 0; pass < 60 / KSTEPS; ++pass) {
+  a.ncand = (1 << D) - 1;
+  for (int pass = 0; pass < 60 / D; ++pass) {
     int rc = level_reduce(a, (double*)b.sums, 0.0, red, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(bisect_pass_kernel, dim3(gz), dim3(256), 0, s, b, nz, npts, mode);
+    hipLaunchKernelGGL((bisect_pass_kernel<D>), dim3(gz), dim3(256), 0, s, b, nz, npts, mode);
   }
   STEAM_LAUNCH_CHECK("bisection");
   return STEAM_OK;
+}
+
+int bisect(LevelArgs a, BisectVec b, int nz, double width, double npts, int mode, double* red,
+           hipStream_t s) {
+  const int64_t cells = (int64_t)a.nx * a.ny * a.nz;
+  return cells > KWIDE_CELLS ? bisect_depth<3>(a, b, nz, width, npts, mode, red, s)
+                             : bisect_depth<4>(a, b, nz, width, npts, mode, red, s);
 }
 
 }  // namespace

@@ -183,14 +183,16 @@ def test_cascade_rejects_what_the_reference_rejects():
         sm.cascade_loop(*args, st['class_seeds'], ledger=RunLedger(2))
 
 
-def test_projection_bisection_equals_one_step_per_pass_bisection():
+@pytest.mark.parametrize("shape,window", [((40, 36, 12), (3, 37, 2, 30)),
+                                          ((512, 512, 130), (8, 500, 16, 480))])
+def test_projection_bisection_equals_one_step_per_pass_bisection(shape, window):
     """The projection takes 4 bisection steps per pass over the field (15
-    candidate midpoints at once); driving a one-step-per-pass bisection from
-    the host with the same per-level sums ends on the same mu bit for bit."""
+    candidate midpoints at once; 3 steps and 7 candidates above 2^25 cells);
+    driving a one-step-per-pass bisection from the host with the same
+    per-level sums ends on the same mu bit for bit."""
     from steam_amd import cascade as C
     from steam_amd import scalars as S
-    pert, mean, lo, hi = _projection_case(seed=7, shape=(40, 36, 12))
-    window = (3, 37, 2, 30)
+    pert, mean, lo, hi = _projection_case(seed=7, shape=shape)
     dev = torch.from_numpy(pert).cuda()
     led = C.project_onto_bounds(dev.clone(), mean, lo, hi, window)
     mean_t = torch.from_numpy(mean).cuda()
@@ -198,8 +200,8 @@ def test_projection_bisection_equals_one_step_per_pass_bisection():
     npts = float((window[1] - window[0]) * (window[3] - window[2]))
     target = red(dev, S.LV_E_SUM, window, mean_1d=mean_t).cpu().numpy() / npts
     width = hi - lo
-    mu_lo = np.full(12, -width)
-    mu_hi = np.full(12, width)
+    mu_lo = np.full(shape[2], -width)
+    mu_hi = np.full(shape[2], width)
     for _ in range(60):
         mid = 0.5 * (mu_lo + mu_hi)
         sums = red(dev, S.LV_E_CLAMP, window, mean_1d=mean_t,

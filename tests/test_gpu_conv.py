@@ -32,6 +32,8 @@ def _rel(a, b):
     ((1024, 8, 4), (3, 3, 3)), ((2048, 8, 4), (5, 3, 3)),
     ((8, 2048, 4), (3, 5, 3)), ((16, 1024, 5), (3, 3, 5)),
     ((64, 96, 40), (13, 13, 13)), ((96, 64, 40), (13, 13, 13)),
+    # odd level counts on the register z path (n_fft 128, 64, 256)
+    ((32, 32, 115), (13, 13, 13)), ((40, 24, 53), (9, 9, 9)), ((16, 24, 201), (13, 13, 13)),
 ])
 def test_convolution_matches_oracle(fs, ks):
     from steam_amd.utils import convolve_fft_xy_oa_z

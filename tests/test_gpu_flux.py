@@ -41,6 +41,9 @@ def _region(key, origin, world):
     ((48, 40, 30), (2, 2, 2), 3, (4, 44, 3, 37), None),
     ((64, 64, 32), (1, 1, 1), 4, None, 1.6),
     ((24, 20, 18), (3, 1, 1), 3, None, None),
+    # odd level counts on the register z path
+    ((32, 32, 115), (1, 1, 1), 3, (3, 29, 2, 30), None),
+    ((40, 24, 53), (1, 1, 1), 2, None, None),
 ])
 def test_advance_matches_fp64_oracle(shape, s, sf, window, alpha):
     from steam_amd.simulate import _advance_flux
