@@ -63,6 +63,9 @@ def parse():
                     help="skip the full-simulate form (flux + h + qt, all classes)")
     ap.add_argument("--full-simulate-only", action="store_true",
                     help="only run the full-simulate form; prints one JSON line")
+    ap.add_argument("--production-square", action="store_true",
+                    help="only time the scalar-path operations at 2048x2048x115; "
+                         "prints one JSON line")
     ap.add_argument("--precision-form", action="store_true",
                     help="only run the precision form: one class advance of a "
                          "1024x1024x256 float32 field (alpha 1.6, C1 0.12, 129^3 "
@@ -463,6 +466,84 @@ def full_simulate_form(reps=3):
             "data": "synthetic linear h/qt profiles, seeds 0..reps", "runs": out}
 
 
+# The reference's own quoted costs at its production square 2048x2048x115
+# (an unnamed CUDA card; the bounded add and the projection include their
+# host/device transfers): steam/utils.py:89-90, steam/simulate.py:2995, 3311.
+PUBLISHED_PRODUCTION_S = {"convolve": 0.64, "bounded_add": 1.5, "projection_qt": 6.28}
+
+
+def production_form(reps=3):
+    """The scalar-path operations at the reference's production square
+    2048x2048x115 (a 13^3 kernel at s = 1), timed on device-resident fields:
+    CONVOLVE, the bounded amplitude-preserving add, the joint pattern with
+    its product norm and normalisation, and the projection of a qt-like
+    field whose levels mostly violate qt_min = 0.  The fields are synthetic
+    (seeded normal draws), so the rescale loop and the violation pattern are
+    those of this data, not of a cascade state."""
+    import numpy as np
+    import torch
+    from steam_amd import cascade as C
+    from steam_amd import scalars as SC
+    from steam_amd import simulate as S
+    from steam_amd.utils import convolve_fft_xy_oa_z
+
+    shape = (2048, 2048, 115)
+    gen = torch.Generator(device='cuda')
+    gen.manual_seed(0)
+
+    def rand(scale):
+        return (scale * torch.randn(shape, device='cuda', generator=gen)).contiguous()
+
+    def timed(fn, setup=None):
+        args = setup() if setup else ()
+        fn(*args)
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(reps):
+            args = setup() if setup else ()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn(*args)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best
+
+    field = rand(1.0)
+    kern = S._turbulon_envelope(1, 0.5, 0.5, 0.5, as_tensor=True)
+    conv_s = timed(lambda: convolve_fft_xy_oa_z(field, kern))
+    del field
+    mean = np.linspace(340e3, 320e3, 115).astype(np.float32)
+    lo, hi = 315 * 1004, 355 * 1004
+    pert0, inc0 = rand(300.0), rand(200.0)
+    ba_s = timed(lambda p, d: SC.bounded_amplitude_add(p, mean, d, lo, hi, record_applied=True),
+                 lambda: (pert0.clone(), inc0.clone()))
+    sk = rand(1.0)
+    z = np.arange(115) * 100.0
+    b_z = np.full(115, 3000.0, np.float32)
+    aspect = np.full(115, 0.3, np.float32)
+    c_k = np.full(115, 500.0, np.float32)
+
+    def pattern():
+        W = SC.joint_pattern(pert0, mean, sk, b_z, aspect, z, 3000.0, 3000.0, lo, hi)
+        SC.normalize_pattern(W, SC.product_norm(W).level_mean, c_k)
+
+    pat_s = timed(pattern)
+    del pert0, inc0, sk
+    qmean = np.linspace(0.018, 0.002, 115).astype(np.float32)
+    qt0 = rand(0.004)
+    proj_s = timed(lambda q: C.project_onto_bounds(q, qmean, 0.0, 0.03), lambda: (qt0.clone(),))
+    out = {"convolve": conv_s, "bounded_add": ba_s, "pattern_norm_normalise": pat_s,
+           "projection_qt": proj_s}
+    return {"workload": "scalar-path operations at 2048x2048x115 (13^3 kernel)",
+            "best_ms": {k: v * 1e3 for k, v in out.items()},
+            "reference_quoted_s": PUBLISHED_PRODUCTION_S,
+            "ratio_vs_reference_quoted": {k: PUBLISHED_PRODUCTION_S[k] / out[k]
+                                          for k in PUBLISHED_PRODUCTION_S},
+            "data": "synthetic: h-like field 340-320 kJ/kg + 300 N(0,1), increment "
+                    "200 N(0,1), qt-like 0.018-0.002 + 0.004 N(0,1), torch seed 0"}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -472,6 +553,11 @@ def main():
         from steam_amd import _lib
         _lib.require_gpu('cuda')
         print(json.dumps(precision_form()))
+        return
+    if args.production_square:
+        from steam_amd import _lib
+        _lib.require_gpu('cuda')
+        print(json.dumps(production_form()))
         return
     if args.full_simulate_only:
         from steam_amd import _lib

@@ -32,6 +32,47 @@ struct ZDiff {
   const float* c;
 };
 
+// Position of a grid-stride loop over a (lines, nz) field: the element i,
+// its level z and line, and the line's (x, y) when ny is given, advanced by
+// additions instead of a 64-bit division per element.
+struct ZWalk {
+  int64_t i, line, dline;
+  int z, dz, nz, x, y, ny, dx_, dy_;
+  __device__ ZWalk(int64_t start, int64_t stride, int nz_, int ny_ = 0) : nz(nz_), ny(ny_) {
+    i = start;
+    line = start / nz;
+    z = (int)(start - line * nz);
+    dline = stride / nz;
+    dz = (int)(stride - dline * nz);
+    if (ny > 0) {
+      x = (int)(line / ny);
+      y = (int)(line - (int64_t)x * ny);
+      dx_ = (int)(dline / ny);
+      dy_ = (int)(dline - (int64_t)dx_ * ny);
+    } else {
+      x = y = dx_ = dy_ = 0;
+    }
+  }
+  __device__ __forceinline__ void next(int64_t stride) {
+    i += stride;
+    int carry = 0;
+    z += dz;
+    if (z >= nz) {
+      z -= nz;
+      carry = 1;
+    }
+    line += dline + carry;
+    if (ny > 0) {
+      y += dy_ + carry;
+      x += dx_;
+      if (y >= ny) {
+        y -= ny;
+        ++x;
+      }
+    }
+  }
+};
+
 // ---------------------------------------------------------------------------
 // joint pattern W * S_k * g  (steam/simulate.py:1935-1963 with _advective_weight
 // 2839-2868 / 2787-2836 and _bound_taper 2905-2930): per cell, in the
@@ -43,11 +84,12 @@ __global__ __launch_bounds__(256) void pattern_kernel(
     const float* __restrict__ aspect_z, ZDiff zd, int nx, int ny, int nz,
     float two_dx, float two_dy, float lo, float span, float* __restrict__ out) {
   const int64_t n = (int64_t)nx * ny * nz;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(i % nz);
-    const int64_t line = i / nz;
-    const int x = (int)(line / ny), y = (int)(line - (int64_t)x * ny);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (ZWalk zw((int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride, nz, ny); zw.i < n;
+       zw.next(stride)) {
+    const int64_t i = zw.i;
+    const int z = zw.z;
+    const int x = zw.x, y = zw.y;
     const int xp = x + 1 < nx ? x + 1 : 0, xm = x > 0 ? x - 1 : nx - 1;
     const int yp = y + 1 < ny ? y + 1 : 0, ym = y > 0 ? y - 1 : ny - 1;
     const float mz = mean_z[z];
@@ -106,11 +148,12 @@ __global__ __launch_bounds__(256) void pattern_precise_kernel(
     const float* __restrict__ aspect_z, ZDiff zd, int nx, int ny, int nz, float two_dx,
     float two_dy, float* __restrict__ out) {
   const int64_t n = (int64_t)nx * ny * nz;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(i % nz);
-    const int64_t line = i / nz;
-    const int x = (int)(line / ny), y = (int)(line - (int64_t)x * ny);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (ZWalk zw((int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride, nz, ny); zw.i < n;
+       zw.next(stride)) {
+    const int64_t i = zw.i;
+    const int z = zw.z;
+    const int x = zw.x, y = zw.y;
     const int xp = x + 1 < nx ? x + 1 : 0, xm = x > 0 ? x - 1 : nx - 1;
     const int yp = y + 1 < ny ? y + 1 : 0, ym = y > 0 ? y - 1 : ny - 1;
     auto p = [&](int xx, int yy, int zz) { return pert[((int64_t)xx * ny + yy) * nz + zz]; };
@@ -146,9 +189,11 @@ __global__ __launch_bounds__(256) void pattern_precise_kernel(
 __global__ __launch_bounds__(256) void pattern_normalize_kernel(
     float* __restrict__ w, const float* __restrict__ level_mean,
     const float* __restrict__ c_z, int64_t n, int nz) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(i % nz);
+  const int64_t stride_ = (int64_t)gridDim.x * blockDim.x;
+  for (ZWalk zw((int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride_, nz); zw.i < n;
+       zw.next(stride_)) {
+    const int64_t i = zw.i;
+    const int z = zw.z;
     const float lm = level_mean[z];
     w[i] = __fmul_rn(__fdiv_rn(w[i], lm > 0.f ? lm : 1.f), c_z[z]);
   }
@@ -462,9 +507,11 @@ __global__ __launch_bounds__(256) void ba_shift_clip_kernel(
     const double* __restrict__ shift, int stride, int64_t n, int nz, float lo, float hi,
     const int* stop) {
   if (stop && *stop) return;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(i % nz);
+  const int64_t stride_ = (int64_t)gridDim.x * blockDim.x;
+  for (ZWalk zw((int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride_, nz); zw.i < n;
+       zw.next(stride_)) {
+    const int64_t i = zw.i;
+    const int z = zw.z;
     const float phi = __fadd_rn(pert[i], mean_z[z]);
     float v = d[i];
     if (shift) v = __fsub_rn(v, (float)shift[(int64_t)z * stride]);
@@ -479,9 +526,10 @@ __global__ __launch_bounds__(256) void ba_scale_kernel(float* __restrict__ d,
                                                        int stride, int64_t n, int nz,
                                                        const int* stop) {
   if (stop && *stop) return;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    d[i] = __fmul_rn(d[i], (float)scale[(int64_t)(i % nz) * stride]);
+  const int64_t stride_ = (int64_t)gridDim.x * blockDim.x;
+  for (ZWalk zw((int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride_, nz); zw.i < n;
+       zw.next(stride_))
+    d[zw.i] = __fmul_rn(d[zw.i], (float)scale[(int64_t)zw.z * stride]);
 }
 
 // e = d + phi (in place on d)
@@ -489,9 +537,11 @@ __global__ __launch_bounds__(256) void ba_to_field_kernel(float* __restrict__ d,
                                                           const float* __restrict__ pert,
                                                           const float* __restrict__ mean_z,
                                                           int64_t n, int nz) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(i % nz);
+  const int64_t stride_ = (int64_t)gridDim.x * blockDim.x;
+  for (ZWalk zw((int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride_, nz); zw.i < n;
+       zw.next(stride_)) {
+    const int64_t i = zw.i;
+    const int z = zw.z;
     d[i] = __fadd_rn(d[i], __fadd_rn(pert[i], mean_z[z]));
   }
 }
@@ -504,9 +554,11 @@ __global__ __launch_bounds__(256) void ba_finish_kernel(const float* e, float* _
                                                         const double* __restrict__ mu,
                                                         int64_t n, int nz, float lo, float hi,
                                                         int record) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(i % nz);
+  const int64_t stride_ = (int64_t)gridDim.x * blockDim.x;
+  for (ZWalk zw((int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride_, nz); zw.i < n;
+       zw.next(stride_)) {
+    const int64_t i = zw.i;
+    const int z = zw.z;
     float v = fminf(fmaxf(__fsub_rn(e[i], (float)mu[z]), lo), hi);
     v = __fsub_rn(v, mean_z[z]);
     if (record) inc[i] = __fsub_rn(v, pert[i]);
@@ -600,9 +652,11 @@ __global__ __launch_bounds__(256) void ba_final_kernel(float* __restrict__ d,
                                                        const double* __restrict__ pend,
                                                        const double* __restrict__ shift,
                                                        int64_t n, int nz, float lo, float hi) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(i % nz);
+  const int64_t stride_ = (int64_t)gridDim.x * blockDim.x;
+  for (ZWalk zw((int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride_, nz); zw.i < n;
+       zw.next(stride_)) {
+    const int64_t i = zw.i;
+    const int z = zw.z;
     const float phi = __fadd_rn(pert[i], mean_z[z]);
     const float u = __fsub_rn(__fmul_rn(d[i], (float)pend[z]), (float)shift[z]);
     const float e = fminf(fmaxf(__fadd_rn(u, phi), lo), hi);
@@ -620,9 +674,11 @@ __global__ __launch_bounds__(256) void deficit_kernel(float* __restrict__ defici
                                                       const float* __restrict__ before,
                                                       const float* __restrict__ coef_z,
                                                       int64_t n, int nz) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(i % nz);
+  const int64_t stride_ = (int64_t)gridDim.x * blockDim.x;
+  for (ZWalk zw((int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride_, nz); zw.i < n;
+       zw.next(stride_)) {
+    const int64_t i = zw.i;
+    const int z = zw.z;
     float inc = after[i];
     if (before) inc = __fsub_rn(inc, before[i]);
     deficit[i] = __fadd_rn(deficit[i], __fmul_rn(coef_z[z], inc));
@@ -644,9 +700,11 @@ __global__ __launch_bounds__(256) void compose_kernel(const float* src,
                                                       const float* __restrict__ mean_z,
                                                       float* out, int64_t n,
                                                       int nz, float lo, float hi) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(i % nz);
+  const int64_t stride_ = (int64_t)gridDim.x * blockDim.x;
+  for (ZWalk zw((int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride_, nz); zw.i < n;
+       zw.next(stride_)) {
+    const int64_t i = zw.i;
+    const int z = zw.z;
     out[i] = fminf(fmaxf(__fadd_rn(src[i], mean_z[z]), lo), hi);
   }
 }
@@ -674,9 +732,11 @@ __global__ __launch_bounds__(256) void proj_apply_kernel(float* __restrict__ f,
                                                          const double* __restrict__ active,
                                                          int64_t n, int nz, double lo,
                                                          double hi) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(i % nz);
+  const int64_t stride_ = (int64_t)gridDim.x * blockDim.x;
+  for (ZWalk zw((int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride_, nz); zw.i < n;
+       zw.next(stride_)) {
+    const int64_t i = zw.i;
+    const int z = zw.z;
     if (active[z] == 0.0) continue;
     const double m = (double)mean_z[z];
     const double e = (double)f[i] + m;
