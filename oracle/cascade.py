@@ -13,7 +13,10 @@ against.  Only ``tests/`` may import it.
   compose_output        steam/simulate.py:3558-3599
   compose_flux_output   steam/simulate.py:3602-3649
   refine_with_plan      steam/simulate.py:4092-4360 (the nest's cascade from
-                        its parent's state and per-class increments)
+                        its parent's state and per-class increments) and
+                        4366-4481 (the nest's record for a nest of it)
+  stage_nest_increments steam/simulate.py:3491-3555 (the whole-ladder
+                        increments a nest stores)
 
 Arithmetic follows the reference's float32 operation order on the fields and
 float64 for every reduction and solved scalar; CONVOLVE is the float64
@@ -25,7 +28,10 @@ root_dyadic / root_s2 / root_half (host path) and root_cuda (GPU path) pin the
 whole chain end to end (h, qt, flux and the three states); the projection
 against its three defining properties (bounds, level means, untouched
 levels) and brute force on a tiny level; the composition against its closed
-forms.
+forms; the nest's ladder against the identity of a spanning nest, the
+finest inherited class against the parent's own window, and the ladder
+replayed down its regrid chain against the nest's state (the reference's
+tests/test_refine.py:675-715 property).
 """
 
 import numpy as np
@@ -230,12 +236,57 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
     return h, qt, fl, dh, dqt, dfl, final, records
 
 
-def refine_with_plan(parent, plan):
+def stage_nest_increments(parent, plan, own):
+    """steam/simulate.py:3491-3555: the nest's class increments over the
+    whole root ladder.  Inherited class j is cut on its own grid by index:
+    along x, first = round(x_start / parent_nx * size_j) and
+    count = max(1, round((x_stop - x_start) / parent_nx * size_j)), wrapped
+    modulo size_j on an axis the nest spans and clipped to [0, size_j)
+    otherwise (y alike); its levels are those in [z_min, z_max).  The nest's
+    own class i is its padded increment cut to inner_windows[i] and the
+    same z range.  Returns a list of dicts (h, qt, flux, z)."""
+    zlo, zhi = plan['z_min'] - 1e-6, plan['z_max'] - 1e-6
+    xs, xe = plan['x_range']
+    ys, ye = plan['y_range']
+    wrap = plan['periodic']
+    out = []
+    for j in range(int(parent['n_classes_consumed'])):
+        cls = parent['class_increments'][j]
+        z = np.asarray(cls['z'], np.float64)
+        zk = [m for m in range(len(z)) if zlo <= z[m] < zhi]
+        idx = []
+        for axis, (a, b, n) in enumerate(((xs, xe, int(parent['nx'])),
+                                          (ys, ye, int(parent['ny'])))):
+            size = np.asarray(cls['h']).shape[axis]
+            first = int(round(a / n * size))
+            count = max(1, int(round((b - a) / n * size)))
+            idx.append([(first + t) % size if wrap[axis] else min(max(first + t, 0), size - 1)
+                        for t in range(count)])
+        entry = {'z': z[zk]}
+        for name in ('h', 'qt', 'flux'):
+            a = np.asarray(cls[name], F32)
+            entry[name] = a[np.asarray(idx[0])][:, np.asarray(idx[1])][:, :, np.asarray(zk, int)]
+        out.append(entry)
+    for i, cls in enumerate(own):
+        wx0, wx1, wy0, wy1 = plan['inner_windows'][i]
+        z = np.asarray(plan['grids']['z_arrays'][i], np.float64)
+        zk = [m for m in range(len(z)) if zlo <= z[m] < zhi]
+        entry = {'z': z[zk]}
+        for name in ('h', 'qt', 'flux'):
+            entry[name] = np.asarray(cls[name], F32)[wx0:wx1, wy0:wy1, zk[0]:zk[-1] + 1].copy()
+        out.append(entry)
+    return out
+
+
+def refine_with_plan(parent, plan, save_for_refinement=False):
     """steam/simulate.py:4092-4360 given the host plan of the nest (its
     grids, windows, anchoring, ladders and parent slab indices): the parent
     slab, the inherited deficits carried down the parent's regrid chain, the
     cascade over the nest's classes, the trim to the inner window and z
-    range, and the composition.  Returns the six fields."""
+    range, and the composition.  Returns the six fields, and with
+    ``save_for_refinement`` the nest's own parent record under 'parent'
+    (steam/simulate.py:4366-4481: trimmed state, whole-ladder increments,
+    continuation attributes)."""
     ix = np.ix_(plan['x_indices'], plan['y_indices'], plan['z_indices'])
     state = {k: np.ascontiguousarray(np.asarray(parent[k], dtype=F32)[ix])
              for k in ('h_perturbation', 'qt_perturbation', 'flux_state')}
@@ -254,6 +305,7 @@ def refine_with_plan(parent, plan):
             acc = acc + dj * inc
         inherited[name] = None if acc is None else np.ascontiguousarray(acc[ix])
     h_min, h_max, qt_min, qt_max = parent['bounds']
+    own = [] if save_for_refinement else None
     h, qt, fl, dh, dqt, dfl, final, _ = cascade_loop(
         parent['h_profile'], parent['qt_profile'], parent['z_profile'], plan['grids'],
         plan['C_h_k'], plan['C_qt_k'], plan['b_h_k'], plan['b_qt_k'], plan['aspect_k'],
@@ -266,7 +318,7 @@ def refine_with_plan(parent, plan):
         world_origins=plan['world_origins'], world_shapes=plan['world_shapes'],
         flux_noise_scale=float(parent['flux_noise_scale']),
         turbulon_shape=plan['turbulon_shape'], zero_bottom=plan['at_ground'],
-        zero_top=plan['at_top'], comp_k=plan['comp_k'])
+        zero_top=plan['at_top'], comp_k=plan['comp_k'], increments=own)
     wx0, wx1, wy0, wy1 = plan['inner_windows'][-1]
     z_full = np.asarray(final['z'])
     zi = np.where((z_full >= plan['z_min'] - 1e-6) & (z_full < plan['z_max'] - 1e-6))[0]
@@ -282,5 +334,23 @@ def refine_with_plan(parent, plan):
     H, _ = compose_output(inner(h), inner(dh), hm, h_min, h_max)
     Q, _ = compose_output(inner(qt), inner(dqt), qm, qt_min, qt_max)
     F, _ = compose_flux_output(inner(fl), inner(dfl))
-    return {'h': H, 'qt': Q, 'flux': F, 'h_perturbation': inner(h),
-            'qt_perturbation': inner(qt), 'flux_state': inner(fl)}
+    out = {'h': H, 'qt': Q, 'flux': F, 'h_perturbation': inner(h),
+           'qt_perturbation': inner(qt), 'flux_state': inner(fl)}
+    if save_for_refinement:
+        nx, ny = out['h'].shape[0], out['h'].shape[1]
+        ext_x, ext_y = plan['inner_extent']
+        rec = dict(parent)
+        rec.update({k: out[k] for k in ('h_perturbation', 'qt_perturbation', 'flux_state')})
+        # x, y of the first written cell: float32 axis values
+        rec['x0'] = float(F32(plan['x_range'][0] * float(parent['dx'])) + F32(parent['x0']))
+        rec['y0'] = float(F32(plan['y_range'][0] * float(parent['dy'])) + F32(parent['y0']))
+        rec.update(nx=nx, ny=ny, dx=ext_x / nx, dy=ext_y / ny,
+                   z=z_final.astype(np.float64), domain_z_min=plan['z_min'],
+                   domain_height=plan['z_max'] - plan['z_min'],
+                   n_classes_consumed=int(parent['n_classes_consumed']) + plan['n_classes'],
+                   sparsity_factors=plan['sparsity_factors'],
+                   turbulon_shape=plan['turbulon_shape'], anisotropy=plan['anisotropy'],
+                   periodic_x=plan['periodic'][0], periodic_y=plan['periodic'][1],
+                   class_increments=stage_nest_increments(parent, plan, own))
+        out['parent'] = rec
+    return out

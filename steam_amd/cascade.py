@@ -26,9 +26,11 @@ from . import simulate as _sim
 from ._device import check_device
 from .ledger import FluxOutputLedger, MeanReduction, ProjectionLedger, RunLedger
 from .params import _interpolation_compensation
-from .scalars import (LevelReducer, ZDifferences, _window, bounded_amplitude_add,
-                      joint_pattern, normalize_pattern, product_norm)
-from .utils import convolve_fft_xy_oa_z, regrid
+from .scalars import (LV_ABS, LV_NONZERO, LevelReducer, ZDifferences, _window,
+                      bounded_amplitude_add, joint_pattern, normalize_pattern,
+                      z_difference_coefficients)
+from .ledger import PatternLedger
+from .utils import conv_operator, convolve_fft_xy_oa_z, kernel_key, regrid
 
 
 def _dev32(a):
@@ -183,7 +185,22 @@ def _convolve(W, kernel_t, kernel_np, device):
 
 def _advance(flux, region, kernel_t, kernel_np, c, n_c, s, n_zero, zb, zt, device, window,
              ledger):
+    """The class's flux advance: (S_k, ledger or None, diagnostics).  With the
+    default seams the fused advance runs without waiting for its ledger (the
+    rescale and the S_k scale are applied on the device); a None ledger is
+    then read from the device once the class is done (_flux_ledger)."""
     adv = _sim.__dict__['_advance_flux']
+    fused = (adv is _sim._advance_flux_default
+             and _sim.CONVOLVE is convolve_fft_xy_oa_z
+             and _sim.__dict__['_keyed_sparse_levy'] is _sim._keyed_sparse_levy_impl)
+    if fused:
+        op = conv_operator(tuple(flux.shape), tuple(kernel_t.shape))
+        spec = op.spectrum(kernel_t, key=kernel_key(kernel_t))
+        sk = torch.empty_like(flux)
+        _sim.advance_flux_device(flux, region, kernel_t, c, n_c, s, n_zero, zb, zt,
+                                 window=window, ledger=ledger, sk_t=sk, op=op, spec=spec,
+                                 sync_ledger=False)
+        return sk, ledger, None
     if adv is _sim._advance_flux_default:
         return adv(flux, region, kernel_t, c, n_c, s, n_zero, zb, zt, device=device,
                    window=window, ledger=ledger)
@@ -291,6 +308,7 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
             h, qt, fl, dh, dqt, dfl = carried
         h_mean = np.interp(z_k, z_profile, h_profile).astype(np.float32)
         qt_mean = np.interp(z_k, z_profile, qt_profile).astype(np.float32)
+        nzk = shape[2]
         region = _sim.NoiseRegion(
             keys[i],
             (0, 0, 0) if world_origins is None else tuple(int(v) for v in world_origins[i]),
@@ -308,36 +326,61 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
             increments.append({})
         need_increment = has_deficit or record
         before = fl.clone() if need_increment else None
+        # every per-level vector of the class in one upload: the mean
+        # profiles, aspect ratios, taper buffers, amplitudes, the deficit
+        # factor and the z-difference coefficients
+        zc = z_difference_coefficients(z_k)
+        rows = [h_mean, qt_mean, aspect_k[i], b_h_k[i], b_qt_k[i], C_h_k[i], C_qt_k[i],
+                deficit_i]
+        if not zc[0]:
+            rows += [np.concatenate([c_, np.zeros(2, np.float32)]) for c_ in zc[2:5]]
+        vec = torch.from_numpy(np.stack([np.asarray(r, dtype=np.float32).reshape(nzk)
+                                         for r in rows])).cuda()
+        (h_mean_t, qt_mean_t, aspect_t, b_h_t, b_qt_t, C_h_t, C_qt_t, deficit_t) = vec[:8]
+        zdiff = ZDifferences(z_k, coefficients=None if zc[0] else vec[8:11, :nzk - 2])
         S_k, class_ledger.flux, _ = _advance(
             fl, region, kernel_t, kernel_np, c, n_scale_classes_per_dyad, sparsity_factors,
             n_zero, zero_bottom, zero_top, device, window, class_ledger.flux)
         if need_increment:
             if has_deficit:
-                deficit_accumulate(dfl, fl, deficit_i, before=before)
+                deficit_accumulate(dfl, fl, deficit_t, before=before)
             if record:
                 _record(increment_dir, increments, i, 'flux', (fl - before).cpu().numpy())
             del before
-        zdiff = ZDifferences(z_k)
         reducer = LevelReducer(shape)
-        for name, pert, dfield, mean_1d, C_i, b_i, lo, hi in (
-                ('h', h, dh, h_mean, C_h_k[i], b_h_k[i], h_min, h_max),
-                ('qt', qt, dqt, qt_mean, C_qt_k[i], b_qt_k[i], qt_min, qt_max)):
-            mean_t = _dev32(mean_1d)
-            W = joint_pattern(pert, mean_t, S_k, b_i, aspect_k[i], z_k, dx_k, dy_k, lo, hi,
+        measured = {}
+        for name, pert, dfield, mean_t, C_t, b_t, lo, hi in (
+                ('h', h, dh, h_mean_t, C_h_t, b_h_t, h_min, h_max),
+                ('qt', qt, dqt, qt_mean_t, C_qt_t, b_qt_t, qt_min, qt_max)):
+            W = joint_pattern(pert, mean_t, S_k, b_t, aspect_t, z_k, dx_k, dy_k, lo, hi,
                               zdiff=zdiff)
             if class_ledger.pattern.get(name) is None:
-                class_ledger.pattern[name] = product_norm(W, window, reducer)
-            normalize_pattern(W, class_ledger.pattern[name].level_mean, C_i)
+                # the product norm stays on the device until the class is done
+                total = reducer(W, LV_ABS, window)
+                count = reducer(W, LV_NONZERO, window)
+                measured[name] = (total, count)
+                level_mean = (total / torch.clamp(count, min=1.0)).to(torch.float32)
+            else:
+                level_mean = class_ledger.pattern[name].level_mean
+            normalize_pattern(W, level_mean, C_t)
             increment = _convolve(W, kernel_t, kernel_np, device)
             del W
             class_ledger.bounded_add[name] = bounded_amplitude_add(
                 pert, mean_t, increment, lo, hi, window=window, record_applied=True,
                 ledger=class_ledger.bounded_add.get(name))
             if dfield is not None:
-                deficit_accumulate(dfield, increment, deficit_i)
+                deficit_accumulate(dfield, increment, deficit_t)
             if record:
                 _record(increment_dir, increments, i, name, increment.cpu().numpy())
             del increment
+        if measured:
+            got = torch.stack([t for pair in measured.values() for t in pair]).cpu().numpy()
+            for k, name in enumerate(measured):
+                class_ledger.pattern[name] = PatternLedger(
+                    got[2 * k], np.rint(got[2 * k + 1]).astype(np.int64))
+        if class_ledger.flux is None:
+            class_ledger.flux, _ = _sim.read_device_ledger(tuple(fl.shape),
+                                                           tuple(kernel_t.shape))
         del S_k
         prev = {'dx': dx_k, 'dy': dy_k, 'dz': float(grids['dz'][i]),
                 'ext_x': float(grids['padded_extent_x'][i]),
@@ -426,7 +469,9 @@ def simulate_fields(h_profile, qt_profile, nx, ny, dx, dy, outer_scale, spherosc
             'bound_buffer_multiple': st['bound_buffer_multiple'],
             'sparsity_factors': tuple(sparsity_factors), 'turbulon_shape': turbulon_shape,
             'anisotropy': anisotropy, 'periodic_x': True, 'periodic_y': True,
-            'class_increments': [dict(inc, z=np.asarray(g['z_arrays'][j], dtype=np.float64))
+            'class_increments': [dict(inc, z=np.asarray(g['z_arrays'][j], dtype=np.float64),
+                                      k=float(g['k'][j]), dx=float(g['dx'][j]),
+                                      dy=float(g['dy'][j]))
                                  for j, inc in enumerate(increments)],
         }
     if not as_tensors:

@@ -9,7 +9,9 @@ class ladder, pads, grids and inner windows, the parent slab indices, the
 world anchoring, the continued seeds and the amplitude ladders);
 ``refine_fields`` gathers the parent slab and the inherited compensation
 deficits on the GPU, runs the device cascade over the nest's classes, trims
-the halo, and composes the written fields.  Reading and writing the file,
+the halo, and composes the written fields; with ``save_for_refinement`` it
+also returns the nest's own record (``nest_class_ladder``,
+``nest_parent_record``), so nests refine recursively.  Reading and writing the file,
 and the pressure diagnostics that only feed the written ``p_bottom``, are not
 part of this path.
 """
@@ -155,7 +157,91 @@ def plan_nest(parent, x_start, x_stop, y_start, y_stop, dx, dy, seed=None,
             'sparsity_factors': s, 'turbulon_shape': shape_name, 'anisotropy': aniso,
             'z_min': z_min, 'z_max': z_max, 'at_ground': at_ground, 'at_top': at_top,
             'inner_extent': (ext_x, ext_y), 'new_outer_scale': new_outer,
-            'n_classes': n_classes}
+            'n_classes': n_classes, 'x_range': (int(x_start), int(x_stop)),
+            'y_range': (int(y_start), int(y_stop)), 'periodic': (spans_x, spans_y)}
+
+
+def nest_class_ladder(parent, plan, own):
+    """The nest's stored class increments over the whole ladder
+    (steam/simulate.py:3491-3555): the parent's classes restricted by index
+    to the nest's extent on each class's own grid, then the nest's own
+    classes (``own``: per class dicts of h, qt, flux on the padded working
+    grid) cut to their inner windows; both keep the levels in
+    [z_min, z_max).  A nest of this nest then sees one uniform ladder."""
+    z_min, z_max = plan['z_min'], plan['z_max']
+    x_start, x_stop = plan['x_range']
+    y_start, y_stop = plan['y_range']
+    per_x, per_y = plan['periodic']
+    pnx, pny = int(parent['nx']), int(parent['ny'])
+    gap = 2.0 ** (1.0 / int(parent['n_scale_classes_per_dyad']))
+
+    def levels(z):
+        return np.where((z >= z_min - 1e-6) & (z < z_max - 1e-6))[0]
+
+    ladder = []
+    for j, cls in enumerate(parent['class_increments'][:int(parent['n_classes_consumed'])]):
+        z_j = np.asarray(cls['z'], dtype=np.float64)
+        keep_z = levels(z_j)
+        size_x, size_y = np.shape(cls['h'])[:2]
+        keep = []
+        for size, start, stop, extent, periodic in (
+                (size_x, x_start, x_stop, pnx, per_x), (size_y, y_start, y_stop, pny, per_y)):
+            first = int(round(start / extent * size))
+            count = max(1, int(round((stop - start) / extent * size)))
+            index = np.arange(first, first + count)
+            keep.append(index % size if periodic else np.clip(index, 0, size - 1))
+        ix = np.ix_(keep[0], keep[1], keep_z)
+        entry = {name: np.ascontiguousarray(np.asarray(cls[name], dtype=np.float32)[ix])
+                 for name in ('h', 'qt', 'flux')}
+        entry.update(k=float(cls.get('k', float(parent['root_outer_scale']) / gap ** j)),
+                     dx=float(cls.get('dx', np.nan)), dy=float(cls.get('dy', np.nan)),
+                     z=z_j[keep_z])
+        ladder.append(entry)
+    grids = plan['grids']
+    for i, cls in enumerate(own):
+        x0, x1, y0, y1 = plan['inner_windows'][i]
+        z_i = np.asarray(grids['z_arrays'][i], dtype=np.float64)
+        keep_z = levels(z_i)
+        z0, z1 = int(keep_z[0]), int(keep_z[-1]) + 1
+        entry = {name: np.ascontiguousarray(np.asarray(cls[name], dtype=np.float32)
+                                            [x0:x1, y0:y1, z0:z1])
+                 for name in ('h', 'qt', 'flux')}
+        entry.update(k=float(grids['k'][i]), dx=float(grids['dx'][i]),
+                     dy=float(grids['dy'][i]), z=z_i[keep_z])
+        ladder.append(entry)
+    return ladder
+
+
+def nest_parent_record(parent, plan, state, z_final, ladder):
+    """What the reference writes into a nest's group for a nest of it to
+    continue from (steam/simulate.py:4366-4481, save_for_refinement): the
+    trimmed state, the whole ladder's class increments and the continuation
+    attributes, with the world position of the nest's first cell and the
+    per-axis periodicity (a nest wraps only along an axis it spans)."""
+    nx_out, ny_out = (int(v) for v in np.shape(state['h_perturbation'])[:2])
+    ext_x, ext_y = plan['inner_extent']
+    x_start, y_start = plan['x_range'][0], plan['y_range'][0]
+    # the first written coordinate, as the file's float32 axis holds it
+    x0 = float(np.float32(np.float32(x_start * float(parent['dx']))
+                          + np.float32(parent.get('x0', 0.0))))
+    y0 = float(np.float32(np.float32(y_start * float(parent['dy']))
+                          + np.float32(parent.get('y0', 0.0))))
+    rec = {k: parent[k] for k in (
+        'h_profile', 'qt_profile', 'z_profile', 'spheroscale_profile', 'profile_dz',
+        'bounds', 'min_distance_to_ground', 'n_scale_classes_per_dyad', 'root_domain_x',
+        'root_domain_y', 'root_domain_height', 'root_outer_scale', 'root_seed', 'H_h',
+        'haar_to_mhat', 'flux_noise_scale', 'bound_buffer_multiple') if k in parent}
+    rec.update({name: np.asarray(state[name], dtype=np.float32)
+                for name in ('h_perturbation', 'qt_perturbation', 'flux_state')})
+    rec.update(x0=x0, y0=y0, nx=nx_out, ny=ny_out, dx=ext_x / nx_out, dy=ext_y / ny_out,
+               z=np.asarray(z_final, dtype=np.float32).astype(np.float64),
+               domain_height=plan['z_max'] - plan['z_min'], domain_z_min=plan['z_min'],
+               n_classes_consumed=int(parent['n_classes_consumed']) + plan['n_classes'],
+               sparsity_factors=tuple(plan['sparsity_factors']),
+               turbulon_shape=plan['turbulon_shape'], anisotropy=plan['anisotropy'],
+               periodic_x=bool(plan['periodic'][0]), periodic_y=bool(plan['periodic'][1]),
+               class_increments=ladder)
+    return rec
 
 
 def _slab(t, plan):
@@ -189,12 +275,15 @@ def _inherited_deficits(parent, plan):
 
 def refine_fields(parent, x_start, x_stop, y_start, y_stop, dx, dy, seed=None,
                   sparsity_factors=None, turbulon_shape=None, z_min=None, z_max=None,
-                  anisotropy=None, device='cuda', ledger=None, as_tensors=False):
+                  anisotropy=None, device='cuda', ledger=None, as_tensors=False,
+                  save_for_refinement=False):
     """The nest's fields (steam/simulate.py:3682-4420 without the file): the
     cascade continued over the subdomain below the parent's finest class,
     the halo trimmed, the written h, qt and flux composed.  Returns a dict
     like ``simulate_fields``: 'h', 'qt', 'flux', 'h_perturbation',
-    'qt_perturbation', 'flux_state', 'ledger', 'grids', 'plan'."""
+    'qt_perturbation', 'flux_state', 'ledger', 'grids', 'plan', and with
+    ``save_for_refinement`` the nest's own 'parent' record, from which a
+    nest of this nest continues."""
     check_device(device)
     plan = plan_nest(parent, x_start, x_stop, y_start, y_stop, dx, dy, seed,
                      sparsity_factors, turbulon_shape, z_min, z_max, anisotropy)
@@ -204,6 +293,7 @@ def refine_fields(parent, x_start, x_stop, y_start, y_stop, dx, dy, seed=None,
                          .cuda(), plan)
              for name in ('h_perturbation', 'qt_perturbation', 'flux_state')}
     inherited = _inherited_deficits(p, plan)
+    own = [] if save_for_refinement else None
     (h, qt, fl, dh, dqt, dfl, final, run_ledger) = cascade_loop(
         p['h_profile'], p['qt_profile'], p['z_profile'], plan['grids'], plan['C_h_k'],
         plan['C_qt_k'], plan['b_h_k'], plan['b_qt_k'], plan['aspect_k'], h_min, h_max,
@@ -215,7 +305,8 @@ def refine_fields(parent, x_start, x_stop, y_start, y_stop, dx, dy, seed=None,
         world_origins=plan['world_origins'], world_shapes=plan['world_shapes'],
         ledger=ledger, flux_noise_scale=float(p['flux_noise_scale']),
         turbulon_shape=plan['turbulon_shape'], zero_bottom=plan['at_ground'],
-        zero_top=plan['at_top'], device=device, comp_k=plan['comp_k'], as_tensors=True)
+        zero_top=plan['at_top'], device=device, comp_k=plan['comp_k'], as_tensors=True,
+        increments=own)
     wx0, wx1, wy0, wy1 = plan['inner_windows'][-1]
     z_full = np.asarray(final['z'])
     zi = np.where((z_full >= plan['z_min'] - 1e-6) & (z_full < plan['z_max'] - 1e-6))[0]
@@ -244,7 +335,15 @@ def refine_fields(parent, x_start, x_stop, y_start, y_stop, dx, dy, seed=None,
         run_ledger.projection['qt'] = proj_qt
     out = {'h': h_out, 'qt': qt_out, 'flux': flux_out, 'h_perturbation': h_in,
            'qt_perturbation': qt_in, 'flux_state': fl_in}
+    record = None
+    if save_for_refinement:
+        state = {k: out[k].cpu().numpy()
+                 for k in ('h_perturbation', 'qt_perturbation', 'flux_state')}
+        record = nest_parent_record(p, plan, state, z_final,
+                                    nest_class_ladder(p, plan, own))
     if not as_tensors:
         out = {k: v.cpu().numpy() for k, v in out.items()}
     out.update(ledger=run_ledger, grids=plan['grids'], plan=plan)
+    if record is not None:
+        out['parent'] = record
     return out

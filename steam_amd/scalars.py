@@ -45,9 +45,11 @@ def z_difference_coefficients(z_coords):
 
 
 class ZDifferences:
-    """Device form of the z differences of one z grid."""
+    """Device form of the z differences of one z grid.  ``coefficients``
+    optionally supplies the non-uniform grid's a, b, c rows already on the
+    device (a (3, nz-2) float32 tensor whose rows are contiguous)."""
 
-    def __init__(self, z_coords, device='cuda'):
+    def __init__(self, z_coords, device='cuda', coefficients=None):
         uniform, two_dz, a, b, c, first, last = z_difference_coefficients(z_coords)
         self.native = _lib.ZDiff()
         self.native.uniform = 1 if uniform else 0
@@ -56,8 +58,14 @@ class ZDifferences:
         self.native.dz_last = float(last)
         self._keep = []
         if not uniform:
-            for name, arr in (("a_dev", a), ("b_dev", b), ("c_dev", c)):
-                t = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float32)).to(device)
+            if coefficients is not None:
+                rows = [coefficients[k] for k in range(3)]
+            else:
+                rows = [torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float32)).to(device)
+                        for arr in (a, b, c)]
+            for name, t in zip(("a_dev", "b_dev", "c_dev"), rows):
+                if t.dim() != 1 or t.stride(0) != 1 or t.numel() != len(a):
+                    raise ValueError("z-difference coefficient rows must be contiguous")
                 self._keep.append(t)
                 setattr(self.native, name, t.data_ptr())
 

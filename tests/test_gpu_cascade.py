@@ -232,3 +232,45 @@ def test_gpu_nest_matches_reference_fixture_and_replays():
     again = refine_fields(parent, 4, 12, 4, 12, 62.5, 62.5, ledger=out['ledger'])
     for field in FIELDS:
         np.testing.assert_array_equal(again[field], out[field], err_msg=field)
+
+
+def test_gpu_nest_of_a_nest_matches_the_oracle():
+    """Recursive refinement on the GPU: root_dyadic, a nest over [0, 8)^2 at
+    dx 62.5 saved for refinement, and a nest of that nest in the reference's
+    test geometry at dx 31.25; every stage's fields and the nest's stored
+    ladder against the oracle's chain, and the grandchild replayed."""
+    from steam_amd import cascade as C
+    from steam_amd.refine import plan_nest, refine_fields
+    from test_oracle_cascade import oracle_nest, oracle_parent
+    root = C.simulate_fields(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0,
+                             800.0, 30.0, seed=42, save_for_refinement=True)
+    o_parent = oracle_parent(fixture_setup((1, 1, 1), 1))
+    r0 = refine_fields(root['parent'], 0, 8, 0, 8, 62.5, 62.5, seed=11,
+                       save_for_refinement=True)
+    _, o_r0 = oracle_nest(o_parent, 0, 8, 62.5, seed=11)
+    for field, tol in TOL.items():
+        assert rel(r0[field], o_r0[field]) < tol, ('r0', field, rel(r0[field], o_r0[field]))
+    nest, o_nest = r0['parent'], o_r0['parent']
+    for key in ('nx', 'ny', 'dx', 'dy', 'x0', 'y0', 'n_classes_consumed', 'periodic_x',
+                'periodic_y', 'domain_z_min', 'domain_height'):
+        assert nest[key] == o_nest[key], key
+    np.testing.assert_array_equal(nest['z'], o_nest['z'])
+    assert len(nest['class_increments']) == len(o_nest['class_increments']) == 5
+    for j, (a, b) in enumerate(zip(nest['class_increments'], o_nest['class_increments'])):
+        np.testing.assert_array_equal(a['z'], b['z'])
+        for name in ('h', 'qt', 'flux'):
+            assert a[name].shape == b[name].shape, (j, name)
+            if np.any(b[name]):
+                assert rel(a[name], b[name]) < 5e-5, (j, name, rel(a[name], b[name]))
+    start, width, dx2 = 6, 2, 31.25
+    r1 = refine_fields(nest, start, start + width, start, start + width, dx2, dx2, seed=13)
+    o_plan = plan_nest(o_nest, start, start + width, start, start + width, dx2, dx2, seed=13)
+    o_r1 = OC.refine_with_plan(o_nest, o_plan)
+    assert r1['plan']['n_classes'] == 1 and np.all(r1['plan']['k_values'] < 125.0)
+    for field, tol in TOL.items():
+        assert r1[field].shape == o_r1[field].shape, field
+        assert rel(r1[field], o_r1[field]) < 2 * tol, ('r1', field, rel(r1[field], o_r1[field]))
+    again = refine_fields(nest, start, start + width, start, start + width, dx2, dx2, seed=13,
+                          ledger=r1['ledger'])
+    for field in FIELDS:
+        np.testing.assert_array_equal(again[field], r1[field], err_msg=field)

@@ -313,3 +313,133 @@ def test_plan_nest_rejects_what_the_reference_rejects():
         plan_nest(parent, 4, 12, 4, 12, 62.5, 62.5, z_max=900.0)
     with pytest.raises(ValueError):
         plan_nest(dict(parent, anisotropy='bogus'), 4, 12, 4, 12, 62.5, 62.5)
+
+
+# ---------------------------------------------------------------------------
+# recursive refinement: a nest's whole-ladder record and a nest of a nest
+# (tests/test_refine.py:455-510, 675-715 of the reference)
+# ---------------------------------------------------------------------------
+
+def oracle_nest(parent, x0, x1, dx, seed, **kw):
+    from steam_amd.refine import plan_nest
+    plan = plan_nest(parent, x0, x1, x0, x1, dx, dx, seed=seed, **kw)
+    return plan, OC.refine_with_plan(parent, plan, save_for_refinement=True)
+
+
+def replay_ladder(ladder, name):
+    """Each stored class carried down the rest of the ladder's regrid chain
+    and summed (float64)."""
+    total = None
+    for j, cls in enumerate(ladder):
+        cur = np.asarray(cls[name], np.float32)
+        for m in range(j + 1, len(ladder)):
+            cur = OC._regrid32(cur, np.shape(ladder[m][name]))
+        total = cur.astype(np.float64) if total is None else total + cur
+    return total
+
+
+@pytest.fixture(scope="module")
+def dyadic_parent():
+    return oracle_parent(fixture_setup((1, 1, 1), 1))
+
+
+def test_spanning_nest_ladder_is_the_identity_and_replays_to_the_state(dyadic_parent):
+    parent = dyadic_parent
+    plan, out = oracle_nest(parent, 0, 16, 62.5, seed=5)
+    rec = out['parent']
+    ladder = rec['class_increments']
+    assert len(ladder) == 5 and rec['n_classes_consumed'] == 5
+    assert rec['periodic_x'] and rec['periodic_y']
+    for j in range(4):      # spanning the parent: the restriction is the identity
+        for name in ('h', 'qt', 'flux', 'z'):
+            np.testing.assert_array_equal(ladder[j][name], parent['class_increments'][j][name])
+    # the sum of the ladder replayed down its regrid chain is the nest's state
+    for name, state in (('h', 'h_perturbation'), ('qt', 'qt_perturbation')):
+        total = replay_ladder(ladder, name)
+        st = np.asarray(rec[state], np.float64)
+        assert total.shape == st.shape
+        scale = np.abs(st).mean()
+        assert scale > 0
+        # (the regrids are linear: equal up to float32 rounding; the
+        # reference's own bound is 5e-3 * scale)
+        assert np.abs(total - st).max() < 1e-5 * scale, name
+        # a dropped class breaks it
+        short = replay_ladder(ladder[:2] + ladder[3:], name)
+        assert np.abs(short - st).max() > 5e-3 * scale
+
+
+def test_subrange_nest_ladder_restricts_each_class_on_its_own_grid(dyadic_parent):
+    parent = dyadic_parent
+    plan, out = oracle_nest(parent, 4, 12, 62.5, seed=7)
+    rec = out['parent']
+    assert not rec['periodic_x'] and not rec['periodic_y']
+    assert rec['x0'] == 500.0 and rec['nx'] == 16 and rec['dx'] == 62.5
+    ladder = rec['class_increments']
+    # the nest's extent [500, 1500) m on classes of 2, 4, 8, 16 cells: the
+    # index cut (half-way starts round to even)
+    expected = {2: [0], 4: [1, 2], 8: [2, 3, 4, 5], 16: list(range(4, 12))}
+    for j in range(4):
+        src = parent['class_increments'][j]
+        idx = expected[src['h'].shape[0]]
+        for name in ('h', 'qt', 'flux'):
+            np.testing.assert_array_equal(ladder[j][name], src[name][idx][:, idx])
+    # the nest's own class: its padded increment cut to the inner window
+    assert ladder[4]['h'].shape == rec['h_perturbation'].shape
+    # the finest inherited class replayed alone onto the nest grid equals the
+    # same replay of the parent's class cut to the nest's window
+    a = OC._regrid32(ladder[3]['h'], ladder[4]['h'].shape)
+    b = OC._regrid32(np.ascontiguousarray(parent['class_increments'][3]['h'][4:12, 4:12]),
+                     ladder[4]['h'].shape)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_product_ladder_staging_equals_the_oracle(dyadic_parent):
+    """steam_amd.refine.nest_class_ladder (host code) against the oracle's
+    restatement on the same parent and own increments, spanning and not,
+    and on an elevated nest (z cut)."""
+    from steam_amd.refine import nest_class_ladder, plan_nest
+    rng = np.random.default_rng(3)
+    for x0, x1, kw in ((0, 16, {}), (4, 12, {}), (2, 9, {}), (4, 12, {'z_min': 240.0})):
+        try:
+            plan = plan_nest(dyadic_parent, x0, x1, x0, x1, 62.5, 62.5, **kw)
+        except ValueError:
+            continue
+        own = [{n: rng.standard_normal((int(g0), int(g1), int(g2))).astype(np.float32)
+                for n in ('h', 'qt', 'flux')}
+               for g0, g1, g2 in zip(plan['grids']['nx'], plan['grids']['ny'],
+                                     plan['grids']['nz'])]
+        a = nest_class_ladder(dyadic_parent, plan, own)
+        b = OC.stage_nest_increments(dyadic_parent, plan, own)
+        assert len(a) == len(b)
+        for ca, cb in zip(a, b):
+            for name in ('h', 'qt', 'flux', 'z'):
+                np.testing.assert_array_equal(ca[name], cb[name])
+
+
+def test_recursive_refinement_inside_a_nest(dyadic_parent):
+    """A nest of a nest (the reference's test geometry): the grandchild's
+    classes are strictly finer, its fields finite and bounded, its ladder
+    one class longer; a grandchild reaching past the nest's edge, or
+    spanning it, is refused (the nest is not periodic there)."""
+    from steam_amd.refine import plan_nest
+    from steam_amd.simulate import SUPPORT_FACTOR
+    _, r0 = oracle_nest(dyadic_parent, 0, 8, 62.5, seed=11)
+    nest = r0['parent']
+    k_finest = 125.0
+    pad_cells = int(np.ceil(SUPPORT_FACTOR * k_finest / nest['dx']))
+    inner_cells = int(round(k_finest / nest['dx']))
+    start = max(pad_cells, nest['nx'] // 4)
+    assert start + inner_cells <= nest['nx'] - pad_cells
+    plan, r1 = oracle_nest(nest, start, start + inner_cells, nest['dx'] / 2, seed=13)
+    assert np.all(plan['k_values'] < k_finest)
+    assert r1['parent']['n_classes_consumed'] == 6
+    assert len(r1['parent']['class_increments']) == 6
+    hmin, hmax, qmin, qmax = nest['bounds']
+    for f in ('h', 'qt', 'flux', 'h_perturbation', 'qt_perturbation', 'flux_state'):
+        assert np.all(np.isfinite(r1[f])), f
+    assert r1['h'].min() >= np.float32(hmin) and r1['h'].max() <= np.float32(hmax)
+    assert r1['flux'].min() >= 0
+    with pytest.raises(ValueError, match="non-periodic parent"):
+        plan_nest(nest, 0, inner_cells, 0, inner_cells, nest['dx'] / 2, nest['dy'] / 2, seed=19)
+    with pytest.raises(ValueError, match="non-periodic parent"):
+        plan_nest(nest, 0, nest['nx'], 0, nest['ny'], nest['dx'] / 2, nest['dy'] / 2, seed=21)
