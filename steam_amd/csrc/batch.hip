@@ -94,7 +94,7 @@ std::tuple<int, int, int> variant_of(const ItemPlan& ip, int pass) {
       if (g.reg_y) return {1, g.reg_y, 256};
       return {2, radix_set(g.ay), FFT_NTH};
     case BP_X:
-      if (g.reg_x) return {1, g.reg_x, (g.nx / 8) * g.Tr};
+      if (g.reg_x) return {1, g.reg_x, (g.nx / 8 / xreg_lanes(g.reg_x)) * g.Tr};
       return {2, radix_set(g.ax), FFT_NTH};
     case BP_FIN: return {0, 0, FFT_NTH};
     default: return {ip.post ? 0 : -1, ip.post_rescale ? 1 : 0, FFT_NTH};
@@ -193,8 +193,9 @@ int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
         break;
       case BP_X:
         if (gr.variant == 1) {
-          STEAM_LOGN_DISPATCH(gr.arg, hipLaunchKernelGGL((xpass_reg_kernel<BatchSrc, 0, LN_>), grid,
-                                                         dim3(gr.threads), 0, s, src));
+          STEAM_LOGN_DISPATCH(gr.arg, hipLaunchKernelGGL(
+              (xpass_reg_kernel<BatchSrc, 0, LN_, xreg_lanes(LN_)>), grid, dim3(gr.threads), 0, s,
+              src));
         } else {
           STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((xpass_kernel<BatchSrc, 0, RS_>), grid,
                                                        dim3(FFT_NTH), gr.lds, s, src));

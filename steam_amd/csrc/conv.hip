@@ -95,8 +95,7 @@ static void set_tiles(ConvGeom& g, int nz_acc, int kahan, size_t& lds_z,
     g.wg_y = (int)((P * g.nx + g.Ty - 1) / g.Ty);
   }
   if (g.reg_x) {
-    const int pth = g.nx / 8;
-    g.Tr = (1024 / pth < XREG_COLS) ? 1024 / pth : XREG_COLS;
+    g.Tr = xreg_cols(g.reg_x);
     g.wg_x = (int)(P * ((g.ny + g.Tr - 1) / g.Tr));
   } else {
     g.Tr = 0;

@@ -49,9 +49,41 @@
 #ifndef STEAM_ZINV_ATTR
 #define STEAM_ZINV_ATTR
 #endif
+// register z passes: move the spectral planes through a workgroup tile in LDS
+// so that each wave access covers all the workgroup's consecutive lines of
+// a plane (1), or access them straight from the lanes' registers (0)
+#ifndef STEAM_ZR_TILE
+#define STEAM_ZR_TILE 1
+#endif
+// register x pass: issue the kernel-spectrum loads with the line's loads,
+// before the forward transform (1), or after it (0)
+#ifndef STEAM_XREG_KPREFETCH
+#define STEAM_XREG_KPREFETCH 1
+#endif
+// register x pass with two transform lanes per thread: y columns per workgroup
+#ifndef STEAM_XREG_COLS2
+#define STEAM_XREG_COLS2 16
+#endif
+// register x pass: largest log2(nx) taking two lanes per thread (0: none).
+// Off: the forward * spectrum * inverse form of two lanes needs more than
+// the 128 registers of two 512-thread workgroups per CU and spills.
+#ifndef STEAM_XREG_C2_MAXLOG
+#define STEAM_XREG_C2_MAXLOG 0
+#endif
 
 namespace steam {
 constexpr int XREG_COLS = STEAM_XREG_COLS;  // y columns per register x-pass workgroup
+constexpr int XREG_COLS2 = STEAM_XREG_COLS2;  // the same with two lanes per thread
+// transform lanes per thread of the register x pass for an nx of 2^logn
+__host__ __device__ constexpr int xreg_lanes(int logn) {
+  return logn >= 6 && logn <= STEAM_XREG_C2_MAXLOG ? 2 : 1;
+}
+// y columns per workgroup of the register x pass
+__host__ __device__ constexpr int xreg_cols(int logn) {
+  const int pt = (1 << logn) / 8 / xreg_lanes(logn);
+  const int cap = xreg_lanes(logn) == 2 ? XREG_COLS2 : XREG_COLS;
+  return 1024 / pt < cap ? 1024 / pt : cap;
+}
 // z passes: 512 threads holding 8 values each (same tile as 256 x 16, half
 // the registers per thread)
 constexpr int Z_NTH = 512;
@@ -389,46 +421,72 @@ __global__ __launch_bounds__(256) void ypass_reg_kernel(Src src, int inv) {
   }
 }
 
-// register-FFT x pass (power-of-two nx): P = nx/8 threads per x-line, Tr
-// adjacent y columns per workgroup (lane t = column, so each row access is a
-// Tr*8-byte run); XMODE 0: forward, * kernel spectrum, inverse; 1: forward,
-// * scale.
-template <class Src, int XMODE, int LOGN>
+// register-FFT x pass (power-of-two nx): each x-line is held by P / C
+// threads (P = nx/8 lanes of the radix-8 transform, C lanes per thread), Tr
+// adjacent y columns per workgroup (thread t % Tr = column, so each row
+// access is a Tr*8-byte run); XMODE 0: forward, * kernel spectrum, inverse;
+// 1: forward, * scale.
+template <class Src, int XMODE, int LOGN, int C>
 __global__ __launch_bounds__(1024) void xpass_reg_kernel(Src src) {
   using F = RegFFT<LOGN>;
-  constexpr int P = F::P;
-  constexpr int T = (1024 / P < XREG_COLS) ? 1024 / P : XREG_COLS;
+  constexpr int P = F::P, PT = P / C;
+  constexpr int T = (1024 / PT < (C == 1 ? XREG_COLS : XREG_COLS2))
+                        ? 1024 / PT : (C == 1 ? XREG_COLS : XREG_COLS2);
   __shared__ float2 lds[T * F::LDSN];
   int wg = xcd_remap(blockIdx.x, gridDim.x);
   const PassDesc& d = src.get(PASS_X, wg);
   const ConvGeom& g = d.g;
   const int tid = threadIdx.x;
-  const int t = tid % T, q = tid / T;
+  const int t = tid % T, q0 = tid / T;
   const int nyt = (g.ny + T - 1) / T;
   const int p = wg / nyt;
   const int y = (wg - p * nyt) * T + t;
   const bool valid = y < g.ny;
   const int nx = g.nx, ny = g.ny;
-  float2* Sp = d.S + (int64_t)p * nx * ny + (valid ? y : 0);
-  float2 v[8];
+  // plane bases are uniform over the workgroup; lane offsets (< nx * ny) are
+  // 32-bit, so the addresses of the final stores are cheap to re-form
+  float2* Sb = d.S + (int64_t)p * nx * ny;
+  const int yo = valid ? y : 0;
+  auto off = [&](int c, int m) { return (q0 + c * PT + m * P) * ny + yo; };
+  float2 v[C][8], kv[C][8];
 #pragma unroll
-  for (int m = 0; m < 8; ++m)
-    v[m] = valid ? Sp[(int64_t)(q + m * P) * ny] : make_float2(0.f, 0.f);
-  fft_reg<LOGN>(v, q, lds, T, t, g.ax.tw, false);
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[c][m] = valid ? Sb[off(c, m)] : make_float2(0.f, 0.f);
+  const float2* Kb = nullptr;
+  auto load_k = [&]() {
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int m = 0; m < 8; ++m) kv[c][m] = valid ? Kb[off(c, m)] : make_float2(0.f, 0.f);
+  };
+  // one lane per thread: the spectrum loads go out with the line's; two
+  // lanes: after the forward transform (the registers of both would spill)
+  constexpr bool KPRE = STEAM_XREG_KPREFETCH && C == 1;
   if (XMODE == 0) {
-    const float2* Kp = d.K + (int64_t)(p % g.nzc) * nx * ny + (valid ? y : 0);
+    Kb = d.K + (int64_t)(p % g.nzc) * nx * ny;
+    if (KPRE) load_k();
+  }
+  fft_reg_lanes<LOGN, C>(v, q0, PT, lds, T, t, g.ax.tw, false);
+  if (XMODE == 0) {
+    if (!KPRE) load_k();
 #pragma unroll
-    for (int m = 0; m < 8; ++m)
-      v[m] = cmul(v[m], valid ? Kp[(int64_t)(q + m * P) * ny] : make_float2(0.f, 0.f));
-    fft_reg<LOGN>(v, q, lds, T, t, g.ax.tw, true);
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[c][m] = cmul(v[c][m], kv[c][m]);
+    fft_reg_lanes<LOGN, C>(v, q0, PT, lds, T, t, g.ax.tw, true);
   } else {
     const float2 f = kernel_bin_factor(d, p % g.nzc);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], f);
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[c][m] = cmul(v[c][m], f);
   }
   if (valid) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) Sp[(int64_t)(q + m * P) * ny] = v[m];
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int m = 0; m < 8; ++m) Sb[off(c, m)] = v[c][m];
   }
 }
 
@@ -886,6 +944,17 @@ inline bool zplan_lines(int n2, int& lines) {
   return with_zplan(n2, [&](auto zp) { lines = decltype(zp)::LINES * ZR_WAVES; });
 }
 
+// The workgroup's plane tile: planes k = 0..N of its LW = ZR_WAVES * LINES
+// consecutive lines, tile[k * (LW + 1) + l], in the space of the wave lines.
+template <class ZP>
+struct ZTile {
+  static constexpr int LW = ZR_WAVES * ZP::LINES;
+  static constexpr int TP = LW + 1;
+  static constexpr int NF = (ZP::N + 1) * LW;
+  static constexpr int ITER = (NF + ZR_NTH - 1) / ZR_NTH;
+  static_assert((ZP::N + 1) * TP <= LW * ZP::PITCH, "plane tile exceeds the line space");
+};
+
 // W_2N^k (k < 2N) from the r2c table W_2N^k (k <= N): W_2N^{k+N} = -W_2N^k
 __device__ __forceinline__ void stage_tw2(const ConvGeom& g, float2* tw2, int N) {
   for (int i = threadIdx.x; i < 2 * N; i += ZR_NTH) {
@@ -957,7 +1026,10 @@ __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
   const int nz = g.nz;
   const int64_t zoff = (valid ? line : 0) * nz;
   const float* srcp = (MODE == MODE_FLUX) ? d.out : d.in;
-  float2 v[V], nv[V];
+  float2 v[V];
+  double acc[3] = {0.0, 0.0, 0.0};
+  float2* L = lds_lines + (w * LPW + t) * ZP::PITCH;
+  float2 nv[V];
 #pragma unroll
   for (int m = 0; m < V; ++m) {
     const int n2i = 2 * (q + m * P);
@@ -969,7 +1041,6 @@ __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
     }
   }
   stage_tw2(g, tw2, N);
-  double acc[3] = {0.0, 0.0, 0.0};
   if (MODE == MODE_FLUX) {
     const Pending pd = pending_of(d);
     const int x = (int)(line / g.ny), y = (int)(line - (int64_t)x * g.ny);
@@ -978,7 +1049,11 @@ __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
     for (int m = 0; m < V; ++m) {
       const int n2i = 2 * (q + m * P);
       if (valid && n2i < nz) {
-        const float f0 = entering_value(pd, v[m].x), f1 = entering_value(pd, v[m].y);
+        // with an odd level count the last pair's second level is past the
+        // top: it holds no cell and enters no sum
+        const bool top = ODD && n2i + 1 >= nz;
+        const float f0 = entering_value(pd, v[m].x);
+        const float f1 = top ? 0.f : entering_value(pd, v[m].y);
         const float2 p = make_float2(__fmul_rn(nv[m].x, f0), __fmul_rn(nv[m].y, f1));
         if (win) {
           acc[0] += (double)f0 + (double)f1;
@@ -991,7 +1066,6 @@ __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
     }
   }
   __syncthreads();
-  float2* L = lds_lines + (w * LPW + t) * ZP::PITCH;
   ZP::template fft<false>(v, q, L, tw2);
   // r2c split: X2[k] = (Z[k] + conj Z[N-k]) - i W_2N^k (Z[k] - conj Z[N-k])
 #pragma unroll
@@ -999,6 +1073,7 @@ __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
   wave_lds_sync();
   const int64_t plane = nlines;
   float2* So = d.S + line;
+  float2 nyq = make_float2(0.f, 0.f);
 #pragma unroll
   for (int m = 0; m < V; ++m) {
     const int k = q + m * P;
@@ -1006,7 +1081,8 @@ __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
     float2 X;
     if (k == 0) {
       X = make_float2(2.f * (zk.x + zk.y), 0.f);
-      if (valid) So[(int64_t)N * plane] = make_float2(2.f * (zk.x - zk.y), 0.f);
+      nyq = make_float2(2.f * (zk.x - zk.y), 0.f);
+      if (!STEAM_ZR_TILE && valid) So[(int64_t)N * plane] = nyq;
     } else {
       const float2 zm = L[zpad(N - k)];
       const float2 wk = tw2[k];
@@ -1015,7 +1091,28 @@ __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
       const float2 wB = cmul(wk, B);
       X = make_float2(A.x + wB.y, A.y - wB.x);
     }
-    if (valid) So[(int64_t)k * plane] = X;
+    if (STEAM_ZR_TILE)
+      v[m] = X;
+    else if (valid)
+      So[(int64_t)k * plane] = X;
+  }
+  if (STEAM_ZR_TILE) {
+    // planes k = 0..N of the workgroup's lines, one plane run per wave access
+    using TL = ZTile<ZP>;
+    float2* tile = lds_lines;
+    const int l = w * LPW + t;
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < V; ++m) tile[(q + m * P) * TL::TP + l] = v[m];
+    if (q == 0) tile[N * TL::TP + l] = nyq;
+    __syncthreads();
+    const int64_t line0 = (int64_t)wg * TL::LW;
+#pragma unroll
+    for (int i = 0; i < TL::ITER; ++i) {
+      const int f = tid + i * ZR_NTH;
+      const int k = f / TL::LW, c = f - k * TL::LW;
+      if (f < TL::NF && line0 + c < nlines) d.S[(int64_t)k * plane + line0 + c] = tile[k * TL::TP + c];
+    }
   }
   if (MODE == MODE_FLUX && !d.apply) {
     zr_block_sum<3>(acc, red);
@@ -1046,10 +1143,37 @@ __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src sr
   const int64_t plane = nlines;
   const float2* Si = d.S + (valid ? line : 0);
   float2 v[V];
-#pragma unroll
-  for (int m = 0; m < V; ++m) v[m] = valid ? Si[(int64_t)(q + m * P) * plane] : make_float2(0.f, 0.f);
   float2 yn = make_float2(0.f, 0.f);
-  if (q == 0 && valid) yn = Si[(int64_t)N * plane];
+  if (STEAM_ZR_TILE) {
+    // planes k = 0..N of the workgroup's lines, one plane run per wave access
+    using TL = ZTile<ZP>;
+    float2* tile = lds_lines;
+    const int64_t line0 = (int64_t)wg * TL::LW;
+    float2 u[TL::ITER];
+#pragma unroll
+    for (int i = 0; i < TL::ITER; ++i) {
+      const int f = tid + i * ZR_NTH;
+      const int k = f / TL::LW, c = f - k * TL::LW;
+      u[i] = (f < TL::NF && line0 + c < nlines) ? d.S[(int64_t)k * plane + line0 + c]
+                                                 : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < TL::ITER; ++i) {
+      const int f = tid + i * ZR_NTH;
+      const int k = f / TL::LW, c = f - k * TL::LW;
+      if (f < TL::NF) tile[k * TL::TP + c] = u[i];
+    }
+    __syncthreads();
+    const int l = w * LPW + t;
+#pragma unroll
+    for (int m = 0; m < V; ++m) v[m] = tile[(q + m * P) * TL::TP + l];
+    if (q == 0) yn = tile[N * TL::TP + l];
+    __syncthreads();
+  } else {
+#pragma unroll
+    for (int m = 0; m < V; ++m) v[m] = valid ? Si[(int64_t)(q + m * P) * plane] : make_float2(0.f, 0.f);
+    if (q == 0 && valid) yn = Si[(int64_t)N * plane];
+  }
   // output levels z = 2n, 2n+1 of natural index n (the kernel spectrum
   // carries the 'same' trim shift); with an odd nz the last pair's second
   // level is past the top and is neither written nor counted

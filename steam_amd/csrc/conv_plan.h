@@ -83,10 +83,10 @@ template <class SrcT, int XMODE>
 inline void launch_xpass(const ConvGeom& g, size_t lds_x, hipStream_t s,
                          const SrcT& src) {
   if (g.reg_x) {
-    const int threads = (g.nx / 8) * g.Tr;
-    STEAM_LOGN_DISPATCH(g.reg_x, hipLaunchKernelGGL((xpass_reg_kernel<SrcT, XMODE, LN_>),
-                                                    dim3(g.wg_x), dim3(threads), 0,
-                                                    s, src));
+    const int threads = (g.nx / 8 / xreg_lanes(g.reg_x)) * g.Tr;
+    STEAM_LOGN_DISPATCH(g.reg_x, hipLaunchKernelGGL(
+        (xpass_reg_kernel<SrcT, XMODE, LN_, xreg_lanes(LN_)>), dim3(g.wg_x), dim3(threads),
+        0, s, src));
   } else {
     const int rsx = radix_set(g.ax);
     STEAM_RS_DISPATCH(rsx, hipLaunchKernelGGL((xpass_kernel<SrcT, XMODE, RS_>),

@@ -88,4 +88,79 @@ __device__ __forceinline__ void fft_reg(float2 (&v)[8], int q, float2* lds,
   }
 }
 
+// The same transform with each thread holding C lanes of the line: lane
+// q0 + c * qstep (c < C) holds v[c].  The LDS exchanges of all C lanes share
+// one pair of workgroup barriers per stage.
+template <int LOGN, int C>
+__device__ __forceinline__ void fft_reg_lanes(float2 (&v)[C][8], int q0, int qstep,
+                                              float2* lds, int lstride, int loff,
+                                              const float2* __restrict__ tw, bool inv) {
+  using F = RegFFT<LOGN>;
+  constexpr int N = F::N, P = F::P;
+#pragma unroll
+  for (int s = 0; s < F::N8; ++s) {
+    const int Ns = 1 << (3 * s);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int q = q0 + c * qstep;
+      const int k = q & (Ns - 1);
+      if (s > 0) {
+        const int step = N / (Ns * 8);
+#pragma unroll
+        for (int r = 1; r < 8; ++r) {
+          float2 w = tw[r * k * step];
+          if (inv) w.y = -w.y;
+          v[c][r] = cmul(v[c][r], w);
+        }
+      }
+      dft_small<8>(v[c], inv);
+    }
+    const bool last = (s == F::NST - 1);
+    if (!last) {
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int q = q0 + c * qstep;
+        const int k = q & (Ns - 1);
+        const int base = (q - k) * 8 + k;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) lds[pad8(base + r * Ns) * lstride + loff] = v[c][r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int q = q0 + c * qstep;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v[c][m] = lds[pad8(q + m * P) * lstride + loff];
+      }
+    }
+  }
+  if constexpr (F::TAIL > 1) {
+    constexpr int R = F::TAIL;
+    constexpr int NB = 8 / R;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int q = q0 + c * qstep;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int j = q + b * P;
+        float2 u[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          float2 x = v[c][b + NB * r];
+          if (r > 0) {
+            float2 w = tw[r * j];
+            if (inv) w.y = -w.y;
+            x = cmul(x, w);
+          }
+          u[r] = x;
+        }
+        dft_small<R>(u, inv);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[c][b + NB * r] = u[r];
+      }
+    }
+  }
+}
+
 }  // namespace steam
