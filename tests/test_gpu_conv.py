@@ -49,6 +49,30 @@ def test_convolution_matches_oracle(fs, ks):
         np.testing.assert_allclose(ref, M.conv_direct(f, k), rtol=1e-9, atol=1e-9)
 
 
+@pytest.mark.parametrize("fs,ks", [
+    ((64, 64, 32), (17, 17, 17)), ((16, 16, 60), (9, 9, 11)), ((128, 64, 16), (9, 9, 5)),
+    ((4, 4, 6), (9, 9, 3)), ((8, 16, 8), (11, 21, 5)), ((32, 32, 115), (13, 13, 13)),
+    ((12, 14, 8), (7, 5, 3)),
+])
+def test_symmetric_kernel(fs, ks):
+    """Odd, point-symmetric kernels (the turbulon envelopes are), including
+    ones folded onto a narrower field: against the float64 oracle, and
+    continuous in the taps (one tap moved by one ulp)."""
+    from steam_amd.utils import convolve_fft_xy_oa_z
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(fs).astype(np.float32)
+    k = rng.standard_normal(ks).astype(np.float32)
+    k = ((k + k[::-1, ::-1, ::-1]) * np.float32(0.5)).astype(np.float32)
+    assert np.array_equal(k, k[::-1, ::-1, ::-1])
+    out = convolve_fft_xy_oa_z(f, k)
+    ref = M.conv_fft(f, k)
+    assert _rel(out, ref) < CONV_REL_L2
+    k2 = k.copy()
+    k2[0, 0, 0] = np.nextafter(k2[0, 0, 0], np.float32(np.inf))
+    out2 = convolve_fft_xy_oa_z(f, k2)
+    assert _rel(out, out2) < CONV_REL_L2
+
+
 @pytest.mark.parametrize("n_fft", [16, 24, 32, 64])
 def test_overlap_add_block_lengths_agree(n_fft):
     """Several z blocks (including a last partial one) give the single-block
