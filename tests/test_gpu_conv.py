@@ -34,6 +34,10 @@ def _rel(a, b):
     ((64, 96, 40), (13, 13, 13)), ((96, 64, 40), (13, 13, 13)),
     # odd level counts on the register z path (n_fft 128, 64, 256)
     ((32, 32, 115), (13, 13, 13)), ((40, 24, 53), (9, 9, 9)), ((16, 24, 201), (13, 13, 13)),
+    # register x, y and z paths on wide planes, odd level counts, a folded kernel
+    ((64, 128, 20), (9, 9, 5)), ((128, 256, 12), (7, 7, 7)), ((64, 512, 8), (5, 5, 5)),
+    ((256, 128, 33), (9, 9, 9)), ((64, 1024, 6), (3, 5, 3)), ((64, 2048, 4), (3, 3, 3)),
+    ((128, 128, 40), (140, 9, 5)),
 ])
 def test_convolution_matches_oracle(fs, ks):
     from steam_amd.utils import convolve_fft_xy_oa_z
@@ -71,6 +75,25 @@ def test_symmetric_kernel(fs, ks):
     k2[0, 0, 0] = np.nextafter(k2[0, 0, 0], np.float32(np.inf))
     out2 = convolve_fft_xy_oa_z(f, k2)
     assert _rel(out, out2) < CONV_REL_L2
+
+
+@pytest.mark.parametrize("fs,ks", [((64, 128, 16), (9, 9, 5)), ((64, 128, 15), (5, 5, 5)),
+                                   ((64, 64, 16), (5, 5, 5))])
+def test_misaligned_field_takes_the_generic_z_passes(fs, ks):
+    """A field 4 bytes off 8-byte alignment cannot take the register z passes'
+    8-byte pairs; the generic z passes then read and write the planes the
+    register x and y passes use."""
+    from steam_amd.utils import convolve_fft_xy_oa_z
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal(fs).astype(np.float32)
+    k = rng.standard_normal(ks).astype(np.float32)
+    n = int(np.prod(fs))
+    buf = torch.zeros(n + 1, dtype=torch.float32, device='cuda')
+    ft = buf[1:].view(fs)
+    ft.copy_(torch.from_numpy(f))
+    assert ft.data_ptr() % 8 == 4
+    out = convolve_fft_xy_oa_z(ft, torch.from_numpy(k).cuda())
+    assert _rel(out.cpu().numpy(), M.conv_fft(f, k)) < CONV_REL_L2
 
 
 @pytest.mark.parametrize("n_fft", [16, 24, 32, 64])
