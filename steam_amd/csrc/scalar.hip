@@ -228,8 +228,11 @@ enum LevelOp {
 // G partials of each (candidate, level) in a fixed order.  A level's result therefore depends
 // only on the field, the window and the shift, never on how many candidate
 // shifts share the launch.
+#ifndef STEAM_RD_MIN_LINES
+#define STEAM_RD_MIN_LINES 128
+#endif
 constexpr int RD_MAX_GROUPS = 4096;
-constexpr int RD_MIN_LINES = 32;
+constexpr int RD_MIN_LINES = STEAM_RD_MIN_LINES;
 // lines whose loads a lane issues before it works on them
 constexpr int RD_U = 4;
 // candidate shifts of one bisection pass: the 2^D - 1 midpoints of a D-deep
@@ -789,6 +792,9 @@ int grid_1d(int64_t n) {
 }
 
 // lines per workgroup run for a window of wlines lines, and the run count
+// The runs depend on the window alone, not on the candidate count, so a
+// many-candidate pass sums each candidate in the order a one-candidate pass
+// does.
 int64_t rd_lines_per_group(int64_t wlines) {
   int64_t g = (wlines + RD_MIN_LINES - 1) / RD_MIN_LINES;
   g = std::min<int64_t>(std::max<int64_t>(g, 1), RD_MAX_GROUPS);
