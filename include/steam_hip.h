@@ -364,8 +364,8 @@ int steam_level_reduce(const float* f_dev, const float* mean_z_dev,
                        steam_stream_t stream);
 
 /* The per-level scalars one bounded-add solve realized
- * (BoundedAddLedger, steam/ledger.py:145-190); host arrays, demean and
- * scale row-major (nz, n_rescale). */
+ * (BoundedAddLedger, steam/ledger.py:145-190); host arrays (device arrays
+ * with replay == 2), demean and scale row-major (nz, n_rescale). */
 typedef struct steam_bounded_add_ledger {
   double* a0;
   double* demean;
@@ -382,9 +382,11 @@ int64_t steam_bounded_add_workspace_bytes(int64_t nx, int64_t ny, int64_t nz);
  * (zero window mean, window mean |increment| preserved, bounds respected):
  * demean / clip / rescale loop, then a 60-step bisection of the zero-mean
  * shift, as _bounded_amplitude_add's GPU path (steam/simulate.py:2985-3111).
- * replay != 0 applies the recorded ledger with no reductions.  With
- * record_applied, inc is left holding the increment actually added.
- * Synchronises (once per rescale step and at the end). */
+ * replay == 1 applies the recorded ledger with no reductions; replay == 0
+ * records it into host arrays (synchronising once, for the copy); replay ==
+ * 2 records it into device arrays in stream order, with no
+ * synchronisation.  With record_applied, inc is left holding the increment
+ * actually added. */
 int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float* inc_dev,
                       int64_t nx, int64_t ny, int64_t nz, const int64_t* window,
                       double phi_min, double phi_max, int n_rescale,

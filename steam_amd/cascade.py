@@ -26,9 +26,9 @@ from . import simulate as _sim
 from ._device import check_device
 from .ledger import FluxOutputLedger, MeanReduction, ProjectionLedger, RunLedger
 from .params import _interpolation_compensation
-from .scalars import (LV_ABS, LV_NONZERO, LevelReducer, ZDifferences, _window,
-                      bounded_amplitude_add, joint_pattern, normalize_pattern,
-                      z_difference_coefficients)
+from .scalars import (LV_ABS, LV_NONZERO, DeviceBoundedAddLedger, LevelReducer,
+                      ZDifferences, _window, bounded_amplitude_add, joint_pattern,
+                      normalize_pattern, z_difference_coefficients)
 from .ledger import PatternLedger
 from .utils import conv_operator, convolve_fft_xy_oa_z, kernel_key, regrid
 
@@ -365,9 +365,12 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
             normalize_pattern(W, level_mean, C_t)
             increment = _convolve(W, kernel_t, kernel_np, device)
             del W
+            # a new solve is recorded on the device and read with the class's
+            # other ledgers once the class is done
             class_ledger.bounded_add[name] = bounded_amplitude_add(
                 pert, mean_t, increment, lo, hi, window=window, record_applied=True,
-                ledger=class_ledger.bounded_add.get(name))
+                ledger=class_ledger.bounded_add.get(name),
+                device_ledger=class_ledger.bounded_add.get(name) is None)
             if dfield is not None:
                 deficit_accumulate(dfield, increment, deficit_t)
             if record:
@@ -381,6 +384,9 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
         if class_ledger.flux is None:
             class_ledger.flux, _ = _sim.read_device_ledger(tuple(fl.shape),
                                                            tuple(kernel_t.shape))
+        for name, led in list(class_ledger.bounded_add.items()):
+            if isinstance(led, DeviceBoundedAddLedger):
+                class_ledger.bounded_add[name] = led.to_host()
         del S_k
         prev = {'dx': dx_k, 'dy': dy_k, 'dz': float(grids['dz'][i]),
                 'ext_x': float(grids['padded_extent_x'][i]),

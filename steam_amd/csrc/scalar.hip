@@ -242,7 +242,10 @@ constexpr int RD_U = 4;
 constexpr int KCAND = 15;
 constexpr int KCAND_WIDE = 7;
 // fields above this many cells take the 3-deep tree
-constexpr int64_t KWIDE_CELLS = int64_t(1) << 25;
+#ifndef STEAM_KWIDE_LOG2
+#define STEAM_KWIDE_LOG2 20
+#endif
+constexpr int64_t KWIDE_CELLS = int64_t(1) << STEAM_KWIDE_LOG2;
 
 struct LevelArgs {
   const float* f;
@@ -649,6 +652,16 @@ __global__ __launch_bounds__(256) void ba_check_kernel(BaVec v, int nz, int step
 
 // the final demean and clip, then e = d + phi, in one pass:
 // d' = clamp(f32(d * f32(pend)) - f32(shift) + phi, lo, hi) - phi; d = d' + phi
+// the loop counts of a device ledger: every level took the same number of
+// rescale steps, flags[1]
+__global__ __launch_bounds__(256) void ba_counts_kernel(const int* flags, int32_t* n_loop,
+                                                         int32_t* n_scale, int nz) {
+  for (int z = blockIdx.x * 256 + threadIdx.x; z < nz; z += gridDim.x * 256) {
+    n_loop[z] = flags[1];
+    n_scale[z] = flags[1];
+  }
+}
+
 __global__ __launch_bounds__(256) void ba_final_kernel(float* __restrict__ d,
                                                        const float* __restrict__ pert,
                                                        const float* __restrict__ mean_z,
@@ -1039,6 +1052,8 @@ extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float
     return set_error(STEAM_EINVAL, "steam_bounded_add: NULL argument");
   if (!shape_ok(nx, ny, nz) || n_rescale < 0 || n_rescale > 64)
     return set_error(STEAM_EINVAL, "steam_bounded_add: bad shape or n_rescale");
+  if (replay < 0 || replay > 2)
+    return set_error(STEAM_EINVAL, "steam_bounded_add: replay must be 0, 1 or 2");
   if (!ledger->a0 || !ledger->demean || !ledger->scale || !ledger->n_loop ||
       !ledger->n_scale || !ledger->final_demean || !ledger->mu)
     return set_error(STEAM_EINVAL, "steam_bounded_add: ledger arrays missing");
@@ -1094,7 +1109,7 @@ extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float
   };
   const size_t vbytes = (size_t)nz * 8;
 
-  if (replay) {
+  if (replay == 1) {
     // apply-only: the recorded per-level scalars, no reductions
     int nloop = 0;
     for (int z = 0; z < nz; ++z) nloop = std::max(nloop, (int)ledger->n_loop[z]);
@@ -1157,25 +1172,28 @@ extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float
     c.f = inc_dev;
     c.op = LV_CLAMP;
     if ((rc = bisect(c, b, (int)nz, width, npts, 0, red, s))) return rc;
-    int flags[2];
-    STEAM_HIP(hipMemcpyAsync(ledger->a0, v.a0, vbytes, hipMemcpyDeviceToHost, s));
-    STEAM_HIP(hipMemcpyAsync(ledger->demean, v.hist_demean, vbytes * n_rescale,
-                             hipMemcpyDeviceToHost, s));
-    STEAM_HIP(hipMemcpyAsync(ledger->scale, v.hist_scale, vbytes * n_rescale,
-                             hipMemcpyDeviceToHost, s));
-    STEAM_HIP(hipMemcpyAsync(ledger->final_demean, v.shift, vbytes, hipMemcpyDeviceToHost, s));
-    STEAM_HIP(hipMemcpyAsync(ledger->mu, b.mu, vbytes, hipMemcpyDeviceToHost, s));
-    STEAM_HIP(hipMemcpyAsync(flags, v.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
-    STEAM_HIP(hipStreamSynchronize(s));
-    for (int z = 0; z < nz; ++z) {
-      ledger->n_loop[z] = flags[1];
-      ledger->n_scale[z] = flags[1];
+    const hipMemcpyKind kind = replay == 2 ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    STEAM_HIP(hipMemcpyAsync(ledger->a0, v.a0, vbytes, kind, s));
+    STEAM_HIP(hipMemcpyAsync(ledger->demean, v.hist_demean, vbytes * n_rescale, kind, s));
+    STEAM_HIP(hipMemcpyAsync(ledger->scale, v.hist_scale, vbytes * n_rescale, kind, s));
+    STEAM_HIP(hipMemcpyAsync(ledger->final_demean, v.shift, vbytes, kind, s));
+    STEAM_HIP(hipMemcpyAsync(ledger->mu, b.mu, vbytes, kind, s));
+    if (replay == 2) {
+      hipLaunchKernelGGL(ba_counts_kernel, dim3(gz), dim3(256), 0, s, v.flags, ledger->n_loop,
+                         ledger->n_scale, (int)nz);
+    } else {
+      int flags[2];
+      STEAM_HIP(hipMemcpyAsync(flags, v.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+      STEAM_HIP(hipStreamSynchronize(s));
+      for (int z = 0; z < nz; ++z) {
+        ledger->n_loop[z] = flags[1];
+        ledger->n_scale[z] = flags[1];
+      }
     }
   }
   hipLaunchKernelGGL(ba_finish_kernel, dim3(gn), dim3(256), 0, s, inc_dev, pert_dev, inc_dev,
                      mean_z_dev, b.mu, n, (int)nz, lo, hi, record_applied);
   STEAM_LAUNCH_CHECK("bounded add");
-  STEAM_HIP(hipStreamSynchronize(s));
   return STEAM_OK;
 }
 

@@ -200,15 +200,54 @@ def normalize_pattern(pattern, level_mean, c_k):
     return pattern
 
 
+class DeviceBoundedAddLedger:
+    """A bounded-add ledger recorded into device arrays in stream order;
+    ``to_host()`` copies it into a BoundedAddLedger (one synchronisation)."""
+
+    def __init__(self, nz, n_rescale):
+        self.nz, self.n_rescale = nz, n_rescale
+        self.f64 = torch.empty((3 + 2 * n_rescale) * nz, dtype=torch.float64, device='cuda')
+        self.i32 = torch.empty(2 * nz, dtype=torch.int32, device='cuda')
+
+    def native(self):
+        nz, nr = self.nz, self.n_rescale
+        base, ib = self.f64.data_ptr(), self.i32.data_ptr()
+        L = _lib.BoundedAddLedgerC()
+        L.a0 = base
+        L.demean = base + 8 * nz
+        L.scale = base + 8 * nz * (1 + nr)
+        L.final_demean = base + 8 * nz * (1 + 2 * nr)
+        L.mu = base + 8 * nz * (2 + 2 * nr)
+        L.n_loop = ib
+        L.n_scale = ib + 4 * nz
+        return L
+
+    def to_host(self):
+        nz, nr = self.nz, self.n_rescale
+        f = self.f64.cpu().numpy()
+        i = self.i32.cpu().numpy()
+        out = BoundedAddLedger(nz, nr)
+        out.a0 = f[:nz].copy()
+        out.demean = f[nz:nz * (1 + nr)].reshape(nz, nr).copy()
+        out.scale = f[nz * (1 + nr):nz * (1 + 2 * nr)].reshape(nz, nr).copy()
+        out.final_demean = f[nz * (1 + 2 * nr):nz * (2 + 2 * nr)].copy()
+        out.mu = f[nz * (2 + 2 * nr):].copy()
+        out.n_loop = i[:nz].copy()
+        out.n_scale = i[nz:].copy()
+        return out
+
+
 def bounded_amplitude_add(perturbation, mean_1d, increment, phi_min, phi_max,
                           window=None, n_rescale=N_RESCALE_DEFAULT,
-                          record_applied=False, ledger=None):
+                          record_applied=False, ledger=None, device_ledger=False):
     """Add ``increment`` to ``perturbation`` (both device tensors, the first
     updated in place) under the three deposit conditions per level
     (steam/simulate.py:3114-3283, the GPU path at 2985-3111).  ``increment``
     is scratch; with ``record_applied`` it is left holding the increment the
     field actually took.  With ``ledger`` the recorded solve is replayed
-    (apply-only).  Returns the BoundedAddLedger."""
+    (apply-only).  Returns the BoundedAddLedger; with ``device_ledger`` (no
+    replay) the solve is recorded on the device with no synchronisation and
+    a DeviceBoundedAddLedger is returned instead."""
     lib = _lib.load()
     _check_field(perturbation, "perturbation")
     _check_field(increment, "increment")
@@ -218,6 +257,18 @@ def bounded_amplitude_add(perturbation, mean_1d, increment, phi_min, phi_max,
         raise ValueError("increment must not alias the perturbation")
     nx, ny, nz = perturbation.shape
     replay = ledger is not None
+    if device_ledger and not replay:
+        dev = DeviceBoundedAddLedger(nz, n_rescale)
+        native = dev.native()
+        mean_t = _f32_dev(mean_1d)
+        nbytes = int(lib.steam_bounded_add_workspace_bytes(nx, ny, nz))
+        ws = torch.empty(nbytes, dtype=torch.uint8, device='cuda')
+        _lib.check(lib.steam_bounded_add(
+            _lib.ptr(perturbation), _lib.ptr(mean_t), _lib.ptr(increment), nx, ny, nz,
+            _window(window), float(phi_min), float(phi_max), int(n_rescale),
+            1 if record_applied else 0, ctypes.byref(native), 2,
+            _lib.ptr(ws), nbytes, _lib.stream_handle()), "steam_bounded_add")
+        return dev
     solve = ledger if replay else BoundedAddLedger(nz, n_rescale)
     n_rescale = solve.demean.shape[1]
     for name in ('a0', 'demean', 'scale', 'final_demean', 'mu'):
