@@ -71,6 +71,9 @@ def parse():
                          "1024x1024x256 float32 field (alpha 1.6, C1 0.12, 129^3 "
                          "kernel) timed on the GPU and checked against the FP64 "
                          "oracle; prints one JSON line")
+    ap.add_argument("--streamed-form", action="store_true",
+                    help="only run the streamed form: the 2048x128x115 root cascade "
+                         "resident and tile by tile; prints one JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / "
@@ -466,6 +469,65 @@ def full_simulate_form(reps=3):
             "data": "synthetic linear h/qt profiles, seeds 0..reps", "runs": out}
 
 
+def streamed_form(reps=3, plans=((5, 2, 1), (4, 4, 2), (4, 8, 4))):
+    """The whole root cascade (flux, h, qt and the deficits) resident and
+    streamed tile by tile (streamed_cascade_loop: per tile the halo regrid,
+    the flux advance with the raw-product pattern measure, the per-tile
+    convolution of the normalised pattern; per-level passes on the whole
+    fields) on the 2048x128x115 ladder of the cost-comparison grids.  Each
+    plan is (horizon, tiles_x, tiles_y): the classes from the horizon on are
+    tiled, the narrower ones before it run as one tile (plan_tiling's forced
+    plan).  Reports the times, the tiling overhead and the streamed fields'
+    rel-L2 against the resident ones."""
+    import numpy as np
+    import torch
+    from steam_amd.cascade import cascade_loop
+    from steam_amd.params import root_setup
+    from steam_amd.streaming import plan_tiling, streamed_cascade_loop
+
+    z = np.arange(201) * 100.0
+    hp = 340e3 - 20e3 * z / z.max()
+    qp = 0.018 - 0.016 * z / z.max()
+    st = root_setup(hp, qp, 2048, 128, 3000.0, 3000.0, 384000.0, 10.0, 20000.0, 100.0,
+                    seed=0)
+    hmin, hmax, qmin, qmax = st['bounds']
+    args = (st['h_profile'], st['qt_profile'], st['z_profile'], st['grids'], st['C_h_k'],
+            st['C_qt_k'], st['b_h_k'], st['b_qt_k'], st['aspect_k'], hmin, hmax, qmin, qmax,
+            st['min_distance_to_ground'], st['sparsity_factors'], st['class_seeds'])
+    kw = dict(comp_k=st['comp_k'], as_tensors=True)
+
+    def timed(fn):
+        res = fn()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            res = fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return res, best
+
+    resident, t_res = timed(lambda: cascade_loop(*args, **kw))
+    runs = []
+    for force in plans:
+        tiles = plan_tiling(st['grids'], (13, 13, 13), force=force)
+        streamed, t = timed(lambda: streamed_cascade_loop(*args, tiles, **kw))
+        rel = {}
+        for name, k in (("h_perturbation", 0), ("qt_perturbation", 1), ("flux_state", 2)):
+            a = streamed[k].double()
+            b = resident[k].double()
+            rel[name] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+        runs.append({"plan": {"horizon": force[0], "tiles": [force[1], force[2]]},
+                     "best_ms": t * 1e3, "vs_resident": t / t_res,
+                     "rel_l2_vs_resident": rel})
+    g = st['grids']
+    return {"workload": "root cascade 2048x128x115 (7 classes), resident and streamed",
+            "class_voxels": int(sum(int(a) * int(b) * int(c)
+                                    for a, b, c in zip(g['nx'], g['ny'], g['nz']))),
+            "resident_ms": t_res * 1e3, "streamed": runs,
+            "data": "synthetic linear h/qt profiles, seed 0"}
+
+
 # The reference's own quoted costs at its production square 2048x2048x115
 # (an unnamed CUDA card; the bounded add and the projection include their
 # host/device transfers): steam/utils.py:89-90, steam/simulate.py:2995, 3311.
@@ -563,6 +625,11 @@ def main():
         from steam_amd import _lib
         _lib.require_gpu('cuda')
         print(json.dumps(full_simulate_form()))
+        return
+    if args.streamed_form:
+        from steam_amd import _lib
+        _lib.require_gpu('cuda')
+        print(json.dumps(streamed_form()))
         return
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (got --gpus {args.gpus}); the "

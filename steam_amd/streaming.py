@@ -191,12 +191,21 @@ def plan_tiling(grids, kernel_shape, budget_bytes=None, force=None):
         tx, ty = tiles.get(index, (1, 1))
         nx, ny, nz = (int(grids['nx'][index]), int(grids['ny'][index]),
                       int(grids['nz'][index]))
-        tile = (-(-nx // tx) + 2 * halo[0] * (tx > 1 or ty > 1),
-                -(-ny // ty) + 2 * halo[1] * (tx > 1 or ty > 1))
+        tile = (-(-nx // tx) + 2 * halo[0] * (tx > 1), -(-ny // ty) + 2 * halo[1] * (ty > 1))
         previous = 6 * _field_bytes(grids, index - 1) if index > 0 else 0
         resident = previous + 9 * _field_bytes(grids, index)
         peak = max(peak, resident + _working_bytes(tile[0], tile[1], nz, kz))
     return TilePlan(horizon, tiles, halo, peak)
+
+
+def _check_split(index, shape, tiles, kernel_shape):
+    """A split axis must be at least as wide as the kernel: the tiles' halo
+    form of the convolution equals the in-RAM operator only there."""
+    for axis, (n, t, k) in enumerate(zip(shape[:2], tiles, kernel_shape[:2])):
+        if t > 1 and n < k:
+            raise ValueError(
+                f"class {index}: axis {'xy'[axis]} has {n} cells, fewer than the "
+                f"{k}-tap kernel, and cannot be split into {t} tiles; run it as one tile")
 
 
 def streamed_flux_cascade(grids, tiles, sparsity_factors=(1, 1, 1), seed=None,
@@ -210,12 +219,15 @@ def streamed_flux_cascade(grids, tiles, sparsity_factors=(1, 1, 1), seed=None,
     (one per class); a count above a class's extent is clamped to it.
     Returns ``(flux, diagnostics)`` like ``simulate_flux_only``; the result
     equals the resident cascade to rounding (the interiors are exact; the
-    sums are reassociated over tiles) for classes at least as wide as the
-    kernel.  A class narrower than the kernel is convolved here with the
-    periodic sum of the kernel, whereas the in-RAM operator folds the kernel
-    with a one-cell centring offset on even widths (steam/utils.py:222-242
-    rolled by (n-1)//2 at :366-372); the reference's streamed driver never
-    folds either.
+    sums are reassociated over tiles).  An axis in one tile takes no halo:
+    the tile is the whole periodic axis and is convolved as in RAM, folding
+    a kernel wider than it, as the reference's driver does for the classes
+    before its horizon (steam/streaming.py:725-760).  A split axis takes the
+    halo and never folds; there the in-RAM fold of a kernel wider than an
+    even axis would differ by its one-cell centring offset
+    (steam/utils.py:222-242, rolled by (n-1)//2 at :366-372), so tiles are
+    kept wider than the kernel, as the reference's planner keeps them wider
+    than two halos.
     """
     check_device(device)
     c = _check_flux_only_inputs(grids, sparsity_factors, flux_noise_scale,
@@ -234,7 +246,7 @@ def streamed_flux_cascade(grids, tiles, sparsity_factors=(1, 1, 1), seed=None,
     sf = SUPPORT_FACTOR if support_factor is None else support_factor
     kt = _turbulon_envelope(1, 1 / (2 * s_x), 1 / (2 * s_y), 1 / (2 * s_z),
                             support_factor=sf, shape=turbulon_shape, as_tensor=True)
-    hx, hy = class_halo(kt.shape)
+    hx0, hy0 = class_halo(kt.shape)
     ledger = RunLedger(n_classes)
     state = None
     sigma = np.float32(1.0)
@@ -244,6 +256,14 @@ def streamed_flux_cascade(grids, tiles, sparsity_factors=(1, 1, 1), seed=None,
         tx = min(int(per_class_tiles[i][0]), shape[0])
         ty = min(int(per_class_tiles[i][1]), shape[1])
         windows = tile_windows(shape[0], shape[1], tx, ty)
+        # an axis in one tile is the whole periodic axis: no halo, so the
+        # class is convolved exactly as in RAM (the reference runs the classes
+        # before its horizon in RAM, steam/streaming.py:725-760; with a kernel
+        # wider than an even axis the fold and a halo differ by the fold's
+        # centring)
+        hx = 0 if tx == 1 else hx0
+        hy = 0 if ty == 1 else hy0
+        _check_split(i, shape, (tx, ty), kt.shape)
         # pass 0: regrid the previous state window by window (rescale folded in)
         if state is None:
             entering_state = torch.ones(shape, dtype=torch.float32, device='cuda')
@@ -350,7 +370,7 @@ def streamed_cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k
     kt = _turbulon_envelope(1, 1 / (2 * s_x), 1 / (2 * s_y), 1 / (2 * s_z),
                             support_factor=SUPPORT_FACTOR, shape=turbulon_shape,
                             as_tensor=True)
-    hx, hy = class_halo(kt.shape)
+    hx0, hy0 = class_halo(kt.shape)
     ledger = RunLedger(n_classes)
     z_profile = np.asarray(z_profile, dtype=np.float64)
     fields = None
@@ -361,6 +381,14 @@ def streamed_cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k
         tx = min(int(per_class_tiles[i][0]), shape[0])
         ty = min(int(per_class_tiles[i][1]), shape[1])
         windows = tile_windows(shape[0], shape[1], tx, ty)
+        # an axis in one tile is the whole periodic axis: no halo, so the
+        # class is convolved exactly as in RAM (the reference runs the classes
+        # before its horizon in RAM, steam/streaming.py:725-760; with a kernel
+        # wider than an even axis the fold and a halo differ by the fold's
+        # centring)
+        hx = 0 if tx == 1 else hx0
+        hy = 0 if ty == 1 else hy0
+        _check_split(i, shape, (tx, ty), kt.shape)
         class_ledger = ledger.classes[i]
         # regrid of every carried field, window by window
         if fields is None:

@@ -128,6 +128,30 @@ def full_resident():
     return st, cascade_loop(*args, **kw)
 
 
+@pytest.mark.parametrize("force", [None, (3, 2, 2), (3, 2, 1), (3, 1, 4)])
+def test_streamed_cascade_with_classes_narrower_than_the_kernel(force):
+    """root_dyadic's ladder (2, 4, 8 and 16 columns against a 13-tap kernel):
+    the classes in one tile run as in RAM (the fold), the wider ones tiled;
+    streamed equals resident to the reference's bound.  Splitting a class
+    narrower than the kernel is refused."""
+    from steam_amd.cascade import cascade_loop
+    from steam_amd.streaming import plan_tiling, streamed_cascade_loop
+    from test_oracle_cascade import fixture_setup
+    st = fixture_setup((1, 1, 1), 1)
+    args, kw = _cascade_args(st)
+    res = cascade_loop(*args, **kw)
+    tiles = (1, 1) if force is None else plan_tiling(st['grids'], (13, 13, 13), force=force)
+    out = streamed_cascade_loop(*args, tiles, **kw)
+    for k in range(3):
+        rel = np.linalg.norm(out[k].astype(np.float64) - res[k]) / np.linalg.norm(
+            res[k].astype(np.float64))
+        assert rel < 1e-5, (force, k, rel)
+    if force is None:
+        with pytest.raises(ValueError):
+            streamed_cascade_loop(*args, plan_tiling(st['grids'], (13, 13, 13), force=(2, 2, 1)),
+                                  **kw)
+
+
 @pytest.mark.parametrize("tiles", [(1, 1), (2, 2), (3, 2), [(1, 1), (2, 3), (4, 4)]])
 def test_streamed_full_cascade_equals_resident(full_resident, tiles):
     """Streamed vs resident within 1e-5 (the reference's own bound,
