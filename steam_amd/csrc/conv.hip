@@ -19,23 +19,26 @@ static bool is_smooth7(int64_t n) {
   return n == 1;
 }
 
+// Stages of a length: radix 8 first, then 4, 2, 3, 5, 7, and the product of
+// the prime factors above 7 as one direct stage (cost: that product per
+// element); the whole line as one direct stage when the slots run out.
 static void factor_axis(int n, AxisPlan& ap) {
   ap.n = n;
   ap.nst = 0;
   if (n <= 1) return;
-  if (!is_smooth7(n)) {
-    ap.radix[ap.nst++] = 0;
-    return;
-  }
   int m = n;
   while (m % 8 == 0 && ap.nst < FFT_MAXST) { ap.radix[ap.nst++] = 8; m /= 8; }
   while (m % 4 == 0 && ap.nst < FFT_MAXST) { ap.radix[ap.nst++] = 4; m /= 4; }
   while (m % 2 == 0 && ap.nst < FFT_MAXST) { ap.radix[ap.nst++] = 2; m /= 2; }
   for (int p : {3, 5, 7})
     while (m % p == 0 && ap.nst < FFT_MAXST) { ap.radix[ap.nst++] = p; m /= p; }
-  if (m != 1) {  // more factors than stage slots: one direct-DFT stage
-    ap.nst = 1;
-    ap.radix[0] = 0;
+  if (m != 1) {
+    if (ap.nst < FFT_MAXST && m > 8) {
+      ap.radix[ap.nst++] = m;  // prime factors above 7
+    } else {
+      ap.nst = 1;
+      ap.radix[0] = 0;
+    }
   }
 }
 

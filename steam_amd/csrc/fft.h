@@ -3,7 +3,7 @@
 // A workgroup of NTH threads holds T lines of length N (N*T <= EM*nth complex
 // values; EM values per thread) in LDS and transforms all of them together.  Mixed radix
 // 2/3/4/5/7/8 stages (Stockham autosort, natural order in and out); a length
-// with a prime factor above 7 is transformed by one direct-DFT stage.  Each
+// with prime factors above 7 takes one direct stage of their product.  Each
 // stage stages its butterflies' inputs in registers, synchronises, and
 // writes the outputs back in place, so one LDS buffer suffices.
 //
@@ -25,7 +25,8 @@ constexpr int FFT_MAXST = 6;   // stages per length (factor 8 first)
 struct AxisPlan {
   int n;                   // transform length
   int nst;                 // number of stages
-  int radix[FFT_MAXST];    // 2,3,4,5,7,8; 0 = direct DFT of the line
+  int radix[FFT_MAXST];    // 2,3,4,5,7,8; above 8: direct stage of that radix;
+                           // 0: direct DFT of the whole line
   const float2* tw;        // n entries exp(-2 pi i m / n)
 };
 
@@ -180,13 +181,20 @@ __device__ __noinline__ void fft_stage_call(float2* buf, int N, int T, int logT,
   fft_stage<R, TFAST, EM, NTH>(buf, N, T, logT, ldn, ldt, Ns, tw, inv, tid);
 }
 
-// Whole-line direct DFT (lengths with a prime factor > 7).
+// Direct stage of a runtime radix R (the product of a length's prime
+// factors above 7; R = N is the whole line): output q of a line, written as
+// q = (j - k) R + k + s Ns with k = j mod Ns, j < N/R and s < R, is
+//   sum_r buf[j + r N/R] exp(-2 pi i (r k / (Ns R) + r s / R)),
+// the stage twiddle and the R-point DFT in one exponent mod N of the axis
+// table.  Cost: R complex products per element.
 template <bool TFAST, int EM, int NTH>
-__device__ __noinline__ void fft_direct(float2* buf, int N, int T, int logT, int ldn,
-                           int ldt, const float2* __restrict__ tw, bool inv,
-                           int tid) {
+__device__ __noinline__ void fft_direct(float2* buf, int N, int R, int Ns, int T, int logT,
+                                        int ldn, int ldt, const float2* __restrict__ tw,
+                                        bool inv, int tid) {
   float2 o[EM];
   const int ne = N * T;
+  const int NR = N / R;
+  const int step = N / (Ns * R);
   constexpr int nth = NTH;
 #pragma unroll
   for (int i = 0; i < EM; ++i) {
@@ -194,14 +202,18 @@ __device__ __noinline__ void fft_direct(float2* buf, int N, int T, int logT, int
     if (e < ne) {
       int t, q;
       line_index<TFAST>(e, T, logT, N, t, q);
-      const float2* src = buf + t * ldt;
+      const int k = q % Ns;
+      const int s = (q / Ns) % R;
+      const int j = (q / (Ns * R)) * Ns + k;
+      const float2* src = buf + t * ldt + j * ldn;
+      const int dm = (k * step + s * NR) % N;
       float2 acc = make_float2(0.f, 0.f);
       int m = 0;
-      for (int n = 0; n < N; ++n) {
+      for (int r = 0; r < R; ++r) {
         float2 w = tw[m];
         if (inv) w.y = -w.y;
-        acc = cadd(acc, cmul(src[n * ldn], w));
-        m += q;
+        acc = cadd(acc, cmul(src[r * NR * ldn], w));
+        m += dm;
         if (m >= N) m -= N;
       }
       o[i] = acc;
@@ -230,7 +242,7 @@ inline int radix_set(const AxisPlan& ap) {
   for (int s = 0; s < ap.nst; ++s) {
     int r = ap.radix[s];
     if (r == 3 || r == 5) rs = rs < 1 ? 1 : rs;
-    if (r == 7 || r == 0) rs = 2;
+    if (r == 7 || r == 0 || r > 8) rs = 2;
   }
   return rs;
 }
@@ -262,7 +274,8 @@ __device__ __forceinline__ void fft_lines(float2* buf, const AxisPlan& ap, int T
         case 5: fft_stage_call<5, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
         case 7: fft_stage_call<7, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
         case 8: fft_stage_call<8, TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, Ns, ap.tw, inv, tid); break;
-        default: fft_direct<TFAST, EM, NTH>(buf, ap.n, T, logT, ldn, ldt, ap.tw, inv, tid); break;
+        default: fft_direct<TFAST, EM, NTH>(buf, ap.n, R == 0 ? ap.n : R, Ns, T, logT, ldn, ldt,
+                                            ap.tw, inv, tid); break;
       }
     }
     Ns *= (R == 0) ? ap.n : R;

@@ -274,3 +274,20 @@ def test_gpu_nest_of_a_nest_matches_the_oracle():
                           ledger=r1['ledger'])
     for field in FIELDS:
         np.testing.assert_array_equal(again[field], r1[field], err_msg=field)
+
+
+def test_gpu_spanning_nest_is_the_root_run_at_its_resolution():
+    """paper/supplement.tex:393 on the GPU: root_dyadic refined over its whole
+    domain to 62.5 m, and that nest again to 31.25 m, against root runs
+    straight to those resolutions."""
+    from steam_amd import cascade as C
+    from steam_amd.refine import refine_fields
+    root = C.simulate_fields(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0,
+                             800.0, 30.0, seed=42, save_for_refinement=True)
+    nest = refine_fields(root['parent'], 0, 16, 0, 16, 62.5, 62.5, save_for_refinement=True)
+    nn = refine_fields(nest['parent'], 0, 32, 0, 32, 31.25, 31.25)
+    for out, n, dx in ((nest, 32, 62.5), (nn, 64, 31.25)):
+        direct = C.simulate_fields(H_PROFILE, QT_PROFILE, n, n, dx, dx, 2000.0, 100.0, 800.0,
+                                   30.0, seed=42)
+        for field in FIELDS:
+            np.testing.assert_array_equal(out[field], direct[field], err_msg=(dx, field))

@@ -38,6 +38,9 @@ def _rel(a, b):
     ((64, 128, 20), (9, 9, 5)), ((128, 256, 12), (7, 7, 7)), ((64, 512, 8), (5, 5, 5)),
     ((256, 128, 33), (9, 9, 9)), ((64, 1024, 6), (3, 5, 3)), ((64, 2048, 4), (3, 3, 3)),
     ((128, 128, 40), (140, 9, 5)),
+    # lengths with prime factors above 7: a direct stage of their product
+    ((22, 26, 8), (5, 5, 3)), ((260, 16, 6), (7, 7, 3)), ((51, 88, 6), (9, 5, 3)),
+    ((2060, 4, 4), (13, 3, 3)), ((8, 286, 5), (3, 13, 3)), ((37, 41, 7), (5, 5, 5)),
 ])
 def test_convolution_matches_oracle(fs, ks):
     from steam_amd.utils import convolve_fft_xy_oa_z

@@ -443,3 +443,19 @@ def test_recursive_refinement_inside_a_nest(dyadic_parent):
         plan_nest(nest, 0, inner_cells, 0, inner_cells, nest['dx'] / 2, nest['dy'] / 2, seed=19)
     with pytest.raises(ValueError, match="non-periodic parent"):
         plan_nest(nest, 0, nest['nx'], 0, nest['ny'], nest['dx'] / 2, nest['dy'] / 2, seed=21)
+
+
+def test_spanning_nest_is_the_root_run_at_its_resolution(dyadic_parent):
+    """paper/supplement.tex:393: a nest spanning the whole parent domain is,
+    cell for cell, the root run taken straight to the nest's resolution; and
+    a spanning nest of that nest the root run one class deeper."""
+    from steam_amd.refine import plan_nest
+    nest = OC.refine_with_plan(dyadic_parent, plan_nest(dyadic_parent, 0, 16, 0, 16, 62.5, 62.5),
+                               save_for_refinement=True)
+    nn = OC.refine_with_plan(nest['parent'],
+                             plan_nest(nest['parent'], 0, 32, 0, 32, 31.25, 31.25))
+    for out, n, dx in ((nest, 32, 62.5), (nn, 64, 31.25)):
+        root = run_oracle(P.root_setup(H_PROFILE, QT_PROFILE, n, n, dx, dx, 2000.0, 100.0,
+                                       800.0, 30.0, seed=42))
+        for field in TOL:
+            np.testing.assert_array_equal(out[field], root[field], err_msg=(dx, field))
