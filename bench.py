@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--streamed-form", action="store_true",
                     help="only run the streamed form: the 2048x128x115 root cascade "
                          "resident and tile by tile; prints one JSON line")
+    ap.add_argument("--refine-form", action="store_true",
+                    help="only run the refine form: a nest of the 2048x128x115 root and a "
+                         "nest of that nest; prints one JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / "
@@ -528,6 +531,60 @@ def streamed_form(reps=3, plans=((5, 2, 1), (4, 4, 2), (4, 8, 4))):
             "data": "synthetic linear h/qt profiles, seed 0"}
 
 
+def refine_form(reps=3):
+    """refine at the size of the cost-comparison runs: the 2048x128x115 root
+    (saved for refinement) refined over x in [512, 1536), y in [32, 96) to
+    1.5 km (one class, the halo padding the nest to 2060x140 columns), and
+    that nest refined again over its middle to 750 m; device-resident
+    outputs.  Checks the bounds and that a replay from the nest's ledger is
+    bit-identical."""
+    import numpy as np
+    import torch
+    from steam_amd import cascade as C
+    from steam_amd.refine import refine_fields
+
+    z = np.arange(201) * 100.0
+    hp = 340e3 - 20e3 * z / z.max()
+    qp = 0.018 - 0.016 * z / z.max()
+    root = C.simulate_fields(hp, qp, 2048, 128, 3000.0, 3000.0, 384000.0, 10.0, 20000.0,
+                             100.0, seed=0, save_for_refinement=True)
+    parent = root['parent']
+
+    def timed(fn):
+        out = fn()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return out, best
+
+    nest, t1 = timed(lambda: refine_fields(parent, 512, 1536, 32, 96, 1500.0, 1500.0,
+                                           as_tensors=True))
+    replay = refine_fields(parent, 512, 1536, 32, 96, 1500.0, 1500.0, as_tensors=True,
+                           ledger=nest['ledger'])
+    identical = all(torch.equal(replay[k], nest[k]) for k in ('h', 'qt', 'flux'))
+    hmin, hmax, qmin, qmax = parent['bounds']
+    in_bounds = bool(nest['h'].min() >= hmin and nest['h'].max() <= hmax
+                     and nest['qt'].min() >= qmin and nest['qt'].max() <= qmax
+                     and nest['flux'].min() >= 0)
+    rec = refine_fields(parent, 512, 1536, 32, 96, 1500.0, 1500.0, as_tensors=True,
+                        save_for_refinement=True)['parent']
+    nx = rec['nx']
+    nn, t2 = timed(lambda: refine_fields(rec, nx // 4, nx // 4 + 512, 16, 112, 750.0, 750.0,
+                                         as_tensors=True))
+    return {"workload": "refine of the 2048x128x115 root and of its nest",
+            "nest": {"region": "x [512, 1536), y [32, 96) of the root, to 1.5 km",
+                     "shape": list(nest['h'].shape), "best_ms": t1 * 1e3,
+                     "replay_bit_identical": identical, "in_bounds": in_bounds},
+            "nest_of_nest": {"region": "x [nx/4, nx/4 + 512), y [16, 112) of the nest, "
+                                       "to 750 m",
+                             "shape": list(nn['h'].shape), "best_ms": t2 * 1e3},
+            "data": "synthetic linear h/qt profiles, seed 0"}
+
+
 # The reference's own quoted costs at its production square 2048x2048x115
 # (an unnamed CUDA card; the bounded add and the projection include their
 # host/device transfers): steam/utils.py:89-90, steam/simulate.py:2995, 3311.
@@ -630,6 +687,11 @@ def main():
         from steam_amd import _lib
         _lib.require_gpu('cuda')
         print(json.dumps(streamed_form()))
+        return
+    if args.refine_form:
+        from steam_amd import _lib
+        _lib.require_gpu('cuda')
+        print(json.dumps(refine_form()))
         return
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (got --gpus {args.gpus}); the "
