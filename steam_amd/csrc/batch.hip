@@ -240,6 +240,7 @@ int make_item(const steam_batch_item& it, PassDesc& d, ItemPlan& ip) {
   d = base_desc(p, it.workspace_dev);
   d.mode = MODE_FLUX;
   d.K = (const float2*)it.spectrum_dev;
+  d.ktile = p->ktile;
   d.out = it.flux_dev;
   d.sigma_in = it.sigma_in;
   d.pend = (it.flags & STEAM_CHAIN_PENDING)

@@ -67,7 +67,9 @@ enum PassMode { MODE_PLAIN = 0, MODE_FLUX = 1, MODE_KERNEL = 2 };
 struct PassDesc {
   ConvGeom g;
   float2* S;            // spectral workspace [nblocks][nzc][nx][ny]
-  const float2* K;      // kernel spectrum [nzc][nx][ny]
+  const float2* K;      // kernel spectrum [nzc][nx][ny], or with ktile > 0 in the
+                        // register x pass's column tiles [nzc][ny/ktile][nx][ktile]
+  int ktile;
   const float* in;      // plain input / kernel taps
   float* out;           // plain output / flux state (in place)
   int mode;

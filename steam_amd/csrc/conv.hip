@@ -261,7 +261,18 @@ extern "C" int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz,
   p->nf_off = p->led_off + align256(sizeof(DevLedger));
   p->ws_bytes = p->nf_off + align256(nx * ny * nz * 4);
   p->wg_noise = (int)((nx * ny + 3) / 4);
-  p->spec_bytes = align256((int64_t)g.nzc * nx * ny * 8);
+  // the register x pass reads the kernel spectrum in its own column tiles,
+  // each workgroup's block one contiguous run; the build writes the usual
+  // [nzc][nx][ny] order after them and is then permuted
+  p->ktile = g.reg_x ? g.Tr : 0;
+  if (p->ktile) {
+    const int64_t nyt = (ny + p->ktile - 1) / p->ktile;
+    p->kstd_off = align256((int64_t)g.nzc * nyt * p->ktile * nx * 8);
+    p->spec_bytes = p->kstd_off + align256((int64_t)g.nzc * nx * ny * 8);
+  } else {
+    p->kstd_off = 0;
+    p->spec_bytes = align256((int64_t)g.nzc * nx * ny * 8);
+  }
   p->wg_post = (int)((nx * ny * nz + 4 * FFT_NTH - 1) / (4 * FFT_NTH));
   *out = p;
   return STEAM_OK;
@@ -287,6 +298,26 @@ extern "C" int steam_conv_plan_info(const steam_conv_plan* p,
 }
 
 
+namespace steam {
+
+// kernel spectrum [nzc][nx][ny] -> the register x pass's column tiles
+// [nzc][ceil(ny/T)][nx][T] (the tail of a partial last tile is never read)
+__global__ __launch_bounds__(256) void ktile_kernel(const float2* __restrict__ src,
+                                                    float2* __restrict__ dst, int nzc, int nx,
+                                                    int ny, int T) {
+  const int64_t plane = (int64_t)nx * ny;
+  const int64_t n = plane * nzc;
+  const int64_t nyt = (ny + T - 1) / T;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const int64_t p = i / plane, r = i - p * plane;
+    const int x = (int)(r / ny), y = (int)(r - (int64_t)x * ny);
+    dst[((p * nyt + y / T) * nx + x) * T + y % T] = src[i];
+  }
+}
+
+}  // namespace steam
+
 extern "C" int steam_conv_set_kernel(const steam_conv_plan* p,
                                      const float* kernel_dev,
                                      void* spectrum_dev, void* workspace_dev,
@@ -299,7 +330,7 @@ extern "C" int steam_conv_set_kernel(const steam_conv_plan* p,
   memset(&d, 0, sizeof(d));
   d.g = p->gk;
   d.mode = MODE_KERNEL;
-  d.S = (float2*)spectrum_dev;
+  d.S = (float2*)((char*)spectrum_dev + p->kstd_off);
   d.in = kernel_dev;
   d.kscale = (float)(1.0 / (4.0 * (double)p->g.nfft * (double)p->nx * (double)p->ny));
   SingleSrc src{d};
@@ -312,6 +343,13 @@ extern "C" int steam_conv_set_kernel(const steam_conv_plan* p,
   STEAM_LAUNCH_CHECK("ypass_kernel(kernel)");
   launch_xpass<SingleSrc, 1>(g, p->lds_x, s, src);
   STEAM_LAUNCH_CHECK("xpass_kernel(kernel)");
+  if (p->ktile) {
+    const int64_t ne = (int64_t)g.nzc * p->nx * p->ny;
+    hipLaunchKernelGGL(ktile_kernel, dim3((unsigned)std::min<int64_t>((ne + 255) / 256, 1 << 16)),
+                       dim3(256), 0, s, (const float2*)d.S, (float2*)spectrum_dev, (int)g.nzc,
+                       (int)p->nx, (int)p->ny, p->ktile);
+    STEAM_LAUNCH_CHECK("ktile_kernel");
+  }
   return STEAM_OK;
 }
 
@@ -323,6 +361,7 @@ extern "C" int steam_conv_execute(const steam_conv_plan* p, const float* in_dev,
   PassDesc d = base_desc(p, workspace_dev);
   d.mode = MODE_PLAIN;
   d.K = (const float2*)spectrum_dev;
+  d.ktile = p->ktile;
   d.in = in_dev;
   d.out = out_dev;
   return run_operator<MODE_PLAIN>(p, d, (hipStream_t)stream);
@@ -444,6 +483,7 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
   PassDesc d = base_desc(p, workspace_dev);
   d.mode = MODE_FLUX;
   d.K = (const float2*)spectrum_dev;
+  d.ktile = p->ktile;
   d.out = flux_dev;
   d.sigma_in = sigma_in;
   d.pend = chain_pending ? (const DevLedger*)((const char*)workspace_dev + p->led_off)

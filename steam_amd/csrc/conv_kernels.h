@@ -453,18 +453,23 @@ __global__ __launch_bounds__(1024) void xpass_reg_kernel(Src src) {
   for (int c = 0; c < C; ++c)
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[c][m] = valid ? Sb[off(c, m)] : make_float2(0.f, 0.f);
+  // the kernel spectrum: in this pass's column tiles [kz][y/T][x][T] when
+  // the plan stores it so (the workgroup's block is then one contiguous run)
   const float2* Kb = nullptr;
+  const bool kt = d.ktile != 0;
+  auto koff = [&](int c, int m) { return kt ? (q0 + c * PT + m * P) * T + t : off(c, m); };
   auto load_k = [&]() {
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
-      for (int m = 0; m < 8; ++m) kv[c][m] = valid ? Kb[off(c, m)] : make_float2(0.f, 0.f);
+      for (int m = 0; m < 8; ++m) kv[c][m] = valid ? Kb[koff(c, m)] : make_float2(0.f, 0.f);
   };
   // one lane per thread: the spectrum loads go out with the line's; two
   // lanes: after the forward transform (the registers of both would spill)
   constexpr bool KPRE = STEAM_XREG_KPREFETCH && C == 1;
   if (XMODE == 0) {
-    Kb = d.K + (int64_t)(p % g.nzc) * nx * ny;
+    Kb = kt ? d.K + ((int64_t)(p % g.nzc) * nyt + (wg - p * nyt)) * nx * T
+            : d.K + (int64_t)(p % g.nzc) * nx * ny;
     if (KPRE) load_k();
   }
   fft_reg_lanes<LOGN, C>(v, q0, PT, lds, T, t, g.ax.tw, false);

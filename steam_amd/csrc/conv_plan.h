@@ -11,6 +11,8 @@ struct steam_conv_plan {
   float2* tw_dev;    // twiddle tables
   int64_t nx, ny, nz, kx, ky, kz;
   int64_t s_bytes, pz_off, po_off, led_off, nf_off, ws_bytes, spec_bytes;
+  int ktile;         // columns of the kernel spectrum's x-pass tiles (0: [nzc][nx][ny])
+  int64_t kstd_off;  // spectrum build area in [nzc][nx][ny] order
   int wg_noise;
   size_t lds_z, lds_zinv, lds_y, lds_x;
   int wg_post;
