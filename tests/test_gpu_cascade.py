@@ -291,3 +291,23 @@ def test_gpu_spanning_nest_is_the_root_run_at_its_resolution():
                                    30.0, seed=42)
         for field in FIELDS:
             np.testing.assert_array_equal(out[field], direct[field], err_msg=(dx, field))
+
+
+@pytest.mark.parametrize("z_min,z_max", [(240.0, 600.0), (0.0, 450.0), (330.0, 800.0)])
+def test_gpu_elevated_nest_matches_the_oracle(z_min, z_max):
+    """A nest over part of the column (z pads below and above, zero_bottom /
+    zero_top off where it floats; the reference's tests/test_refine.py:515-560
+    geometry): the GPU refine against the oracle's on the same plan."""
+    from steam_amd import cascade as C
+    from steam_amd.refine import plan_nest, refine_fields
+    from test_oracle_cascade import oracle_parent
+    root = C.simulate_fields(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0,
+                             800.0, 30.0, seed=42, save_for_refinement=True)
+    out = refine_fields(root['parent'], 4, 12, 4, 12, 62.5, 62.5, z_min=z_min, z_max=z_max)
+    o_parent = oracle_parent(fixture_setup((1, 1, 1), 1))
+    plan = plan_nest(o_parent, 4, 12, 4, 12, 62.5, 62.5, z_min=z_min, z_max=z_max)
+    assert plan['at_ground'] == (z_min == 0.0) and plan['at_top'] == (z_max == 800.0)
+    ref = OC.refine_with_plan(o_parent, plan)
+    for field, tol in TOL.items():
+        assert out[field].shape == ref[field].shape, field
+        assert rel(out[field], ref[field]) < tol, (field, rel(out[field], ref[field]))
