@@ -473,18 +473,37 @@ __global__ void bisect_init_kernel(BisectVec v, int nz, double width) {
   }
 }
 
+// The combine of one level's 2^D - 1 candidate sums (wave c joins candidate
+// c's run partials exactly as level_combine_kernel does) and the D bisection
+// steps of that level, in one launch: one workgroup per level.
 // mode 0: positive = sum / npts - target > 0 (the bounded add);
 // mode 1: positive = sum / npts > target (the projection)
 template <int D>
-__global__ void bisect_pass_kernel(BisectVec v, int nz, double npts, int mode) {
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  if (z >= nz) return;
+__global__ __launch_bounds__(1024) void bisect_combine_kernel(const double* __restrict__ part,
+                                                              int groups, int nz, int op,
+                                                              BisectVec v, double npts,
+                                                              int mode) {
+  constexpr int NC = (1 << D) - 1;
+  __shared__ double sums[NC];
+  const int z = blockIdx.x;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  {
+    const double* row = part + ((int64_t)w * nz + z) * groups;
+    double acc = lv_init(op);
+#pragma unroll 8
+    for (int g = lane; g < groups; g += 64) acc = lv_join(op, acc, row[g]);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc = lv_join(op, acc, __shfl_xor(acc, off, 64));
+    if (lane == 0) sums[w] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   double lo = v.lo[z], hi = v.hi[z];
   const double target = v.target[z];
   int n = 1;
   for (int s = 0; s < D; ++s) {
     const double mid = 0.5 * (lo + hi);
-    const double mean = v.sums[(int64_t)(n - 1) * nz + z] / npts;
+    const double mean = sums[n - 1] / npts;
     const bool positive = mode == 0 ? (mean - target > 0.0) : (mean > target);
     if (positive) {
       lo = mid;
@@ -829,10 +848,13 @@ int64_t level_scratch_bytes(int64_t nlines, int64_t nz, int nc) {
 
 // out[c][z] = per-level reduction (divided by div when div > 0) of a's op
 // over its window, for each of a.ncand shifts
-int level_reduce(const LevelArgs& a, double* out, double div, double* scratch, hipStream_t s) {
+// the partial pass of a level reduction: per run of lines, one partial per
+// (candidate, level) in scratch[(c * nz + z) * groups + run]
+int level_partials(const LevelArgs& a, double* scratch, hipStream_t s, int64_t* groups_out) {
   const int64_t wlines = (int64_t)(a.win[1] - a.win[0]) * (a.win[3] - a.win[2]);
   const int64_t lpg = rd_lines_per_group(wlines);
   const int64_t groups = rd_groups(wlines);
+  *groups_out = groups;
   const dim3 grid((unsigned)groups), block(256);
   if (a.ncand > 1) {
     if (a.ncand == KCAND && a.op == LV_CLAMP)
@@ -874,6 +896,13 @@ int level_reduce(const LevelArgs& a, double* out, double div, double* scratch, h
     }
   }
   STEAM_LAUNCH_CHECK("level_partial_kernel");
+  return STEAM_OK;
+}
+
+int level_reduce(const LevelArgs& a, double* out, double div, double* scratch, hipStream_t s) {
+  int64_t groups = 0;
+  int rc = level_partials(a, scratch, s, &groups);
+  if (rc) return rc;
   const int64_t pairs = (int64_t)a.ncand * a.nz;
   hipLaunchKernelGGL(level_combine_kernel, dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, s,
                      scratch, (int)groups, a.ncand, a.nz, a.op, div, out, a.stop);
@@ -906,10 +935,13 @@ int bisect_depth(LevelArgs a, BisectVec b, int nz, double width, double npts, in
   hipLaunchKernelGGL((bisect_init_kernel<D>), dim3(gz), dim3(256), 0, s, b, nz, width);
   a.mu = b.cand;
   a.ncand = (1 << D) - 1;
+  constexpr int NC = (1 << D) - 1;
   for (int pass = 0; pass < 60 / D; ++pass) {
-    int rc = level_reduce(a, (double*)b.sums, 0.0, red, s);
+    int64_t groups = 0;
+    int rc = level_partials(a, red, s, &groups);
     if (rc) return rc;
-    hipLaunchKernelGGL((bisect_pass_kernel<D>), dim3(gz), dim3(256), 0, s, b, nz, npts, mode);
+    hipLaunchKernelGGL((bisect_combine_kernel<D>), dim3(nz), dim3(64 * NC), 0, s, red,
+                       (int)groups, nz, a.op, b, npts, mode);
   }
   STEAM_LAUNCH_CHECK("bisection");
   return STEAM_OK;
