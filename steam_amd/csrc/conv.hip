@@ -19,9 +19,10 @@ static bool is_smooth7(int64_t n) {
   return n == 1;
 }
 
-// Stages of a length: radix 8 first, then 4, 2, 3, 5, 7, and the product of
-// the prime factors above 7 as one direct stage (cost: that product per
-// element); the whole line as one direct stage when the slots run out.
+// Stages of a length: radix 8 first, then 4, 2, 3, 5, 7, then each prime
+// factor above 7 as a direct stage (cost: that prime per element; the last
+// free slot takes the product of what remains); the whole line as one direct
+// stage when the slots run out.
 static void factor_axis(int n, AxisPlan& ap) {
   ap.n = n;
   ap.nst = 0;
@@ -32,14 +33,14 @@ static void factor_axis(int n, AxisPlan& ap) {
   while (m % 2 == 0 && ap.nst < FFT_MAXST) { ap.radix[ap.nst++] = 2; m /= 2; }
   for (int p : {3, 5, 7})
     while (m % p == 0 && ap.nst < FFT_MAXST) { ap.radix[ap.nst++] = p; m /= p; }
-  if (m != 1) {
-    if (ap.nst < FFT_MAXST && m > 8) {
-      ap.radix[ap.nst++] = m;  // prime factors above 7
-    } else {
-      ap.nst = 1;
-      ap.radix[0] = 0;
-    }
+  if (m != 1 && (ap.nst == FFT_MAXST || m <= 8)) {
+    ap.nst = 1;
+    ap.radix[0] = 0;
+    return;
   }
+  for (int p = 11; m != 1 && p * p <= m && ap.nst < FFT_MAXST - 1; p += 2)
+    while (m % p == 0 && m != p && ap.nst < FFT_MAXST - 1) { ap.radix[ap.nst++] = p; m /= p; }
+  if (m != 1) ap.radix[ap.nst++] = m;  // a prime, or the product of the rest
 }
 
 // log2 of a power of two in [64, 2048], else 0 (no register FFT path)

@@ -41,6 +41,7 @@ def _rel(a, b):
     # lengths with prime factors above 7: a direct stage of their product
     ((22, 26, 8), (5, 5, 3)), ((260, 16, 6), (7, 7, 3)), ((51, 88, 6), (9, 5, 3)),
     ((2060, 4, 4), (13, 3, 3)), ((8, 286, 5), (3, 13, 3)), ((37, 41, 7), (5, 5, 5)),
+    ((143, 8, 4), (9, 3, 3)), ((1034, 4, 4), (5, 3, 3)), ((8, 1573, 3), (3, 7, 3)),
 ])
 def test_convolution_matches_oracle(fs, ks):
     from steam_amd.utils import convolve_fft_xy_oa_z
