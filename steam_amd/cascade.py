@@ -269,6 +269,8 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
                                                  deficit_h, deficit_qt, deficit_flux)]
     prev = None
     z_profile = np.asarray(z_profile, dtype=np.float64)
+    pattern_pending = []
+    flux_pending = {}
     for i in range(n_classes):
         shape = (int(grids['nx'][i]), int(grids['ny'][i]), int(grids['nz'][i]))
         dx_k, dy_k = float(grids['dx'][i]), float(grids['dy'][i])
@@ -334,13 +336,21 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
                 deficit_i]
         if not zc[0]:
             rows += [np.concatenate([c_, np.zeros(2, np.float32)]) for c_ in zc[2:5]]
+        # from pinned memory without blocking: the host never waits for the
+        # device inside the class loop
         vec = torch.from_numpy(np.stack([np.asarray(r, dtype=np.float32).reshape(nzk)
-                                         for r in rows])).cuda()
+                                         for r in rows])).pin_memory().cuda(non_blocking=True)
         (h_mean_t, qt_mean_t, aspect_t, b_h_t, b_qt_t, C_h_t, C_qt_t, deficit_t) = vec[:8]
         zdiff = ZDifferences(z_k, coefficients=None if zc[0] else vec[8:11, :nzk - 2])
+        op_key = (tuple(fl.shape), tuple(kernel_t.shape))
+        if op_key in flux_pending:
+            # this operator's ledger slot is about to be rewritten
+            flux_pending.pop(op_key).flux, _ = _sim.read_device_ledger(*op_key)
         S_k, class_ledger.flux, _ = _advance(
             fl, region, kernel_t, kernel_np, c, n_scale_classes_per_dyad, sparsity_factors,
             n_zero, zero_bottom, zero_top, device, window, class_ledger.flux)
+        if class_ledger.flux is None:
+            flux_pending[op_key] = class_ledger
         if need_increment:
             if has_deficit:
                 deficit_accumulate(dfl, fl, deficit_t, before=before)
@@ -377,22 +387,27 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
                 _record(increment_dir, increments, i, name, increment.cpu().numpy())
             del increment
         if measured:
-            got = torch.stack([t for pair in measured.values() for t in pair]).cpu().numpy()
-            for k, name in enumerate(measured):
-                class_ledger.pattern[name] = PatternLedger(
-                    got[2 * k], np.rint(got[2 * k + 1]).astype(np.int64))
-        if class_ledger.flux is None:
-            class_ledger.flux, _ = _sim.read_device_ledger(tuple(fl.shape),
-                                                           tuple(kernel_t.shape))
-        for name, led in list(class_ledger.bounded_add.items()):
-            if isinstance(led, DeviceBoundedAddLedger):
-                class_ledger.bounded_add[name] = led.to_host()
+            pattern_pending.append((class_ledger, measured))
         del S_k
         prev = {'dx': dx_k, 'dy': dy_k, 'dz': float(grids['dz'][i]),
                 'ext_x': float(grids['padded_extent_x'][i]),
                 'ext_y': float(grids['padded_extent_y'][i]),
                 'height': float(grids['padded_height'][i]),
                 'z_min': float(grids['z_min_per_class'][i])}
+    # the ledgers the classes left on the device, read once the cascade is
+    # queued (they feed no later class: the device path passes its level
+    # means on the device)
+    for class_ledger, measured in pattern_pending:
+        got = torch.stack([t for pair in measured.values() for t in pair]).cpu().numpy()
+        for k, name in enumerate(measured):
+            class_ledger.pattern[name] = PatternLedger(
+                got[2 * k], np.rint(got[2 * k + 1]).astype(np.int64))
+    for op_key, class_ledger in flux_pending.items():
+        class_ledger.flux, _ = _sim.read_device_ledger(*op_key)
+    for class_ledger in ledger.classes:
+        for name, led in list(class_ledger.bounded_add.items()):
+            if isinstance(led, DeviceBoundedAddLedger):
+                class_ledger.bounded_add[name] = led.to_host()
     final_grid_info = {'nx': shape[0], 'ny': shape[1], 'nz': shape[2], 'dx': dx_k,
                        'dy': dy_k, 'dz': grids['dz_arrays'][n_classes - 1],
                        'z': grids['z_arrays'][n_classes - 1]}
