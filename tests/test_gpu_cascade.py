@@ -311,3 +311,40 @@ def test_gpu_elevated_nest_matches_the_oracle(z_min, z_max):
     for field, tol in TOL.items():
         assert out[field].shape == ref[field].shape, field
         assert rel(out[field], ref[field]) < tol, (field, rel(out[field], ref[field]))
+
+
+@pytest.mark.parametrize("shape_name,aniso,nx,ls,s,n", [
+    ('morlet_omega0_6', 'piecewise_isotropic_below_spheroscale', 16, 100.0, (1, 1, 1), 1),
+    ('mexican_hat', 'canonical', 32, 200.0, (1, 1, 1), 1),
+    ('mexican_hat', 'piecewise_isotropic_below_spheroscale', 32, 200.0, (1, 1, 1), 1),
+    ('morlet_omega0_6', 'canonical', 16, 200.0, (2, 2, 2), 2),
+])
+def test_gpu_root_run_options_match_the_oracle(shape_name, aniso, nx, ls, s, n):
+    """Options the reference's fixtures leave at their defaults (the Morlet
+    turbulon, the canonical anisotropy with classes below a 200 m
+    spheroscale, together with sparsity 2 and two classes per dyad): the GPU
+    root run against the oracle's."""
+    from steam_amd import cascade as C
+    from steam_amd import params as P
+    dx = 2000.0 / nx
+    st = P.root_setup(H_PROFILE, QT_PROFILE, nx, nx, dx, dx, 2000.0, ls, 800.0, 30.0,
+                      sparsity_factors=s, n_scale_classes_per_dyad=n, seed=42, anisotropy=aniso)
+    out = C.simulate_fields(H_PROFILE, QT_PROFILE, nx, nx, dx, dx, 2000.0, ls, 800.0,
+                            30.0, sparsity_factors=s, n_scale_classes_per_dyad=n, seed=42,
+                            turbulon_shape=shape_name, anisotropy=aniso)
+    hmin, hmax, qmin, qmax = st['bounds']
+    h, qt, fl, dh, dqt, dfl, fin, _ = OC.cascade_loop(
+        st['h_profile'], st['qt_profile'], st['z_profile'], st['grids'], st['C_h_k'],
+        st['C_qt_k'], st['b_h_k'], st['b_qt_k'], st['aspect_k'], hmin, hmax, qmin, qmax,
+        st['min_distance_to_ground'], st['sparsity_factors'], st['class_seeds'],
+        n_scale_classes_per_dyad=n, comp_k=st['comp_k'], turbulon_shape=shape_name)
+    zf = fin['z'].astype(np.float32)
+    hm = np.interp(zf, st['z_profile'], st['h_profile']).astype(np.float32)
+    qm = np.interp(zf, st['z_profile'], st['qt_profile']).astype(np.float32)
+    ref = {'h_perturbation': h, 'qt_perturbation': qt, 'flux_state': fl,
+           'h': OC.compose_output(h, dh, hm, hmin, hmax)[0],
+           'qt': OC.compose_output(qt, dqt, qm, qmin, qmax)[0],
+           'flux': OC.compose_flux_output(fl, dfl)[0]}
+    for field, tol in TOL.items():
+        assert out[field].shape == ref[field].shape, field
+        assert rel(out[field], ref[field]) < tol, (field, rel(out[field], ref[field]))
