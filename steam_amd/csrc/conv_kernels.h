@@ -426,8 +426,17 @@ __global__ __launch_bounds__(256) void ypass_reg_kernel(Src src, int inv) {
 // adjacent y columns per workgroup (thread t % Tr = column, so each row
 // access is a Tr*8-byte run); XMODE 0: forward, * kernel spectrum, inverse;
 // 1: forward, * scale.
+// register x pass: waves per SIMD the compiler is held to (0: its own choice)
+#ifndef STEAM_XREG_WAVES
+#define STEAM_XREG_WAVES 0
+#endif
+#if STEAM_XREG_WAVES > 0
+#define STEAM_XREG_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(STEAM_XREG_WAVES, STEAM_XREG_WAVES)))
+#else
+#define STEAM_XREG_WAVES_ATTR
+#endif
 template <class Src, int XMODE, int LOGN, int C>
-__global__ __launch_bounds__(1024) void xpass_reg_kernel(Src src) {
+__global__ __launch_bounds__(1024) STEAM_XREG_WAVES_ATTR void xpass_reg_kernel(Src src) {
   using F = RegFFT<LOGN>;
   constexpr int P = F::P, PT = P / C;
   constexpr int T = (1024 / PT < (C == 1 ? XREG_COLS : XREG_COLS2))
