@@ -83,9 +83,11 @@ class ConvOperator:
         return ws
 
     def spectrum(self, kernel_t, key=None):
-        """Kernel spectrum (cached under ``key`` when given)."""
+        """Kernel spectrum (cached under ``key`` when given).  An entry keyed
+        by a device tensor's address holds that tensor, so no other kernel
+        can be allocated at the address while the entry lives."""
         if key is not None and key in self._spectra:
-            return self._spectra[key]
+            return self._spectra[key][0]
         lib = _lib.load()
         if tuple(kernel_t.shape) != self.kernel_shape:
             raise ValueError(f"kernel shape {tuple(kernel_t.shape)} does not "
@@ -99,7 +101,7 @@ class ConvOperator:
         if key is not None:
             if len(self._spectra) > 8:
                 self._spectra.pop(next(iter(self._spectra)))
-            self._spectra[key] = spec
+            self._spectra[key] = (spec, kernel_t if key[0] == 't' else None)
         return spec
 
     def execute(self, field_t, out_t, spec):
@@ -129,9 +131,12 @@ def conv_operator(field_shape, kernel_shape, n_fft=0, precision='plain'):
 
 
 def kernel_key(kernel):
-    """Content key of a kernel for the spectrum cache."""
+    """Key of a kernel for the spectrum cache: a device tensor by its address,
+    layout and version counter (in-place writes bump it), which the cache
+    entry keeps valid by holding the tensor; a host array by its content."""
     if isinstance(kernel, torch.Tensor):
-        return ('t', kernel.data_ptr(), kernel._version, tuple(kernel.shape))
+        return ('t', kernel.data_ptr(), kernel._version, tuple(kernel.shape),
+                tuple(kernel.stride()), str(kernel.dtype), str(kernel.device))
     arr = np.ascontiguousarray(kernel, dtype=np.float32)
     return ('h', arr.shape, hashlib.sha1(arr.tobytes()).hexdigest())
 

@@ -220,3 +220,25 @@ def test_regrid_window_equals_full_zoom():
     for win in [(0, 5, 0, 4), (7, 33, 3, 18), (30, 33, 0, 18)]:
         part = regrid(src, (33, 18, 8), window=win).cpu().numpy()
         np.testing.assert_array_equal(part, full[win[0]:win[1], win[2]:win[3]])
+
+
+def test_spectrum_cache_never_serves_a_freed_kernels_spectrum():
+    """Kernels of one shape uploaded, cached by address and freed in turn
+    (the allocator hands a freed block straight back): every convolution
+    equals the uncached one, since a cache entry keyed by a device address
+    holds its tensor and no later kernel can take that address."""
+    from steam_amd.utils import conv_operator, kernel_key
+    rng = np.random.default_rng(5)
+    shape, kshape = (32, 32, 16), (5, 5, 5)
+    op = conv_operator(shape, kshape)
+    field = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)).cuda()
+    seen = set()
+    for _ in range(4):
+        k = torch.from_numpy(rng.standard_normal(kshape).astype(np.float32)).cuda()
+        key = kernel_key(k)
+        assert key not in seen
+        seen.add(key)
+        out = op.execute(field, torch.empty_like(field), op.spectrum(k, key=key))
+        ref = op.execute(field, torch.empty_like(field), op.spectrum(k))
+        np.testing.assert_array_equal(out.cpu().numpy(), ref.cpu().numpy())
+        del k
