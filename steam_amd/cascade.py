@@ -168,11 +168,14 @@ def _crop(carried, axis, keep, start):
     return out
 
 
-def _record(increment_dir, increments, i, name, arr):
+def _record(increment_dir, increments, i, name, t):
+    """A class increment (a device tensor of its own) into the file directory
+    and/or the in-memory list; the list keeps the device tensor, so recording
+    never waits for the device inside the class loop."""
     if increment_dir is not None:
-        np.save(os.path.join(increment_dir, f"c{i:02d}_{name}.npy"), arr)
+        np.save(os.path.join(increment_dir, f"c{i:02d}_{name}.npy"), t.cpu().numpy())
     if increments is not None:
-        increments[-1][name] = arr
+        increments[-1][name] = t
 
 
 def _convolve(W, kernel_t, kernel_np, device):
@@ -227,7 +230,7 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
     A recorded ``ledger`` replays the run apply-only.  ``increment_dir``
     receives each class's applied increments as c{i:02d}_{h,qt,flux}.npy,
     as in the reference; ``increments`` (a list) receives them in memory,
-    one dict of NumPy arrays per class."""
+    one dict of device tensors per class."""
     check_device(device)
     s_x, s_y, s_z = sparsity_factors
     n_classes = len(grids['k'])
@@ -355,7 +358,7 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
             if has_deficit:
                 deficit_accumulate(dfl, fl, deficit_t, before=before)
             if record:
-                _record(increment_dir, increments, i, 'flux', (fl - before).cpu().numpy())
+                _record(increment_dir, increments, i, 'flux', fl - before)
             del before
         reducer = LevelReducer(shape)
         measured = {}
@@ -384,7 +387,7 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
             if dfield is not None:
                 deficit_accumulate(dfield, increment, deficit_t)
             if record:
-                _record(increment_dir, increments, i, name, increment.cpu().numpy())
+                _record(increment_dir, increments, i, name, increment)
             del increment
         if measured:
             pattern_pending.append((class_ledger, measured))
@@ -432,7 +435,7 @@ def simulate_fields(h_profile, qt_profile, nx, ny, dx, dy, outer_scale, spherosc
     'flux_state' (the cascade state), 'ledger', 'grids', 'setup', and with
     ``save_for_refinement`` a 'parent' record (what the reference's file
     holds for refine: the state, the per-class increments, the run's
-    attributes).  The NetCDF file itself and the streamed/resumable drivers
+    attributes), whose arrays stay on the device with ``as_tensors``.  The NetCDF file itself and the streamed/resumable drivers
     are not part of this path."""
     from .params import root_setup
     check_device(device)
@@ -468,10 +471,13 @@ def simulate_fields(h_profile, qt_profile, nx, ny, dx, dy, outer_scale, spherosc
         g = st['grids']
         nxf = int(h_pert.shape[0])
         nyf = int(h_pert.shape[1])
+        # device tensors with ``as_tensors`` (the state shared with the
+        # returned fields, nothing copied), else NumPy arrays as the file holds
+        keep = (lambda t: t) if as_tensors else (lambda t: t.cpu().numpy())
         parent = {
-            'h_perturbation': h_pert.cpu().numpy(),
-            'qt_perturbation': qt_pert.cpu().numpy(),
-            'flux_state': flux_state.cpu().numpy(),
+            'h_perturbation': keep(h_pert),
+            'qt_perturbation': keep(qt_pert),
+            'flux_state': keep(flux_state),
             'x0': 0.0, 'y0': 0.0, 'nx': nxf, 'ny': nyf,
             'dx': (nx * dx) / nxf, 'dy': (ny * dy) / nyf,
             # the file stores z in float32
@@ -490,7 +496,8 @@ def simulate_fields(h_profile, qt_profile, nx, ny, dx, dy, outer_scale, spherosc
             'bound_buffer_multiple': st['bound_buffer_multiple'],
             'sparsity_factors': tuple(sparsity_factors), 'turbulon_shape': turbulon_shape,
             'anisotropy': anisotropy, 'periodic_x': True, 'periodic_y': True,
-            'class_increments': [dict(inc, z=np.asarray(g['z_arrays'][j], dtype=np.float64),
+            'class_increments': [dict({name: keep(t) for name, t in inc.items()},
+                                      z=np.asarray(g['z_arrays'][j], dtype=np.float64),
                                       k=float(g['k'][j]), dx=float(g['dx'][j]),
                                       dy=float(g['dy'][j]))
                                  for j, inc in enumerate(increments)],

@@ -161,13 +161,39 @@ def plan_nest(parent, x_start, x_stop, y_start, y_stop, dx, dy, seed=None,
             'y_range': (int(y_start), int(y_stop)), 'periodic': (spans_x, spans_y)}
 
 
-def nest_class_ladder(parent, plan, own):
+def _shape(a):
+    return tuple(a.shape) if isinstance(a, torch.Tensor) else np.shape(a)
+
+
+def _take(a, keep):
+    """a[ix_(*keep)] (one index array per axis) for a NumPy array or a
+    device tensor, in the input's form."""
+    if isinstance(a, torch.Tensor):
+        for axis, idx in enumerate(keep):
+            a = a.index_select(axis, torch.from_numpy(np.asarray(idx, np.int64)).to(a.device))
+        return a
+    return np.asarray(a, dtype=np.float32)[np.ix_(*keep)]
+
+
+def _as_record(a, on_device):
+    """A float32 array in the record's form: a contiguous device tensor
+    (``on_device``) or a NumPy array."""
+    if isinstance(a, torch.Tensor):
+        a = a.to(torch.float32).contiguous()
+        return a if on_device else a.cpu().numpy()
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return torch.from_numpy(a).cuda() if on_device else a
+
+
+def nest_class_ladder(parent, plan, own, on_device=False):
     """The nest's stored class increments over the whole ladder
     (steam/simulate.py:3491-3555): the parent's classes restricted by index
     to the nest's extent on each class's own grid, then the nest's own
     classes (``own``: per class dicts of h, qt, flux on the padded working
     grid) cut to their inner windows; both keep the levels in
-    [z_min, z_max).  A nest of this nest then sees one uniform ladder."""
+    [z_min, z_max).  A nest of this nest then sees one uniform ladder.
+    Either input may hold NumPy arrays or device tensors; the ladder holds
+    device tensors with ``on_device``, NumPy arrays otherwise."""
     z_min, z_max = plan['z_min'], plan['z_max']
     x_start, x_stop = plan['x_range']
     y_start, y_stop = plan['y_range']
@@ -182,7 +208,7 @@ def nest_class_ladder(parent, plan, own):
     for j, cls in enumerate(parent['class_increments'][:int(parent['n_classes_consumed'])]):
         z_j = np.asarray(cls['z'], dtype=np.float64)
         keep_z = levels(z_j)
-        size_x, size_y = np.shape(cls['h'])[:2]
+        size_x, size_y = _shape(cls['h'])[:2]
         keep = []
         for size, start, stop, extent, periodic in (
                 (size_x, x_start, x_stop, pnx, per_x), (size_y, y_start, y_stop, pny, per_y)):
@@ -190,8 +216,8 @@ def nest_class_ladder(parent, plan, own):
             count = max(1, int(round((stop - start) / extent * size)))
             index = np.arange(first, first + count)
             keep.append(index % size if periodic else np.clip(index, 0, size - 1))
-        ix = np.ix_(keep[0], keep[1], keep_z)
-        entry = {name: np.ascontiguousarray(np.asarray(cls[name], dtype=np.float32)[ix])
+        keep.append(keep_z)
+        entry = {name: _as_record(_take(cls[name], keep), on_device)
                  for name in ('h', 'qt', 'flux')}
         entry.update(k=float(cls.get('k', float(parent['root_outer_scale']) / gap ** j)),
                      dx=float(cls.get('dx', np.nan)), dy=float(cls.get('dy', np.nan)),
@@ -203,8 +229,7 @@ def nest_class_ladder(parent, plan, own):
         z_i = np.asarray(grids['z_arrays'][i], dtype=np.float64)
         keep_z = levels(z_i)
         z0, z1 = int(keep_z[0]), int(keep_z[-1]) + 1
-        entry = {name: np.ascontiguousarray(np.asarray(cls[name], dtype=np.float32)
-                                            [x0:x1, y0:y1, z0:z1])
+        entry = {name: _as_record(cls[name][x0:x1, y0:y1, z0:z1], on_device)
                  for name in ('h', 'qt', 'flux')}
         entry.update(k=float(grids['k'][i]), dx=float(grids['dx'][i]),
                      dy=float(grids['dy'][i]), z=z_i[keep_z])
@@ -217,8 +242,9 @@ def nest_parent_record(parent, plan, state, z_final, ladder):
     continue from (steam/simulate.py:4366-4481, save_for_refinement): the
     trimmed state, the whole ladder's class increments and the continuation
     attributes, with the world position of the nest's first cell and the
-    per-axis periodicity (a nest wraps only along an axis it spans)."""
-    nx_out, ny_out = (int(v) for v in np.shape(state['h_perturbation'])[:2])
+    per-axis periodicity (a nest wraps only along an axis it spans).  The
+    state keeps its form (device tensors or NumPy arrays)."""
+    nx_out, ny_out = (int(v) for v in _shape(state['h_perturbation'])[:2])
     ext_x, ext_y = plan['inner_extent']
     x_start, y_start = plan['x_range'][0], plan['y_range'][0]
     # the first written coordinate, as the file's float32 axis holds it
@@ -231,7 +257,7 @@ def nest_parent_record(parent, plan, state, z_final, ladder):
         'bounds', 'min_distance_to_ground', 'n_scale_classes_per_dyad', 'root_domain_x',
         'root_domain_y', 'root_domain_height', 'root_outer_scale', 'root_seed', 'H_h',
         'haar_to_mhat', 'flux_noise_scale', 'bound_buffer_multiple') if k in parent}
-    rec.update({name: np.asarray(state[name], dtype=np.float32)
+    rec.update({name: _as_record(state[name], isinstance(state[name], torch.Tensor))
                 for name in ('h_perturbation', 'qt_perturbation', 'flux_state')})
     rec.update(x0=x0, y0=y0, nx=nx_out, ny=ny_out, dx=ext_x / nx_out, dy=ext_y / ny_out,
                z=np.asarray(z_final, dtype=np.float32).astype(np.float64),
@@ -244,11 +270,23 @@ def nest_parent_record(parent, plan, state, z_final, ladder):
     return rec
 
 
-def _slab(t, plan):
+def _slab(t, plan, axes=(0, 1, 2)):
     dev = t.device
-    return (t.index_select(0, torch.from_numpy(plan['x_indices']).to(dev))
-             .index_select(1, torch.from_numpy(plan['y_indices']).to(dev))
-             .index_select(2, torch.from_numpy(plan['z_indices']).to(dev)).contiguous())
+    for axis in axes:
+        idx = plan[('x_indices', 'y_indices', 'z_indices')[axis]]
+        t = t.index_select(axis, torch.from_numpy(np.asarray(idx, np.int64)).to(dev))
+    return t.contiguous()
+
+
+def _parent_slab(a, plan):
+    """A parent state field restricted to the nest's slab, on the device.  A
+    NumPy field is cut along x on the host first, so only the slab's rows
+    cross to the device; a device field is cut in place."""
+    if isinstance(a, torch.Tensor):
+        return _slab(a.to('cuda', torch.float32), plan)
+    rows = np.take(np.asarray(a, dtype=np.float32),
+                   np.asarray(plan['x_indices'], np.int64), axis=0)
+    return _slab(torch.from_numpy(np.ascontiguousarray(rows)).cuda(), plan, axes=(1, 2))
 
 
 def _inherited_deficits(parent, plan):
@@ -262,8 +300,9 @@ def _inherited_deficits(parent, plan):
             deficit_j = np.asarray(plan['comp_inherited'][j], dtype=np.float32) - np.float32(1.0)
             if acc is None and not np.any(deficit_j):
                 continue
-            inc = torch.from_numpy(np.ascontiguousarray(
-                parent['class_increments'][j][name], dtype=np.float32)).cuda()
+            inc = parent['class_increments'][j][name]
+            inc = (inc.to('cuda', torch.float32) if isinstance(inc, torch.Tensor) else
+                   torch.from_numpy(np.ascontiguousarray(inc, dtype=np.float32)).cuda())
             if acc is None:
                 acc = torch.zeros_like(inc)
             elif tuple(acc.shape) != tuple(inc.shape):
@@ -283,14 +322,14 @@ def refine_fields(parent, x_start, x_stop, y_start, y_stop, dx, dy, seed=None,
     like ``simulate_fields``: 'h', 'qt', 'flux', 'h_perturbation',
     'qt_perturbation', 'flux_state', 'ledger', 'grids', 'plan', and with
     ``save_for_refinement`` the nest's own 'parent' record, from which a
-    nest of this nest continues."""
+    nest of this nest continues (device tensors with ``as_tensors``, NumPy
+    arrays otherwise).  The parent record may hold either form."""
     check_device(device)
     plan = plan_nest(parent, x_start, x_stop, y_start, y_stop, dx, dy, seed,
                      sparsity_factors, turbulon_shape, z_min, z_max, anisotropy)
     p = parent
     h_min, h_max, qt_min, qt_max = p['bounds']
-    state = {name: _slab(torch.from_numpy(np.ascontiguousarray(p[name], dtype=np.float32))
-                         .cuda(), plan)
+    state = {name: _parent_slab(p[name], plan)
              for name in ('h_perturbation', 'qt_perturbation', 'flux_state')}
     inherited = _inherited_deficits(p, plan)
     own = [] if save_for_refinement else None
@@ -335,14 +374,13 @@ def refine_fields(parent, x_start, x_stop, y_start, y_stop, dx, dy, seed=None,
         run_ledger.projection['qt'] = proj_qt
     out = {'h': h_out, 'qt': qt_out, 'flux': flux_out, 'h_perturbation': h_in,
            'qt_perturbation': qt_in, 'flux_state': fl_in}
-    record = None
-    if save_for_refinement:
-        state = {k: out[k].cpu().numpy()
-                 for k in ('h_perturbation', 'qt_perturbation', 'flux_state')}
-        record = nest_parent_record(p, plan, state, z_final,
-                                    nest_class_ladder(p, plan, own))
     if not as_tensors:
         out = {k: v.cpu().numpy() for k, v in out.items()}
+    record = None
+    if save_for_refinement:
+        # the record shares the returned state arrays
+        record = nest_parent_record(p, plan, out, z_final,
+                                    nest_class_ladder(p, plan, own, on_device=as_tensors))
     out.update(ledger=run_ledger, grids=plan['grids'], plan=plan)
     if record is not None:
         out['parent'] = record

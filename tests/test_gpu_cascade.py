@@ -293,6 +293,44 @@ def test_gpu_spanning_nest_is_the_root_run_at_its_resolution():
             np.testing.assert_array_equal(out[field], direct[field], err_msg=(dx, field))
 
 
+def test_gpu_device_resident_record_chain_equals_the_host_record_chain():
+    """A parent record kept on the device (``as_tensors``) refines exactly
+    as its NumPy copy: root -> nest (not spanning, saved) -> nest of the
+    nest, every field and every stored increment bit-identical, and mixed
+    forms (a host record refined to a device one and back) too."""
+    import torch
+    from steam_amd import cascade as C
+    from steam_amd.refine import refine_fields
+    args = (H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0, 800.0, 30.0)
+    host = C.simulate_fields(*args, seed=42, save_for_refinement=True)['parent']
+    dev = C.simulate_fields(*args, seed=42, save_for_refinement=True,
+                            as_tensors=True)['parent']
+    assert isinstance(dev['h_perturbation'], torch.Tensor)
+    assert isinstance(dev['class_increments'][-1]['flux'], torch.Tensor)
+
+    def same(a, b, what):
+        assert isinstance(a, torch.Tensor) and isinstance(b, np.ndarray), what
+        np.testing.assert_array_equal(a.cpu().numpy(), b, err_msg=str(what))
+
+    for name in ('h_perturbation', 'qt_perturbation', 'flux_state'):
+        same(dev[name], host[name], name)
+    n_h = refine_fields(host, 0, 8, 2, 10, 62.5, 62.5, seed=11, save_for_refinement=True)
+    n_d = refine_fields(dev, 0, 8, 2, 10, 62.5, 62.5, seed=11, save_for_refinement=True,
+                        as_tensors=True)
+    for field in FIELDS:
+        same(n_d[field], n_h[field], field)
+    rh, rd = n_h['parent'], n_d['parent']
+    for j, (a, b) in enumerate(zip(rd['class_increments'], rh['class_increments'])):
+        for name in ('h', 'qt', 'flux'):
+            same(a[name], b[name], (j, name))
+    nn_h = refine_fields(rh, 6, 8, 6, 8, 31.25, 31.25, seed=13)
+    nn_d = refine_fields(rd, 6, 8, 6, 8, 31.25, 31.25, seed=13, as_tensors=True)
+    nn_x = refine_fields(rd, 6, 8, 6, 8, 31.25, 31.25, seed=13)
+    for field in FIELDS:
+        same(nn_d[field], nn_h[field], field)
+        np.testing.assert_array_equal(nn_x[field], nn_h[field], err_msg=field)
+
+
 @pytest.mark.parametrize("z_min,z_max", [(240.0, 600.0), (0.0, 450.0), (330.0, 800.0)])
 def test_gpu_elevated_nest_matches_the_oracle(z_min, z_max):
     """A nest over part of the column (z pads below and above, zero_bottom /
