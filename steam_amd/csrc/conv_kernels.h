@@ -414,7 +414,9 @@ __global__ __launch_bounds__(256) void ypass_reg_kernel(Src src, int inv) {
   float2 v[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = valid ? Sl[q + m * P] : make_float2(0.f, 0.f);
-  fft_reg<LOGN>(v, q, lds, 1, lw * F::LDSN, g.ay.tw, inv != 0);
+  RegTw<LOGN> twb;
+  reg_tw_load<LOGN>(twb, q, g.ay.tw);
+  fft_reg<LOGN>(v, q, lds, 1, lw * F::LDSN, g.ay.tw, twb, inv != 0);
   if (valid) {
 #pragma unroll
     for (int m = 0; m < 8; ++m) Sl[q + m * P] = v[m];
@@ -481,14 +483,18 @@ __global__ __launch_bounds__(1024) STEAM_XREG_WAVES_ATTR void xpass_reg_kernel(S
             : d.K + (int64_t)(p % g.nzc) * nx * ny;
     if (KPRE) load_k();
   }
-  fft_reg_lanes<LOGN, C>(v, q0, PT, lds, T, t, g.ax.tw, false);
+  // the twiddle bases, shared by the forward and the inverse transform
+  RegTw<LOGN> twb[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) reg_tw_load<LOGN>(twb[c], q0 + c * PT, g.ax.tw);
+  fft_reg_lanes<LOGN, C>(v, q0, PT, lds, T, t, g.ax.tw, twb, false);
   if (XMODE == 0) {
     if (!KPRE) load_k();
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[c][m] = cmul(v[c][m], kv[c][m]);
-    fft_reg_lanes<LOGN, C>(v, q0, PT, lds, T, t, g.ax.tw, true);
+    fft_reg_lanes<LOGN, C>(v, q0, PT, lds, T, t, g.ax.tw, twb, true);
   } else {
     const float2 f = kernel_bin_factor(d, p % g.nzc);
 #pragma unroll

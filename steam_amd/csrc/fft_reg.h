@@ -28,29 +28,64 @@ struct RegFFT {
 
 __device__ __forceinline__ int pad8(int i) { return i + (i >> 3); }
 
-// Transform the line held in v (thread q of the line).  LDS element i of the
-// line lives at lds[pad8(i) * lstride + loff].  Every thread of the
-// workgroup must call this (it synchronises the workgroup between stages).
+// Twiddle bases of a thread lane q for the radix-8 stages s >= 1: W^e, W^2e
+// and W^4e with e = k * step (k = q mod 8^s, step = N / 8^(s+1)), read from
+// the table once (a pass that transforms twice reuses them).  A stage's other
+// four twiddles are products of these, at most two deep, so each stays
+// within a few float32 ulps of the table's correctly rounded value.
+template <int LOGN>
+struct RegTw {
+  static constexpr int NS = RegFFT<LOGN>::N8 - 1;
+  float2 b[NS > 0 ? NS : 1][3];
+};
+
+template <int LOGN>
+__device__ __forceinline__ void reg_tw_load(RegTw<LOGN>& t, int q,
+                                            const float2* __restrict__ tw) {
+  using F = RegFFT<LOGN>;
+#pragma unroll
+  for (int s = 1; s < F::N8; ++s) {
+    const int Ns = 1 << (3 * s);
+    const int e = (q & (Ns - 1)) * (F::N / (Ns * 8));
+    t.b[s - 1][0] = tw[e];
+    t.b[s - 1][1] = tw[2 * e];
+    t.b[s - 1][2] = tw[4 * e];
+  }
+}
+
+// v[r] *= W^(r e) (conjugated for the inverse), r = 1..7, from the bases
+__device__ __forceinline__ void stage_twiddle8(float2 (&v)[8], const float2 (&b)[3], bool inv) {
+  float2 w[8];
+  w[1] = b[0];
+  w[2] = b[1];
+  w[4] = b[2];
+  w[3] = cmul(b[0], b[1]);
+  w[5] = cmul(b[0], b[2]);
+  w[6] = cmul(b[1], b[2]);
+  w[7] = cmul(w[3], b[2]);
+#pragma unroll
+  for (int r = 1; r < 8; ++r) {
+    if (inv) w[r].y = -w[r].y;
+    v[r] = cmul(v[r], w[r]);
+  }
+}
+
+// Transform the line held in v (thread q of the line; twb its twiddle bases,
+// reg_tw_load).  LDS element i of the line lives at
+// lds[pad8(i) * lstride + loff].  Every thread of the workgroup must call
+// this (it synchronises the workgroup between stages).
 template <int LOGN>
 __device__ __forceinline__ void fft_reg(float2 (&v)[8], int q, float2* lds,
                                         int lstride, int loff,
                                         const float2* __restrict__ tw,
-                                        bool inv) {
+                                        const RegTw<LOGN>& twb, bool inv) {
   using F = RegFFT<LOGN>;
-  constexpr int N = F::N, P = F::P;
+  constexpr int P = F::P;
 #pragma unroll
   for (int s = 0; s < F::N8; ++s) {
     const int Ns = 1 << (3 * s);
     const int k = q & (Ns - 1);
-    if (s > 0) {
-      const int step = N / (Ns * 8);
-#pragma unroll
-      for (int r = 1; r < 8; ++r) {
-        float2 w = tw[r * k * step];
-        if (inv) w.y = -w.y;
-        v[r] = cmul(v[r], w);
-      }
-    }
+    if (s > 0) stage_twiddle8(v, twb.b[s - 1], inv);
     dft_small<8>(v, inv);
     const bool last = (s == F::NST - 1);
     if (!last) {
@@ -94,25 +129,16 @@ __device__ __forceinline__ void fft_reg(float2 (&v)[8], int q, float2* lds,
 template <int LOGN, int C>
 __device__ __forceinline__ void fft_reg_lanes(float2 (&v)[C][8], int q0, int qstep,
                                               float2* lds, int lstride, int loff,
-                                              const float2* __restrict__ tw, bool inv) {
+                                              const float2* __restrict__ tw,
+                                              const RegTw<LOGN> (&twb)[C], bool inv) {
   using F = RegFFT<LOGN>;
-  constexpr int N = F::N, P = F::P;
+  constexpr int P = F::P;
 #pragma unroll
   for (int s = 0; s < F::N8; ++s) {
     const int Ns = 1 << (3 * s);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const int q = q0 + c * qstep;
-      const int k = q & (Ns - 1);
-      if (s > 0) {
-        const int step = N / (Ns * 8);
-#pragma unroll
-        for (int r = 1; r < 8; ++r) {
-          float2 w = tw[r * k * step];
-          if (inv) w.y = -w.y;
-          v[c][r] = cmul(v[c][r], w);
-        }
-      }
+      if (s > 0) stage_twiddle8(v[c], twb[c].b[s - 1], inv);
       dft_small<8>(v[c], inv);
     }
     const bool last = (s == F::NST - 1);
