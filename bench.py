@@ -480,8 +480,10 @@ def streamed_form(reps=3, plans=((5, 2, 1), (4, 4, 2), (4, 8, 4))):
     fields) on the 2048x128x115 ladder of the cost-comparison grids.  Each
     plan is (horizon, tiles_x, tiles_y): the classes from the horizon on are
     tiled, the narrower ones before it run as one tile (plan_tiling's forced
-    plan).  Reports the times, the tiling overhead and the streamed fields'
-    rel-L2 against the resident ones."""
+    plan).  Reports the times, the tiling overhead, the streamed fields'
+    rel-L2 against the resident ones, and the h differences in units of the
+    float32 spacing of phi = perturbation + mean, the resolution h is carried
+    at."""
     import numpy as np
     import torch
     from steam_amd.cascade import cascade_loop
@@ -511,6 +513,13 @@ def streamed_form(reps=3, plans=((5, 2, 1), (4, 4, 2), (4, 8, 4))):
         return res, best
 
     resident, t_res = timed(lambda: cascade_loop(*args, **kw))
+    # the resolution the h state is carried at: the bounded add forms
+    # phi = perturbation + mean in float32, so h moves in steps of ulp(phi)
+    z_fin = np.asarray(resident[6]['z'], dtype=np.float32)
+    h_mean = torch.from_numpy(np.interp(z_fin, st['z_profile'], st['h_profile'])
+                              .astype(np.float32)).cuda()
+    phi = (resident[0] + h_mean).abs()
+    ulp_phi = (torch.nextafter(phi, torch.full_like(phi, float("inf"))) - phi).double()
     runs = []
     for force in plans:
         tiles = plan_tiling(st['grids'], (13, 13, 13), force=force)
@@ -520,14 +529,21 @@ def streamed_form(reps=3, plans=((5, 2, 1), (4, 4, 2), (4, 8, 4))):
             a = streamed[k].double()
             b = resident[k].double()
             rel[name] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+        steps = (streamed[0].double() - resident[0].double()).abs() / ulp_phi
         runs.append({"plan": {"horizon": force[0], "tiles": [force[1], force[2]]},
                      "best_ms": t * 1e3, "vs_resident": t / t_res,
-                     "rel_l2_vs_resident": rel})
+                     "rel_l2_vs_resident": rel,
+                     "h_difference_in_ulps_of_phi": {
+                         "max": float(steps.max()),
+                         "cells_above_1": float((steps > 1.0).double().mean()),
+                         "cells_above_4": float((steps > 4.0).double().mean())}})
     g = st['grids']
     return {"workload": "root cascade 2048x128x115 (7 classes), resident and streamed",
             "class_voxels": int(sum(int(a) * int(b) * int(c)
                                     for a, b, c in zip(g['nx'], g['ny'], g['nz']))),
             "resident_ms": t_res * 1e3, "streamed": runs,
+            "h_rms_over_ulp_phi": float(resident[0].double().pow(2).mean().sqrt()
+                                        / ulp_phi.mean()),
             "data": "synthetic linear h/qt profiles, seed 0"}
 
 

@@ -178,6 +178,30 @@ def _record(increment_dir, increments, i, name, t):
         increments[-1][name] = t
 
 
+_kernel_memo = {}
+
+
+def _cascade_kernel(s_x, s_y, s_z, shape):
+    """The cascade's turbulon kernel (one for every class of a run) on the
+    device and on the host.  Synthesis is deterministic in its arguments, so
+    runs with the same sparsity and shape share one kernel tensor, and with it
+    the spectra cached per class shape; an entry is rebuilt if the synthesis
+    function was replaced or the tensor written in place."""
+    synth = _sim._turbulon_envelope
+    key = (float(s_x), float(s_y), float(s_z), float(_sim.SUPPORT_FACTOR), shape, id(synth))
+    hit = _kernel_memo.get(key)
+    if hit is not None and hit[0]._version == hit[2] and hit[3] is synth:
+        return hit[0], hit[1]
+    kernel_t = synth(1, 1 / (2 * s_x), 1 / (2 * s_y), 1 / (2 * s_z),
+                     support_factor=_sim.SUPPORT_FACTOR, shape=shape, as_tensor=True)
+    kernel_np = kernel_t.cpu().numpy()
+    kernel_np.flags.writeable = False
+    if len(_kernel_memo) >= 8:
+        _kernel_memo.pop(next(iter(_kernel_memo)))
+    _kernel_memo[key] = (kernel_t, kernel_np, kernel_t._version, synth)
+    return kernel_t, kernel_np
+
+
 def _convolve(W, kernel_t, kernel_np, device):
     conv = _sim.CONVOLVE
     if conv is convolve_fft_xy_oa_z:
@@ -263,10 +287,7 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
     if as_tensors is None:
         as_tensors = isinstance(h_perturbation, torch.Tensor)
     c = _sim.FLUX_SCALE if flux_noise_scale is None else flux_noise_scale
-    kernel_t = _sim._turbulon_envelope(1, 1 / (2 * s_x), 1 / (2 * s_y), 1 / (2 * s_z),
-                                       support_factor=_sim.SUPPORT_FACTOR,
-                                       shape=turbulon_shape, as_tensor=True)
-    kernel_np = kernel_t.cpu().numpy()
+    kernel_t, kernel_np = _cascade_kernel(s_x, s_y, s_z, turbulon_shape)
     # the carried fields are the run's own copies: the caller's are not mutated
     h, qt, fl, dh, dqt, dfl = [_own(a) for a in (h_perturbation, qt_perturbation, flux,
                                                  deficit_h, deficit_qt, deficit_flux)]
