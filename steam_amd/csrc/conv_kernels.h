@@ -937,7 +937,16 @@ using ZP64 = ZPlan<64, 8, 8, 8>;
 using ZP80 = ZPlan<80, 20, 4, 20>;
 using ZP96 = ZPlan<96, 24, 8, 12>;
 using ZP128 = ZPlan<128, 16, 8, 16>;
+// 160 points: 20 values on 8 lanes per line, or (STEAM_ZP160_V 10) 10 values
+// on 16 lanes, radix 10 then four radix-2 stages
+#ifndef STEAM_ZP160_V
+#define STEAM_ZP160_V 20
+#endif
+#if STEAM_ZP160_V == 10
+using ZP160 = ZPlan<160, 10, 10, 2, 2, 2, 2>;
+#else
 using ZP160 = ZPlan<160, 20, 4, 4, 10>;
+#endif
 using ZP192 = ZPlan<192, 24, 8, 24>;
 using ZP256 = ZPlan<256, 16, 16, 16>;
 using ZP320 = ZPlan<320, 20, 4, 4, 20>;
