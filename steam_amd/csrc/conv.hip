@@ -452,9 +452,13 @@ extern "C" int steam_flux_noise(const steam_conv_plan* p,
   d.nfield = out_dev;
   d.noise = make_noise_dev(noise);
   {
+    // the field generated ahead, usually beside another class's transforms:
+    // at most STEAM_NOISE_WGS workgroups (0: one per four lines)
     ProfScope ps(STEAM_KF_NOISE, s);
-    hipLaunchKernelGGL((noise_field_kernel<SingleSrc>), dim3(p->wg_noise), dim3(256), 0,
-                       s, SingleSrc{d});
+    const int grid = (STEAM_NOISE_WGS > 0 && p->wg_noise > STEAM_NOISE_WGS) ? STEAM_NOISE_WGS
+                                                                              : p->wg_noise;
+    hipLaunchKernelGGL((noise_field_kernel<SingleSrc, true>), dim3(grid), dim3(256), 0, s,
+                       SingleSrc{d});
   }
   STEAM_LAUNCH_CHECK("noise_field_kernel");
   return STEAM_OK;

@@ -189,7 +189,17 @@ __device__ __forceinline__ float2 kernel_bin_factor(const PassDesc& d, int kz) {
 // ahead of the advance on another stream.  4 lines per workgroup, threads
 // along z (coalesced stores).
 // ---------------------------------------------------------------------------
-template <class Src>
+// workgroups of a noise field generated ahead of its advance (0: one per
+// four lines).  1.5 per CU of the 256: beside the transforms of the class
+// before, 384 took a 512x512x256 step from 0.999 to 0.979 ms (256: 1.004,
+// 512: 0.990, 1024: 1.036, 2048: 1.090)
+#ifndef STEAM_NOISE_WGS
+#define STEAM_NOISE_WGS 384
+#endif
+// STRIDE: a grid smaller than the field's line groups walks them with a
+// grid stride (a few workgroups per CU keep the VALU busy beside memory-bound
+// passes on another stream without taking their slots)
+template <class Src, bool STRIDE = false>
 __global__ __launch_bounds__(256) void noise_field_kernel(Src src) {
   int wg = blockIdx.x;
   const PassDesc& d = src.get(PASS_NOISE, wg);
@@ -197,14 +207,17 @@ __global__ __launch_bounds__(256) void noise_field_kernel(Src src) {
   const int nz = g.nz;
   const int64_t lines = (int64_t)g.nx * g.ny;
   const int lpw = 4;
-  for (int l = 0; l < lpw; ++l) {
-    const int64_t line = (int64_t)wg * lpw + l;
-    if (line >= lines) break;
-    const int x = (int)(line / g.ny);
-    const int y = (int)(line - (int64_t)x * g.ny);
-    const LineNoise ln = noise_line(d.noise, x, y);
-    float* o = d.nfield + line * nz;
-    for (int z = threadIdx.x; z < nz; z += 256) o[z] = noise_at(d.noise, ln, z, nz);
+  const int64_t stride = STRIDE ? (int64_t)gridDim.x * lpw : lines;
+  for (int64_t base = (int64_t)wg * lpw; base < lines; base += stride) {
+    for (int l = 0; l < lpw; ++l) {
+      const int64_t line = base + l;
+      if (line >= lines) break;
+      const int x = (int)(line / g.ny);
+      const int y = (int)(line - (int64_t)x * g.ny);
+      const LineNoise ln = noise_line(d.noise, x, y);
+      float* o = d.nfield + line * nz;
+      for (int z = threadIdx.x; z < nz; z += 256) o[z] = noise_at(d.noise, ln, z, nz);
+    }
   }
 }
 
