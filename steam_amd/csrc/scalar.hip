@@ -241,11 +241,20 @@ constexpr int RD_U = 4;
 // large fields, where the candidates' arithmetic would bound the pass
 constexpr int KCAND = 15;
 constexpr int KCAND_WIDE = 7;
+constexpr int KCAND_NARROW = 3;
 // fields above this many cells take the 3-deep tree
 #ifndef STEAM_KWIDE_LOG2
 #define STEAM_KWIDE_LOG2 20
 #endif
 constexpr int64_t KWIDE_CELLS = int64_t(1) << STEAM_KWIDE_LOG2;
+// above 2^STEAM_KNARROW_LOG2 cells the tree is 2 deep (3 candidates, 30
+// passes).  Off by default: at 2048x2048x115 it took the bounded add from
+// 18.2 to 19.1 ms and the projection from 20.7 to 22.2 ms (the 20 passes of
+// 7 candidates cost less than 30 of 3)
+#ifndef STEAM_KNARROW_LOG2
+#define STEAM_KNARROW_LOG2 40
+#endif
+constexpr int64_t KNARROW_CELLS = int64_t(1) << STEAM_KNARROW_LOG2;
 
 struct LevelArgs {
   const float* f;
@@ -869,6 +878,12 @@ int level_partials(const LevelArgs& a, double* scratch, hipStream_t s, int64_t* 
     else if (a.ncand == KCAND_WIDE && a.op == LV_E_CLAMP)
       hipLaunchKernelGGL((level_partial_kernel<KCAND_WIDE, LV_E_CLAMP>), grid, block, 0, s, a,
                          scratch, lpg);
+    else if (a.ncand == KCAND_NARROW && a.op == LV_CLAMP)
+      hipLaunchKernelGGL((level_partial_kernel<KCAND_NARROW, LV_CLAMP>), grid, block, 0, s, a,
+                         scratch, lpg);
+    else if (a.ncand == KCAND_NARROW && a.op == LV_E_CLAMP)
+      hipLaunchKernelGGL((level_partial_kernel<KCAND_NARROW, LV_E_CLAMP>), grid, block, 0, s, a,
+                         scratch, lpg);
     else
       return set_error(STEAM_EINVAL, "level reduction: candidates need a clamp op");
   } else {
@@ -950,6 +965,7 @@ int bisect_depth(LevelArgs a, BisectVec b, int nz, double width, double npts, in
 int bisect(LevelArgs a, BisectVec b, int nz, double width, double npts, int mode, double* red,
            hipStream_t s) {
   const int64_t cells = (int64_t)a.nx * a.ny * a.nz;
+  if (cells > KNARROW_CELLS) return bisect_depth<2>(a, b, nz, width, npts, mode, red, s);
   return cells > KWIDE_CELLS ? bisect_depth<3>(a, b, nz, width, npts, mode, red, s)
                              : bisect_depth<4>(a, b, nz, width, npts, mode, red, s);
 }
