@@ -960,6 +960,10 @@ using ZP160 = ZPlan<160, 10, 10, 2, 2, 2, 2>;
 #else
 using ZP160 = ZPlan<160, 20, 4, 4, 10>;
 #endif
+// 72 and 144 points (z lengths 144 and 288: a 13-tap kernel over 128 and
+// 256 levels pads to them instead of 160 and 320): 18 values per lane
+using ZP72 = ZPlan<72, 18, 6, 6, 2>;
+using ZP144 = ZPlan<144, 18, 6, 6, 2, 2>;
 using ZP192 = ZPlan<192, 24, 8, 24>;
 using ZP256 = ZPlan<256, 16, 16, 16>;
 using ZP320 = ZPlan<320, 20, 4, 4, 20>;
@@ -971,9 +975,11 @@ inline bool with_zplan(int n2, F&& f) {
   switch (n2) {
     case 32: f(ZP32{}); return true;
     case 64: f(ZP64{}); return true;
+    case 72: f(ZP72{}); return true;
     case 80: f(ZP80{}); return true;
     case 96: f(ZP96{}); return true;
     case 128: f(ZP128{}); return true;
+    case 144: f(ZP144{}); return true;
     case 160: f(ZP160{}); return true;
     case 192: f(ZP192{}); return true;
     case 256: f(ZP256{}); return true;

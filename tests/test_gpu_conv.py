@@ -242,3 +242,28 @@ def test_spectrum_cache_never_serves_a_freed_kernels_spectrum():
         ref = op.execute(field, torch.empty_like(field), op.spectrum(k))
         np.testing.assert_array_equal(out.cpu().numpy(), ref.cpu().numpy())
         del k
+
+
+@pytest.mark.parametrize("fs,ks,n_fft", [
+    ((32, 24, 128), (13, 13, 13), 144), ((24, 32, 131), (9, 9, 13), 144),
+    ((20, 12, 100), (7, 7, 9), 144), ((16, 16, 256), (13, 13, 13), 288),
+    ((16, 8, 275), (5, 5, 13), 288), ((8, 16, 201), (3, 3, 7), 288),
+])
+def test_register_z_plans_at_18_values_per_lane(fs, ks, n_fft):
+    """The 72- and 144-point z plans (radix 6·6·2 and 6·6·2·2, 18 values on 4
+    or 8 lanes per line), plain and compensated, one z block, odd level
+    counts, against the float64 oracle; the length model now picks them for
+    a 13-tap kernel over 128 and 256 levels."""
+    from steam_amd.batch import nfft_candidates
+    from steam_amd.utils import ConvOperator
+    rng = np.random.default_rng(11)
+    f = torch.from_numpy(rng.standard_normal(fs).astype(np.float32)).cuda()
+    k = torch.from_numpy(rng.standard_normal(ks).astype(np.float32)).cuda()
+    ref = M.conv_fft(f.cpu().numpy(), k.cpu().numpy())
+    for prec in ('plain', 'kahan'):
+        op = ConvOperator(fs, ks, n_fft=n_fft, precision=prec)
+        assert op.n_fft == n_fft and op.n_blocks == 1
+        out = torch.empty_like(f)
+        op.execute(f, out, op.spectrum(k))
+        assert _rel(out.cpu().numpy(), ref) < CONV_REL_L2, prec
+    assert nfft_candidates(128, 13)[0] == 144 and nfft_candidates(256, 13)[0] == 288

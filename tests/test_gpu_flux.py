@@ -44,6 +44,9 @@ def _region(key, origin, world):
     # odd level counts on the register z path
     ((32, 32, 115), (1, 1, 1), 3, (3, 29, 2, 30), None),
     ((40, 24, 53), (1, 1, 1), 2, None, None),
+    # 13-tap kernel over 128 and 256 levels: the 72- and 144-point z plans
+    ((32, 32, 128), (1, 1, 1), 3, None, None),
+    ((16, 16, 256), (1, 1, 1), 3, (2, 14, 3, 13), 1.6),
 ])
 def test_advance_matches_fp64_oracle(shape, s, sf, window, alpha):
     from steam_amd.simulate import _advance_flux
