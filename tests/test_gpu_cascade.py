@@ -386,3 +386,32 @@ def test_gpu_root_run_options_match_the_oracle(shape_name, aniso, nx, ls, s, n):
     for field, tol in TOL.items():
         assert out[field].shape == ref[field].shape, field
         assert rel(out[field], ref[field]) < tol, (field, rel(out[field], ref[field]))
+
+
+def test_cascade_kernel_memo_is_keyed_and_invalidated(monkeypatch):
+    """The cascade's kernel is synthesised once per (sparsity, support,
+    shape, synthesis function); another sparsity, a replaced synthesis
+    function or an in-place write to the kept tensor builds it again, and
+    the host copy is read-only."""
+    from steam_amd import cascade as C
+    from steam_amd import simulate as sm
+    calls = []
+    real = sm._turbulon_envelope
+
+    def counting(*a, **k):
+        calls.append(1)
+        return real(*a, **k)
+
+    monkeypatch.setattr(sm, '_turbulon_envelope', counting)
+    t1, n1 = C._cascade_kernel(1, 1, 1, 'mexican_hat')
+    t2, n2 = C._cascade_kernel(1, 1, 1, 'mexican_hat')
+    assert t1 is t2 and n1 is n2 and len(calls) == 1
+    C._cascade_kernel(2, 2, 2, 'mexican_hat')
+    assert len(calls) == 2
+    t1.mul_(1.0)
+    t4, n4 = C._cascade_kernel(1, 1, 1, 'mexican_hat')
+    assert t4 is not t1 and len(calls) == 3
+    np.testing.assert_array_equal(n4, n1)
+    assert not n1.flags.writeable
+    with pytest.raises(ValueError):
+        n1[0, 0, 0] = 1.0
