@@ -1225,15 +1225,18 @@ __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src sr
   // output levels z = 2n, 2n+1 of natural index n (the kernel spectrum
   // carries the 'same' trim shift); with an odd nz the last pair's second
   // level is past the top and is neither written nor counted
+  // STEAM_ZR_PREFETCH 1: all of it before the transform, 2: the first half
+  // before and the rest after
   float2 fl[V];
-  auto load_flux = [&]() {
-#pragma unroll
-    for (int m = 0; m < V; ++m) {
-      const int z0 = 2 * (q + m * P);
-      fl[m] = (valid && z0 < nz) ? ld_zpair<ODD>(d.out, zoff, z0, nz) : make_float2(0.f, 0.f);
-    }
+  constexpr int PRE = STEAM_ZR_PREFETCH == 1 ? V : (STEAM_ZR_PREFETCH == 2 ? V / 2 : 0);
+  auto load_flux = [&](int m) {
+    const int z0 = 2 * (q + m * P);
+    fl[m] = (valid && z0 < nz) ? ld_zpair<ODD>(d.out, zoff, z0, nz) : make_float2(0.f, 0.f);
   };
-  if (MODE == MODE_FLUX && STEAM_ZR_PREFETCH) load_flux();
+  if (MODE == MODE_FLUX) {
+#pragma unroll
+    for (int m = 0; m < PRE; ++m) load_flux(m);
+  }
   stage_tw2(g, tw2, N);
   __syncthreads();
   float2* L = lds_lines + (w * LPW + t) * ZP::PITCH;
@@ -1262,7 +1265,8 @@ __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src sr
       if (valid && z0 < nz) st_zpair<ODD>(d.out, zoff, z0, nz, v[m]);
     }
   } else {
-    if (!STEAM_ZR_PREFETCH) load_flux();
+#pragma unroll
+    for (int m = PRE; m < V; ++m) load_flux(m);
     const Pending pd = pending_of(d);
     const int x = (int)(line / g.ny), y = (int)(line - (int64_t)x * g.ny);
     const bool win = valid && x >= d.win[0] && x < d.win[1] && y >= d.win[2] && y < d.win[3];
