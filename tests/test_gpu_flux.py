@@ -166,13 +166,15 @@ def test_flux_only_validation_errors():
     assert d['n_clipped'] > 0 and np.all(f >= 0)
 
 
-def test_pregenerated_noise_and_chained_rescale_are_bit_identical():
+@pytest.mark.parametrize("shape", [(48, 40, 36), (256, 160, 12)])
+def test_pregenerated_noise_and_chained_rescale_are_bit_identical(shape):
     """The pipelined form (noise generated ahead into a caller buffer, the
     pending rescale chained on the device) reproduces the plain form bit for
-    bit over three deferred advances."""
+    bit over three deferred advances.  The ahead-of-time noise strides its
+    line groups over a capped grid: 480 groups (a second pass for some
+    workgroups) and 10240 (about 27 passes each)."""
     from steam_amd import simulate as S
     from steam_amd.utils import conv_operator
-    shape = (48, 40, 36)
     kern = S._turbulon_envelope(1, .5, .5, .5, as_tensor=True)
     op = conv_operator(shape, tuple(kern.shape))
     regions = [_region(M.class_key(sq), (0, 0, 0), shape)
