@@ -23,7 +23,7 @@ def lib():
         L.oracle_philox4x32_10.argtypes = [P, u64, u64, P]
         L.oracle_keyed_uniforms.argtypes = [P, i64, P, i64, P, i64, u64, u64,
                                             u64, P, P]
-        L.oracle_levy_f32.argtypes = [ctypes.c_float, P, P, P, i64]
+        L.oracle_levy_f32.argtypes = [ctypes.c_double, P, P, P, i64]
         L.oracle_levy_f64.argtypes = [ctypes.c_double, P, P, P, i64]
         L.oracle_keyed_levy.argtypes = [P, i64, P, i64, P, i64, u64, u64, u64,
                                         ctypes.c_double, ctypes.c_int, P, P]

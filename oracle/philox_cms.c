@@ -123,9 +123,11 @@ static inline double cms_f64(float u0f, float u1f, double alpha) {
   return sign * sin(shifted) * factor * tail;
 }
 
-void oracle_levy_f32(float alpha, const float* u0, const float* u1, float* out,
+void oracle_levy_f32(double alpha, const float* u0, const float* u1, float* out,
                      int64_t n) {
-  double a = (double)alpha;
+  /* the float32 constants are rounded from float64 expressions of alpha, as
+   * the reference forms them (steam/simulate.py:471-473, 421-433) */
+  double a = alpha;
   float phi0 = (float)(-(3.141592653589793 / 2.0) * (1.0 - fabs(1.0 - a)) / a);
   float sign = a > 1.0 ? 1.0f : -1.0f;
   float abs_am1 = (float)fabs(a - 1.0);

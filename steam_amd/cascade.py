@@ -192,8 +192,8 @@ def _cascade_kernel(s_x, s_y, s_z, shape):
     hit = _kernel_memo.get(key)
     if hit is not None and hit[0]._version == hit[2] and hit[3] is synth:
         return hit[0], hit[1]
-    kernel_t = synth(1, 1 / (2 * s_x), 1 / (2 * s_y), 1 / (2 * s_z),
-                     support_factor=_sim.SUPPORT_FACTOR, shape=shape, as_tensor=True)
+    kernel_t = _sim.synth_kernel_device(1, 1 / (2 * s_x), 1 / (2 * s_y), 1 / (2 * s_z),
+                                        _sim.SUPPORT_FACTOR, shape)
     kernel_np = kernel_t.cpu().numpy()
     kernel_np.flags.writeable = False
     if len(_kernel_memo) >= 8:
@@ -369,12 +369,16 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
         op_key = (tuple(fl.shape), tuple(kernel_t.shape))
         if op_key in flux_pending:
             # this operator's ledger slot is about to be rewritten
-            flux_pending.pop(op_key).flux, _ = _sim.read_device_ledger(*op_key)
+            held, op_held, ws_held = flux_pending.pop(op_key)
+            held.flux, _ = _sim.read_operator_ledger(op_held, ws_held)
         S_k, class_ledger.flux, _ = _advance(
             fl, region, kernel_t, kernel_np, c, n_scale_classes_per_dyad, sparsity_factors,
             n_zero, zero_bottom, zero_top, device, window, class_ledger.flux)
         if class_ledger.flux is None:
-            flux_pending[op_key] = class_ledger
+            # the operator and workspace the advance just used: held here, so
+            # the final read cannot meet an evicted or rebuilt operator
+            op_used = conv_operator(*op_key)
+            flux_pending[op_key] = (class_ledger, op_used, op_used.workspace())
         if need_increment:
             if has_deficit:
                 deficit_accumulate(dfl, fl, deficit_t, before=before)
@@ -426,8 +430,8 @@ def cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k, 
         for k, name in enumerate(measured):
             class_ledger.pattern[name] = PatternLedger(
                 got[2 * k], np.rint(got[2 * k + 1]).astype(np.int64))
-    for op_key, class_ledger in flux_pending.items():
-        class_ledger.flux, _ = _sim.read_device_ledger(*op_key)
+    for class_ledger, op_held, ws_held in flux_pending.values():
+        class_ledger.flux, _ = _sim.read_operator_ledger(op_held, ws_held)
     for class_ledger in ledger.classes:
         for name, led in list(class_ledger.bounded_add.items()):
             if isinstance(led, DeviceBoundedAddLedger):

@@ -393,10 +393,17 @@ def generate_noise_device(op, region, flux_noise_scale, n_scale_classes_per_dyad
 
 def read_device_ledger(flux_shape, kernel_shape, precision='plain'):
     """The ledger the last fused advance on this operator left on the device."""
-    lib = _lib.load()
     op = conv_operator(flux_shape, kernel_shape, precision=precision)
+    return read_operator_ledger(op, op.workspace())
+
+
+def read_operator_ledger(op, workspace):
+    """The ledger the last fused advance left in ``workspace`` of ``op``.  A
+    caller that defers the read holds both objects, so neither can be
+    evicted or reused in between."""
+    lib = _lib.load()
     L = _lib.FluxLedger()
-    _lib.check(lib.steam_flux_ledger_read(op.plan, _lib.ptr(op.workspace()),
+    _lib.check(lib.steam_flux_ledger_read(op.plan, _lib.ptr(workspace),
                                           ctypes.byref(L), _lib.stream_handle()),
                "steam_flux_ledger_read")
     return FluxAdvanceLedger.from_native(L), L
@@ -419,7 +426,7 @@ def _advance_flux(flux, noise_region, kernel, flux_noise_scale,
         kt, _ = to_device(kernel)
         sk = torch.empty_like(ft)
         op = conv_operator(ft.shape, kt.shape, precision=precision)
-        spec = op.spectrum(kt, key=kernel_key(kernel))
+        spec = op.spectrum(kt, key=kernel_key(kernel), owner=kernel)
         led, L = advance_flux_device(
             ft, noise_region, kt, flux_noise_scale, n_scale_classes_per_dyad,
             sparsity_factors, n_zero, zero_bottom, zero_top, window, ledger,
@@ -446,6 +453,20 @@ def _advance_flux(flux, noise_region, kernel, flux_noise_scale,
 
 # The default advance, kept so a driver can tell whether the seam was rebound.
 _advance_flux_default = _advance_flux
+_turbulon_envelope_default = _turbulon_envelope
+
+
+def synth_kernel_device(k, dx, dy, dz, support_factor, shape):
+    """The turbulon kernel as a device tensor through the module's current
+    ``_turbulon_envelope``: the GPU synthesis, or a replacement with the
+    reference's signature (steam/simulate.py:2503) returning a host array."""
+    synth = globals()['_turbulon_envelope']
+    if synth is _turbulon_envelope_default:
+        return synth(k, dx, dy, dz, support_factor=support_factor, shape=shape,
+                     as_tensor=True)
+    out, _ = to_device(np.ascontiguousarray(synth(k, dx, dy, dz, support_factor, shape),
+                                            dtype=np.float32))
+    return out
 
 
 def _advance_flux_composed(ft, noise_region, kernel, flux_noise_scale,
@@ -559,9 +580,8 @@ def simulate_flux_only(grids, sparsity_factors=(1, 1, 1), seed=None,
     n_zero = int(round(2 * s_z * min_distance_to_ground))
     sf = SUPPORT_FACTOR if support_factor is None else support_factor
     if kernel is None:
-        kt = _turbulon_envelope(1, 1 / (2 * s_x), 1 / (2 * s_y), 1 / (2 * s_z),
-                                support_factor=sf, shape=turbulon_shape,
-                                as_tensor=True)
+        kt = synth_kernel_device(1, 1 / (2 * s_x), 1 / (2 * s_y), 1 / (2 * s_z),
+                                 sf, turbulon_shape)
     else:
         kt, _ = to_device(kernel)
     flux = None

@@ -29,38 +29,56 @@ from .utils import regrid
 
 
 def tile_bounds(n, n_tiles):
-    """Split n cells into n_tiles as evenly as possible, low tiles taking the
-    remainder (steam/streaming.py:249-256)."""
-    if n_tiles < 1 or n_tiles > n:
+    """Contiguous runs covering an axis of ``n`` cells with ``n_tiles`` tiles.
+
+    Cut ``i`` sits at floor(i * n / n_tiles), so neighbouring runs differ by at
+    most one cell and the longer runs fall where the floor steps by one more
+    (the cuts of steam/streaming.py:249-256).  More tiles than cells, or none,
+    is a ValueError."""
+    n = int(n)
+    n_tiles = int(n_tiles)
+    if not 1 <= n_tiles <= n:
         raise ValueError(f"cannot split {n} cells into {n_tiles} tiles")
-    edges = [(index * n) // n_tiles for index in range(n_tiles + 1)]
-    return [(edges[i], edges[i + 1]) for i in range(n_tiles)]
+    cuts = (np.arange(n_tiles + 1, dtype=np.int64) * n) // n_tiles
+    return [(int(a), int(b)) for a, b in zip(cuts[:-1], cuts[1:])]
 
 
 def tile_windows(nx, ny, tiles_x, tiles_y):
-    """Every tile's (x_lo, x_hi, y_lo, y_hi) in world raster order, x-major
-    (steam/streaming.py:259-269); the class sums are accumulated in it."""
-    return [(x_lo, x_hi, y_lo, y_hi)
-            for x_lo, x_hi in tile_bounds(nx, tiles_x)
-            for y_lo, y_hi in tile_bounds(ny, tiles_y)]
+    """(x_lo, x_hi, y_lo, y_hi) of every tile, ordered with x outer and y
+    inner.  The class's float64 sums are accumulated tile by tile in this
+    order, so two runs with the same tiling agree to the bit
+    (steam/streaming.py:259-269 fixes the same order)."""
+    runs_y = tile_bounds(ny, tiles_y)
+    order = []
+    for xr in tile_bounds(nx, tiles_x):
+        order.extend(xr + yr for yr in runs_y)
+    return order
 
 
 def axis_regrid_plan(n_in, n_out, lo, hi):
-    """Source slab and crop reproducing target cells [lo, hi) of an
-    align_corners=False upsample (steam/streaming.py:132-167): returns
-    (src_lo, src_hi, crop_lo, out_len) with one extra block of
-    n_in/gcd source cells on each side."""
+    """Which source cells a tiled regrid reads for target cells [lo, hi).
+
+    With align_corners=False the source coordinate of target cell j is
+    (j + 1/2) n_in / n_out - 1/2.  Writing n_in / n_out = p / r in lowest
+    terms, every r target cells start on a source cell boundary, p source
+    cells later.  So the read is a whole number of such r-cell groups
+    covering [lo, hi), widened by one group on each side for the half-cell
+    overhang of the first and last target cell, and the regridded slab is
+    cropped back to [lo, hi).  Returns (src_lo, src_hi, crop_lo, out_len):
+    source cells [src_lo, src_hi) (taken modulo the extent on a periodic
+    axis), out_len target cells from them, of which [crop_lo, crop_lo + hi
+    - lo) are the window (the plan of steam/streaming.py:132-167).  A coarser
+    target raises NotImplementedError (the cascade only refines)."""
     if n_out < n_in:
         raise NotImplementedError(
-            f"tiled regrid downsamples ({n_in} -> {n_out}); the cascade only "
-            f"ever goes finer")
-    common = gcd(n_in, n_out)
-    b = n_in // common
-    a = n_out // common
-    block_lo = lo // a
-    block_hi = -(-hi // a)
-    return (block_lo * b - b, block_hi * b + b,
-            a + (lo - block_lo * a), (block_hi - block_lo) * a + 2 * a)
+            f"the tiled regrid downsamples ({n_in} -> {n_out}) here; cascade "
+            f"classes only get finer")
+    g = gcd(n_in, n_out)
+    p, r = n_in // g, n_out // g
+    first = lo // r                # first r-cell group touching the window
+    last = (hi + r - 1) // r       # one past the last group
+    return ((first - 1) * p, (last + 1) * p, lo - (first - 1) * r,
+            (last - first + 2) * r)
 
 
 def regrid_window(source, target_shape, window, scale=1.0):

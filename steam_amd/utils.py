@@ -82,12 +82,16 @@ class ConvOperator:
             self._workspaces[key] = ws
         return ws
 
-    def spectrum(self, kernel_t, key=None):
+    def spectrum(self, kernel_t, key=None, owner=None):
         """Kernel spectrum (cached under ``key`` when given).  An entry keyed
-        by a device tensor's address holds that tensor, so no other kernel
-        can be allocated at the address while the entry lives."""
+        by a device tensor's address holds that tensor (``owner``, the object
+        the key was taken from; ``kernel_t`` when None), so no other kernel
+        can be allocated at the address while the entry lives.  Entries are
+        evicted least recently used first."""
         if key is not None and key in self._spectra:
-            return self._spectra[key][0]
+            entry = self._spectra.pop(key)
+            self._spectra[key] = entry
+            return entry[0]
         lib = _lib.load()
         if tuple(kernel_t.shape) != self.kernel_shape:
             raise ValueError(f"kernel shape {tuple(kernel_t.shape)} does not "
@@ -101,7 +105,8 @@ class ConvOperator:
         if key is not None:
             if len(self._spectra) > 8:
                 self._spectra.pop(next(iter(self._spectra)))
-            self._spectra[key] = (spec, kernel_t if key[0] == 't' else None)
+            held = kernel_t if owner is None else owner
+            self._spectra[key] = (spec, held if key[0] == 't' else None)
         return spec
 
     def execute(self, field_t, out_t, spec):
@@ -118,15 +123,18 @@ _operators = {}
 
 
 def conv_operator(field_shape, kernel_shape, n_fft=0, precision='plain'):
-    """Cached ConvOperator for a shape pair."""
+    """Cached ConvOperator for a shape pair (least recently used evicted
+    first).  A caller that reads an operator's workspace later (a deferred
+    ledger) keeps the operator object itself rather than looking it up
+    again."""
     key = (tuple(int(n) for n in field_shape), tuple(int(n) for n in kernel_shape),
            int(n_fft), precision)
-    op = _operators.get(key)
+    op = _operators.pop(key, None)
     if op is None:
         op = ConvOperator(field_shape, kernel_shape, n_fft, precision)
         if len(_operators) > 64:
             _operators.pop(next(iter(_operators)))
-        _operators[key] = op
+    _operators[key] = op
     return op
 
 
@@ -181,7 +189,7 @@ def convolve_fft_xy_oa_z(field, kernel, device='cuda', precision='plain'):
     ft, was_np = to_device(field)
     kt, _ = to_device(kernel)
     op = conv_operator(ft.shape, kt.shape, precision=precision)
-    spec = op.spectrum(kt, key=kernel_key(kernel))
+    spec = op.spectrum(kt, key=kernel_key(kernel), owner=kernel)
     try:
         out = torch.empty_like(ft)
     except torch.OutOfMemoryError as e:

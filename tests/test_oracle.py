@@ -189,6 +189,58 @@ def test_conv_fft_matches_brute_force(fs, ks):
                                rtol=1e-10, atol=1e-10)
 
 
+@pytest.mark.parametrize("fs,ks", [
+    ((8, 8, 8), (3, 3, 3)),      # one overlap-add block
+    ((10, 12, 41), (5, 3, 7)),   # odd level count, three z blocks (n_fft 16)
+    ((4, 5, 37), (9, 7, 3)),     # kernel wider than the field: folded
+    ((6, 7, 70), (5, 5, 17)),    # n_fft 64, two blocks, ragged last block
+])
+def test_conv_oa_torch_f32_matches_brute_force(fs, ks):
+    """The float32 overlap-add operator timed as the CPU baseline is the same
+    operator as the float64 brute force, to float32 rounding.  A dropped
+    block, a wrong trim offset or a missing fold moves it by O(1)."""
+    rng = np.random.default_rng(62)
+    f = rng.standard_normal(fs).astype(np.float32)
+    k = rng.standard_normal(ks).astype(np.float32)
+    got = M.conv_oa_torch_f32(f, k)
+    ref = M.conv_direct(f, k)
+    assert got.dtype == np.float32 and got.shape == f.shape
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 2e-6
+    np.testing.assert_allclose(got, ref, rtol=0, atol=2e-5 * np.abs(ref).max())
+
+
+def test_conv_oa_torch_f32_block_seams():
+    """A field of one nonzero level at each block boundary: the overlap-add
+    tails must land on the next block's levels, not be dropped or doubled."""
+    kz = 9
+    n_fft = 1 << max(4, (2 * kz - 1).bit_length())
+    L = n_fft - kz + 1
+    f = np.zeros((4, 4, 3 * L + 2), np.float32)
+    for z in (L - 1, L, 2 * L - 1, 2 * L):
+        f[1, 2, z] = 1.0
+    k = np.random.default_rng(3).standard_normal((3, 3, kz)).astype(np.float32)
+    np.testing.assert_allclose(M.conv_oa_torch_f32(f, k), M.conv_direct(f, k),
+                               rtol=0, atol=1e-5)
+
+
+@pytest.mark.parametrize("alpha", [1.5, 1.8])
+def test_keyed_levy_is_cms_of_keyed_uniforms(alpha):
+    """The fused C keyed Levy draw equals the two-step form: the keyed
+    uniforms of the same box and key (pinned by the Random123 answers and the
+    lattice tests above), then the CMS transform (float32 bit for bit;
+    float64 to its last bit too, both legs being one C expression order)."""
+    key = (0x9E3779B9, 0x7F4A7C15, 0x00C0FFEE)
+    ix, iy, iz = np.arange(-3, 9), np.arange(5, 12), np.arange(0, 11)
+    u0, u1 = M.keyed_uniforms(ix, iy, iz, key)
+    f32 = N.keyed_levy(ix, iy, iz, key, alpha, dtype=np.float32)
+    f64 = N.keyed_levy(ix, iy, iz, key, alpha, dtype=np.float64)
+    assert np.array_equal(f32.ravel(), N.levy_f32(alpha, u0.ravel(), u1.ravel()))
+    assert np.array_equal(f64.ravel(), N.levy_f64(alpha, u0.ravel(), u1.ravel()))
+    # a sub-box is the restriction of the box (counter-based: no stream order)
+    sub = N.keyed_levy(ix[2:7], iy[1:4], iz[3:9], key, alpha, dtype=np.float32)
+    assert np.array_equal(sub, f32[2:7, 1:4, 3:9])
+
+
 def test_conv_identity_and_shift():
     rng = np.random.default_rng(30)
     f = rng.standard_normal((10, 10, 8))
