@@ -148,6 +148,12 @@ int steam_conv_plan_destroy(steam_conv_plan* plan);
 int steam_conv_nfft_candidates(int64_t nz, int64_t kz, int64_t* out,
                                int32_t max_out, int32_t* count);
 
+/* The half lengths n_fft / 2 that have a compiled register z plan (a
+ * single-block transform whose lines are held in registers); writes up to
+ * max_out values in increasing order.  No counterpart in the reference,
+ * which transforms with torch.fft (steam/utils.py:372-417). */
+int steam_conv_zplans(int32_t* out, int32_t max_out, int32_t* count);
+
 typedef struct steam_conv_info {
   int64_t n_fft;           /* z transform length */
   int64_t n_blocks;        /* overlap-add blocks along z */
@@ -267,6 +273,16 @@ int steam_batch_destroy(steam_batch* batch);
  * for the previous advance's descriptor upload to finish). */
 int steam_batch_set_noise(steam_batch* batch, int32_t i,
                           const steam_noise_params* noise);
+
+/* Mark, on stream, the point after a replay of a captured
+ * steam_batch_advance.  A replayed graph copies the pinned descriptors when
+ * the replay runs, not when it was captured, so the next
+ * steam_batch_set_noise must not rewrite them before that copy: it waits for
+ * the last marked point.  Call it after every replay (eager advances mark
+ * themselves).  No counterpart in the reference, which has no batched or
+ * captured advance (it advances one instance per call,
+ * steam/simulate.py:2066-2228). */
+int steam_batch_mark_replay(steam_batch* batch, steam_stream_t stream);
 
 /* Kernel launches per advance (descriptor upload included) and the bytes of
  * descriptors uploaded per advance. */

@@ -98,6 +98,7 @@ _SIGNATURES = [
     ("steam_conv_nfft_candidates", ctypes.c_int,
      [_I64, _I64, ctypes.POINTER(_I64), _I32, ctypes.POINTER(_I32)]),
     ("steam_conv_plan_info", ctypes.c_int, [_P, ctypes.POINTER(ConvInfo)]),
+    ("steam_conv_zplans", ctypes.c_int, [ctypes.POINTER(_I32), _I32, ctypes.POINTER(_I32)]),
     ("steam_conv_set_kernel", ctypes.c_int, [_P, _P, _P, _P, _P]),
     ("steam_conv_execute", ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     ("steam_flux_noise", ctypes.c_int, [_P, ctypes.POINTER(NoiseParams), _P, _P]),
@@ -113,6 +114,7 @@ _SIGNATURES = [
     ("steam_batch_set_noise", ctypes.c_int, [_P, _I32, ctypes.POINTER(NoiseParams)]),
     ("steam_batch_info", ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),
     ("steam_batch_advance", ctypes.c_int, [_P, _P]),
+    ("steam_batch_mark_replay", ctypes.c_int, [_P, _P]),
     ("steam_scalar_pattern", ctypes.c_int,
      [_P, _P, _P, _P, _P, ctypes.POINTER(ZDiff), _I64, _I64, _I64, _F64, _F64,
       _F64, _F64, _P, _P]),

@@ -189,7 +189,9 @@ class BatchedAdvance:
                    "steam_batch_advance")
 
     def capture(self):
-        """A torch CUDAGraph (hipGraph) holding one advance."""
+        """One advance captured into a hipGraph (torch CUDAGraph), returned as
+        a ``CapturedAdvance`` whose ``replay`` marks each replay for
+        ``set_regions``."""
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
@@ -199,7 +201,7 @@ class BatchedAdvance:
         torch.cuda.synchronize()
         with torch.cuda.graph(g):
             self.advance()
-        return g
+        return CapturedAdvance(self, g)
 
     def ledgers(self):
         """Each instance's ledger of the last advance (synchronises)."""
@@ -213,6 +215,24 @@ class BatchedAdvance:
                        "steam_flux_ledger_read")
             out.append(FluxAdvanceLedger.from_native(L))
         return out
+
+
+class CapturedAdvance:
+    """A captured batched advance.  A replay's descriptor upload reads the
+    batch's pinned staging copy when the replay runs, so each ``replay``
+    marks its end on the stream, and the batch's next ``set_regions`` waits
+    for that mark before it rewrites the copy (no caller synchronisation
+    needed between a replay and new keys)."""
+
+    def __init__(self, batch, graph):
+        self.batch = batch
+        self.graph = graph
+
+    def replay(self):
+        self.graph.replay()
+        _lib.check(_lib.load().steam_batch_mark_replay(self.batch.handle,
+                                                       _lib.stream_handle()),
+                   "steam_batch_mark_replay")
 
 
 def draw_batch_config(n=48, seed=48):

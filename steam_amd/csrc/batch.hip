@@ -339,6 +339,15 @@ extern "C" int steam_batch_set_noise(steam_batch* b, int i,
   return STEAM_OK;
 }
 
+extern "C" int steam_batch_mark_replay(steam_batch* b, steam_stream_t stream) {
+  if (!b) return set_error(STEAM_EINVAL, "steam_batch_mark_replay: NULL batch");
+  // a replay's captured upload reads the staging copy when the replay runs:
+  // the next steam_batch_set_noise waits for this point of the stream
+  STEAM_HIP(hipEventRecord(b->uploaded, (hipStream_t)stream));
+  b->pending_upload = true;
+  return STEAM_OK;
+}
+
 extern "C" int steam_batch_info(const steam_batch* b, int* launches,
                                 int64_t* descriptor_bytes) {
   if (!b) return set_error(STEAM_EINVAL, "steam_batch_info: NULL batch");

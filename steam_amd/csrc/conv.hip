@@ -165,6 +165,20 @@ extern "C" int steam_conv_nfft_candidates(int64_t nz, int64_t kz, int64_t* out,
   return STEAM_OK;
 }
 
+extern "C" int steam_conv_zplans(int32_t* out, int32_t max_out, int32_t* count) {
+  if (!count || (max_out > 0 && !out))
+    return set_error(STEAM_EINVAL, "steam_conv_zplans: bad arguments");
+  int n = 0;
+  for (int n2 = 2; n2 <= FFT_LDSE; n2 += 2) {
+    int lines = 0;
+    if (!zplan_lines(n2, lines)) continue;
+    if (n < max_out) out[n] = n2;
+    ++n;
+  }
+  *count = std::min(n, std::max(0, (int)max_out));
+  return STEAM_OK;
+}
+
 extern "C" int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz,
                                       int64_t kx, int64_t ky, int64_t kz,
                                       int64_t n_fft, int precision,

@@ -100,6 +100,37 @@ def test_graph_replays_match_eager_and_follow_new_keys():
             np.testing.assert_array_equal(a.flux.cpu().numpy(), b.flux.cpu().numpy())
 
 
+def test_replay_then_new_keys_without_synchronising():
+    """set_regions right after a replay, with no synchronisation in between:
+    each replay must still run with the keys set before it (the staging
+    copy is not rewritten until the replay's upload has read it)."""
+    from steam_amd import simulate as S
+    from steam_amd.batch import BatchedAdvance
+    insts = _instances(seed=4)
+    twins = _instances(seed=4)
+    g_batch = BatchedAdvance(insts, autotune=False)
+    e_batch = BatchedAdvance(twins, autotune=False)
+    graph = g_batch.capture()
+    e_batch.advance()
+    torch.cuda.synchronize()
+
+    def regions(rep):
+        return [S.NoiseRegion(tuple(np.uint64(k) for k in
+                                    M.class_key(np.random.SeedSequence(11 * rep + i))),
+                              (0, 0, 0), tuple(inst.flux.shape))
+                for i, inst in enumerate(insts)]
+    for rep in range(4):
+        g_batch.set_regions(regions(rep))
+        graph.replay()
+    for rep in range(4):
+        e_batch.set_regions(regions(rep))
+        e_batch.advance()
+        torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    for a, b in zip(insts, twins):
+        np.testing.assert_array_equal(a.flux.cpu().numpy(), b.flux.cpu().numpy())
+
+
 def test_deferred_batch_leaves_rescale_pending():
     from steam_amd.batch import BatchedAdvance
     insts = _instances(seed=3, defer=True)

@@ -244,6 +244,27 @@ def test_spectrum_cache_never_serves_a_freed_kernels_spectrum():
         del k
 
 
+def test_spectrum_cache_with_converted_device_kernels():
+    """float64 and non-contiguous device kernels are converted before their
+    spectrum is taken; the cache entry must hold the caller's tensor (the
+    one its key names), so a fresh kernel allocated where a freed one lived
+    never gets the stale spectrum.  Each CONVOLVE equals an uncached one."""
+    from steam_amd.utils import ConvOperator, convolve_fft_xy_oa_z
+    rng = np.random.default_rng(9)
+    shape, kshape = (24, 20, 16), (5, 5, 5)
+    field = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)).cuda()
+    fresh = ConvOperator(shape, kshape)
+    for i in range(6):
+        k64 = torch.from_numpy(rng.standard_normal(kshape)).cuda()
+        k = k64 if i % 2 == 0 else k64.permute(2, 1, 0).contiguous().permute(2, 1, 0)
+        assert k.dtype == torch.float64
+        out = convolve_fft_xy_oa_z(field, k, device='cuda')
+        k32 = k.to(torch.float32).contiguous()
+        ref = fresh.execute(field, torch.empty_like(field), fresh.spectrum(k32))
+        np.testing.assert_array_equal(out.cpu().numpy(), ref.cpu().numpy())
+        del k, k64, k32
+
+
 @pytest.mark.parametrize("fs,ks,n_fft", [
     ((32, 24, 128), (13, 13, 13), 144), ((24, 32, 131), (9, 9, 13), 144),
     ((20, 12, 100), (7, 7, 9), 144), ((16, 16, 256), (13, 13, 13), 288),
