@@ -231,8 +231,10 @@ extern "C" int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz,
   p->gk.nblocks = 1;
   p->gk.L = g.nfft;
   {
-    size_t a, b, c, d2;
-    set_tiles(p->gk, 1, 0, a, b, c, d2);
+    // its own tile heights: one accumulated level and no compensation let
+    // the build take taller z tiles than the field's, with more LDS
+    size_t unused;
+    set_tiles(p->gk, 1, 0, p->lds_zk, unused, p->lds_yk, p->lds_xk);
   }
 
   // twiddles: [x: nx][y: ny][z: n2][r2c: n2+1]
@@ -352,11 +354,11 @@ extern "C" int steam_conv_set_kernel(const steam_conv_plan* p,
   const ConvGeom& g = d.g;
   const int rsz = radix_set(g.az);
   STEAM_RS_DISPATCH(rsz, hipLaunchKernelGGL((zfwd_kernel<SingleSrc, MODE_KERNEL, RS_>),
-                                            dim3(g.wg_z), dim3(Z_NTH), p->lds_z, s, src));
+                                            dim3(g.wg_z), dim3(Z_NTH), p->lds_zk, s, src));
   STEAM_LAUNCH_CHECK("zfwd_kernel(kernel)");
-  launch_ypass(g, p->lds_y, s, src, 0);
+  launch_ypass(g, p->lds_yk, s, src, 0);
   STEAM_LAUNCH_CHECK("ypass_kernel(kernel)");
-  launch_xpass<SingleSrc, 1>(g, p->lds_x, s, src);
+  launch_xpass<SingleSrc, 1>(g, p->lds_xk, s, src);
   STEAM_LAUNCH_CHECK("xpass_kernel(kernel)");
   if (p->ktile) {
     const int64_t ne = (int64_t)g.nzc * p->nx * p->ny;

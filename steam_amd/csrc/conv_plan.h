@@ -15,6 +15,7 @@ struct steam_conv_plan {
   int64_t kstd_off;  // spectrum build area in [nzc][nx][ny] order
   int wg_noise;
   size_t lds_z, lds_zinv, lds_y, lds_x;
+  size_t lds_zk, lds_yk, lds_xk;  // the kernel-spectrum build's (geometry gk)
   int wg_post;
 };
 
