@@ -122,6 +122,20 @@ int steam_turbulon_envelope(double k, double dx, double dy, double dz,
                             double support_factor, int shape_id,
                             float* out_dev, steam_stream_t stream);
 
+/* steam_turbulon_envelope without allocating: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels and grid, so the same taps bit for bit; the
+ * float64 partial sums of the correction (at most 4096 workgroups) and its
+ * ratio live in one library-owned device array of 65,792 bytes per GPU,
+ * held until the process ends (the grid never needs more; a larger need
+ * would be refused with STEAM_EINVAL).  Asynchronous: it does not
+ * synchronise.  Calls on one stream run in order; calls that overlap on one
+ * GPU (other streams or host threads) race on the shared array. */
+int steam_turbulon_envelope_noalloc(double k, double dx, double dy, double dz,
+                                    double support_factor, int shape_id,
+                                    float* out_dev, steam_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * (c) FFT overlap-add convolution: periodic x/y, zero-padded z, centred
  *     'same' output (steam/utils.py:222-242, 293-427).

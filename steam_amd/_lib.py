@@ -91,6 +91,8 @@ _SIGNATURES = [
      [_F64, _F64, _F64, _F64, _F64, ctypes.POINTER(_I64)]),
     ("steam_turbulon_envelope", ctypes.c_int,
      [_F64, _F64, _F64, _F64, _F64, ctypes.c_int, _P, _P]),
+    ("steam_turbulon_envelope_noalloc", ctypes.c_int,
+     [_F64, _F64, _F64, _F64, _F64, ctypes.c_int, _P, _P]),
     ("steam_conv_plan_create", ctypes.c_int,
      [_I64, _I64, _I64, _I64, _I64, _I64, _I64, ctypes.c_int,
       ctypes.POINTER(_P)]),

@@ -4,6 +4,9 @@
 // fixed-order reduction.
 #include <math.h>
 
+#include <mutex>
+#include <vector>
+
 #include "common.h"
 
 namespace steam {
@@ -102,6 +105,60 @@ __global__ __launch_bounds__(256) void env_correct_kernel(EnvParams p, int64_t n
   }
 }
 
+// Workgroups of the tap and correction passes (each takes a grid stride
+// over the taps); their float64 partial sums are the scratch a call needs.
+constexpr int kEnvMaxWgs = 4096;
+constexpr size_t kEnvScratchBytes = (size_t)kEnvMaxWgs * 2 * sizeof(double) + 256;
+
+// Scratch of steam_turbulon_envelope_noalloc: the partial sums of up to
+// kEnvMaxWgs workgroups, then the float32 correction ratio.  One copy per
+// GPU, held until the process ends.  Host code never uses this name as an
+// address: it reads each device's address once (envelope_scratch).
+static __device__ __align__(256) unsigned char scratch[kEnvScratchBytes];
+
+// Device address of scratch on the current device, looked up on the first
+// call on that device and kept.
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;
+
+static int envelope_scratch(void** out) {
+  int dev = 0;
+  STEAM_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return check_hip(e, "hipGetSymbolAddress(envelope scratch)");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return STEAM_OK;
+}
+
+// Parameters and tap count of a kernel, with the argument checks of the
+// public entries.
+static int envelope_setup(double k, double dx, double dy, double dz, double sf,
+                          int shape_id, const float* out, EnvParams& p, int64_t& n);
+
+// The three passes: taps and their two sums per workgroup into part, the
+// fixed-order reduction into the ratio (after the 2 * nblk partials), the
+// correction.  part holds env_scratch_bytes(nblk).
+static size_t env_scratch_bytes(int nblk) { return (size_t)nblk * 2 * sizeof(double) + 256; }
+
+static hipError_t envelope_passes(const EnvParams& p, int64_t n, int nblk, float* out,
+                                  double* part, hipStream_t s) {
+  float* ratio = (float*)(part + 2 * nblk);
+  hipLaunchKernelGGL(env_taps_kernel, dim3(nblk), dim3(256), 0, s, p, n, out, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(env_reduce_kernel, dim3(1), dim3(256), 0, s, part, nblk, ratio);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(env_correct_kernel, dim3(nblk), dim3(256), 0, s, p, n, ratio, out);
+  return hipGetLastError();
+}
+
 }  // namespace steam
 
 using namespace steam;
@@ -117,9 +174,8 @@ extern "C" int steam_envelope_halfwidths(double k, double dx, double dy,
   return STEAM_OK;
 }
 
-extern "C" int steam_turbulon_envelope(double k, double dx, double dy, double dz,
-                                       double sf, int shape_id, float* out,
-                                       steam_stream_t stream) {
+int steam::envelope_setup(double k, double dx, double dy, double dz, double sf,
+                          int shape_id, const float* out, EnvParams& p, int64_t& n) {
   if (shape_id != STEAM_SHAPE_MEXICAN_HAT && shape_id != STEAM_SHAPE_MORLET_OMEGA0_6)
     return set_error(STEAM_EINVAL, "envelope: unknown shape");
   int64_t h[3];
@@ -128,7 +184,6 @@ extern "C" int steam_turbulon_envelope(double k, double dx, double dy, double dz
   if (!out) return set_error(STEAM_EINVAL, "envelope: NULL output");
   if (h[0] > 4096 || h[1] > 4096 || h[2] > 4096)
     return set_error(STEAM_EINVAL, "envelope: support too large");
-  EnvParams p;
   p.hx = (int)h[0];
   p.hy = (int)h[1];
   p.hz = (int)h[2];
@@ -140,25 +195,47 @@ extern "C" int steam_turbulon_envelope(double k, double dx, double dy, double dz
   p.g2 = (float)(2.0 * sig * sig);
   p.omega = (float)(6.0 / k);
   p.shape = shape_id;
-  const int64_t n = (2 * h[0] + 1) * (2 * h[1] + 1) * (2 * h[2] + 1);
-  int nblk = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  n = (2 * h[0] + 1) * (2 * h[1] + 1) * (2 * h[2] + 1);
+  return STEAM_OK;
+}
+
+extern "C" int steam_turbulon_envelope(double k, double dx, double dy, double dz,
+                                       double sf, int shape_id, float* out,
+                                       steam_stream_t stream) {
+  EnvParams p;
+  int64_t n = 0;
+  int rc = envelope_setup(k, dx, dy, dz, sf, shape_id, out, p, n);
+  if (rc) return rc;
+  const int nblk = (int)std::min<int64_t>((n + 255) / 256, kEnvMaxWgs);
   hipStream_t s = (hipStream_t)stream;
   double* part = nullptr;
-  STEAM_HIP(hipMallocAsync((void**)&part, (size_t)nblk * 2 * sizeof(double) + 256, s));
-  float* ratio = (float*)(part + 2 * nblk);
-  hipLaunchKernelGGL(env_taps_kernel, dim3(nblk), dim3(256), 0, s, p, n, out, part);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(env_reduce_kernel, dim3(1), dim3(256), 0, s, part, nblk, ratio);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(env_correct_kernel, dim3(nblk), dim3(256), 0, s, p, n, ratio, out);
-    e = hipGetLastError();
-  }
+  STEAM_HIP(hipMallocAsync((void**)&part, env_scratch_bytes(nblk), s));
+  hipError_t e = envelope_passes(p, n, nblk, out, part, s);
   hipError_t ef = hipFreeAsync(part, s);
   if (e != hipSuccess) return check_hip(e, "envelope kernels");
   if (ef != hipSuccess) return check_hip(ef, "hipFreeAsync(envelope)");
   STEAM_HIP(hipStreamSynchronize(s));
+  return STEAM_OK;
+}
+
+extern "C" int steam_turbulon_envelope_noalloc(double k, double dx, double dy, double dz,
+                                               double sf, int shape_id, float* out,
+                                               steam_stream_t stream) {
+  EnvParams p;
+  int64_t n = 0;
+  int rc = envelope_setup(k, dx, dy, dz, sf, shape_id, out, p, n);
+  if (rc) return rc;
+  // the same grid as steam_turbulon_envelope, so the same summation order
+  const int nblk = (int)std::min<int64_t>((n + 255) / 256, kEnvMaxWgs);
+  if (env_scratch_bytes(nblk) > kEnvScratchBytes)
+    return set_error(STEAM_EINVAL, "envelope_noalloc: the kernel needs more scratch than "
+                                   "the library's static array holds");
+  void* ptr = nullptr;
+  rc = envelope_scratch(&ptr);
+  if (rc) return rc;
+  // every partial slot and the ratio are written before they are read, in
+  // stream order: nothing in the scratch needs clearing between calls
+  hipError_t e = envelope_passes(p, n, nblk, out, (double*)ptr, (hipStream_t)stream);
+  if (e != hipSuccess) return check_hip(e, "envelope kernels (noalloc)");
   return STEAM_OK;
 }
