@@ -77,6 +77,11 @@ def parse():
     ap.add_argument("--refine-form", action="store_true",
                     help="only run the refine form: a nest of the 2048x128x115 root and a "
                          "nest of that nest; prints one JSON line")
+    ap.add_argument("--pipelined-only", action="store_true",
+                    help="only run the reported (pipelined) timed region of the class "
+                         "advance, for profiling it alone; prints one JSON line")
+    ap.add_argument("--noise-ahead", type=int, default=2,
+                    help="classes whose noise the pipelined form generates ahead")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / "
@@ -130,20 +135,30 @@ def measure_copy_gbs(nbytes=1 << 30, reps=5):
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
-def family_bytes(info, shape, kernel_elems_planes):
+def family_bytes(info, shape, kernel_bytes):
     """Algorithmic HBM bytes per launch of each kernel family (fp32 state,
-    complex64 spectra): the bytes the pass must move at minimum."""
+    complex64 spectra): the bytes the pass must move at minimum.
+    ``kernel_bytes``: the kernel spectrum the x pass reads (complex64 per
+    bin, or float32 per bin x <= nx/2 for a mirror-symmetric kernel)."""
     nx, ny, nz = shape
     N = nx * ny * nz
     E = info.spectral_elems
-    K = kernel_elems_planes
     return {
         "noise": 4 * N,                 # write the noise field
         "zfwd": 8 * N + 8 * E,          # read flux + noise, write planes
         "ypass": 16 * E,                # read + write planes
-        "xpass": 16 * E + 8 * K,        # read + write planes, read kernel spectrum
+        "xpass": 16 * E + kernel_bytes,  # read + write planes, read kernel spectrum
         "zinv": 8 * E + 8 * N,          # read planes, read + write flux
     }
+
+
+def kernel_spectrum_bytes(op, spec, shape):
+    """Bytes of kernel spectrum one x pass reads, from the spectrum's format."""
+    nx, ny, _ = shape
+    nzc = op.n_fft // 2 + 1
+    if op.spectrum_format(spec) == 'real_even':
+        return 4 * nzc * (nx // 2 + 1) * ny
+    return 8 * nzc * nx * ny
 
 
 def cpu_baseline(sample, c, alpha, kernel_np, seed_key, repeats=3):
@@ -746,7 +761,7 @@ def main():
     lib.steam_conv_plan_info(op.plan, ctypes.byref(info))
 
     total = args.warmup + args.steps
-    seeds = np.random.SeedSequence(0).spawn(total + 1)
+    seeds = np.random.SeedSequence(0).spawn(total + args.steps + args.noise_ahead + 4)
     regions = [S.NoiseRegion(S.class_key(sq), (0, 0, 0), shape) for sq in seeds]
 
     # serial form: each advance generates its own noise
@@ -767,46 +782,60 @@ def main():
         return fam
 
     # serial timed region: per-kernel HIP-event durations without overlap
-    for i in range(args.warmup):
-        serial_step(i)
-    torch.cuda.synchronize()
-    _lib.check(lib.steam_profile_enable(1))
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        serial_step(args.warmup + i)
-    torch.cuda.synchronize()
-    serial_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    fam = read_profile()
-    _lib.check(lib.steam_profile_enable(0))
+    fam = {}
+    serial_ms = None
+    if not args.pipelined_only:
+        for i in range(args.warmup):
+            serial_step(i)
+        torch.cuda.synchronize()
+        _lib.check(lib.steam_profile_enable(1))
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            serial_step(args.warmup + i)
+        torch.cuda.synchronize()
+        serial_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        fam = read_profile()
+        _lib.check(lib.steam_profile_enable(0))
 
-    # pipelined form: class i+1's noise (flux-independent, VALU-bound) is
-    # generated on a side stream into the other of two buffers while class i
-    # runs its memory-bound FFT passes on the main stream
+    # pipelined form: the noise fields of the next classes (flux-independent,
+    # VALU-bound) are generated on a side stream, ``ahead`` classes in
+    # advance into a ring of ahead + 1 buffers, while class i runs its
+    # memory-bound FFT passes on the main stream; the side stream only waits
+    # for the advance that last read the buffer it refills
+    ahead = max(1, args.noise_ahead)
+    nbuf = ahead + 1
     main = torch.cuda.current_stream()
     side = torch.cuda.Stream()
-    bufs = [torch.empty(shape, dtype=torch.float32, device='cuda') for _ in range(2)]
-    ev_noise = [torch.cuda.Event() for _ in range(2)]
-    ev_adv = [torch.cuda.Event() for _ in range(2)]
+    bufs = [torch.empty(shape, dtype=torch.float32, device='cuda') for _ in range(nbuf)]
+    ev_noise = [torch.cuda.Event() for _ in range(nbuf)]
+    ev_adv = [torch.cuda.Event() for _ in range(nbuf)]
+    queued = [-1]
 
     def gen(i):
-        b = i % 2
-        if i >= 2:
+        b = i % nbuf
+        if i >= nbuf:
             side.wait_event(ev_adv[b])
         S.generate_noise_device(op, regions[i], c, 1, (1, 1, 1), bufs[b],
                                 alpha=alpha, stream=side)
         ev_noise[b].record(side)
+        queued[0] = i
 
     def piped_step(i):
-        b = i % 2
-        gen(i + 1)
+        b = i % nbuf
+        while queued[0] < i + ahead:
+            gen(queued[0] + 1)
         main.wait_event(ev_noise[b])
         S.advance_flux_device(flux, regions[i], kern, c, 1, (1, 1, 1), alpha=alpha,
                               defer_rescale=True, sync_ledger=False, op=op,
                               spec=spec, chain_pending=True, noise_t=bufs[b])
         ev_adv[b].record(main)
 
-    # pipelined timed region (the reported value)
-    gen(0)
+    # pipelined timed region (the reported value), uninstrumented; every
+    # pipelined advance chains the pending rescale the previous one left, so
+    # the chain starts from one unchained advance when the serial region
+    # did not run
+    if args.pipelined_only:
+        serial_step(0)
     for i in range(args.warmup):
         piped_step(i)
     torch.cuda.synchronize()
@@ -817,30 +846,64 @@ def main():
     t1 = time.perf_counter()
     led, _ = S.read_device_ledger(shape, tuple(kern.shape))
     assert np.isfinite(led.realized.mean) and led.realized.mean > 0
-
     ms_step = (t1 - t0) * 1e3 / args.steps
     voxels = int(np.prod(shape))
     value = voxels / (ms_step * 1e-3)
-    kplanes = (info.n_fft // 2 + 1) * shape[0] * shape[1]
-    fbytes = family_bytes(info, shape, kplanes)
-    # dominant memory-bound pass (the noise kernel is VALU-bound and is
-    # reported beside it with its instruction ceiling)
-    dom = max((k for k in fam if k in fbytes and k != "noise"),
-              key=lambda k: fam[k]["total_ms"])
-    achieved = fbytes[dom] / (fam[dom]["avg_ms"] * 1e-3) / 1e9
+    if args.pipelined_only:
+        print(json.dumps({"form": "pipelined class advance only", "grid": list(shape),
+                          "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": ms_step, "voxels_per_s": value}))
+        return
+
+    # the same pipelined region again with every launch bracketed by HIP
+    # events on its own stream: the per-pass durations of the reported form
+    # (the events add a little host work per launch, so the value above comes
+    # from the uninstrumented run)
+    _lib.check(lib.steam_profile_enable(1))
+    t0 = time.perf_counter()
+    for i in range(total, total + args.steps):
+        piped_step(i)
+    torch.cuda.synchronize()
+    piped_instr_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    fam_piped = read_profile()
+    _lib.check(lib.steam_profile_enable(0))
+
+    kbytes = kernel_spectrum_bytes(op, spec, shape)
+    fbytes = family_bytes(info, shape, kbytes)
+
+    def per_pass(f):
+        return {k: {"avg_ms": round(f[k]["avg_ms"], 5),
+                    "ms_per_step": round(f[k]["total_ms"] / args.steps, 5),
+                    "GBs": round(fbytes[k] / (f[k]["avg_ms"] * 1e-3) / 1e9, 1),
+                    "frac": round(fbytes[k] / (f[k]["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                for k in fbytes if k in f and k != "noise"}
+    # dominant memory-bound pass of the reported form: the most device time
+    # per step on the main stream (the noise kernel runs beside the passes on
+    # its own stream and is VALU-bound)
+    dom = max((k for k in fam_piped if k in fbytes and k != "noise"),
+              key=lambda k: fam_piped[k]["total_ms"])
+    achieved = fbytes[dom] / (fam_piped[dom]["avg_ms"] * 1e-3) / 1e9
     traffic = None
     if shape == (512, 512, 256):
         traffic = read_traffic(os.path.join(ROOT, "profiles",
                                             "traffic_flux_advance_512x512x256.txt")).get(dom)
     copy_gbs = measure_copy_gbs()
+    # end to end: 16 algorithmic bytes per voxel per class (read the entering
+    # flux, write the advanced flux, the convolution's input and output)
+    e2e_gbs = 16 * voxels / (ms_step * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes_per_launch": fbytes[dom],
+                "avg_launch_ms": round(fam_piped[dom]["avg_ms"], 5),
+                "timed_in": "pipelined region (HIP events on the launch stream)",
                 "measured_copy_GBs": round(copy_gbs, 1),
                 "frac_of_copy": round(achieved / copy_gbs, 4),
-                "per_kernel_GBs": {k: round(fbytes[k] / (fam[k]["avg_ms"] * 1e-3) / 1e9, 1)
-                                   for k in fbytes if k in fam and k != "noise"}}
+                "per_pass_pipelined": per_pass(fam_piped),
+                "per_pass_serial": per_pass(fam),
+                "end_to_end": {"bytes_per_voxel": 16, "GBs": round(e2e_gbs, 1),
+                               "frac": round(e2e_gbs / HBM_PEAK_GBS, 4)},
+                "instrumented_ms_per_step": round(piped_instr_ms, 4)}
 
     plain = None
     if not args.no_plain:
@@ -892,6 +955,7 @@ def main():
         "serial_resident_voxels_per_s": voxels / (serial_ms * 1e-3),
         "serial_resident_ms_per_step": serial_ms,
         "kernels": fam,
+        "kernels_pipelined": fam_piped,
         "batched": batched,
         "served": served,
         "full_simulate": full,

@@ -239,7 +239,8 @@ int make_item(const steam_batch_item& it, PassDesc& d, ItemPlan& ip) {
   if (rc) return rc;
   d = base_desc(p, it.workspace_dev);
   d.mode = MODE_FLUX;
-  d.K = (const float2*)it.spectrum_dev;
+  d.K = (const float2*)((const char*)it.spectrum_dev + KSPEC_DATA_OFF);
+  d.kfmt = (const int*)it.spectrum_dev;
   d.ktile = p->ktile;
   d.out = it.flux_dev;
   d.sigma_in = it.sigma_in;

@@ -64,12 +64,23 @@ struct DevLedger {
 
 enum PassMode { MODE_PLAIN = 0, MODE_FLUX = 1, MODE_KERNEL = 2 };
 
+// Kernel-spectrum formats (the word at the start of a spectrum buffer).
+// COMPLEX: complex64 [kz][x][y], or in the register x pass's column tiles.
+// REAL_EVEN: the spectrum of a kernel with mirror symmetry on every axis is
+// real and even in x; floats [kz][y/T][nx/2 + 1][T] (register x pass only).
+enum KernelFormat { KFMT_COMPLEX = 0, KFMT_REAL_EVEN = 1 };
+// Layout of a spectrum buffer: the format word, the per-workgroup mismatch
+// counts of the symmetry test, then the spectrum itself.
+constexpr int KSYM_WGS = 1024;
+constexpr int64_t KSPEC_DATA_OFF = 256 + 4 * KSYM_WGS;
+
 struct PassDesc {
   ConvGeom g;
   float2* S;            // spectral workspace [nblocks][nzc][nx][ny]
   const float2* K;      // kernel spectrum [nzc][nx][ny], or with ktile > 0 in the
                         // register x pass's column tiles [nzc][ny/ktile][nx][ktile]
   int ktile;
+  const int* kfmt;      // format word of the kernel spectrum (KFMT_*)
   const float* in;      // plain input / kernel taps
   float* out;           // plain output / flux state (in place)
   int mode;

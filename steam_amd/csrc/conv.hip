@@ -3,6 +3,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -281,15 +282,21 @@ extern "C" int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz,
   // the register x pass reads the kernel spectrum in its own column tiles,
   // each workgroup's block one contiguous run; the build writes the usual
   // [nzc][nx][ny] order after them and is then permuted
+  // a spectrum buffer starts with its format word and the symmetry test's
+  // counts (KSPEC_DATA_OFF bytes), then holds the spectrum
   p->ktile = g.reg_x ? g.Tr : 0;
   if (p->ktile) {
     const int64_t nyt = (ny + p->ktile - 1) / p->ktile;
-    p->kstd_off = align256((int64_t)g.nzc * nyt * p->ktile * nx * 8);
+    p->kstd_off = KSPEC_DATA_OFF + align256((int64_t)g.nzc * nyt * p->ktile * nx * 8);
     p->spec_bytes = p->kstd_off + align256((int64_t)g.nzc * nx * ny * 8);
   } else {
-    p->kstd_off = 0;
-    p->spec_bytes = align256((int64_t)g.nzc * nx * ny * 8);
+    p->kstd_off = KSPEC_DATA_OFF;
+    p->spec_bytes = KSPEC_DATA_OFF + align256((int64_t)g.nzc * nx * ny * 8);
   }
+  // the real, x-even format needs the register x pass and a kernel that is
+  // neither folded nor even-sized on any axis (then mirror symmetry of its
+  // taps makes the spectrum, with the 'same' trim phase, real and even)
+  p->ksym_shape = p->ktile && (kx & 1) && (ky & 1) && (kz & 1) && kx <= nx && ky <= ny;
   p->wg_post = (int)((nx * ny * nz + 4 * FFT_NTH - 1) / (4 * FFT_NTH));
   *out = p;
   return STEAM_OK;
@@ -318,19 +325,70 @@ extern "C" int steam_conv_plan_info(const steam_conv_plan* p,
 namespace steam {
 
 // kernel spectrum [nzc][nx][ny] -> the register x pass's column tiles
-// [nzc][ceil(ny/T)][nx][T] (the tail of a partial last tile is never read)
+// [nzc][ceil(ny/T)][nx][T] (the tail of a partial last tile is never read),
+// or with the real, x-even format the real parts of bins x <= nx/2 as
+// floats [nzc][ceil(ny/T)][nx/2 + 1][T]
 __global__ __launch_bounds__(256) void ktile_kernel(const float2* __restrict__ src,
-                                                    float2* __restrict__ dst, int nzc, int nx,
-                                                    int ny, int T) {
+                                                    void* __restrict__ dst_v, const int* fmt,
+                                                    int nzc, int nx, int ny, int T) {
   const int64_t plane = (int64_t)nx * ny;
   const int64_t n = plane * nzc;
   const int64_t nyt = (ny + T - 1) / T;
+  const bool real = *fmt == KFMT_REAL_EVEN;
+  const int xh = nx / 2 + 1;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * 256) {
     const int64_t p = i / plane, r = i - p * plane;
     const int x = (int)(r / ny), y = (int)(r - (int64_t)x * ny);
-    dst[((p * nyt + y / T) * nx + x) * T + y % T] = src[i];
+    if (real) {
+      if (x < xh)
+        reinterpret_cast<float*>(dst_v)[((p * nyt + y / T) * xh + x) * T + y % T] = src[i].x;
+    } else {
+      reinterpret_cast<float2*>(dst_v)[((p * nyt + y / T) * nx + x) * T + y % T] = src[i];
+    }
   }
+}
+
+// mirror-symmetry test of the kernel taps (kx, ky, kz): per workgroup, the
+// number of taps that differ from their mirror image on some axis
+__global__ __launch_bounds__(256) void ksym_count_kernel(const float* __restrict__ k, int kx,
+                                                         int ky, int kz, int* counts) {
+  __shared__ int red[256];
+  const int64_t n = (int64_t)kx * ky * kz;
+  int bad = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const int c = (int)(i % kz);
+    const int64_t ab = i / kz;
+    const int b = (int)(ab % ky), a = (int)(ab / ky);
+    const float v = k[i];
+    const float mx = k[((int64_t)(kx - 1 - a) * ky + b) * kz + c];
+    const float my = k[((int64_t)a * ky + (ky - 1 - b)) * kz + c];
+    const float mz = k[((int64_t)a * ky + b) * kz + (kz - 1 - c)];
+    bad += (v != mx) || (v != my) || (v != mz);
+  }
+  red[threadIdx.x] = bad;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[blockIdx.x] = red[0];
+}
+
+// the format word: REAL_EVEN when the shape allows it and no tap failed
+__global__ __launch_bounds__(256) void ksym_format_kernel(const int* counts, int n,
+                                                          int shape_ok, int* fmt) {
+  __shared__ int red[256];
+  int bad = 0;
+  for (int i = threadIdx.x; i < n; i += 256) bad += counts[i];
+  red[threadIdx.x] = bad;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *fmt = (shape_ok && red[0] == 0) ? KFMT_REAL_EVEN : KFMT_COMPLEX;
 }
 
 }  // namespace steam
@@ -360,11 +418,25 @@ extern "C" int steam_conv_set_kernel(const steam_conv_plan* p,
   STEAM_LAUNCH_CHECK("ypass_kernel(kernel)");
   launch_xpass<SingleSrc, 1>(g, p->lds_xk, s, src);
   STEAM_LAUNCH_CHECK("xpass_kernel(kernel)");
+  // the format word, then the spectrum in that format
+  int* fmt = (int*)spectrum_dev;
+  int* counts = fmt + 64;
+  int nsym = 0;
+  if (p->ksym_shape) {
+    nsym = (int)std::min<int64_t>((p->kx * p->ky * p->kz + 255) / 256, KSYM_WGS);
+    hipLaunchKernelGGL(ksym_count_kernel, dim3(nsym), dim3(256), 0, s, kernel_dev, (int)p->kx,
+                       (int)p->ky, (int)p->kz, counts);
+    STEAM_LAUNCH_CHECK("ksym_count_kernel");
+  }
+  hipLaunchKernelGGL(ksym_format_kernel, dim3(1), dim3(256), 0, s, (const int*)counts, nsym,
+                     (int)p->ksym_shape, fmt);
+  STEAM_LAUNCH_CHECK("ksym_format_kernel");
   if (p->ktile) {
     const int64_t ne = (int64_t)g.nzc * p->nx * p->ny;
     hipLaunchKernelGGL(ktile_kernel, dim3((unsigned)std::min<int64_t>((ne + 255) / 256, 1 << 16)),
-                       dim3(256), 0, s, (const float2*)d.S, (float2*)spectrum_dev, (int)g.nzc,
-                       (int)p->nx, (int)p->ny, p->ktile);
+                       dim3(256), 0, s, (const float2*)d.S,
+                       (void*)((char*)spectrum_dev + KSPEC_DATA_OFF), (const int*)fmt,
+                       (int)g.nzc, (int)p->nx, (int)p->ny, p->ktile);
     STEAM_LAUNCH_CHECK("ktile_kernel");
   }
   return STEAM_OK;
@@ -377,7 +449,8 @@ extern "C" int steam_conv_execute(const steam_conv_plan* p, const float* in_dev,
     return set_error(STEAM_EINVAL, "steam_conv_execute: NULL argument");
   PassDesc d = base_desc(p, workspace_dev);
   d.mode = MODE_PLAIN;
-  d.K = (const float2*)spectrum_dev;
+  d.K = (const float2*)((const char*)spectrum_dev + KSPEC_DATA_OFF);
+  d.kfmt = (const int*)spectrum_dev;
   d.ktile = p->ktile;
   d.in = in_dev;
   d.out = out_dev;
@@ -454,6 +527,16 @@ void ledger_to_host(const DevLedger& L, steam_flux_ledger* h) {
 
 }  // namespace steam
 
+// Workgroup cap of a noise field generated ahead: STEAM_NOISE_WGS from the
+// environment when set (read once), else the compiled default.
+static int noise_wgs_cap() {
+  static const int cap = [] {
+    const char* e = getenv("STEAM_NOISE_WGS");
+    return e && *e ? atoi(e) : STEAM_NOISE_WGS;
+  }();
+  return cap;
+}
+
 extern "C" int steam_flux_noise(const steam_conv_plan* p,
                                 const steam_noise_params* noise, float* out_dev,
                                 steam_stream_t stream) {
@@ -471,8 +554,8 @@ extern "C" int steam_flux_noise(const steam_conv_plan* p,
     // the field generated ahead, usually beside another class's transforms:
     // at most STEAM_NOISE_WGS workgroups (0: one per four lines)
     ProfScope ps(STEAM_KF_NOISE, s);
-    const int grid = (STEAM_NOISE_WGS > 0 && p->wg_noise > STEAM_NOISE_WGS) ? STEAM_NOISE_WGS
-                                                                              : p->wg_noise;
+    const int cap = noise_wgs_cap();
+    const int grid = (cap > 0 && p->wg_noise > cap) ? cap : p->wg_noise;
     hipLaunchKernelGGL((noise_field_kernel<SingleSrc, true>), dim3(grid), dim3(256), 0, s,
                        SingleSrc{d});
   }
@@ -503,7 +586,8 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
   hipStream_t s = (hipStream_t)stream;
   PassDesc d = base_desc(p, workspace_dev);
   d.mode = MODE_FLUX;
-  d.K = (const float2*)spectrum_dev;
+  d.K = (const float2*)((const char*)spectrum_dev + KSPEC_DATA_OFF);
+  d.kfmt = (const int*)spectrum_dev;
   d.ktile = p->ktile;
   d.out = flux_dev;
   d.sigma_in = sigma_in;

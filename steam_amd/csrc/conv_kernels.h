@@ -24,6 +24,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -48,12 +49,6 @@
 #endif
 #ifndef STEAM_ZINV_ATTR
 #define STEAM_ZINV_ATTR
-#endif
-// register z passes: move the spectral planes through a workgroup tile in LDS
-// so that each wave access covers all the workgroup's consecutive lines of
-// a plane (1), or access them straight from the lanes' registers (0)
-#ifndef STEAM_ZR_TILE
-#define STEAM_ZR_TILE 1
 #endif
 // register x pass: issue the kernel-spectrum loads with the line's loads,
 // before the forward transform (1), or after it (0)
@@ -450,15 +445,15 @@ __global__ __launch_bounds__(256) void ypass_reg_kernel(Src src, int inv) {
 #else
 #define STEAM_XREG_WAVES_ATTR
 #endif
-template <class Src, int XMODE, int LOGN, int C>
-__global__ __launch_bounds__(1024) STEAM_XREG_WAVES_ATTR void xpass_reg_kernel(Src src) {
+// One workgroup of the register x pass.  KREAL: the kernel spectrum is real
+// and even in x (a kernel with mirror symmetry on every axis), stored as
+// floats [kz][y/T][nx/2 + 1][T]; bin x takes the value of min(x, nx - x).
+template <int XMODE, int LOGN, int C, bool KREAL>
+__device__ __forceinline__ void xpass_reg_body(const PassDesc& d, int wg, float2* lds) {
   using F = RegFFT<LOGN>;
   constexpr int P = F::P, PT = P / C;
   constexpr int T = (1024 / PT < (C == 1 ? XREG_COLS : XREG_COLS2))
                         ? 1024 / PT : (C == 1 ? XREG_COLS : XREG_COLS2);
-  __shared__ float2 lds[T * F::LDSN];
-  int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const PassDesc& d = src.get(PASS_X, wg);
   const ConvGeom& g = d.g;
   const int tid = threadIdx.x;
   const int t = tid % T, q0 = tid / T;
@@ -472,28 +467,42 @@ __global__ __launch_bounds__(1024) STEAM_XREG_WAVES_ATTR void xpass_reg_kernel(S
   float2* Sb = d.S + (int64_t)p * nx * ny;
   const int yo = valid ? y : 0;
   auto off = [&](int c, int m) { return (q0 + c * PT + m * P) * ny + yo; };
-  float2 v[C][8], kv[C][8];
+  using KV = typename std::conditional<KREAL, float, float2>::type;
+  float2 v[C][8];
+  KV kv[C][8];
 #pragma unroll
   for (int c = 0; c < C; ++c)
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[c][m] = valid ? Sb[off(c, m)] : make_float2(0.f, 0.f);
   // the kernel spectrum: in this pass's column tiles [kz][y/T][x][T] when
   // the plan stores it so (the workgroup's block is then one contiguous run)
-  const float2* Kb = nullptr;
+  const KV* Kb = nullptr;
   const bool kt = d.ktile != 0;
-  auto koff = [&](int c, int m) { return kt ? (q0 + c * PT + m * P) * T + t : off(c, m); };
+  auto koff = [&](int c, int m) {
+    const int x = q0 + c * PT + m * P;
+    if (KREAL) return (x <= nx - x ? x : nx - x) * T + t;
+    return kt ? x * T + t : off(c, m);
+  };
   auto load_k = [&]() {
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
-      for (int m = 0; m < 8; ++m) kv[c][m] = valid ? Kb[koff(c, m)] : make_float2(0.f, 0.f);
+      for (int m = 0; m < 8; ++m) {
+        if constexpr (KREAL)
+          kv[c][m] = valid ? Kb[koff(c, m)] : 0.f;
+        else
+          kv[c][m] = valid ? Kb[koff(c, m)] : make_float2(0.f, 0.f);
+      }
   };
   // one lane per thread: the spectrum loads go out with the line's; two
   // lanes: after the forward transform (the registers of both would spill)
   constexpr bool KPRE = STEAM_XREG_KPREFETCH && C == 1;
   if (XMODE == 0) {
-    Kb = kt ? d.K + ((int64_t)(p % g.nzc) * nyt + (wg - p * nyt)) * nx * T
-            : d.K + (int64_t)(p % g.nzc) * nx * ny;
+    const int64_t tile = (int64_t)(p % g.nzc) * nyt + (wg - p * nyt);
+    if constexpr (KREAL)
+      Kb = reinterpret_cast<const KV*>(d.K) + tile * (nx / 2 + 1) * T;
+    else
+      Kb = kt ? d.K + tile * nx * T : d.K + (int64_t)(p % g.nzc) * nx * ny;
     if (KPRE) load_k();
   }
   // the twiddle bases, shared by the forward and the inverse transform
@@ -506,7 +515,12 @@ __global__ __launch_bounds__(1024) STEAM_XREG_WAVES_ATTR void xpass_reg_kernel(S
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[c][m] = cmul(v[c][m], kv[c][m]);
+      for (int m = 0; m < 8; ++m) {
+        if constexpr (KREAL)
+          v[c][m] = make_float2(v[c][m].x * kv[c][m], v[c][m].y * kv[c][m]);
+        else
+          v[c][m] = cmul(v[c][m], kv[c][m]);
+      }
     fft_reg_lanes<LOGN, C>(v, q0, PT, lds, T, t, g.ax.tw, twb, true);
   } else {
     const float2 f = kernel_bin_factor(d, p % g.nzc);
@@ -521,6 +535,22 @@ __global__ __launch_bounds__(1024) STEAM_XREG_WAVES_ATTR void xpass_reg_kernel(S
 #pragma unroll
       for (int m = 0; m < 8; ++m) Sb[off(c, m)] = v[c][m];
   }
+}
+
+template <class Src, int XMODE, int LOGN, int C>
+__global__ __launch_bounds__(1024) STEAM_XREG_WAVES_ATTR void xpass_reg_kernel(Src src) {
+  using F = RegFFT<LOGN>;
+  constexpr int P = F::P, PT = P / C;
+  constexpr int T = (1024 / PT < (C == 1 ? XREG_COLS : XREG_COLS2))
+                        ? 1024 / PT : (C == 1 ? XREG_COLS : XREG_COLS2);
+  __shared__ float2 lds[T * F::LDSN];
+  int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const PassDesc& d = src.get(PASS_X, wg);
+  // the spectrum's format word is uniform over the launch's instance
+  if (XMODE == 0 && d.kfmt && *d.kfmt == KFMT_REAL_EVEN)
+    xpass_reg_body<XMODE, LOGN, C, true>(d, wg, lds);
+  else
+    xpass_reg_body<XMODE, LOGN, C, false>(d, wg, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1057,112 +1087,162 @@ __device__ __forceinline__ void st_zpair(float* p, int64_t off, int z0, int nz, 
   }
 }
 
-template <class Src, int MODE, class ZP, bool ODD>
-__global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
+// A pair whose two levels both lie below nz: no bound checks (with an odd
+// level count the line start is only 4-byte aligned).
+template <bool ODD>
+__device__ __forceinline__ float2 ld_pair_in(const float* p, unsigned off) {
+  if constexpr (!ODD) return *reinterpret_cast<const float2*>(p + off);
+  return make_float2(p[off], p[off + 1]);
+}
+template <bool ODD>
+__device__ __forceinline__ void st_pair_in(float* p, unsigned off, float2 v) {
+  if constexpr (!ODD) {
+    *reinterpret_cast<float2*>(p + off) = v;
+  } else {
+    p[off] = v.x;
+    p[off + 1] = v.y;
+  }
+}
+
+// Lane layout shared by the register z passes.  A workgroup holds LW
+// consecutive lines from line0; lane (w, t, q) works on line l = w LINES + t
+// of them, natural indices q + m P.  Pair m (levels 2(q + mP), +1) lies
+// below nz on every lane when m < m_full, a count uniform over the launch,
+// so those pairs need neither a bound check nor a lane mask; lines and level
+// offsets inside one workgroup fit 32 bits (x/y extents <= 4096, nz <= 16384).
+template <class ZP>
+struct ZLanes {
+  static constexpr int LW = ZTile<ZP>::LW;
+  int t, q, l, line0, nlines, m_full;
+  bool valid;
+  __device__ __forceinline__ ZLanes(const ConvGeom& g, int wg, bool full) {
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    t = lane / ZP::P;
+    q = lane % ZP::P;
+    l = w * ZP::LINES + t;
+    line0 = wg * LW;
+    nlines = g.nx * g.ny;
+    valid = full || line0 + l < nlines;
+    m_full = min(ZP::V, g.nz / (2 * ZP::P));
+  }
+  // the lane's line is inside the x/y window (32-bit division)
+  __device__ __forceinline__ bool in_window(const ConvGeom& g, const int (&win)[4]) const {
+    const unsigned li = (unsigned)(line0 + l);
+    const int x = (int)(li / (unsigned)g.ny);
+    const int y = (int)(li - (unsigned)x * (unsigned)g.ny);
+    return valid && x >= win[0] && x < win[1] && y >= win[2] && y < win[3];
+  }
+};
+
+template <int MODE, class ZP, bool ODD, bool FULL>
+__device__ __forceinline__ void zfwd_reg_body(const PassDesc& d, int wg, float2* lds_lines,
+                                              float2* tw2, double* red) {
   constexpr int N = ZP::N, V = ZP::V, P = ZP::P, LPW = ZP::LINES;
-  __shared__ float2 lds_lines[ZR_WAVES * LPW * ZP::PITCH];
-  __shared__ float2 tw2[2 * N];
-  __shared__ double red[ZR_WAVES * 3];
-  int wg = blockIdx.x;
-  const PassDesc& d = src.get(PASS_Z, wg);
+  using TL = ZTile<ZP>;
   const ConvGeom& g = d.g;
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int t = lane / P, q = lane % P;
-  const int64_t nlines = (int64_t)g.nx * g.ny;
-  const int64_t line = ((int64_t)wg * ZR_WAVES + w) * LPW + t;
-  const bool valid = line < nlines;
-  const int nz = g.nz;
-  const int64_t zoff = (valid ? line : 0) * nz;
-  const float* srcp = (MODE == MODE_FLUX) ? d.out : d.in;
-  float2 v[V];
-  double acc[3] = {0.0, 0.0, 0.0};
-  float2* L = lds_lines + (w * LPW + t) * ZP::PITCH;
-  float2 nv[V];
+  const int tid = threadIdx.x, w = tid >> 6;
+  const ZLanes<ZP> ln(g, wg, FULL);
+  const int q = ln.q, nz = g.nz;
+  // uniform bases, 32-bit lane offsets
+  const int64_t base = (int64_t)ln.line0 * nz;
+  const float* srcp = ((MODE == MODE_FLUX) ? d.out : d.in) + base;
+  const float* nfp = d.nfield + base;
+  const unsigned loff = ln.valid ? (unsigned)(ln.l * nz) : 0u;
+  float2 v[V], nv[V];
 #pragma unroll
   for (int m = 0; m < V; ++m) {
-    const int n2i = 2 * (q + m * P);
+    const int z0 = 2 * (q + m * P);
     v[m] = make_float2(0.f, 0.f);
     nv[m] = make_float2(0.f, 0.f);
-    if (valid && n2i < nz) {
-      v[m] = ld_zpair<ODD>(srcp, zoff, n2i, nz);
-      if (MODE == MODE_FLUX) nv[m] = ld_zpair<ODD>(d.nfield, zoff, n2i, nz);
+    if (m < ln.m_full) {
+      if (ln.valid) {
+        v[m] = ld_pair_in<ODD>(srcp, loff + z0);
+        if (MODE == MODE_FLUX) nv[m] = ld_pair_in<ODD>(nfp, loff + z0);
+      }
+    } else if (ln.valid && z0 < nz) {
+      v[m] = ld_zpair<ODD>(srcp, loff, z0, nz);
+      if (MODE == MODE_FLUX) nv[m] = ld_zpair<ODD>(nfp, loff, z0, nz);
     }
   }
   stage_tw2(g, tw2, N);
+  double acc0 = 0.0, acc1 = 0.0;
+  int nonzero = 0;
   if (MODE == MODE_FLUX) {
     const Pending pd = pending_of(d);
-    const int x = (int)(line / g.ny), y = (int)(line - (int64_t)x * g.ny);
-    const bool win = valid && x >= d.win[0] && x < d.win[1] && y >= d.win[2] && y < d.win[3];
+    const bool win = ln.in_window(g, d.win);
+    float* skp = d.sk ? d.sk + base : nullptr;
 #pragma unroll
     for (int m = 0; m < V; ++m) {
-      const int n2i = 2 * (q + m * P);
-      if (valid && n2i < nz) {
+      const int z0 = 2 * (q + m * P);
+      const bool full = m < ln.m_full;
+      if (full ? ln.valid : (ln.valid && z0 < nz)) {
         // with an odd level count the last pair's second level is past the
         // top: it holds no cell and enters no sum
-        const bool top = ODD && n2i + 1 >= nz;
+        const bool top = ODD && !full && z0 + 1 >= nz;
         const float f0 = entering_value(pd, v[m].x);
         const float f1 = top ? 0.f : entering_value(pd, v[m].y);
-        const float2 p = make_float2(__fmul_rn(nv[m].x, f0), __fmul_rn(nv[m].y, f1));
+        const float2 pr = make_float2(__fmul_rn(nv[m].x, f0), __fmul_rn(nv[m].y, f1));
         if (win) {
-          acc[0] += (double)f0 + (double)f1;
-          acc[1] += (double)fabsf(nv[m].x) + (double)fabsf(nv[m].y);
-          acc[2] += (double)((nv[m].x != 0.f) + (nv[m].y != 0.f));
+          acc0 += (double)f0 + (double)f1;
+          acc1 += (double)fabsf(nv[m].x) + (double)fabsf(nv[m].y);
+          nonzero += (nv[m].x != 0.f) + (nv[m].y != 0.f);
         }
-        if (d.sk) st_zpair<ODD>(d.sk, zoff, n2i, nz, p);
-        v[m] = p;
+        if (skp) {
+          if (full)
+            st_pair_in<ODD>(skp, loff + z0, pr);
+          else
+            st_zpair<ODD>(skp, loff, z0, nz, pr);
+        }
+        v[m] = pr;
       }
     }
   }
   __syncthreads();
+  float2* L = lds_lines + ln.l * ZP::PITCH;
   ZP::template fft<false>(v, q, L, tw2);
   // r2c split: X2[k] = (Z[k] + conj Z[N-k]) - i W_2N^k (Z[k] - conj Z[N-k])
 #pragma unroll
   for (int m = 0; m < V; ++m) L[zpad(q + m * P)] = v[m];
   wave_lds_sync();
-  const int64_t plane = nlines;
-  float2* So = d.S + line;
   float2 nyq = make_float2(0.f, 0.f);
 #pragma unroll
   for (int m = 0; m < V; ++m) {
     const int k = q + m * P;
     const float2 zk = v[m];
-    float2 X;
-    if (k == 0) {
-      X = make_float2(2.f * (zk.x + zk.y), 0.f);
+    if (m == 0 && q == 0) {
+      v[m] = make_float2(2.f * (zk.x + zk.y), 0.f);
       nyq = make_float2(2.f * (zk.x - zk.y), 0.f);
-      if (!STEAM_ZR_TILE && valid) So[(int64_t)N * plane] = nyq;
     } else {
       const float2 zm = L[zpad(N - k)];
       const float2 wk = tw2[k];
       const float2 A = make_float2(zk.x + zm.x, zk.y - zm.y);
       const float2 B = make_float2(zk.x - zm.x, zk.y + zm.y);
       const float2 wB = cmul(wk, B);
-      X = make_float2(A.x + wB.y, A.y - wB.x);
+      v[m] = make_float2(A.x + wB.y, A.y - wB.x);
     }
-    if (STEAM_ZR_TILE)
-      v[m] = X;
-    else if (valid)
-      So[(int64_t)k * plane] = X;
   }
-  if (STEAM_ZR_TILE) {
-    // planes k = 0..N of the workgroup's lines, one plane run per wave access
-    using TL = ZTile<ZP>;
-    float2* tile = lds_lines;
-    const int l = w * LPW + t;
-    __syncthreads();
+  // planes k = 0..N of the workgroup's lines, one plane run per wave access
+  float2* tile = lds_lines;
+  __syncthreads();
 #pragma unroll
-    for (int m = 0; m < V; ++m) tile[(q + m * P) * TL::TP + l] = v[m];
-    if (q == 0) tile[N * TL::TP + l] = nyq;
-    __syncthreads();
-    const int64_t line0 = (int64_t)wg * TL::LW;
+  for (int m = 0; m < V; ++m) tile[(q + m * P) * TL::TP + ln.l] = v[m];
+  if (q == 0) tile[N * TL::TP + ln.l] = nyq;
+  __syncthreads();
+  const int64_t plane = (int64_t)ln.nlines;
+  float2* So = d.S + ln.line0;
+  constexpr int KSTEP = ZR_NTH / TL::LW;  // planes per iteration
+  const int kl = tid / TL::LW, c = tid % TL::LW;
+  const int64_t lane_off = (int64_t)kl * plane + c;
+  const bool cvalid = FULL || ln.line0 + c < ln.nlines;
 #pragma unroll
-    for (int i = 0; i < TL::ITER; ++i) {
-      const int f = tid + i * ZR_NTH;
-      const int k = f / TL::LW, c = f - k * TL::LW;
-      if (f < TL::NF && line0 + c < nlines) d.S[(int64_t)k * plane + line0 + c] = tile[k * TL::TP + c];
+  for (int i = 0; i < TL::ITER; ++i) {
+    const int k = kl + i * KSTEP;
+    if ((i + 1) * ZR_NTH <= TL::NF || k <= N) {
+      if (cvalid) So[(int64_t)(i * KSTEP) * plane + lane_off] = tile[k * TL::TP + c];
     }
   }
   if (MODE == MODE_FLUX && !d.apply) {
+    double acc[3] = {acc0, acc1, (double)nonzero};
     zr_block_sum<3>(acc, red);
     if (tid == 0) {
       d.part_z[wg * 3 + 0] = acc[0];
@@ -1170,76 +1250,85 @@ __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
       d.part_z[wg * 3 + 2] = acc[2];
     }
   }
+  (void)w;
+  (void)LPW;
 }
 
 template <class Src, int MODE, class ZP, bool ODD>
-__global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src src) {
-  constexpr int N = ZP::N, V = ZP::V, P = ZP::P, LPW = ZP::LINES;
+__global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
+  constexpr int N = ZP::N, LPW = ZP::LINES;
   __shared__ float2 lds_lines[ZR_WAVES * LPW * ZP::PITCH];
   __shared__ float2 tw2[2 * N];
-  __shared__ double red[ZR_WAVES * 2];
+  __shared__ double red[ZR_WAVES * 3];
   int wg = blockIdx.x;
-  const PassDesc& d = src.get(PASS_ZINV, wg);
+  const PassDesc& d = src.get(PASS_Z, wg);
   const ConvGeom& g = d.g;
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int t = lane / P, q = lane % P;
-  const int64_t nlines = (int64_t)g.nx * g.ny;
-  const int64_t line = ((int64_t)wg * ZR_WAVES + w) * LPW + t;
-  const bool valid = line < nlines;
-  const int nz = g.nz;
-  const int64_t zoff = (valid ? line : 0) * nz;
-  const int64_t plane = nlines;
-  const float2* Si = d.S + (valid ? line : 0);
-  float2 v[V];
-  float2 yn = make_float2(0.f, 0.f);
-  if (STEAM_ZR_TILE) {
-    // planes k = 0..N of the workgroup's lines, one plane run per wave access
-    using TL = ZTile<ZP>;
-    float2* tile = lds_lines;
-    const int64_t line0 = (int64_t)wg * TL::LW;
+  // a workgroup whose lines all exist takes the unmasked body
+  if ((int64_t)(wg + 1) * ZTile<ZP>::LW <= (int64_t)g.nx * g.ny)
+    zfwd_reg_body<MODE, ZP, ODD, true>(d, wg, lds_lines, tw2, red);
+  else
+    zfwd_reg_body<MODE, ZP, ODD, false>(d, wg, lds_lines, tw2, red);
+}
+
+template <int MODE, class ZP, bool ODD, bool FULL>
+__device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2* lds_lines,
+                                              float2* tw2, double* red) {
+  constexpr int N = ZP::N, V = ZP::V, P = ZP::P;
+  using TL = ZTile<ZP>;
+  const ConvGeom& g = d.g;
+  const int tid = threadIdx.x;
+  const ZLanes<ZP> ln(g, wg, FULL);
+  const int q = ln.q, nz = g.nz;
+  // planes k = 0..N of the workgroup's lines, one plane run per wave access
+  const int64_t plane = (int64_t)ln.nlines;
+  const float2* Si = d.S + ln.line0;
+  constexpr int KSTEP = ZR_NTH / TL::LW;
+  const int kl = tid / TL::LW, c = tid % TL::LW;
+  const int64_t lane_off = (int64_t)kl * plane + c;
+  const bool cvalid = FULL || ln.line0 + c < ln.nlines;
+  float2* tile = lds_lines;
+  {
     float2 u[TL::ITER];
 #pragma unroll
     for (int i = 0; i < TL::ITER; ++i) {
-      const int f = tid + i * ZR_NTH;
-      const int k = f / TL::LW, c = f - k * TL::LW;
-      u[i] = (f < TL::NF && line0 + c < nlines) ? d.S[(int64_t)k * plane + line0 + c]
-                                                 : make_float2(0.f, 0.f);
+      const int k = kl + i * KSTEP;
+      const bool in = (i + 1) * ZR_NTH <= TL::NF || k <= N;
+      u[i] = (in && cvalid) ? Si[(int64_t)(i * KSTEP) * plane + lane_off] : make_float2(0.f, 0.f);
     }
+    stage_tw2(g, tw2, N);
 #pragma unroll
     for (int i = 0; i < TL::ITER; ++i) {
-      const int f = tid + i * ZR_NTH;
-      const int k = f / TL::LW, c = f - k * TL::LW;
-      if (f < TL::NF) tile[k * TL::TP + c] = u[i];
+      const int k = kl + i * KSTEP;
+      if ((i + 1) * ZR_NTH <= TL::NF || k <= N) tile[k * TL::TP + c] = u[i];
     }
-    __syncthreads();
-    const int l = w * LPW + t;
-#pragma unroll
-    for (int m = 0; m < V; ++m) v[m] = tile[(q + m * P) * TL::TP + l];
-    if (q == 0) yn = tile[N * TL::TP + l];
-    __syncthreads();
-  } else {
-#pragma unroll
-    for (int m = 0; m < V; ++m) v[m] = valid ? Si[(int64_t)(q + m * P) * plane] : make_float2(0.f, 0.f);
-    if (q == 0 && valid) yn = Si[(int64_t)N * plane];
   }
+  __syncthreads();
+  float2 v[V];
+  float2 yn = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int m = 0; m < V; ++m) v[m] = tile[(q + m * P) * TL::TP + ln.l];
+  if (q == 0) yn = tile[N * TL::TP + ln.l];
   // output levels z = 2n, 2n+1 of natural index n (the kernel spectrum
   // carries the 'same' trim shift); with an odd nz the last pair's second
   // level is past the top and is neither written nor counted
-  // STEAM_ZR_PREFETCH 1: all of it before the transform, 2: the first half
-  // before and the rest after
+  const int64_t base = (int64_t)ln.line0 * nz;
+  float* outp = d.out + base;
+  const unsigned loff = ln.valid ? (unsigned)(ln.l * nz) : 0u;
   float2 fl[V];
-  constexpr int PRE = STEAM_ZR_PREFETCH == 1 ? V : (STEAM_ZR_PREFETCH == 2 ? V / 2 : 0);
+  // STEAM_ZR_PREFETCH 1: the flux before the transform, 0: after it
   auto load_flux = [&](int m) {
     const int z0 = 2 * (q + m * P);
-    fl[m] = (valid && z0 < nz) ? ld_zpair<ODD>(d.out, zoff, z0, nz) : make_float2(0.f, 0.f);
+    if (m < ln.m_full)
+      fl[m] = ln.valid ? ld_pair_in<ODD>(outp, loff + z0) : make_float2(0.f, 0.f);
+    else
+      fl[m] = (ln.valid && z0 < nz) ? ld_zpair<ODD>(outp, loff, z0, nz) : make_float2(0.f, 0.f);
   };
-  if (MODE == MODE_FLUX) {
+  if (MODE == MODE_FLUX && STEAM_ZR_PREFETCH) {
 #pragma unroll
-    for (int m = 0; m < PRE; ++m) load_flux(m);
+    for (int m = 0; m < V; ++m) load_flux(m);
   }
-  stage_tw2(g, tw2, N);
   __syncthreads();
-  float2* L = lds_lines + (w * LPW + t) * ZP::PITCH;
+  float2* L = lds_lines + ln.l * ZP::PITCH;
 #pragma unroll
   for (int m = 0; m < V; ++m) L[zpad(q + m * P)] = v[m];
   wave_lds_sync();
@@ -1248,7 +1337,7 @@ __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src sr
   for (int m = 0; m < V; ++m) {
     const int k = q + m * P;
     const float2 yk = v[m];
-    const float2 ym = (k == 0) ? yn : L[zpad(N - k)];
+    const float2 ym = (m == 0 && q == 0) ? yn : L[zpad(N - k)];
     const float2 wk = tw2[k];
     const float2 A = make_float2(yk.x + ym.x, yk.y - ym.y);
     const float2 B = make_float2(yk.x - ym.x, yk.y + ym.y);
@@ -1257,42 +1346,69 @@ __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src sr
   }
   wave_lds_sync();
   ZP::template fft<true>(v, q, L, tw2);
-  double acc[2] = {0.0, 0.0};
   if (MODE == MODE_PLAIN) {
 #pragma unroll
     for (int m = 0; m < V; ++m) {
       const int z0 = 2 * (q + m * P);
-      if (valid && z0 < nz) st_zpair<ODD>(d.out, zoff, z0, nz, v[m]);
-    }
-  } else {
-#pragma unroll
-    for (int m = PRE; m < V; ++m) load_flux(m);
-    const Pending pd = pending_of(d);
-    const int x = (int)(line / g.ny), y = (int)(line - (int64_t)x * g.ny);
-    const bool win = valid && x >= d.win[0] && x < d.win[1] && y >= d.win[2] && y < d.win[3];
-#pragma unroll
-    for (int m = 0; m < V; ++m) {
-      const int z0 = 2 * (q + m * P);
-      float o0 = __fadd_rn(entering_value(pd, fl[m].x), v[m].x);
-      float o1 = __fadd_rn(entering_value(pd, fl[m].y), v[m].y);
-      if (ODD && z0 + 1 >= nz) o1 = 0.f;
-      const int clipped = (o0 < 0.f) + (o1 < 0.f);
-      o0 = (o0 < 0.f) ? 0.f : o0;
-      o1 = (o1 < 0.f) ? 0.f : o1;
-      if (valid && z0 < nz) {
-        if (win || !d.clip_win) acc[1] += (double)clipped;
-        if (win) acc[0] += (double)o0 + (double)o1;
-        st_zpair<ODD>(d.out, zoff, z0, nz, make_float2(o0, o1));
+      if (m < ln.m_full) {
+        if (ln.valid) st_pair_in<ODD>(outp, loff + z0, v[m]);
+      } else if (ln.valid && z0 < nz) {
+        st_zpair<ODD>(outp, loff, z0, nz, v[m]);
       }
     }
-    if (!d.apply) {
-      zr_block_sum<2>(acc, red);
-      if (tid == 0) {
-        d.part_o[wg * 2 + 0] = acc[0];
-        d.part_o[wg * 2 + 1] = acc[1];
-      }
+    return;
+  }
+  if (!STEAM_ZR_PREFETCH) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) load_flux(m);
+  }
+  const Pending pd = pending_of(d);
+  const bool win = ln.in_window(g, d.win);
+  const bool count_clip = win || !d.clip_win;
+  double acc0 = 0.0;
+  int nclip = 0;
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int z0 = 2 * (q + m * P);
+    const bool full = m < ln.m_full;
+    float o0 = __fadd_rn(entering_value(pd, fl[m].x), v[m].x);
+    float o1 = __fadd_rn(entering_value(pd, fl[m].y), v[m].y);
+    if (ODD && !full && z0 + 1 >= nz) o1 = 0.f;
+    const int clipped = (o0 < 0.f) + (o1 < 0.f);
+    o0 = (o0 < 0.f) ? 0.f : o0;
+    o1 = (o1 < 0.f) ? 0.f : o1;
+    if (full ? ln.valid : (ln.valid && z0 < nz)) {
+      if (count_clip) nclip += clipped;
+      if (win) acc0 += (double)o0 + (double)o1;
+      if (full)
+        st_pair_in<ODD>(outp, loff + z0, make_float2(o0, o1));
+      else
+        st_zpair<ODD>(outp, loff, z0, nz, make_float2(o0, o1));
     }
   }
+  if (!d.apply) {
+    double acc[2] = {acc0, (double)nclip};
+    zr_block_sum<2>(acc, red);
+    if (tid == 0) {
+      d.part_o[wg * 2 + 0] = acc[0];
+      d.part_o[wg * 2 + 1] = acc[1];
+    }
+  }
+}
+
+template <class Src, int MODE, class ZP, bool ODD>
+__global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src src) {
+  constexpr int N = ZP::N, LPW = ZP::LINES;
+  __shared__ float2 lds_lines[ZR_WAVES * LPW * ZP::PITCH];
+  __shared__ float2 tw2[2 * N];
+  __shared__ double red[ZR_WAVES * 2];
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_ZINV, wg);
+  const ConvGeom& g = d.g;
+  if ((int64_t)(wg + 1) * ZTile<ZP>::LW <= (int64_t)g.nx * g.ny)
+    zinv_reg_body<MODE, ZP, ODD, true>(d, wg, lds_lines, tw2, red);
+  else
+    zinv_reg_body<MODE, ZP, ODD, false>(d, wg, lds_lines, tw2, red);
 }
 
 // ---------------------------------------------------------------------------

@@ -12,6 +12,7 @@ struct steam_conv_plan {
   int64_t nx, ny, nz, kx, ky, kz;
   int64_t s_bytes, pz_off, po_off, led_off, nf_off, ws_bytes, spec_bytes;
   int ktile;         // columns of the kernel spectrum's x-pass tiles (0: [nzc][nx][ny])
+  int ksym_shape;    // the kernel's shape admits the real, x-even spectrum format
   int64_t kstd_off;  // spectrum build area in [nzc][nx][ny] order
   int wg_noise;
   size_t lds_z, lds_zinv, lds_y, lds_x;

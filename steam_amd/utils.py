@@ -109,6 +109,14 @@ class ConvOperator:
             self._spectra[key] = (spec, held if key[0] == 't' else None)
         return spec
 
+    @staticmethod
+    def spectrum_format(spec):
+        """Format of a built spectrum: 'real_even' (the kernel has mirror
+        symmetry on every axis: real values of bins x <= nx/2 are stored) or
+        'complex'.  Reads the buffer's format word (synchronises)."""
+        word = int(spec[:4].view(torch.int32).item())
+        return 'real_even' if word == 1 else 'complex'
+
     def execute(self, field_t, out_t, spec):
         lib = _lib.load()
         ws = self.workspace()

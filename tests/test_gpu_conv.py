@@ -288,3 +288,31 @@ def test_register_z_plans_at_18_values_per_lane(fs, ks, n_fft):
         op.execute(f, out, op.spectrum(k))
         assert _rel(out.cpu().numpy(), ref) < CONV_REL_L2, prec
     assert nfft_candidates(128, 13)[0] == 144 and nfft_candidates(256, 13)[0] == 288
+
+
+@pytest.mark.parametrize("fs", [(64, 64, 40), (128, 64, 33), (256, 32, 20)])
+def test_mirror_symmetric_kernel_takes_the_real_even_spectrum(fs):
+    """A kernel with mirror symmetry on every axis (the turbulon kernels) has
+    a real spectrum, even in x: the register x pass then reads float32 bins
+    x <= nx/2 only.  One asymmetric tap, an even-sized axis or a folded
+    kernel keeps the complex format; every form matches the float64 oracle."""
+    from steam_amd import simulate as S
+    from steam_amd.utils import ConvOperator
+    rng = np.random.default_rng(17)
+    f = torch.from_numpy(rng.standard_normal(fs).astype(np.float32)).cuda()
+    sym = S._turbulon_envelope(1, .5, .5, .5, as_tensor=True)      # 13^3
+    asym = sym.clone()
+    asym[2, 5, 7] += 0.25
+    even = torch.from_numpy(rng.standard_normal((4, 5, 5)).astype(np.float32)).cuda()
+    wide = S._turbulon_envelope(1, .5, .5, .5, support_factor=20, as_tensor=True)[:, :, 36:45]
+    cases = [(sym, 'real_even'), (asym, 'complex'), (even, 'complex')]
+    if fs[1] < wide.shape[1]:
+        cases.append((wide.contiguous(), 'complex'))   # folded onto the field in y
+    for k, fmt in cases:
+        op = ConvOperator(fs, tuple(k.shape))
+        spec = op.spectrum(k)
+        assert op.spectrum_format(spec) == fmt, (tuple(k.shape), fmt)
+        out = torch.empty_like(f)
+        op.execute(f, out, spec)
+        ref = M.conv_fft(f.cpu().numpy(), k.cpu().numpy())
+        assert _rel(out.cpu().numpy(), ref) < CONV_REL_L2, fmt
