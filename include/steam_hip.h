@@ -109,6 +109,7 @@ int steam_multiplier_noise(int64_t nx, int64_t ny, int64_t nz,
 
 #define STEAM_SHAPE_MEXICAN_HAT 0
 #define STEAM_SHAPE_MORLET_OMEGA0_6 1
+#define STEAM_SHAPE_POWER_LAW 2 /* steam_turbulon_envelope_power only */
 
 /* Half-widths ceil(support_factor*k/d) of the kernel (host-side, no GPU). */
 int steam_envelope_halfwidths(double k, double dx, double dy, double dz,
@@ -121,6 +122,17 @@ int steam_envelope_halfwidths(double k, double dx, double dy, double dz,
 int steam_turbulon_envelope(double k, double dx, double dy, double dz,
                             double support_factor, int shape_id,
                             float* out_dev, steam_stream_t stream);
+
+/* The power-law turbulon T = (1 + rho)^(-p/2), rho = r^2 / (k/pi)^2: regular
+ * at the centre, |r|^(-p) beyond the core (p = d/alpha for a generator
+ * kernel, d - H for a fractional integration), with the same support and
+ * Gaussian-weighted zero-sum correction as steam_turbulon_envelope.  No
+ * counterpart in the reference: an addition to its shape registry
+ * (steam/turbulons.py:20), whose entries are the Mexican hat and Morlet
+ * shapes.  0 < exponent < 16.  Synchronises. */
+int steam_turbulon_envelope_power(double k, double dx, double dy, double dz,
+                                  double support_factor, double exponent,
+                                  float* out_dev, steam_stream_t stream);
 
 /* steam_turbulon_envelope without allocating: no counterpart in the
  * reference.  It is for callers that cannot allocate device memory (code

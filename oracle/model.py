@@ -169,13 +169,15 @@ def compute_all_grids(k_values, inner_extent_x, inner_extent_y, inner_height,
 
 
 def turbulon_envelope(k, dx, dy, dz, support_factor=SUPPORT_FACTOR,
-                      shape='mexican_hat', dtype=np.float32):
+                      shape='mexican_hat', dtype=np.float32, exponent=3.0 / 1.8):
     """steam/simulate.py:2503-2556 with turbulons.py:23-48.
 
     float32 mirrors the reference (float32 taps, float32 sums); float64 keeps
-    everything in double.
+    everything in double.  ``shape='power_law'`` is (1 + rho)^(-exponent/2),
+    an addition with no counterpart in the reference (pinned by closed forms
+    in the tests only).
     """
-    if shape not in SHAPES:
+    if shape not in SHAPES and shape != 'power_law':
         raise ValueError(f"Unknown shape {shape!r}")
     hx = int(np.ceil(support_factor * k / dx))
     hy = int(np.ceil(support_factor * k / dy))
@@ -189,6 +191,8 @@ def turbulon_envelope(k, dx, dy, dz, support_factor=SUPPORT_FACTOR,
     ratio = r2 / sigma2
     if shape == 'mexican_hat':
         env = (dtype(3.0) - ratio) * np.exp(-ratio / dtype(2.0))
+    elif shape == 'power_law':
+        env = np.power(dtype(1.0) + ratio, dtype(-0.5 * exponent))
     else:
         env = np.cos(dtype(6.0 / k) * np.sqrt(r2)) * np.exp(-ratio / dtype(2.0))
     env = env.astype(dtype)

@@ -7,7 +7,7 @@ cascade, as hand-written HIP kernels (libsteam_hip.so) behind the reference's
 Python API:
 
     steam_amd.noise       keyed_uniforms, class_key, center_indices
-    steam_amd.turbulons   SHAPES, mexican_hat, morlet_omega0_6
+    steam_amd.turbulons   SHAPES, mexican_hat, morlet_omega0_6, power_law
     steam_amd.utils       convolve_fft_xy_oa_z, fold_kernel_to_field,
                           zoom_trilinear
     steam_amd.simulate    CONVOLVE, NoiseRegion, _keyed_sparse_levy,
@@ -18,6 +18,9 @@ Python API:
                           bounded_amplitude_add
     steam_amd.cascade     cascade_loop, project_onto_bounds, compose_output,
                           compose_flux_output, simulate_fields
+    steam_amd.api         simulate, refine (the reference's signatures; the run
+                          written to a grouped npz file), read_ledger
+    steam_amd.files       the grouped npz file (NetCDF-style groups)
     steam_amd.params      _compute_normalization, _bound_buffers,
                           _compensation_profiles, root_setup
     steam_amd.ledger      MeanReduction, FluxAdvanceLedger, PatternLedger,
@@ -34,3 +37,21 @@ from . import _lib  # noqa: F401  (binds torch's HIP runtime first)
 
 def library_version():
     return _lib.load().steam_version()
+
+
+def simulate(*args, **kwargs):
+    """steam.simulate (steam/__init__.py:6): see steam_amd.api.simulate."""
+    from .api import simulate as _simulate
+    return _simulate(*args, **kwargs)
+
+
+def refine(*args, **kwargs):
+    """steam.refine (steam/__init__.py:6): see steam_amd.api.refine."""
+    from .api import refine as _refine
+    return _refine(*args, **kwargs)
+
+
+def cascade_loop(*args, **kwargs):
+    """steam.cascade_loop (steam/__init__.py:6): see steam_amd.cascade."""
+    from .cascade import cascade_loop as _cascade_loop
+    return _cascade_loop(*args, **kwargs)

@@ -24,7 +24,7 @@ PREC_PLAIN = 0
 PREC_KAHAN = 1
 MEASURE = 0
 APPLY = 1
-SHAPE_IDS = {'mexican_hat': 0, 'morlet_omega0_6': 1}
+SHAPE_IDS = {'mexican_hat': 0, 'morlet_omega0_6': 1, 'power_law': 2}
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -91,6 +91,8 @@ _SIGNATURES = [
      [_F64, _F64, _F64, _F64, _F64, ctypes.POINTER(_I64)]),
     ("steam_turbulon_envelope", ctypes.c_int,
      [_F64, _F64, _F64, _F64, _F64, ctypes.c_int, _P, _P]),
+    ("steam_turbulon_envelope_power", ctypes.c_int,
+     [_F64, _F64, _F64, _F64, _F64, _F64, _P, _P]),
     ("steam_turbulon_envelope_noalloc", ctypes.c_int,
      [_F64, _F64, _F64, _F64, _F64, ctypes.c_int, _P, _P]),
     ("steam_conv_plan_create", ctypes.c_int,

@@ -188,7 +188,9 @@ def _cascade_kernel(s_x, s_y, s_z, shape):
     the spectra cached per class shape; an entry is rebuilt if the synthesis
     function was replaced or the tensor written in place."""
     synth = _sim._turbulon_envelope
-    key = (float(s_x), float(s_y), float(s_z), float(_sim.SUPPORT_FACTOR), shape, id(synth))
+    expo = float(_sim._turbulons.POWER_LAW_EXPONENT) if shape == 'power_law' else None
+    key = (float(s_x), float(s_y), float(s_z), float(_sim.SUPPORT_FACTOR), shape, expo,
+           id(synth))
     hit = _kernel_memo.get(key)
     if hit is not None and hit[0]._version == hit[2] and hit[3] is synth:
         return hit[0], hit[1]

@@ -17,6 +17,7 @@ struct EnvParams {
   float sig2;   // float32((k/pi)^2)
   float g2;     // float32(2 (k/pi)^2)
   float omega;  // float32(6/k)
+  float pexp;   // float32(-p/2): power law (1 + rho)^(-p/2)
   int shape;
 };
 
@@ -36,8 +37,10 @@ __device__ __forceinline__ void env_tap(const EnvParams& p, int64_t idx,
   const float half = expf(__fdiv_rn(-ratio, 2.0f));
   if (p.shape == STEAM_SHAPE_MEXICAN_HAT)
     env = __fmul_rn(__fsub_rn(3.0f, ratio), half);
-  else
+  else if (p.shape == STEAM_SHAPE_MORLET_OMEGA0_6)
     env = __fmul_rn(cosf(__fmul_rn(p.omega, sqrtf(r2))), half);
+  else
+    env = powf(__fadd_rn(1.0f, ratio), p.pexp);
   gauss = expf(__fdiv_rn(-r2, p.g2));
 }
 
@@ -176,7 +179,8 @@ extern "C" int steam_envelope_halfwidths(double k, double dx, double dy,
 
 int steam::envelope_setup(double k, double dx, double dy, double dz, double sf,
                           int shape_id, const float* out, EnvParams& p, int64_t& n) {
-  if (shape_id != STEAM_SHAPE_MEXICAN_HAT && shape_id != STEAM_SHAPE_MORLET_OMEGA0_6)
+  if (shape_id != STEAM_SHAPE_MEXICAN_HAT && shape_id != STEAM_SHAPE_MORLET_OMEGA0_6 &&
+      shape_id != STEAM_SHAPE_POWER_LAW)
     return set_error(STEAM_EINVAL, "envelope: unknown shape");
   int64_t h[3];
   int rc = steam_envelope_halfwidths(k, dx, dy, dz, sf, h);
@@ -194,18 +198,43 @@ int steam::envelope_setup(double k, double dx, double dy, double dz, double sf,
   p.sig2 = (float)(sig * sig);
   p.g2 = (float)(2.0 * sig * sig);
   p.omega = (float)(6.0 / k);
+  p.pexp = 0.f;
   p.shape = shape_id;
   n = (2 * h[0] + 1) * (2 * h[1] + 1) * (2 * h[2] + 1);
   return STEAM_OK;
 }
 
+static int envelope_allocating(const EnvParams& p, int64_t n, float* out,
+                               steam_stream_t stream);
+
 extern "C" int steam_turbulon_envelope(double k, double dx, double dy, double dz,
                                        double sf, int shape_id, float* out,
                                        steam_stream_t stream) {
+  if (shape_id == STEAM_SHAPE_POWER_LAW)
+    return set_error(STEAM_EINVAL, "envelope: the power-law shape takes its exponent "
+                                   "(steam_turbulon_envelope_power)");
   EnvParams p;
   int64_t n = 0;
   int rc = envelope_setup(k, dx, dy, dz, sf, shape_id, out, p, n);
   if (rc) return rc;
+  return envelope_allocating(p, n, out, stream);
+}
+
+extern "C" int steam_turbulon_envelope_power(double k, double dx, double dy, double dz,
+                                             double sf, double exponent, float* out,
+                                             steam_stream_t stream) {
+  if (!(exponent > 0.0) || !(exponent < 16.0))
+    return set_error(STEAM_EINVAL, "envelope: power-law exponent must lie in (0, 16)");
+  EnvParams p;
+  int64_t n = 0;
+  int rc = envelope_setup(k, dx, dy, dz, sf, STEAM_SHAPE_POWER_LAW, out, p, n);
+  if (rc) return rc;
+  p.pexp = (float)(-0.5 * exponent);
+  return envelope_allocating(p, n, out, stream);
+}
+
+static int envelope_allocating(const EnvParams& p, int64_t n, float* out,
+                               steam_stream_t stream) {
   const int nblk = (int)std::min<int64_t>((n + 255) / 256, kEnvMaxWgs);
   hipStream_t s = (hipStream_t)stream;
   double* part = nullptr;
@@ -221,6 +250,9 @@ extern "C" int steam_turbulon_envelope(double k, double dx, double dy, double dz
 extern "C" int steam_turbulon_envelope_noalloc(double k, double dx, double dy, double dz,
                                                double sf, int shape_id, float* out,
                                                steam_stream_t stream) {
+  if (shape_id == STEAM_SHAPE_POWER_LAW)
+    return set_error(STEAM_EINVAL, "envelope_noalloc: the power-law shape takes its "
+                                   "exponent (steam_turbulon_envelope_power)");
   EnvParams p;
   int64_t n = 0;
   int rc = envelope_setup(k, dx, dy, dz, sf, shape_id, out, p, n);

@@ -381,7 +381,8 @@ def refine_fields(parent, x_start, x_stop, y_start, y_stop, dx, dy, seed=None,
         # the record shares the returned state arrays
         record = nest_parent_record(p, plan, out, z_final,
                                     nest_class_ladder(p, plan, own, on_device=as_tensors))
-    out.update(ledger=run_ledger, grids=plan['grids'], plan=plan)
+    out.update(ledger=run_ledger, grids=plan['grids'], plan=plan,
+               z=z_final.astype(np.float64))
     if record is not None:
         out['parent'] = record
     return out

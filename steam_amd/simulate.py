@@ -294,11 +294,17 @@ def _turbulon_envelope(k, dx, dy, dz, support_factor=SUPPORT_FACTOR,
                                              half), "steam_envelope_halfwidths")
     dims = tuple(2 * int(h) + 1 for h in half)
     out = torch.empty(dims, dtype=torch.float32, device='cuda')
-    _lib.check(lib.steam_turbulon_envelope(float(k), float(dx), float(dy),
-                                           float(dz), float(support_factor),
-                                           _lib.SHAPE_IDS[shape], _lib.ptr(out),
-                                           _lib.stream_handle()),
-               "steam_turbulon_envelope")
+    if shape == 'power_law':
+        _lib.check(lib.steam_turbulon_envelope_power(
+            float(k), float(dx), float(dy), float(dz), float(support_factor),
+            float(_turbulons.POWER_LAW_EXPONENT), _lib.ptr(out), _lib.stream_handle()),
+            "steam_turbulon_envelope_power")
+    else:
+        _lib.check(lib.steam_turbulon_envelope(float(k), float(dx), float(dy),
+                                               float(dz), float(support_factor),
+                                               _lib.SHAPE_IDS[shape], _lib.ptr(out),
+                                               _lib.stream_handle()),
+                   "steam_turbulon_envelope")
     return out if as_tensor else out.cpu().numpy()
 
 
@@ -663,4 +669,13 @@ def __getattr__(name):
         import importlib
         mod, attr = _LAZY[name]
         return getattr(importlib.import_module(f'{__package__}.{mod}'), attr)
+    raise AttributeError(f"module {__name__!r} has no attribute {name!r}")
+
+
+def __getattr__(name):
+    """``simulate`` and ``refine`` (the file-level API, steam/simulate.py:729,
+    3682) live in steam_amd.api and are reached here as in the reference."""
+    if name in ('simulate', 'refine'):
+        from . import api
+        return getattr(api, name)
     raise AttributeError(f"module {__name__!r} has no attribute {name!r}")

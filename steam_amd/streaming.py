@@ -233,8 +233,9 @@ def streamed_flux_cascade(grids, tiles, sparsity_factors=(1, 1, 1), seed=None,
                           alpha=None, support_factor=None, return_tensor=False):
     """The unit-mean flux cascade over root grids, tile by tile.
 
-    ``tiles`` is (tiles_x, tiles_y) for every class, or a list of such pairs
-    (one per class); a count above a class's extent is clamped to it.
+    ``tiles`` is a TilePlan (plan_tiling), (tiles_x, tiles_y) for every
+    class, or a list of such pairs (one per class); a count above a class's
+    extent is clamped to it.
     Returns ``(flux, diagnostics)`` like ``simulate_flux_only``; the result
     equals the resident cascade to rounding (the interiors are exact; the
     sums are reassociated over tiles).  An axis in one tile takes no halo:
@@ -252,7 +253,9 @@ def streamed_flux_cascade(grids, tiles, sparsity_factors=(1, 1, 1), seed=None,
                                 n_scale_classes_per_dyad, min_distance_to_ground)
     s_x, s_y, s_z = sparsity_factors
     n_classes = len(grids['k'])
-    if isinstance(tiles, (list, tuple)) and len(tiles) and isinstance(tiles[0], (list, tuple)):
+    if isinstance(tiles, TilePlan):
+        per_class_tiles = [tiles.for_class(i) for i in range(n_classes)]
+    elif isinstance(tiles, (list, tuple)) and len(tiles) and isinstance(tiles[0], (list, tuple)):
         per_class_tiles = list(tiles)
         if len(per_class_tiles) != n_classes:
             raise ValueError("one (tiles_x, tiles_y) pair per class expected")
@@ -335,15 +338,38 @@ def streamed_flux_cascade(grids, tiles, sparsity_factors=(1, 1, 1), seed=None,
     return (state if return_tensor else state.cpu().numpy()), diagnostics
 
 
+class ResidentStore:
+    """What the streamed cascade returns in place of the reference's scratch
+    ``TileStore`` (steam/streaming.py:276-622): the run's fields stay in
+    device memory, so there is nothing on disk to own.  ``destroy()`` is
+    accepted and does nothing; ``scratch_dir`` is the directory the caller
+    passed (never written)."""
+
+    def __init__(self, scratch_dir=None):
+        self.scratch_dir = scratch_dir
+        self.destroyed = False
+
+    def destroy(self):
+        self.destroyed = True
+
+
 def streamed_cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k, b_h_k,
                           b_qt_k, aspect_k, h_min, h_max, qt_min, qt_max,
-                          min_distance_to_ground, sparsity_factors, class_seeds, tiles,
-                          n_scale_classes_per_dyad=1, flux_noise_scale=None,
+                          min_distance_to_ground, sparsity_factors, class_seeds, plan,
+                          scratch_dir=None, n_scale_classes_per_dyad=1, flux_noise_scale=None,
                           turbulon_shape='mexican_hat', zero_bottom=True, zero_top=True,
-                          device='cuda', comp_k=None, as_tensors=False):
+                          device='cuda', comp_k=None, increment_store=None, completed=None,
+                          _crash_hook=None, as_tensors=False):
     """The whole root cascade (flux, h, qt and the compensation deficits)
     tile by tile: the passes of steam/streaming.py:850-1447 over a state
     held in HBM.
+
+    Call-compatible with the reference's driver as ``simulate`` calls it
+    (steam/simulate.py:1380-1404): ``plan`` and ``scratch_dir`` positional,
+    the rest by keyword, and the reference's 9-tuple returned.  The state
+    never goes to disk, so ``scratch_dir`` is not written and the store
+    returned is a ``ResidentStore``; the disk features (``increment_store``,
+    a resume from ``completed``, ``_crash_hook``) raise NotImplementedError.
 
     Per class: the carried fields are regridded window by window; a first
     pass over the tiles (tile plus halo, in world raster order) advances the
@@ -355,9 +381,16 @@ def streamed_cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k
     (the bounded add of each scalar, the flux rescale, the deficits) then
     run on the whole fields, which a level at a time is what the reference's
     plane pass does.  ``tiles`` is a TilePlan (plan_tiling), (tiles_x,
-    tiles_y) for every class, or one pair per class.  Returns the 8-tuple of ``cascade_loop``.  The disk
-    store, resume protocol and tiling planner are not part of this path.
+    tiles_y) for every class, or one pair per class.  Returns (h, qt, flux,
+    deficit_h, deficit_qt, deficit_flux, final_grid_info, ledger, store).
     """
+    for name, value in (('increment_store', increment_store), ('completed', completed),
+                        ('_crash_hook', _crash_hook)):
+        if value is not None:
+            raise NotImplementedError(
+                f"streamed_cascade_loop: {name} belongs to the on-disk scratch store and "
+                f"its resume protocol; this cascade keeps its state in device memory")
+    tiles = plan
     from .cascade import _own, deficit_accumulate
     from .params import _interpolation_compensation
     from .scalars import (LevelReducer, ZDifferences, bounded_amplitude_add, joint_pattern,
@@ -516,4 +549,4 @@ def streamed_cascade_loop(h_profile, qt_profile, z_profile, grids, C_h_k, C_qt_k
            fields.get('dflux')]
     if not as_tensors:
         out = [None if a is None else a.cpu().numpy() for a in out]
-    return (*out, final, ledger)
+    return (*out, final, ledger, ResidentStore(scratch_dir))

@@ -58,6 +58,26 @@ def test_gpu_root_run_replays_bit_for_bit():
         np.testing.assert_array_equal(fresh[field], first[field], err_msg=field)
 
 
+def test_gpu_root_run_replays_from_a_ledger_file(tmp_path):
+    """The run's ledger written to an npz file and read back drives an
+    apply-only replay bit-identical to the recorded run; one class's file
+    (the reference's per-class format) reads back the same entries."""
+    from steam_amd.ledger import (load_class_ledger, load_run_ledger, save_class_ledger,
+                                  save_run_ledger)
+    first = gpu_fields((2, 2, 2), 1)
+    path = str(tmp_path / "run_ledger.npz")
+    save_run_ledger(path, first['ledger'])
+    again = gpu_fields((2, 2, 2), 1, ledger=load_run_ledger(path))
+    for field in FIELDS:
+        np.testing.assert_array_equal(again[field], first[field], err_msg=field)
+    cpath = str(tmp_path / "class_002.npz")
+    save_class_ledger(cpath, first['ledger'].classes[2])
+    back = load_class_ledger(cpath)
+    assert back.flux.realized == first['ledger'].classes[2].flux.realized
+    np.testing.assert_array_equal(back.bounded_add['qt'].mu,
+                                  first['ledger'].classes[2].bounded_add['qt'].mu)
+
+
 def _projection_case(seed=3, shape=(96, 96, 24)):
     rng = np.random.default_rng(seed)
     lo, hi = 0.0, 0.03
@@ -415,3 +435,49 @@ def test_cascade_kernel_memo_is_keyed_and_invalidated(monkeypatch):
     assert not n1.flags.writeable
     with pytest.raises(ValueError):
         n1[0, 0, 0] = 1.0
+
+
+def test_file_api_simulate_then_refine(tmp_path):
+    """simulate() and refine() with the reference's signatures, through the
+    grouped npz file: the root group holds the fields of simulate_fields
+    bit for bit and matches the reference's fixture; refine() reads the
+    parent group and appends refinements/r0, which matches the reference's
+    nest fixture; a second refine appends r1; the stored ledger replays."""
+    from steam_amd import cascade as C
+    from steam_amd import files as F
+    from steam_amd.api import read_ledger
+    from steam_amd.simulate import refine, simulate
+    path = tmp_path / "run.npz"
+    got = simulate(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0, 800.0, 30.0,
+                   path, seed=42, save_for_refinement=True, device='cuda')
+    assert got == path
+    root = F.read_group(path, '/')
+    direct = C.simulate_fields(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0,
+                               800.0, 30.0, seed=42)
+    ref = np.load(os.path.join(GOLDEN, "root_dyadic.npz"))
+    for field in ('h', 'qt', 'flux'):
+        np.testing.assert_array_equal(root[field], direct[field], err_msg=field)
+        assert rel(root[field], ref[field]) < TOL[field], field
+    assert root['x'].shape == (16,) and root['z'].shape == (root['h'].shape[2],)
+    refine(path, 4, 12, 4, 12, 62.5, 62.5)
+    assert F.groups(path) == ['/', 'refinements/r0']
+    nest = F.read_group(path, 'refinements/r0')
+    nref = np.load(os.path.join(GOLDEN, "nest.npz"))
+    for field in ('h', 'qt', 'flux'):
+        assert nest[field].shape == nref[field].shape, field
+        assert rel(nest[field], nref[field]) < TOL[field], field
+    refine(path, 2, 6, 2, 6, 62.5, 62.5)
+    assert F.groups(path) == ['/', 'refinements/r0', 'refinements/r1']
+    # the root's stored ledger drives an apply-only replay of the root
+    again = C.simulate_fields(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0,
+                              800.0, 30.0, seed=42, ledger=read_ledger(path, '/'))
+    for field in ('h', 'qt', 'flux'):
+        np.testing.assert_array_equal(again[field], root[field], err_msg=field)
+    with pytest.raises(NotImplementedError):
+        simulate(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0, 800.0, 30.0,
+                 tmp_path / "s.npz", seed=42, stream_to_disk=True)
+    plain = tmp_path / "plain.npz"
+    simulate(H_PROFILE, QT_PROFILE, 16, 16, 125.0, 125.0, 2000.0, 100.0, 800.0, 30.0,
+             plain, seed=42)
+    with pytest.raises(ValueError, match="flux_state"):
+        refine(plain, 4, 12, 4, 12, 62.5, 62.5)

@@ -196,3 +196,49 @@ def test_pregenerated_noise_and_chained_rescale_are_bit_identical(shape):
     assert np.float32(L2.rescale) == sigma
     torch.cuda.synchronize()
     np.testing.assert_array_equal(f1.cpu().numpy(), f2.cpu().numpy())
+
+
+@pytest.mark.parametrize("p,sf,s", [(3.0 / 1.8, 4, 1), (3.0 - 1.0 / 3.0, 4, 1), (3.0 / 1.6, 2, 2)])
+def test_power_law_kernel_matches_oracle(p, sf, s):
+    """The power-law turbulon shape on the GPU against the oracle's float64
+    form (an addition with no reference counterpart; the oracle's form is
+    pinned by its closed form in the CPU tests)."""
+    from steam_amd import simulate as S
+    from steam_amd import turbulons as T
+    old = T.POWER_LAW_EXPONENT
+    try:
+        T.POWER_LAW_EXPONENT = p
+        d = 1.0 / (2 * s)
+        g = S._turbulon_envelope(1, d, d, d, support_factor=sf, shape='power_law')
+    finally:
+        T.POWER_LAW_EXPONENT = old
+    ref = M.turbulon_envelope(1, d, d, d, support_factor=sf, shape='power_law',
+                              dtype=np.float64, exponent=p)
+    assert g.shape == ref.shape and g.dtype == np.float32
+    assert np.abs(g - ref).max() < 2e-6 * np.abs(ref).max()
+    assert abs(float(g.astype(np.float64).sum())) < 1e-5 * np.abs(ref).sum()
+
+
+def test_plain_cpu_config_with_the_power_law_kernel():
+    """The CPU-runnable configuration: 64x64x32, alpha 1.8, C1 0.1, a 17^3
+    power-law kernel |r|^(-d/alpha) (support factor 4 at s = 1), one class
+    advance on the GPU against the float64 oracle on the same keyed
+    uniforms."""
+    from steam_amd.simulate import _advance_flux
+    shape = (64, 64, 32)
+    p = 3.0 / 1.8
+    c = 0.1 ** (1 / 1.8)
+    key = M.class_key(np.random.SeedSequence(0).spawn(1)[0])
+    kern = M.turbulon_envelope(1, .5, .5, .5, support_factor=4, shape='power_law',
+                               dtype=np.float64, exponent=p)
+    assert kern.shape == (17, 17, 17)
+    rng = np.random.default_rng(0)
+    flux = np.abs(1 + 0.3 * rng.standard_normal(shape)).astype(np.float32)
+    fo, so, lo, _ = M.advance_flux(flux, key, (0, 0, 0), shape, kern, c, 1, (1, 1, 1),
+                                   alpha=1.8)
+    fg = flux.copy()
+    sk, led, _ = _advance_flux(fg, _region(key, (0, 0, 0), shape), kern.astype(np.float32),
+                               c, 1, (1, 1, 1))
+    assert _rel(fg, fo) < ADVANCE_REL_L2
+    assert _rel(sk, so) < ADVANCE_REL_L2
+    assert led.noise_abs.count == lo['noise_abs'][1]

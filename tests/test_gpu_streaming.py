@@ -185,3 +185,62 @@ def test_streamed_full_cascade_from_a_tile_plan(full_resident):
     for a, b in zip(out[:3], res[:3]):
         rel = np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64))
         assert rel < 1e-5
+
+
+# the reference's committed fixtures, streamed with forced tilings (the
+# reference's own _force_tiling tests, tests/test_streaming.py:293-390, on
+# its in-RAM fixtures): only classes at least as wide as the kernel can be
+# split, so the finest class of each ladder streams
+FIXTURE_PLANS = [("root_dyadic", (1, 1, 1), 1, (3, 2, 2)),
+                 ("root_dyadic", (1, 1, 1), 1, (3, 4, 3)),
+                 ("root_s2", (2, 2, 2), 1, (3, 2, 2)),
+                 ("root_half", (1, 1, 1), 2, (6, 2, 2)),
+                 ("root_cuda", (1, 1, 1), 1, (3, 3, 2))]
+
+
+@pytest.mark.parametrize("name,s,n,force", FIXTURE_PLANS)
+def test_streamed_cascade_matches_reference_fixtures(name, s, n, force):
+    """streamed_cascade_loop, called as the reference's simulate calls it
+    (plan and scratch directory positional, the rest by keyword), against the
+    cascade states of the reference's fixtures at the resident tests'
+    tolerances."""
+    import os
+    from conftest import GOLDEN
+    from steam_amd.streaming import ResidentStore, plan_tiling, streamed_cascade_loop
+    from test_oracle_cascade import TOL, fixture_setup, rel
+    st = fixture_setup(s, n)
+    args, kw = _cascade_args(st)
+    kshape = (13, 13, 13) if s == (1, 1, 1) else (25, 25, 25)
+    plan = plan_tiling(st['grids'], kshape, force=force)
+    assert plan.tiles, "the finest class must stream"
+    out = streamed_cascade_loop(*args, plan, "/nonexistent-scratch",
+                                n_scale_classes_per_dyad=n, turbulon_shape='mexican_hat',
+                                device='cuda', comp_k=kw['comp_k'], increment_store=None,
+                                completed=None, _crash_hook=None)
+    assert len(out) == 9 and isinstance(out[8], ResidentStore)
+    out[8].destroy()
+    ref = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    for k, field in ((0, 'h_perturbation'), (1, 'qt_perturbation'), (2, 'flux_state')):
+        assert out[k].shape == ref[field].shape, field
+        assert rel(out[k], ref[field]) < TOL[field], (name, force, field, rel(out[k], ref[field]))
+
+
+@pytest.mark.parametrize("name,s,force", [("root_dyadic", (1, 1, 1), (3, 2, 2)),
+                                          ("root_s2", (2, 2, 2), (3, 2, 2)),
+                                          ("root_dyadic", (1, 1, 1), (3, 4, 4))])
+def test_streamed_flux_cascade_matches_reference_fixtures(name, s, force):
+    """The flux-only streamed cascade against the fixtures' flux_state (the
+    flux path of the fixture run, simulate_flux_only semantics)."""
+    import os
+    from conftest import GOLDEN
+    from steam_amd.streaming import plan_tiling, streamed_flux_cascade
+    from test_oracle_cascade import TOL, fixture_setup, rel
+    st = fixture_setup(s, 1)
+    kshape = (13, 13, 13) if s == (1, 1, 1) else (25, 25, 25)
+    plan = plan_tiling(st['grids'], kshape, force=force)
+    f, d = streamed_flux_cascade(st['grids'], plan, sparsity_factors=s, seed=42,
+                                 min_distance_to_ground=1, zero_bottom=True, zero_top=True)
+    ref = np.load(os.path.join(GOLDEN, f"{name}.npz"))['flux_state']
+    assert f.shape == ref.shape
+    assert rel(f, ref) < TOL['flux_state'], (name, force, rel(f, ref))
+    assert d['steps'][-1]['tiles'] == (force[1], force[2])

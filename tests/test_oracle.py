@@ -406,3 +406,34 @@ def test_plan_tiling():
             'nz': np.array([16])}
     plan = ST.plan_tiling(wide, (13, 13, 13), budget_bytes=1 << 50)
     assert plan.horizon == 0 and -(-8192 // plan.tiles[0][0]) + 14 <= ST.MAX_PLAN_AXIS
+
+
+@pytest.mark.parametrize("p,s", [(3.0 / 1.8, 1), (3.0 - 1.0 / 3.0, 1), (3.0 / 1.6, 2)])
+def test_power_law_envelope_closed_form(p, s):
+    """The power-law shape (an addition with no reference counterpart):
+    before the zero-sum correction T = (1 + rho)^(-p/2) exactly, so the
+    corrected kernel plus c times the Gaussian is that closed form for one
+    constant c; T(0) = 1, T(r = k/pi) = 2^(-p/2); the corrected kernel sums to
+    zero and is symmetric; the tail's log-slope tends to -p."""
+    k = 1.0
+    d = 1.0 / (2 * s)
+    env = M.turbulon_envelope(k, d, d, d, support_factor=4, shape='power_law',
+                              dtype=np.float64, exponent=p)
+    h = env.shape[0] // 2
+    x = (np.arange(env.shape[0]) - h) * d
+    r2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
+    sig2 = (k / np.pi) ** 2
+    g = np.exp(-r2 / (2 * sig2))
+    c = 1.0 - env[h, h, h]             # T(0) = 1 before the correction
+    closed = env + c * g
+    np.testing.assert_allclose(closed, (1 + r2 / sig2) ** (-p / 2), rtol=1e-12, atol=1e-14)
+    assert abs(env.sum()) < 1e-9 * np.abs(env).sum()
+    np.testing.assert_array_equal(env, env[::-1, ::-1, ::-1])
+    np.testing.assert_array_equal(env, env.transpose(1, 0, 2))
+    # the oracle's own taps along an axis: the log-slope of the far tail is -p
+    # (a wrong sign, r for r^2 or a missing core shows here)
+    i1, i2 = h + int(round(2.5 / d)), h + int(round(3.5 / d))
+    slope = (np.log(closed[i2, h, h]) - np.log(closed[i1, h, h])) / np.log(x[i2] / x[i1])
+    assert slope == pytest.approx(-p, rel=0.03)
+    # isotropic in index space: equal radii, equal taps
+    assert closed[h + 2, h + 1, h] == closed[h + 1, h, h + 2]
