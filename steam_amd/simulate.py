@@ -661,6 +661,9 @@ _LAZY = {
     'HAAR_TO_MHAT': ('params', 'HAAR_TO_MHAT'),
     '_advective_weight': ('scalars', 'advective_weight'),
     '_bounded_amplitude_add': ('scalars', 'bounded_amplitude_add'),
+    # the file-level API (steam/simulate.py:729, 3682)
+    'simulate': ('api', 'simulate'),
+    'refine': ('api', 'refine'),
 }
 
 
@@ -669,13 +672,4 @@ def __getattr__(name):
         import importlib
         mod, attr = _LAZY[name]
         return getattr(importlib.import_module(f'{__package__}.{mod}'), attr)
-    raise AttributeError(f"module {__name__!r} has no attribute {name!r}")
-
-
-def __getattr__(name):
-    """``simulate`` and ``refine`` (the file-level API, steam/simulate.py:729,
-    3682) live in steam_amd.api and are reached here as in the reference."""
-    if name in ('simulate', 'refine'):
-        from . import api
-        return getattr(api, name)
     raise AttributeError(f"module {__name__!r} has no attribute {name!r}")
