@@ -227,7 +227,12 @@ class TileServer:
     def serve(self, trace, keep_results=False):
         """Run a request trace in real time.  Returns (latencies_s,
         wall_s, voxels, results) with results[id] = device tile when
-        ``keep_results``."""
+        ``keep_results`` (True: every request; a set: those ids)."""
+        wanted = keep_results if isinstance(keep_results, (set, frozenset)) else None
+        keep_results = bool(keep_results)
+
+        def keep(rid):
+            return keep_results and (wanted is None or rid in wanted)
         pending = deque(sorted(trace, key=lambda r: r["arrival"]))
         queue = deque()
         inflight = []
@@ -250,7 +255,7 @@ class TileServer:
                     lat[f.req["id"]] = done - f.req["arrival"]
                     voxels += f.out.numel()
                     self._cache_put((f.req["origin"], f.req["shape"]), f.out)
-                    if keep_results:
+                    if keep(f.req["id"]):
                         results[f.req["id"]] = f.out
                     used -= f.bytes
                     f.flux = None
@@ -269,7 +274,7 @@ class TileServer:
                     self.stats["cache_hits"] += 1
                     lat[req["id"]] = (time.perf_counter() - t0) - req["arrival"]
                     voxels += hit.numel()
-                    if keep_results:
+                    if keep(req["id"]):
                         results[req["id"]] = hit
                     continue
                 nb = self.request_bytes(self.world, req["shape"])
