@@ -300,6 +300,17 @@ int steam_batch_destroy(steam_batch* batch);
 int steam_batch_set_noise(steam_batch* batch, int32_t i,
                           const steam_noise_params* noise);
 
+/* The noise fields of the instances created with a noise_dev buffer, for
+ * their next advance: every such instance's multiplier noise (its current
+ * keyed-noise parameters) written into its buffer, one launch per variant,
+ * after a descriptor upload; graph-capturable like steam_batch_advance.  A
+ * caller runs it on another stream ahead of the advance that reads the
+ * buffers, so the VALU-bound noise of one step overlaps the memory-bound
+ * transforms of the step before.  No counterpart in the reference (which
+ * draws each instance's noise inside its own advance,
+ * steam/simulate.py:2146-2161). */
+int steam_batch_noise(steam_batch* batch, steam_stream_t stream);
+
 /* Mark, on stream, the point after a replay of a captured
  * steam_batch_advance.  A replayed graph copies the pinned descriptors when
  * the replay runs, not when it was captured, so the next

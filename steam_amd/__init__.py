@@ -19,7 +19,8 @@ Python API:
     steam_amd.cascade     cascade_loop, project_onto_bounds, compose_output,
                           compose_flux_output, simulate_fields
     steam_amd.api         simulate, refine (the reference's signatures; the run
-                          written to a grouped npz file), read_ledger
+                          written to a grouped npz file), read_ledger; also
+                          reached as steam_amd.simulate.simulate / .refine
     steam_amd.files       the grouped npz file (NetCDF-style groups)
     steam_amd.params      _compute_normalization, _bound_buffers,
                           _compensation_profiles, root_setup
@@ -37,18 +38,6 @@ from . import _lib  # noqa: F401  (binds torch's HIP runtime first)
 
 def library_version():
     return _lib.load().steam_version()
-
-
-def simulate(*args, **kwargs):
-    """steam.simulate (steam/__init__.py:6): see steam_amd.api.simulate."""
-    from .api import simulate as _simulate
-    return _simulate(*args, **kwargs)
-
-
-def refine(*args, **kwargs):
-    """steam.refine (steam/__init__.py:6): see steam_amd.api.refine."""
-    from .api import refine as _refine
-    return _refine(*args, **kwargs)
 
 
 def cascade_loop(*args, **kwargs):

@@ -116,11 +116,20 @@ class BatchedAdvance:
     any advance (``ledgers``).
     """
 
-    def __init__(self, instances, autotune=True):
+    def __init__(self, instances, autotune=True, noise_buffers=None, workspaces=None):
         lib = _lib.load()
         self.instances = list(instances)
         if not self.instances:
             raise ValueError("empty batch")
+        if noise_buffers is not None:
+            noise_buffers = list(noise_buffers)
+            if len(noise_buffers) != len(self.instances) or any(
+                    tuple(b.shape) != tuple(inst.flux.shape) or b.dtype != torch.float32
+                    or not b.is_cuda or not b.is_contiguous()
+                    for b, inst in zip(noise_buffers, self.instances)):
+                raise ValueError("one contiguous float32 device noise buffer of the "
+                                 "instance's shape per instance expected")
+        self.noise_buffers = noise_buffers
         self.ops, self.specs, self.workspaces = [], [], []
         self.n_fft = []
         items = (_lib.BatchItem * len(self.instances))()
@@ -129,7 +138,12 @@ class BatchedAdvance:
             n_fft = autotune_nfft(shape, kshape)[0] if autotune else 0
             op = conv_operator(shape, kshape, n_fft=n_fft)
             spec = op.spectrum(inst.kernel, key=kernel_key(inst.kernel))
-            ws = torch.empty(op.workspace_bytes, dtype=torch.uint8, device='cuda')
+            if workspaces is not None:
+                ws = workspaces[i]
+                if ws.numel() < op.workspace_bytes:
+                    raise ValueError("a shared workspace is smaller than the plan needs")
+            else:
+                ws = torch.empty(op.workspace_bytes, dtype=torch.uint8, device='cuda')
             self.ops.append(op)
             self.specs.append(spec)
             self.workspaces.append(ws)
@@ -139,7 +153,7 @@ class BatchedAdvance:
             it.spectrum_dev = spec.data_ptr()
             it.workspace_dev = ws.data_ptr()
             it.flux_dev = inst.flux.data_ptr()
-            it.noise_dev = None
+            it.noise_dev = None if noise_buffers is None else noise_buffers[i].data_ptr()
             it.sk_dev = None
             it.noise = inst.native_noise()
             if inst.window is not None:
@@ -188,6 +202,17 @@ class BatchedAdvance:
                                                    _lib.stream_handle(stream)),
                    "steam_batch_advance")
 
+    def generate_noise(self, stream=None):
+        """Write every instance's noise for its next advance into its noise
+        buffer (batches created with ``noise_buffers``); asynchronous."""
+        _lib.check(_lib.load().steam_batch_noise(self.handle, _lib.stream_handle(stream)),
+                   "steam_batch_noise")
+
+    def mark_replay(self, stream=None):
+        _lib.check(_lib.load().steam_batch_mark_replay(self.handle,
+                                                       _lib.stream_handle(stream)),
+                   "steam_batch_mark_replay")
+
     def capture(self):
         """One advance captured into a hipGraph (torch CUDAGraph), returned as
         a ``CapturedAdvance`` whose ``replay`` marks each replay for
@@ -233,6 +258,108 @@ class CapturedAdvance:
         _lib.check(_lib.load().steam_batch_mark_replay(self.batch.handle,
                                                        _lib.stream_handle()),
                    "steam_batch_mark_replay")
+
+
+class PipelinedBatch:
+    """The batched advance with each step's noise generated during the step
+    before.  Two batches over the same instances (flux and kernel spectra
+    shared) read precomputed noise from two buffer sets: step i advances
+    with batch i % 2 on the current stream while, on a side stream, batch
+    (i + 1) % 2 writes the noise of step i + 1 (VALU-bound work beside
+    memory-bound transforms).  Each step is one fork and join, so a step
+    captures into one graph (``capture``: one graph per parity, replayed in
+    turn).  Results equal the plain batched advance with the same keys.
+
+    ``set_regions(regions)`` gives the keys of the next step whose noise is
+    not yet generated (step i + 1 while step i is queued)."""
+
+    def __init__(self, instances, autotune=True):
+        import copy
+        self.instances = list(instances)
+        self.buffers = [[torch.empty_like(inst.flux) for inst in self.instances]
+                        for _ in range(2)]
+        # every step leaves its volume-mean rescale pending in the shared
+        # workspace and the next step applies it on read (no separate rescale
+        # pass); the first step has nothing pending to read
+        chained, first = [], []
+        for inst in self.instances:
+            c = copy.copy(inst)
+            c.defer_rescale, c.chain_pending = True, True
+            chained.append(c)
+            f = copy.copy(inst)
+            f.defer_rescale, f.chain_pending = True, False
+            first.append(f)
+        b0 = BatchedAdvance(chained, autotune=autotune, noise_buffers=self.buffers[0])
+        b1 = BatchedAdvance(chained, autotune=autotune, noise_buffers=self.buffers[1],
+                            workspaces=b0.workspaces)
+        self.first = BatchedAdvance(first, autotune=autotune, noise_buffers=self.buffers[0],
+                                    workspaces=b0.workspaces)
+        self.batches = [b0, b1]
+        self.side = torch.cuda.Stream()
+        self.step = 0
+        self.graphs = None
+        self.first.generate_noise()   # step 0's noise
+
+    @property
+    def voxels(self):
+        return self.batches[0].voxels
+
+    @property
+    def launches_per_step(self):
+        """The advance's launches plus the next step's noise (its descriptor
+        upload and one noise launch: every instance shares the noise kernel)."""
+        return self.batches[0].launches_per_advance + 2
+
+    def set_regions(self, regions):
+        """Keys of step ``self.step + 1`` (the next noise to be generated)."""
+        self.batches[(self.step + 1) % 2].set_regions(regions)
+
+    def _work(self, b, first=False):
+        main = torch.cuda.current_stream()
+        self.side.wait_stream(main)          # buffers 1 - b were last read by step i - 1
+        with torch.cuda.stream(self.side):
+            self.batches[1 - b].generate_noise(self.side)
+        (self.first if first else self.batches[b]).advance(main)
+        main.wait_stream(self.side)
+
+    def advance(self):
+        """One step (eager)."""
+        self._work(self.step % 2, first=self.step == 0)
+        self.step += 1
+
+    def finish(self):
+        """Apply the last step's pending rescale to every flux (synchronises)."""
+        for inst, led in zip(self.instances, self.ledgers()):
+            r = led.rescale
+            if r is None:
+                inst.flux.fill_(np.float32(led.entering.mean))
+            else:
+                inst.flux.mul_(float(r))
+
+    def capture(self):
+        """Capture the two step graphs (parity 0 and 1), running nothing;
+        ``replay`` then runs the next step from them.  At least one eager
+        step must have run first (every kernel of the path loaded)."""
+        if self.step < 1:
+            raise ValueError("PipelinedBatch.capture: run one eager step first")
+        torch.cuda.synchronize()
+        self.graphs = [None, None]
+        for b in (0, 1):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._work(b)
+            self.graphs[b] = g
+        return self.graphs
+
+    def replay(self):
+        self.graphs[self.step % 2].replay()
+        for batch in self.batches:
+            batch.mark_replay()
+        self.step += 1
+
+    def ledgers(self):
+        """Each instance's ledger of the last step (synchronises)."""
+        return self.batches[0].ledgers()   # the workspaces are shared
 
 
 def draw_batch_config(n=48, seed=48):

@@ -9,6 +9,7 @@
 // are refreshed from a pinned host copy at the start of every advance, so
 // the whole advance can be captured once into a hipGraph and replayed with
 // new noise keys.
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -20,7 +21,10 @@ using namespace steam;
 
 namespace {
 
-enum BatchPass { BP_NOISE = 0, BP_Z, BP_Y, BP_X, BP_ZINV, BP_FIN, BP_POST, BP_COUNT };
+// BP_NOISE_EXT: the noise fields of the instances created with their own
+// noise buffer, generated by steam_batch_noise (not part of an advance)
+enum BatchPass { BP_NOISE = 0, BP_Z, BP_Y, BP_X, BP_ZINV, BP_FIN, BP_POST, BP_NOISE_EXT,
+                 BP_COUNT };
 
 struct Group {
   int offset;   // first descriptor of the group in the flat array
@@ -36,6 +40,7 @@ struct Group {
 struct ItemPlan {
   const steam_conv_plan* p;
   bool zreg, tile, gen_noise, post, post_rescale;
+  int wg_noise_ext;  // workgroups of its strided noise launch (steam_batch_noise)
 };
 
 }  // namespace
@@ -59,6 +64,7 @@ int wg_count(const ItemPlan& ip, int pass) {
   const ConvGeom& g = ip.p->g;
   switch (pass) {
     case BP_NOISE: return ip.p->wg_noise;
+    case BP_NOISE_EXT: return ip.wg_noise_ext;
     case BP_Z:
     case BP_ZINV: return ip.zreg ? g.wg_zr : g.wg_z;
     case BP_Y: return g.wg_y;
@@ -71,7 +77,8 @@ int wg_count(const ItemPlan& ip, int pass) {
 // pass slot used by the kernels' descriptor lookup
 int desc_pass(int pass) {
   switch (pass) {
-    case BP_NOISE: return PASS_NOISE;
+    case BP_NOISE:
+    case BP_NOISE_EXT: return PASS_NOISE;
     case BP_Z: return PASS_Z;
     case BP_Y: return PASS_Y;
     case BP_X: return PASS_X;
@@ -86,6 +93,7 @@ std::tuple<int, int, int> variant_of(const ItemPlan& ip, int pass) {
   const ConvGeom& g = ip.p->g;
   switch (pass) {
     case BP_NOISE: return {ip.gen_noise ? 0 : -1, 0, 256};
+    case BP_NOISE_EXT: return {ip.gen_noise ? -1 : 0, 0, 256};
     case BP_Z:
     case BP_ZINV:
       if (ip.zreg) return {1, 2 * g.n2 + (g.nz & 1), ZR_NTH};  // plan length, odd nz
@@ -111,9 +119,39 @@ size_t lds_of(const ItemPlan& ip, int pass) {
   }
 }
 
+// Workgroup cap of a batch's noise generated ahead: STEAM_BATCH_NOISE_WGS
+// from the environment when set (read once), else 768 (three per CU; a
+// 48-member batch of 146M cells measured 3.43 ms per pipelined step at 768
+// against 5.8 at 384 and 3.68 at 1536)
+static int batch_noise_wgs_cap() {
+  static const int cap = [] {
+    const char* e = getenv("STEAM_BATCH_NOISE_WGS");
+    return e && *e ? atoi(e) : 768;
+  }();
+  return cap;
+}
+
 void build_groups(steam_batch* b) {
   std::vector<PassDesc> flat;
   b->slot_item.clear();
+  // the noise generated ahead runs beside the transforms of the step
+  // before: its launch gets at most the cap's workgroups, shared out by
+  // instance size, each instance's workgroups striding over its lines
+  {
+    int64_t total = 0;
+    for (int i = 0; i < b->n; ++i)
+      if (!b->plans[i].gen_noise) total += b->plans[i].p->wg_noise;
+    const int cap = batch_noise_wgs_cap();
+    for (int i = 0; i < b->n; ++i) {
+      ItemPlan& ip = b->plans[i];
+      const int full = ip.p->wg_noise;
+      int n = full;
+      if (cap > 0 && total > cap)
+        n = (int)std::max<int64_t>(1, std::min<int64_t>(full, (int64_t)cap * full / total));
+      ip.wg_noise_ext = n;
+      b->items[i].wg_noise_n = n;
+    }
+  }
   for (int pass = 0; pass < BP_COUNT; ++pass) {
     std::map<std::tuple<int, int, int>, std::vector<int>> by;
     for (int i = 0; i < b->n; ++i) {
@@ -163,6 +201,9 @@ int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
     switch (pass) {
       case BP_NOISE:
         hipLaunchKernelGGL((noise_field_kernel<BatchSrc>), grid, dim3(256), 0, s, src);
+        break;
+      case BP_NOISE_EXT:
+        hipLaunchKernelGGL((noise_field_kernel<BatchSrc, true>), grid, dim3(256), 0, s, src);
         break;
       case BP_Z:
         if (gr.variant == 1) {
@@ -354,10 +395,27 @@ extern "C" int steam_batch_info(const steam_batch* b, int* launches,
   if (!b) return set_error(STEAM_EINVAL, "steam_batch_info: NULL batch");
   int n = 1;  // descriptor upload
   for (int p = 0; p < BP_COUNT; ++p)
-    n += (int)b->groups[p].size() * (p == BP_Y ? 2 : 1);
+    if (p != BP_NOISE_EXT) n += (int)b->groups[p].size() * (p == BP_Y ? 2 : 1);
   if (launches) *launches = n;
   if (descriptor_bytes) *descriptor_bytes = (int64_t)b->nslots * (int64_t)sizeof(PassDesc);
   return STEAM_OK;
+}
+
+extern "C" int steam_batch_noise(steam_batch* b, steam_stream_t stream) {
+  if (!b) return set_error(STEAM_EINVAL, "steam_batch_noise: NULL batch");
+  if (b->groups[BP_NOISE_EXT].empty())
+    return set_error(STEAM_EINVAL, "steam_batch_noise: no instance has its own noise buffer");
+  hipStream_t s = (hipStream_t)stream;
+  STEAM_HIP(hipMemcpyAsync(b->dev, b->host, (size_t)b->nslots * sizeof(PassDesc),
+                           hipMemcpyHostToDevice, s));
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  STEAM_HIP(hipStreamIsCapturing(s, &cs));
+  if (cs == hipStreamCaptureStatusNone) {
+    STEAM_HIP(hipEventRecord(b->uploaded, s));
+    b->pending_upload = true;
+  }
+  ProfScope ps(STEAM_KF_NOISE, s);
+  return launch_pass(b, BP_NOISE_EXT, s, 0);
 }
 
 extern "C" int steam_batch_advance(steam_batch* b, steam_stream_t stream) {

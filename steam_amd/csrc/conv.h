@@ -100,6 +100,8 @@ struct PassDesc {
   float kscale;         // kernel-spectrum normalisation (MODE_KERNEL)
   int npart;            // partial-sum slots written by the z passes
   int clip_win;         // count clipped cells inside the window only
+  int wg_noise_n;       // workgroups a strided noise launch gives this instance
+                        // (0: the whole grid)
   int wg_base[7];       // batched: first workgroup of z, y, x, zinv, post,
                         // finalize, noise passes
 };

@@ -527,9 +527,7 @@ void ledger_to_host(const DevLedger& L, steam_flux_ledger* h) {
 
 }  // namespace steam
 
-// Workgroup cap of a noise field generated ahead: STEAM_NOISE_WGS from the
-// environment when set (read once), else the compiled default.
-static int noise_wgs_cap() {
+int steam::noise_wgs_cap() {
   static const int cap = [] {
     const char* e = getenv("STEAM_NOISE_WGS");
     return e && *e ? atoi(e) : STEAM_NOISE_WGS;

@@ -202,7 +202,8 @@ __global__ __launch_bounds__(256) void noise_field_kernel(Src src) {
   const int nz = g.nz;
   const int64_t lines = (int64_t)g.nx * g.ny;
   const int lpw = 4;
-  const int64_t stride = STRIDE ? (int64_t)gridDim.x * lpw : lines;
+  const int64_t stride =
+      STRIDE ? (int64_t)(d.wg_noise_n > 0 ? d.wg_noise_n : (int)gridDim.x) * lpw : lines;
   for (int64_t base = (int64_t)wg * lpw; base < lines; base += stride) {
     for (int l = 0; l < lpw; ++l) {
       const int64_t line = base + l;

@@ -24,6 +24,11 @@ namespace steam {
 
 inline int64_t align256(int64_t v) { return (v + 255) & ~(int64_t)255; }
 
+// Workgroup cap of a noise field generated ahead, beside other work:
+// STEAM_NOISE_WGS from the environment when set (read once), else the
+// compiled default.
+int noise_wgs_cap();
+
 NoiseDev make_noise_dev(const steam_noise_params* np);
 int check_noise_params(const steam_noise_params* np);
 DevLedger ledger_to_dev(const steam_flux_ledger* h);

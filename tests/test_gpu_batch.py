@@ -164,3 +164,45 @@ def test_batch_rejects_bad_items():
     bad[0].window = (0, 999, 0, 4)
     with pytest.raises(ValueError):
         BatchedAdvance(bad, autotune=False)
+
+
+def test_pipelined_batch_equals_the_batched_advance():
+    """Noise generated a step ahead on a side stream into two buffer sets,
+    each step's rescale left pending and applied on the next step's read
+    (eager steps, then graph replays with new keys each step and no
+    synchronisation between them): the batched advance's result bit for
+    bit."""
+    from steam_amd import simulate as S
+    from steam_amd.batch import BatchedAdvance, PipelinedBatch
+    insts = _instances(seed=5)
+    twins = _instances(seed=5)
+    pipe = PipelinedBatch(insts, autotune=False)
+    plain = BatchedAdvance(twins, autotune=False)
+
+    def regions(step):
+        # step 0: the instances' own regions
+        if step == 0:
+            return [inst.region for inst in twins]
+        return [S.NoiseRegion(tuple(np.uint64(k) for k in
+                                    M.class_key(np.random.SeedSequence(1000 * step + i))),
+                              (0, 0, 0), tuple(inst.flux.shape))
+                for i, inst in enumerate(twins)]
+    steps = 7
+    for step in range(steps):
+        pipe.set_regions(regions(step + 1))
+        if step < 2:
+            pipe.advance()
+        else:
+            if pipe.graphs is None:
+                pipe.capture()
+            pipe.replay()
+    pipe.finish()           # the last step's rescale, left pending
+    torch.cuda.synchronize()
+    for step in range(steps):
+        plain.set_regions(regions(step))
+        plain.advance()
+    torch.cuda.synchronize()
+    for a, b in zip(insts, twins):
+        np.testing.assert_array_equal(a.flux.cpu().numpy(), b.flux.cpu().numpy())
+    for la, lb in zip(pipe.ledgers(), plain.ledgers()):
+        assert la.realized == lb.realized and la.n_clipped == lb.n_clipped
