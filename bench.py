@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=1000.0,
                     help="tile requests per second of the served form")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-sweep", type=float, nargs="*", default=None,
+    ap.add_argument("--serve-sweep", type=float, nargs="*", default=[500.0, 2000.0, 4000.0],
                     help="also serve the trace at these rates (req/s) and report "
                          "throughput and p50/p99 per rate")
     ap.add_argument("--serve-world", type=int, nargs=3, default=[2048, 2048, 256])
@@ -343,6 +343,8 @@ def served_form(world, n_req, rate, sweep=None):
         "world": list(w.world_shape), "kernel": list(w.kernel.shape),
         "measure_pass_s": measure_s, "cache_hits": srv.stats["cache_hits"],
         "max_inflight": srv.stats["max_inflight"],
+        "groups": srv.stats["groups"], "max_group": srv.stats["max_group"],
+        "mean_group": srv.stats["batched_requests"] / max(1, srv.stats["groups"]),
         "tiles": "{64^3, 128^3, 256^3, 384x384x192}, 70% periodic neighbours",
     }
 

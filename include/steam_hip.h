@@ -267,9 +267,10 @@ int steam_flux_ledger_read(const steam_conv_plan* plan,
 
 /* ------------------------------------------------------------------------
  * Batched advance: n independent instances (realisations or tiles of
- * different sizes, alpha and noise scale), each exactly the measure-mode
- * steam_flux_advance of its own plan, advanced by one launch per pass and
- * compiled kernel variant.  The reference advances instances one call at a
+ * different sizes, alpha and noise scale), each exactly the
+ * steam_flux_advance of its own plan (measure mode, or apply-only with a
+ * recorded ledger), advanced by one launch per pass and compiled kernel
+ * variant.  The reference advances instances one call at a
  * time (steam/simulate.py:2066-2228 per class and realisation); the batch
  * returns, per instance, the same result as that call.
  * ---------------------------------------------------------------------- */
@@ -287,10 +288,13 @@ typedef struct steam_batch_item {
   float sigma_in;             /* pending rescale applied on read */
   int32_t flags;              /* STEAM_DEFER_RESCALE | STEAM_CHAIN_PENDING |
                                  STEAM_CLIP_IN_WINDOW */
+  const steam_flux_ledger* ledger; /* NULL: measure; else apply-only with these
+                                      reductions (copied at creation) */
 } steam_batch_item;
 
 /* Group the instances by kernel variant and stage their descriptors
- * (pinned host copy + device copy).  Allocates. */
+ * (pinned host copy + device copy).  Allocates.  An item with a ledger runs
+ * the apply-only advance (STEAM_APPLY) of steam_flux_advance. */
 int steam_batch_create(const steam_batch_item* items, int32_t n,
                        steam_batch** batch);
 int steam_batch_destroy(steam_batch* batch);

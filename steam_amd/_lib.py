@@ -72,7 +72,7 @@ class BatchItem(ctypes.Structure):
     _fields_ = [("plan", _P), ("spectrum_dev", _P), ("workspace_dev", _P),
                 ("flux_dev", _P), ("noise_dev", _P), ("sk_dev", _P),
                 ("noise", NoiseParams), ("window", _I64 * 4), ("sigma_in", _F32),
-                ("flags", _I32)]
+                ("flags", _I32), ("ledger", ctypes.POINTER(FluxLedger))]
 
 
 # (name, restype, argtypes)

@@ -80,7 +80,7 @@ class Instance:
     def __init__(self, flux, kernel, region, flux_noise_scale,
                  n_scale_classes_per_dyad=1, sparsity_factors=(1, 1, 1),
                  alpha=None, n_zero=0, zero_bottom=False, zero_top=False,
-                 window=None, defer_rescale=False, chain_pending=False):
+                 window=None, defer_rescale=False, chain_pending=False, ledger=None):
         if not (isinstance(flux, torch.Tensor) and flux.is_cuda
                 and flux.dtype == torch.float32 and flux.is_contiguous()):
             raise ValueError("batched instances need contiguous float32 device "
@@ -98,6 +98,8 @@ class Instance:
         self.window = window
         self.defer_rescale = defer_rescale
         self.chain_pending = chain_pending
+        # a recorded FluxAdvanceLedger: the instance advances apply-only
+        self.ledger = ledger
 
     def native_noise(self, region=None):
         a, pcs, shift = class_scale_shift(self.flux_noise_scale,
@@ -131,6 +133,7 @@ class BatchedAdvance:
                                  "instance's shape per instance expected")
         self.noise_buffers = noise_buffers
         self.ops, self.specs, self.workspaces = [], [], []
+        self._ledgers = []
         self.n_fft = []
         items = (_lib.BatchItem * len(self.instances))()
         for i, inst in enumerate(self.instances):
@@ -162,6 +165,10 @@ class BatchedAdvance:
             it.sigma_in = 1.0
             it.flags = ((_lib.DEFER_RESCALE if inst.defer_rescale else 0)
                         | (_lib.CHAIN_PENDING if inst.chain_pending else 0))
+            if inst.ledger is not None:
+                native = inst.ledger.to_native()
+                self._ledgers.append(native)
+                it.ledger = ctypes.pointer(native)
         self._items = items
         handle = ctypes.c_void_p()
         _lib.check(lib.steam_batch_create(items, len(self.instances),

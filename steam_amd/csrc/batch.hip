@@ -290,7 +290,8 @@ int make_item(const steam_batch_item& it, PassDesc& d, ItemPlan& ip) {
                : nullptr;
   d.noise = make_noise_dev(&it.noise);
   d.sk = it.sk_dev;
-  d.apply = 0;
+  d.apply = it.ledger != nullptr;
+  if (d.apply) d.led_in = ledger_to_dev(it.ledger);
   d.clip_win = (it.flags & STEAM_CLIP_IN_WINDOW) != 0;
   const int64_t* w = it.window;
   if (w[0] || w[1] || w[2] || w[3]) {

@@ -11,13 +11,16 @@ the kernel half-width: the keyed noise is a function of world coordinates
 solve restricted to it, seams included.
 
 The server admits requests as they arrive, against a device-memory budget
-and a pool of streams; admitted requests run concurrently on their own
-streams (their kernels overlap on the device) and new arrivals are admitted
-as soon as a slot and memory free up.  Kernel spectra are cached per padded
-tile shape; finished tiles are kept in a device-resident LRU cache so a
-request for a recently served tile (a neighbour walk coming back) is
-answered without recomputation.  Latency is arrival to completion of the
-tile on the device.
+and a pool of streams.  Waiting requests of the same padded tile shape are
+merged into one batched advance (one launch per pass for the group,
+steam_batch_*), and groups run concurrently on their own streams (their
+kernels overlap on the device); new arrivals are admitted as soon as a
+stream and memory free up.  Kernel spectra are cached per padded tile shape,
+and each (shape, stream) keeps its slot buffers and batch descriptors, so
+nothing is allocated per request; finished tiles are kept in a
+device-resident LRU cache so a request for a recently served tile (a
+neighbour walk coming back) is answered without recomputation.  Latency is
+arrival to completion of the tile on the device.
 """
 
 import time
@@ -168,51 +171,120 @@ def make_trace(n, rate, world_shape, sizes=((64, 64, 64), (128, 128, 128),
 
 
 class _InFlight:
-    __slots__ = ("req", "stream", "event", "out", "flux", "bytes", "start")
+    __slots__ = ("reqs", "stream", "event", "outs", "pool", "bytes")
+
+
+class _SlotPool:
+    """The device buffers of one padded tile shape on one stream: ``slots``
+    flux buffers (views of one tensor, so one fill resets them all) and
+    workspaces, and a batched advance over the first c slots for each group
+    size c, created on first use and kept (their descriptors are refreshed
+    per group from pinned memory, nothing is allocated per request)."""
+
+    def __init__(self, world, padded_shape, slots):
+        self.world = world
+        self.shape = tuple(padded_shape)
+        self.flux = torch.ones((slots,) + self.shape, dtype=torch.float32, device='cuda')
+        op = conv_operator(self.shape, tuple(world.kernel.shape))
+        self.workspaces = [torch.empty(op.workspace_bytes, dtype=torch.uint8, device='cuda')
+                           for _ in range(slots)]
+        self.batches = {}
+
+    @property
+    def nbytes(self):
+        return self.flux.numel() * 4 + sum(w.numel() for w in self.workspaces)
+
+    def batch(self, count):
+        from .batch import BatchedAdvance, Instance
+        b = self.batches.get(count)
+        if b is None:
+            w = self.world
+            insts = [Instance(self.flux[i], w.kernel, w.region((0, 0, 0)), w.flux_noise_scale,
+                              alpha=w.alpha, ledger=w.ledger) for i in range(count)]
+            b = BatchedAdvance(insts, autotune=False, workspaces=self.workspaces[:count])
+            self.batches[count] = b
+        return b
 
 
 class TileServer:
-    """Continuous-batching tile server over one WorldClass."""
+    """Continuous-batching tile server over one WorldClass.
 
-    def __init__(self, world, budget_bytes=8 << 30, streams=4, cache_bytes=2 << 30):
+    Requests wait in arrival order.  Whenever a stream is free, the oldest
+    waiting request is admitted together with up to ``max_batch - 1`` later
+    waiting requests of the same padded tile shape (within the device-memory
+    budget): their apply-only advances run as one batched advance (one
+    launch per pass for the whole group, steam_batch_*) on that stream,
+    while other groups run on the other streams.  A tile's result does not
+    depend on its group: the batched advance of an instance equals its
+    single advance bit for bit."""
+
+    def __init__(self, world, budget_bytes=8 << 30, streams=4, cache_bytes=2 << 30,
+                 max_batch=8, pool_bytes=2 << 30):
         if world.ledger is None:
             raise ValueError("measure the world ledger first (WorldClass.measure)")
         _lib.require_gpu('cuda')
         self.world = world
         self.budget = int(budget_bytes)
         self.streams = [torch.cuda.Stream() for _ in range(max(1, streams))]
+        self.max_batch = max(1, int(max_batch))
+        self.pool_bytes = int(pool_bytes)
+        self._slot_counts = {}
         self.cache_bytes = int(cache_bytes)
         self.cache = OrderedDict()
         self.cache_used = 0
+        self.pools = {}
         self.stats = {"cache_hits": 0, "spectra": 0, "max_inflight": 0,
-                      "max_bytes": 0}
+                      "max_bytes": 0, "groups": 0, "batched_requests": 0, "max_group": 0}
 
     @staticmethod
     def request_bytes(world, shape):
         _, ps, _ = world.padded_box((0, 0, 0), shape)
         return 4 * int(np.prod(ps)) + 4 * int(np.prod(shape))
 
-    def _launch(self, req, stream):
-        w = self.world
-        with torch.cuda.stream(stream):
-            po, ps, (ix, iy, iz) = w.padded_box(req["origin"], req["shape"])
+    def slots(self, ps):
+        """Group size cap of a padded shape: max_batch, or fewer when the
+        slot buffers of one stream would pass pool_bytes."""
+        ps = tuple(ps)
+        n = self._slot_counts.get(ps)
+        if n is None:
+            op = conv_operator(ps, tuple(self.world.kernel.shape))
+            per_slot = 4 * int(np.prod(ps)) + int(op.workspace_bytes)
+            n = max(1, min(self.max_batch, self.pool_bytes // per_slot))
+            self._slot_counts[ps] = n
+        return n
+
+    def _pool(self, ps, stream):
+        key = (tuple(ps), stream.cuda_stream)
+        pool = self.pools.get(key)
+        if pool is None:
+            w = self.world
             op = conv_operator(ps, tuple(w.kernel.shape))
-            key = kernel_key(w.kernel)
-            fresh = key not in op._spectra
-            spec = op.spectrum(w.kernel, key=key)
-            if fresh:
-                # other streams will read this cached spectrum
-                stream.synchronize()
+            kkey = kernel_key(w.kernel)
+            if kkey not in op._spectra:
+                op.spectrum(w.kernel, key=kkey)
+                torch.cuda.current_stream().synchronize()
                 self.stats["spectra"] += 1
-            flux = torch.ones(ps, dtype=torch.float32, device='cuda')
-            advance_flux_device(flux, w.region(po), w.kernel, w.flux_noise_scale, 1,
-                                (1, 1, 1), alpha=w.alpha, ledger=w.ledger, op=op,
-                                spec=spec)
-            tx, ty, tz = req["shape"]
-            out = flux[ix:ix + tx, iy:iy + ty, iz:iz + tz].contiguous()
+            pool = _SlotPool(w, ps, self.slots(ps))
+            self.pools[key] = pool
+        return pool
+
+    def _launch(self, reqs, stream):
+        w = self.world
+        boxes = [w.padded_box(r["origin"], r["shape"]) for r in reqs]
+        pool = self._pool(boxes[0][1], stream)
+        b = pool.batch(len(reqs))
+        with torch.cuda.stream(stream):
+            pool.flux[:len(reqs)].fill_(1.0)
+            # the stream is free: the batch's previous upload has been read
+            b.set_regions([w.region(po) for po, _, _ in boxes])
+            b.advance(stream)
+            outs = []
+            for i, (r, (_, _, (ix, iy, iz))) in enumerate(zip(reqs, boxes)):
+                tx, ty, tz = r["shape"]
+                outs.append(pool.flux[i, ix:ix + tx, iy:iy + ty, iz:iz + tz].contiguous())
             ev = torch.cuda.Event()
             ev.record(stream)
-        return flux, out, ev
+        return pool, outs, ev
 
     def _cache_put(self, key, t):
         nb = t.numel() * 4
@@ -233,8 +305,10 @@ class TileServer:
 
         def keep(rid):
             return keep_results and (wanted is None or rid in wanted)
+
+        w = self.world
         pending = deque(sorted(trace, key=lambda r: r["arrival"]))
-        queue = deque()
+        queue = []
         inflight = []
         free_streams = list(self.streams)
         used = 0
@@ -252,42 +326,64 @@ class TileServer:
             for f in inflight:
                 if f.event.query():
                     done = time.perf_counter() - t0
-                    lat[f.req["id"]] = done - f.req["arrival"]
-                    voxels += f.out.numel()
-                    self._cache_put((f.req["origin"], f.req["shape"]), f.out)
-                    if keep(f.req["id"]):
-                        results[f.req["id"]] = f.out
+                    for r, out in zip(f.reqs, f.outs):
+                        lat[r["id"]] = done - r["arrival"]
+                        voxels += out.numel()
+                        self._cache_put((r["origin"], r["shape"]), out)
+                        if keep(r["id"]):
+                            results[r["id"]] = out
                     used -= f.bytes
-                    f.flux = None
                     free_streams.append(f.stream)
                 else:
                     still.append(f)
             inflight = still
-            # admission: in arrival order, while a stream and memory are free
-            while queue:
-                req = queue[0]
+            # exact repeats of a finished tile are answered from the cache
+            rest = []
+            for req in queue:
                 key = (req["origin"], req["shape"])
                 hit = self.cache.get(key)
-                if hit is not None:
-                    queue.popleft()
-                    self.cache.move_to_end(key)
-                    self.stats["cache_hits"] += 1
-                    lat[req["id"]] = (time.perf_counter() - t0) - req["arrival"]
-                    voxels += hit.numel()
-                    if keep(req["id"]):
-                        results[req["id"]] = hit
+                if hit is None:
+                    rest.append(req)
                     continue
-                nb = self.request_bytes(self.world, req["shape"])
-                if not free_streams or (inflight and used + nb > self.budget):
+                self.cache.move_to_end(key)
+                self.stats["cache_hits"] += 1
+                lat[req["id"]] = (time.perf_counter() - t0) - req["arrival"]
+                voxels += hit.numel()
+                if keep(req["id"]):
+                    results[req["id"]] = hit
+            queue = rest
+            # admission: the oldest waiting request and the later waiting
+            # requests of its padded shape, while a stream and memory are free
+            while queue and free_streams:
+                head = queue[0]
+                ps = w.padded_box(head["origin"], head["shape"])[1]
+                cap = self.slots(ps)
+                group, nb = [], 0
+                for req in queue:
+                    if len(group) == cap:
+                        break
+                    if w.padded_box(req["origin"], req["shape"])[1] != ps:
+                        continue
+                    rb = self.request_bytes(w, req["shape"])
+                    if group and used + nb + rb > self.budget:
+                        break
+                    group.append(req)
+                    nb += rb
+                if inflight and used + nb > self.budget:
                     break
-                queue.popleft()
+                ids = {r["id"] for r in group}
+                queue = [r for r in queue if r["id"] not in ids]
                 s = free_streams.pop()
                 f = _InFlight()
-                f.req, f.stream, f.bytes = req, s, nb
-                f.flux, f.out, f.event = self._launch(req, s)
+                f.reqs, f.stream, f.bytes = group, s, nb
+                f.pool, f.outs, f.event = self._launch(group, s)
                 used += nb
                 inflight.append(f)
-                self.stats["max_inflight"] = max(self.stats["max_inflight"], len(inflight))
+                self.stats["groups"] += 1
+                self.stats["batched_requests"] += len(group)
+                self.stats["max_group"] = max(self.stats["max_group"], len(group))
+                self.stats["max_inflight"] = max(self.stats["max_inflight"],
+                                                 sum(len(x.reqs) for x in inflight))
                 self.stats["max_bytes"] = max(self.stats["max_bytes"], used)
             if not queue and not inflight and pending:
                 wait = pending[0]["arrival"] - (time.perf_counter() - t0)

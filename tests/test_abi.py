@@ -29,7 +29,7 @@ STRUCTS = {
                                                        "n_scale", "final_demean", "mu"]),
     "steam_batch_item": ("BatchItem", ["plan", "spectrum_dev", "workspace_dev",
                                        "flux_dev", "noise_dev", "sk_dev", "noise",
-                                       "window", "sigma_in", "flags"]),
+                                       "window", "sigma_in", "flags", "ledger"]),
 }
 
 

@@ -74,6 +74,8 @@ def test_result_is_the_same_alone_and_in_the_mix():
     _, _, _, r2 = busy.serve(mix, keep_results=True)
     torch.cuda.synchronize()
     assert busy.stats["max_inflight"] > 1
+    # requests waiting together were merged into batched advances
+    assert busy.stats["max_group"] > 1 and busy.stats["groups"] < len(mix)
     np.testing.assert_array_equal(r1[0].cpu().numpy(), r2[probe["id"]].cpu().numpy())
 
 
@@ -97,3 +99,20 @@ def test_server_needs_a_measured_world():
     from steam_amd.serve import TileServer
     with pytest.raises(ValueError):
         TileServer(_world())
+
+
+def test_batched_groups_equal_single_requests():
+    """Every tile of a burst served in batched groups equals the same tile
+    served as a group of one (max_batch=1), bit for bit."""
+    from steam_amd.serve import TileServer, make_trace
+    w = _world()
+    w.measure(tile=(64, 48))
+    burst = [dict(r, arrival=0.0) for r in make_trace(24, 1e6, WORLD, sizes=SIZES, seed=4)]
+    grouped = TileServer(w, streams=2, cache_bytes=0, max_batch=8)
+    _, _, _, a = grouped.serve(burst, keep_results=True)
+    single = TileServer(w, streams=2, cache_bytes=0, max_batch=1)
+    _, _, _, b = single.serve(burst, keep_results=True)
+    torch.cuda.synchronize()
+    assert grouped.stats["max_group"] > 1 and single.stats["max_group"] == 1
+    for r in burst:
+        np.testing.assert_array_equal(a[r["id"]].cpu().numpy(), b[r["id"]].cpu().numpy())
