@@ -12,7 +12,9 @@ import threading
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsteam_hip.so")
+# STEAM_HIP_LIBRARY names another build of the library (a measurement
+# variant compiled with different tuning constants); default: the in-tree one
+LIB_PATH = os.environ.get("STEAM_HIP_LIBRARY") or os.path.join(_HERE, "libsteam_hip.so")
 
 STEAM_OK = 0
 STEAM_EINVAL = 1

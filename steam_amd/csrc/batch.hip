@@ -104,7 +104,7 @@ std::tuple<int, int, int> variant_of(const ItemPlan& ip, int pass) {
     case BP_X:
       if (g.reg_x) return {1, g.reg_x, (g.nx / 8 / xreg_lanes(g.reg_x)) * g.Tr};
       return {2, radix_set(g.ax), FFT_NTH};
-    case BP_FIN: return {0, 0, FFT_NTH};
+    case BP_FIN: return {0, 0, FIN_NTH};
     default: return {ip.post ? 0 : -1, ip.post_rescale ? 1 : 0, FFT_NTH};
   }
 }
@@ -261,7 +261,7 @@ int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
         }
         break;
       case BP_FIN:
-        hipLaunchKernelGGL((finalize_kernel<BatchSrc>), grid, dim3(FFT_NTH), 0, s, src);
+        hipLaunchKernelGGL((finalize_kernel<BatchSrc>), grid, dim3(FIN_NTH), 0, s, src);
         break;
       default:
         hipLaunchKernelGGL((post_kernel<BatchSrc>), grid, dim3(FFT_NTH), 0, s, src, gr.arg);

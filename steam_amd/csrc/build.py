@@ -28,21 +28,27 @@ FLAGS = [
 ]
 
 
-def _compile(src, extra):
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+def _compile(src, extra, obj_dir=None):
+    obj = os.path.join(obj_dir or OBJ, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC, *FLAGS, *SOURCE_FLAGS.get(src, []), *extra, "-c", os.path.join(HERE, src), "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     return src, obj, res
 
 
-def build(jobs=4, resource_usage=False, verbose=False):
-    os.makedirs(OBJ, exist_ok=True)
+def build(jobs=4, resource_usage=False, verbose=False, defines=(), target=None):
+    """Compile every source and link the library.  ``defines`` (NAME=VALUE
+    strings) and ``target`` build a measurement variant elsewhere (its
+    objects beside it), leaving the in-tree library alone."""
+    obj_dir = OBJ if target is None else os.path.join(os.path.dirname(os.path.abspath(target)),
+                                                      "_obj")
+    os.makedirs(obj_dir, exist_ok=True)
     extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
+    extra += [f"-D{d}" for d in defines]
     sources = [s for s in SOURCES if os.path.exists(os.path.join(HERE, s))]
     objs = []
     failed = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as pool:
-        for src, obj, res in pool.map(lambda s: _compile(s, extra), sources):
+        for src, obj, res in pool.map(lambda s: _compile(s, extra, obj_dir), sources):
             if verbose or resource_usage or res.returncode:
                 sys.stderr.write(res.stderr)
             if res.returncode:
@@ -50,11 +56,11 @@ def build(jobs=4, resource_usage=False, verbose=False):
             objs.append(obj)
     if failed:
         raise RuntimeError(f"hipcc failed for {failed}")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o",
-           TARGET + ".tmp"]
+    out = TARGET if target is None else os.path.abspath(target)
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out + ".tmp"]
     subprocess.run(cmd, check=True)
-    os.replace(TARGET + ".tmp", TARGET)
-    return TARGET
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":
@@ -62,5 +68,10 @@ if __name__ == "__main__":
     ap.add_argument("-j", type=int, default=4)
     ap.add_argument("--resource-usage", action="store_true")
     ap.add_argument("-v", action="store_true")
+    ap.add_argument("-D", dest="defines", action="append", default=[],
+                    help="NAME=VALUE for a measurement variant (with --out)")
+    ap.add_argument("--out", default=None, help="library path of a variant build")
     a = ap.parse_args()
-    print(build(a.j, a.resource_usage, a.v))
+    if a.defines and a.out is None:
+        ap.error("-D builds a variant: give its --out path")
+    print(build(a.j, a.resource_usage, a.v, a.defines, a.out))

@@ -615,7 +615,7 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
   SingleSrc src{d};
   {
     ProfScope ps(STEAM_KF_FINALIZE, s);
-    hipLaunchKernelGGL((finalize_kernel<SingleSrc>), dim3(1), dim3(FFT_NTH), 0, s, src);
+    hipLaunchKernelGGL((finalize_kernel<SingleSrc>), dim3(1), dim3(FIN_NTH), 0, s, src);
   }
   STEAM_LAUNCH_CHECK("finalize_kernel");
   if (!defer_rescale || sk_scale) {

@@ -185,11 +185,13 @@ __device__ __forceinline__ float2 kernel_bin_factor(const PassDesc& d, int kz) {
 // along z (coalesced stores).
 // ---------------------------------------------------------------------------
 // workgroups of a noise field generated ahead of its advance (0: one per
-// four lines).  1.5 per CU of the 256: beside the transforms of the class
-// before, 384 took a 512x512x256 step from 0.999 to 0.979 ms (256: 1.004,
-// 512: 0.990, 1024: 1.036, 2048: 1.090)
+// four lines), two per CU of the 256.  Generated two classes ahead beside
+// the transforms, a 512x512x256 step took 0.914 ms at 512 on two boxes
+// (640: 0.927; 384: 0.918 on one box, 1.03-1.16 on another, where the noise
+// no longer kept up with the transforms; the whole grid competes with them
+// for slots: 1024 gave 0.975)
 #ifndef STEAM_NOISE_WGS
-#define STEAM_NOISE_WGS 384
+#define STEAM_NOISE_WGS 512
 #endif
 // STRIDE: a grid smaller than the field's line groups walks them with a
 // grid stride (a few workgroups per CU keep the VALU busy beside memory-bound
@@ -1415,24 +1417,29 @@ __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src sr
 // ---------------------------------------------------------------------------
 // finalize: fixed-order reduction of the partials into the ledger
 // ---------------------------------------------------------------------------
+// threads of the finalize workgroup: the partials of a large field (8192
+// workgroups at 512 x 512) are read eight per thread, and the workgroup
+// sum is a shuffle tree per wave plus one pass over the waves
+constexpr int FIN_NTH = 1024;
+
 template <class Src>
-__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void finalize_kernel(Src src) {
+__global__ __launch_bounds__(FIN_NTH) void finalize_kernel(Src src) {
   int wg = blockIdx.x;
   const PassDesc& d = src.get(PASS_FIN, wg);
   const ConvGeom& g = d.g;
   const int tid = threadIdx.x;
-  __shared__ double red[5 * FFT_NTH];
+  __shared__ double red[5 * (FIN_NTH / 64)];
   DevLedger L;
   if (!d.apply) {
     double v[5] = {0, 0, 0, 0, 0};
-    for (int i = tid; i < d.npart; i += FFT_NTH) {
+    for (int i = tid; i < d.npart; i += FIN_NTH) {
       v[0] += d.part_z[i * 3 + 0];
       v[1] += d.part_z[i * 3 + 1];
       v[2] += d.part_z[i * 3 + 2];
       v[3] += d.part_o[i * 2 + 0];
       v[4] += d.part_o[i * 2 + 1];
     }
-    block_sum<5, FFT_NTH>(v, red);
+    block_sum_w<5, FIN_NTH>(v, red);
     if (tid != 0) return;
     const double cnt = (double)(d.win[1] - d.win[0]) * (double)(d.win[3] - d.win[2]) *
                        (double)g.nz;
