@@ -70,6 +70,18 @@
 #endif
 
 namespace steam {
+
+// Lane addressing of the register passes.  An access is a base that is
+// uniform over the workgroup (its first line, one plane of its columns, one
+// row of a plane) plus a 32-bit lane byte offset plus a compile-time
+// constant, so the compiler forms one lane address and takes the constants
+// as immediates: no per-access 64-bit vector address arithmetic.
+template <class T>
+__device__ __forceinline__ T* lane_at(T* base, unsigned lane_bytes, int const_bytes) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T*>(reinterpret_cast<B*>(base) + (size_t)lane_bytes + const_bytes);
+}
+
 constexpr int XREG_COLS = STEAM_XREG_COLS;  // y columns per register x-pass workgroup
 constexpr int XREG_COLS2 = STEAM_XREG_COLS2;  // the same with two lanes per thread
 // transform lanes per thread of the register x pass for an nx of 2^logn
@@ -137,21 +149,23 @@ __device__ void block_sum(double (&v)[NV], double* red) {
 // The entering flux value of a stored cell: the pending rescale of the
 // previous advance (sigma_in, or chained from the ledger it left on the
 // device) applied on read, or the flat entering mean of an all-clipped class.
+// Both cases are one fma with uniform coefficients: sig * raw + 0 rounds as
+// the product does, and 0 * raw + em is em (stored flux is finite).
 struct Pending {
-  float sig, em;
-  int flat;
+  float a, b;
 };
 __device__ __forceinline__ Pending pending_of(const PassDesc& d) {
-  Pending p{d.sigma_in, 0.f, 0};
+  float sig = d.sigma_in, em = 0.f;
+  int flat = 0;
   if (d.pend) {
-    p.flat = d.pend->flatten;
-    p.sig = d.pend->rescale;
-    p.em = d.pend->entering_mean_f;
+    flat = d.pend->flatten;
+    sig = d.pend->rescale;
+    em = d.pend->entering_mean_f;
   }
-  return p;
+  return flat ? Pending{0.f, em} : Pending{sig, 0.f};
 }
 __device__ __forceinline__ float entering_value(const Pending& p, float raw) {
-  return p.flat ? p.em : __fmul_rn(p.sig, raw);
+  return fmaf(p.a, raw, p.b);
 }
 
 // kp[x, y, z] of the rolled, folded kernel: taps i = x + offx (mod nx),
@@ -241,7 +255,7 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
   const int L = (MODE == MODE_KERNEL) ? g.nfft : g.L;
   double acc[3] = {0.0, 0.0, 0.0};
   bool xin = false;
-  Pending pd{1.f, 0.f, 0};
+  Pending pd{1.f, 0.f};
   if (MODE == MODE_FLUX) {
     xin = x >= d.win[0] && x < d.win[1];
     pd = pending_of(d);
@@ -468,10 +482,15 @@ __device__ __forceinline__ void xpass_reg_body(const PassDesc& d, int wg, float2
   const int y = (wg - p * nyt) * T + t;
   const bool valid = y < g.ny;
   const int nx = g.nx, ny = g.ny;
-  // plane bases are uniform over the workgroup; lane offsets (< nx * ny) are
-  // 32-bit, so the addresses of the final stores are cheap to re-form
+  // Value m of lane c is row x = q + m P (q = q0 + c PT) of the plane: a row
+  // base uniform over the workgroup (Sb + m P ny) plus the lane's 32-bit byte
+  // offset (q ny + y) * 8 < nx ny * 8, so no access re-forms a 64-bit address.
   float2* Sb = d.S + (int64_t)p * nx * ny;
   const int yo = valid ? y : 0;
+  auto row = [&](int m) { return Sb + (int64_t)m * P * ny; };
+  unsigned lb[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) lb[c] = (unsigned)((q0 + c * PT) * ny + yo) * 8u;
   auto off = [&](int c, int m) { return (q0 + c * PT + m * P) * ny + yo; };
   using KV = typename std::conditional<KREAL, float, float2>::type;
   float2 v[C][8];
@@ -479,26 +498,36 @@ __device__ __forceinline__ void xpass_reg_body(const PassDesc& d, int wg, float2
 #pragma unroll
   for (int c = 0; c < C; ++c)
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[c][m] = valid ? Sb[off(c, m)] : make_float2(0.f, 0.f);
+    for (int m = 0; m < 8; ++m)
+      v[c][m] = valid ? *lane_at(row(m), lb[c], 0) : make_float2(0.f, 0.f);
   // the kernel spectrum: in this pass's column tiles [kz][y/T][x][T] when
   // the plan stores it so (the workgroup's block is then one contiguous run)
+  // With the real, x-even format, row x takes bin min(x, nx - x): for
+  // x = q + m P (q < P, nx = 8 P) that is x itself for m < 4 and
+  // (8 - m) P - q for m >= 4 (both give 4 P at x = nx / 2), so both halves
+  // are a uniform base plus a lane offset.
   const KV* Kb = nullptr;
   const bool kt = d.ktile != 0;
-  auto koff = [&](int c, int m) {
-    const int x = q0 + c * PT + m * P;
-    if (KREAL) return (x <= nx - x ? x : nx - x) * T + t;
-    return kt ? x * T + t : off(c, m);
-  };
   auto load_k = [&]() {
 #pragma unroll
-    for (int c = 0; c < C; ++c)
+    for (int c = 0; c < C; ++c) {
+      const int q = q0 + c * PT;
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        if constexpr (KREAL)
-          kv[c][m] = valid ? Kb[koff(c, m)] : 0.f;
-        else
-          kv[c][m] = valid ? Kb[koff(c, m)] : make_float2(0.f, 0.f);
+        if constexpr (KREAL) {
+          const unsigned lo = (unsigned)(q * T + t) * 4u;
+          const unsigned hi = (unsigned)((P - q) * T + t) * 4u;
+          kv[c][m] = !valid ? 0.f
+                     : m < 4 ? *lane_at(Kb + m * P * T, lo, 0)
+                             : *lane_at(Kb + (7 - m) * P * T, hi, 0);
+        } else if (kt) {
+          kv[c][m] = valid ? *lane_at(Kb + m * P * T, (unsigned)(q * T + t) * 8u, 0)
+                           : make_float2(0.f, 0.f);
+        } else {
+          kv[c][m] = valid ? Kb[off(c, m)] : make_float2(0.f, 0.f);
+        }
       }
+    }
   };
   // one lane per thread: the spectrum loads go out with the line's; two
   // lanes: after the forward transform (the registers of both would spill)
@@ -511,11 +540,15 @@ __device__ __forceinline__ void xpass_reg_body(const PassDesc& d, int wg, float2
       Kb = kt ? d.K + tile * nx * T : d.K + (int64_t)(p % g.nzc) * nx * ny;
     if (KPRE) load_k();
   }
-  // the twiddle bases, shared by the forward and the inverse transform
+  // the twiddles, formed once for the forward and the inverse transform
+  // (two lanes per thread keep only the bases: registers)
   RegTw<LOGN> twb[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) reg_tw_load<LOGN>(twb[c], q0 + c * PT, g.ax.tw);
-  fft_reg_lanes<LOGN, C>(v, q0, PT, lds, T, t, g.ax.tw, twb, false);
+  using TWX = typename std::conditional<C == 1, RegTwFull<LOGN>, RegTw<LOGN>>::type;
+  TWX twx[C] = {TWX(twb[0])};
+  if constexpr (C == 2) twx[1] = TWX(twb[1]);
+  fft_reg_lanes<LOGN, C>(v, q0, PT, lds, T, t, g.ax.tw, twx, false);
   if (XMODE == 0) {
     if (!KPRE) load_k();
 #pragma unroll
@@ -527,7 +560,7 @@ __device__ __forceinline__ void xpass_reg_body(const PassDesc& d, int wg, float2
         else
           v[c][m] = cmul(v[c][m], kv[c][m]);
       }
-    fft_reg_lanes<LOGN, C>(v, q0, PT, lds, T, t, g.ax.tw, twb, true);
+    fft_reg_lanes<LOGN, C>(v, q0, PT, lds, T, t, g.ax.tw, twx, true);
   } else {
     const float2 f = kernel_bin_factor(d, p % g.nzc);
 #pragma unroll
@@ -539,7 +572,7 @@ __device__ __forceinline__ void xpass_reg_body(const PassDesc& d, int wg, float2
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
-      for (int m = 0; m < 8; ++m) Sb[off(c, m)] = v[c][m];
+      for (int m = 0; m < 8; ++m) *lane_at(row(m), lb[c], 0) = v[c][m];
   }
 }
 
@@ -645,7 +678,7 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
   }
   double acc[2] = {0.0, 0.0};
   const bool xin = (MODE == MODE_FLUX) && x >= d.win[0] && x < d.win[1];
-  Pending pd{1.f, 0.f, 0};
+  Pending pd{1.f, 0.f};
   if (MODE == MODE_FLUX) pd = pending_of(d);
   for (int t = 0; t < g.Tz; ++t) {
     const int y = y0 + t;
@@ -752,7 +785,7 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
   }
   stage_twiddles(g, twz, twr, tid);
   double acc[3] = {0.0, 0.0, 0.0};
-  Pending pd{1.f, 0.f, 0};
+  Pending pd{1.f, 0.f};
   bool xin = false;
   if (MODE == MODE_FLUX) {
     pd = pending_of(d);
@@ -922,7 +955,7 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
   // output z of line t is real sample m = z + cz of the block transform
   const float* sr = reinterpret_cast<const float*>(smem);
   double acc[2] = {0.0, 0.0};
-  Pending pd{1.f, 0.f, 0};
+  Pending pd{1.f, 0.f};
   bool xin = false;
   if (MODE == MODE_FLUX) {
     pd = pending_of(d);
@@ -1073,42 +1106,43 @@ __device__ __forceinline__ void zr_block_sum(double (&v)[NV], double* red) {
     }
 }
 
-// z pairs (2n, 2n+1) of a line starting at off: one 8-byte access when the
-// field has an even number of levels (every line start is then 8-byte
-// aligned), two 4-byte ones when it is odd (the pair past the top level
-// reads 0 and is not written)
+// The z pair (2n, 2n+1) at lane offset lb plus cb bytes: one 8-byte access
+// when the field has an even number of levels (every line start is then
+// 8-byte aligned), two 4-byte ones when it is odd.  The _top forms are for
+// a pair that may reach past the top level nz: with an odd count its second
+// level reads 0 and is not written.
 template <bool ODD>
-__device__ __forceinline__ float2 ld_zpair(const float* p, int64_t off, int z0, int nz) {
+__device__ __forceinline__ float2 ld_pair(const float* base, unsigned lb, int cb) {
+  const float* p = lane_at(base, lb, cb);
+  if constexpr (!ODD) return *reinterpret_cast<const float2*>(p);
+  return make_float2(p[0], p[1]);
+}
+template <bool ODD>
+__device__ __forceinline__ void st_pair(float* base, unsigned lb, int cb, float2 v) {
+  float* p = lane_at(base, lb, cb);
   if constexpr (!ODD) {
-    return *reinterpret_cast<const float2*>(p + off + z0);
+    *reinterpret_cast<float2*>(p) = v;
   } else {
-    return make_float2(p[off + z0], z0 + 1 < nz ? p[off + z0 + 1] : 0.f);
+    p[0] = v.x;
+    p[1] = v.y;
   }
 }
 template <bool ODD>
-__device__ __forceinline__ void st_zpair(float* p, int64_t off, int z0, int nz, float2 v) {
-  if constexpr (!ODD) {
-    *reinterpret_cast<float2*>(p + off + z0) = v;
-  } else {
-    p[off + z0] = v.x;
-    if (z0 + 1 < nz) p[off + z0 + 1] = v.y;
-  }
-}
-
-// A pair whose two levels both lie below nz: no bound checks (with an odd
-// level count the line start is only 4-byte aligned).
-template <bool ODD>
-__device__ __forceinline__ float2 ld_pair_in(const float* p, unsigned off) {
-  if constexpr (!ODD) return *reinterpret_cast<const float2*>(p + off);
-  return make_float2(p[off], p[off + 1]);
+__device__ __forceinline__ float2 ld_pair_top(const float* base, unsigned lb, int cb, int z0,
+                                              int nz) {
+  if constexpr (!ODD) return ld_pair<false>(base, lb, cb);
+  const float* p = lane_at(base, lb, cb);
+  return make_float2(p[0], z0 + 1 < nz ? p[1] : 0.f);
 }
 template <bool ODD>
-__device__ __forceinline__ void st_pair_in(float* p, unsigned off, float2 v) {
+__device__ __forceinline__ void st_pair_top(float* base, unsigned lb, int cb, int z0, int nz,
+                                            float2 v) {
   if constexpr (!ODD) {
-    *reinterpret_cast<float2*>(p + off) = v;
+    st_pair<false>(base, lb, cb, v);
   } else {
-    p[off] = v.x;
-    p[off + 1] = v.y;
+    float* p = lane_at(base, lb, cb);
+    p[0] = v.x;
+    if (z0 + 1 < nz) p[1] = v.y;
   }
 }
 
@@ -1123,6 +1157,7 @@ struct ZLanes {
   static constexpr int LW = ZTile<ZP>::LW;
   int t, q, l, line0, nlines, m_full;
   bool valid;
+  unsigned lb;  // byte offset of the lane's pair 0 (level 2q of line l) from line0
   __device__ __forceinline__ ZLanes(const ConvGeom& g, int wg, bool full) {
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     t = lane / ZP::P;
@@ -1132,7 +1167,10 @@ struct ZLanes {
     nlines = g.nx * g.ny;
     valid = full || line0 + l < nlines;
     m_full = min(ZP::V, g.nz / (2 * ZP::P));
+    lb = valid ? (unsigned)(l * g.nz + 2 * q) * 4u : 0u;
   }
+  // byte offset of pair m from the lane's pair 0
+  static constexpr int pair_bytes(int m) { return 8 * m * ZP::P; }
   // the lane's line is inside the x/y window (32-bit division)
   __device__ __forceinline__ bool in_window(const ConvGeom& g, const int (&win)[4]) const {
     const unsigned li = (unsigned)(line0 + l);
@@ -1155,21 +1193,22 @@ __device__ __forceinline__ void zfwd_reg_body(const PassDesc& d, int wg, float2*
   const int64_t base = (int64_t)ln.line0 * nz;
   const float* srcp = ((MODE == MODE_FLUX) ? d.out : d.in) + base;
   const float* nfp = d.nfield + base;
-  const unsigned loff = ln.valid ? (unsigned)(ln.l * nz) : 0u;
+  const unsigned lb = ln.lb;
   float2 v[V], nv[V];
 #pragma unroll
   for (int m = 0; m < V; ++m) {
     const int z0 = 2 * (q + m * P);
+    const int cb = ZLanes<ZP>::pair_bytes(m);
     v[m] = make_float2(0.f, 0.f);
     nv[m] = make_float2(0.f, 0.f);
     if (m < ln.m_full) {
       if (ln.valid) {
-        v[m] = ld_pair_in<ODD>(srcp, loff + z0);
-        if (MODE == MODE_FLUX) nv[m] = ld_pair_in<ODD>(nfp, loff + z0);
+        v[m] = ld_pair<ODD>(srcp, lb, cb);
+        if (MODE == MODE_FLUX) nv[m] = ld_pair<ODD>(nfp, lb, cb);
       }
     } else if (ln.valid && z0 < nz) {
-      v[m] = ld_zpair<ODD>(srcp, loff, z0, nz);
-      if (MODE == MODE_FLUX) nv[m] = ld_zpair<ODD>(nfp, loff, z0, nz);
+      v[m] = ld_pair_top<ODD>(srcp, lb, cb, z0, nz);
+      if (MODE == MODE_FLUX) nv[m] = ld_pair_top<ODD>(nfp, lb, cb, z0, nz);
     }
   }
   stage_tw2(g, tw2, N);
@@ -1196,10 +1235,11 @@ __device__ __forceinline__ void zfwd_reg_body(const PassDesc& d, int wg, float2*
           nonzero += (nv[m].x != 0.f) + (nv[m].y != 0.f);
         }
         if (skp) {
+          const int cb = ZLanes<ZP>::pair_bytes(m);
           if (full)
-            st_pair_in<ODD>(skp, loff + z0, pr);
+            st_pair<ODD>(skp, lb, cb, pr);
           else
-            st_zpair<ODD>(skp, loff, z0, nz, pr);
+            st_pair_top<ODD>(skp, lb, cb, z0, nz, pr);
         }
         v[m] = pr;
       }
@@ -1240,13 +1280,14 @@ __device__ __forceinline__ void zfwd_reg_body(const PassDesc& d, int wg, float2*
   float2* So = d.S + ln.line0;
   constexpr int KSTEP = ZR_NTH / TL::LW;  // planes per iteration
   const int kl = tid / TL::LW, c = tid % TL::LW;
-  const int64_t lane_off = (int64_t)kl * plane + c;
+  // (kl plane + c) complex, below 2^30 bytes for planes of at most 2^24 cells
+  const unsigned pb = (unsigned)(kl * ln.nlines + c) * 8u;
   const bool cvalid = FULL || ln.line0 + c < ln.nlines;
 #pragma unroll
   for (int i = 0; i < TL::ITER; ++i) {
     const int k = kl + i * KSTEP;
     if ((i + 1) * ZR_NTH <= TL::NF || k <= N) {
-      if (cvalid) So[(int64_t)(i * KSTEP) * plane + lane_off] = tile[k * TL::TP + c];
+      if (cvalid) *lane_at(So + (int64_t)(i * KSTEP) * plane, pb, 0) = tile[k * TL::TP + c];
     }
   }
   if (MODE == MODE_FLUX && !d.apply) {
@@ -1292,7 +1333,8 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
   const float2* Si = d.S + ln.line0;
   constexpr int KSTEP = ZR_NTH / TL::LW;
   const int kl = tid / TL::LW, c = tid % TL::LW;
-  const int64_t lane_off = (int64_t)kl * plane + c;
+  // (kl plane + c) complex, below 2^30 bytes for planes of at most 2^24 cells
+  const unsigned pb = (unsigned)(kl * ln.nlines + c) * 8u;
   const bool cvalid = FULL || ln.line0 + c < ln.nlines;
   float2* tile = lds_lines;
   {
@@ -1301,7 +1343,8 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
     for (int i = 0; i < TL::ITER; ++i) {
       const int k = kl + i * KSTEP;
       const bool in = (i + 1) * ZR_NTH <= TL::NF || k <= N;
-      u[i] = (in && cvalid) ? Si[(int64_t)(i * KSTEP) * plane + lane_off] : make_float2(0.f, 0.f);
+      u[i] = (in && cvalid) ? *lane_at(Si + (int64_t)(i * KSTEP) * plane, pb, 0)
+                            : make_float2(0.f, 0.f);
     }
     stage_tw2(g, tw2, N);
 #pragma unroll
@@ -1321,15 +1364,17 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
   // level is past the top and is neither written nor counted
   const int64_t base = (int64_t)ln.line0 * nz;
   float* outp = d.out + base;
-  const unsigned loff = ln.valid ? (unsigned)(ln.l * nz) : 0u;
+  const unsigned lb = ln.lb;
   float2 fl[V];
   // STEAM_ZR_PREFETCH 1: the flux before the transform, 0: after it
   auto load_flux = [&](int m) {
     const int z0 = 2 * (q + m * P);
+    const int cb = ZLanes<ZP>::pair_bytes(m);
     if (m < ln.m_full)
-      fl[m] = ln.valid ? ld_pair_in<ODD>(outp, loff + z0) : make_float2(0.f, 0.f);
+      fl[m] = ln.valid ? ld_pair<ODD>(outp, lb, cb) : make_float2(0.f, 0.f);
     else
-      fl[m] = (ln.valid && z0 < nz) ? ld_zpair<ODD>(outp, loff, z0, nz) : make_float2(0.f, 0.f);
+      fl[m] = (ln.valid && z0 < nz) ? ld_pair_top<ODD>(outp, lb, cb, z0, nz)
+                                    : make_float2(0.f, 0.f);
   };
   if (MODE == MODE_FLUX && STEAM_ZR_PREFETCH) {
 #pragma unroll
@@ -1358,10 +1403,11 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
 #pragma unroll
     for (int m = 0; m < V; ++m) {
       const int z0 = 2 * (q + m * P);
+      const int cb = ZLanes<ZP>::pair_bytes(m);
       if (m < ln.m_full) {
-        if (ln.valid) st_pair_in<ODD>(outp, loff + z0, v[m]);
+        if (ln.valid) st_pair<ODD>(outp, lb, cb, v[m]);
       } else if (ln.valid && z0 < nz) {
-        st_zpair<ODD>(outp, loff, z0, nz, v[m]);
+        st_pair_top<ODD>(outp, lb, cb, z0, nz, v[m]);
       }
     }
     return;
@@ -1388,10 +1434,11 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
     if (full ? ln.valid : (ln.valid && z0 < nz)) {
       if (count_clip) nclip += clipped;
       if (win) acc0 += (double)o0 + (double)o1;
+      const int cb = ZLanes<ZP>::pair_bytes(m);
       if (full)
-        st_pair_in<ODD>(outp, loff + z0, make_float2(o0, o1));
+        st_pair<ODD>(outp, lb, cb, make_float2(o0, o1));
       else
-        st_zpair<ODD>(outp, loff, z0, nz, make_float2(o0, o1));
+        st_pair_top<ODD>(outp, lb, cb, z0, nz, make_float2(o0, o1));
     }
   }
   if (!d.apply) {

@@ -28,15 +28,58 @@ struct RegFFT {
 
 __device__ __forceinline__ int pad8(int i) { return i + (i >> 3); }
 
+// v[r] *= w[r] (conjugated for the inverse), r = 1..7
+__device__ __forceinline__ void mul_twiddles8(float2 (&v)[8], const float2 (&w)[8], bool inv) {
+#pragma unroll
+  for (int r = 1; r < 8; ++r) {
+    float2 x = w[r];
+    if (inv) x.y = -x.y;
+    v[r] = cmul(v[r], x);
+  }
+}
+
+// The seven twiddles W^(r e), r = 1..7, of a stage from its bases W^e, W^2e
+// and W^4e: the other four are products, at most two deep, so each stays
+// within a few float32 ulps of the correctly rounded table value.
+__device__ __forceinline__ void expand_twiddles8(const float2 (&b)[3], float2 (&w)[8]) {
+  w[0] = make_float2(1.f, 0.f);
+  w[1] = b[0];
+  w[2] = b[1];
+  w[4] = b[2];
+  w[3] = cmul(b[0], b[1]);
+  w[5] = cmul(b[0], b[2]);
+  w[6] = cmul(b[1], b[2]);
+  w[7] = cmul(w[3], b[2]);
+}
+
 // Twiddle bases of a thread lane q for the radix-8 stages s >= 1: W^e, W^2e
 // and W^4e with e = k * step (k = q mod 8^s, step = N / 8^(s+1)), read from
-// the table once (a pass that transforms twice reuses them).  A stage's other
-// four twiddles are products of these, at most two deep, so each stays
-// within a few float32 ulps of the table's correctly rounded value.
+// the table once; apply() forms a stage's seven twiddles from them.
 template <int LOGN>
 struct RegTw {
   static constexpr int NS = RegFFT<LOGN>::N8 - 1;
   float2 b[NS > 0 ? NS : 1][3];
+  __device__ __forceinline__ void apply(float2 (&v)[8], int s, bool inv) const {
+    float2 w[8];
+    expand_twiddles8(b[s], w);
+    mul_twiddles8(v, w, inv);
+  }
+};
+
+// The same with every stage's seven twiddles formed once, for a pass that
+// transforms twice with one lane's twiddles (forward and inverse): the same
+// values, without re-forming the products in the second transform.
+template <int LOGN>
+struct RegTwFull {
+  static constexpr int NS = RegFFT<LOGN>::N8 - 1;
+  float2 w[NS > 0 ? NS : 1][8];
+  __device__ __forceinline__ explicit RegTwFull(const RegTw<LOGN>& t) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) expand_twiddles8(t.b[s], w[s]);
+  }
+  __device__ __forceinline__ void apply(float2 (&v)[8], int s, bool inv) const {
+    mul_twiddles8(v, w[s], inv);
+  }
 };
 
 template <int LOGN>
@@ -50,23 +93,6 @@ __device__ __forceinline__ void reg_tw_load(RegTw<LOGN>& t, int q,
     t.b[s - 1][0] = tw[e];
     t.b[s - 1][1] = tw[2 * e];
     t.b[s - 1][2] = tw[4 * e];
-  }
-}
-
-// v[r] *= W^(r e) (conjugated for the inverse), r = 1..7, from the bases
-__device__ __forceinline__ void stage_twiddle8(float2 (&v)[8], const float2 (&b)[3], bool inv) {
-  float2 w[8];
-  w[1] = b[0];
-  w[2] = b[1];
-  w[4] = b[2];
-  w[3] = cmul(b[0], b[1]);
-  w[5] = cmul(b[0], b[2]);
-  w[6] = cmul(b[1], b[2]);
-  w[7] = cmul(w[3], b[2]);
-#pragma unroll
-  for (int r = 1; r < 8; ++r) {
-    if (inv) w[r].y = -w[r].y;
-    v[r] = cmul(v[r], w[r]);
   }
 }
 
@@ -85,7 +111,7 @@ __device__ __forceinline__ void fft_reg(float2 (&v)[8], int q, float2* lds,
   for (int s = 0; s < F::N8; ++s) {
     const int Ns = 1 << (3 * s);
     const int k = q & (Ns - 1);
-    if (s > 0) stage_twiddle8(v, twb.b[s - 1], inv);
+    if (s > 0) twb.apply(v, s - 1, inv);
     dft_small<8>(v, inv);
     const bool last = (s == F::NST - 1);
     if (!last) {
@@ -126,11 +152,11 @@ __device__ __forceinline__ void fft_reg(float2 (&v)[8], int q, float2* lds,
 // The same transform with each thread holding C lanes of the line: lane
 // q0 + c * qstep (c < C) holds v[c].  The LDS exchanges of all C lanes share
 // one pair of workgroup barriers per stage.
-template <int LOGN, int C>
+template <int LOGN, int C, class TW>
 __device__ __forceinline__ void fft_reg_lanes(float2 (&v)[C][8], int q0, int qstep,
                                               float2* lds, int lstride, int loff,
                                               const float2* __restrict__ tw,
-                                              const RegTw<LOGN> (&twb)[C], bool inv) {
+                                              const TW (&twb)[C], bool inv) {
   using F = RegFFT<LOGN>;
   constexpr int P = F::P;
 #pragma unroll
@@ -138,7 +164,7 @@ __device__ __forceinline__ void fft_reg_lanes(float2 (&v)[C][8], int q0, int qst
     const int Ns = 1 << (3 * s);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      if (s > 0) stage_twiddle8(v[c], twb[c].b[s - 1], inv);
+      if (s > 0) twb[c].apply(v[c], s - 1, inv);
       dft_small<8>(v[c], inv);
     }
     const bool last = (s == F::NST - 1);
