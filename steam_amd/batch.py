@@ -38,9 +38,12 @@ def nfft_candidates(nz, kz, max_out=8):
     return [int(out[i]) for i in range(n.value)]
 
 
-def autotune_nfft(field_shape, kernel_shape, top=3, reps=5, table=None):
+def autotune_nfft(field_shape, kernel_shape, top=3, reps=10, trials=3, table=None):
     """Fastest z transform length among the ``top`` model candidates, timed
-    with HIP events on the convolution operator (cached per shape pair).
+    with HIP events on the convolution operator (cached per shape pair): the
+    best of ``trials`` runs of ``reps`` transforms each, the candidates taken
+    in turn within a trial, so a transient stall does not decide the choice
+    for a small field.
 
     Returns (n_fft, {n_fft: ms}).
     """
@@ -55,19 +58,24 @@ def autotune_nfft(field_shape, kernel_shape, top=3, reps=5, table=None):
     field = torch.randn(key[0], device='cuda', generator=gen)
     kern = torch.randn(key[1], device='cuda', generator=gen)
     out = torch.empty_like(field)
-    times = {}
+    ops = {}
     for n in cands:
         op = conv_operator(key[0], key[1], n_fft=n)
         spec = op.spectrum(kern)
         op.execute(field, out, spec)
-        start = torch.cuda.Event(enable_timing=True)
-        stop = torch.cuda.Event(enable_timing=True)
-        start.record()
-        for _ in range(reps):
-            op.execute(field, out, spec)
-        stop.record()
-        stop.synchronize()
-        times[n] = start.elapsed_time(stop) / reps
+        ops[n] = (op, spec)
+    times = {}
+    for _ in range(trials):
+        for n, (op, spec) in ops.items():
+            start = torch.cuda.Event(enable_timing=True)
+            stop = torch.cuda.Event(enable_timing=True)
+            start.record()
+            for _ in range(reps):
+                op.execute(field, out, spec)
+            stop.record()
+            stop.synchronize()
+            t = start.elapsed_time(stop) / reps
+            times[n] = min(times.get(n, t), t)
     best = min(times, key=times.get)
     table[key] = (best, times)
     return best, times

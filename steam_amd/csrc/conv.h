@@ -110,6 +110,17 @@ struct PassDesc {
 
 namespace steam {
 
+// Work split of the noise-field kernel: lines handled side by side per pass
+// (256 / nz of them when a line has at most 128 levels, else one) and lines
+// per workgroup (a whole number of passes, at least four lines).
+__host__ __device__ inline int noise_lines_per_pass(int nz) {
+  return (nz > 0 && nz <= 128) ? 256 / nz : 1;
+}
+__host__ __device__ inline int noise_lines_per_wg(int nz) {
+  const int u = noise_lines_per_pass(nz);
+  return u * ((4 + u - 1) / u);
+}
+
 // Per-line part of the keyed multiplier noise: the wrapped world x/y of the
 // line and whether it lies on the x/y centre sub-lattice.  World coordinates
 // are 32-bit here (the host rejects worlds or origins beyond +-2^30).

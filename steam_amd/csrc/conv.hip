@@ -278,7 +278,10 @@ extern "C" int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz,
   p->led_off = p->po_off + align256(nparts * 2 * 8);
   p->nf_off = p->led_off + align256(sizeof(DevLedger));
   p->ws_bytes = p->nf_off + align256(nx * ny * nz * 4);
-  p->wg_noise = (int)((nx * ny + 3) / 4);
+  {
+    const int lpw = noise_lines_per_wg((int)nz);
+    p->wg_noise = (int)((nx * ny + lpw - 1) / lpw);
+  }
   // the register x pass reads the kernel spectrum in its own column tiles,
   // each workgroup's block one contiguous run; the build writes the usual
   // [nzc][nx][ny] order after them and is then permuted
@@ -554,8 +557,12 @@ extern "C" int steam_flux_noise(const steam_conv_plan* p,
     ProfScope ps(STEAM_KF_NOISE, s);
     const int cap = noise_wgs_cap();
     const int grid = (cap > 0 && p->wg_noise > cap) ? cap : p->wg_noise;
-    hipLaunchKernelGGL((noise_field_kernel<SingleSrc, true>), dim3(grid), dim3(256), 0, s,
-                       SingleSrc{d});
+    if (noise_lines_per_pass((int)p->nz) > 1)
+      hipLaunchKernelGGL((noise_field_kernel<SingleSrc, true, NOISE_SHORT>), dim3(grid),
+                         dim3(256), 0, s, SingleSrc{d});
+    else
+      hipLaunchKernelGGL((noise_field_kernel<SingleSrc, true, NOISE_LONG>), dim3(grid),
+                         dim3(256), 0, s, SingleSrc{d});
   }
   STEAM_LAUNCH_CHECK("noise_field_kernel");
   return STEAM_OK;
@@ -606,8 +613,12 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
     d.nfield = const_cast<float*>(noise_dev);
   } else {
     ProfScope ps(STEAM_KF_NOISE, s);
-    hipLaunchKernelGGL((noise_field_kernel<SingleSrc>), dim3(p->wg_noise), dim3(256), 0,
-                       s, SingleSrc{d});
+    if (noise_lines_per_pass((int)p->nz) > 1)
+      hipLaunchKernelGGL((noise_field_kernel<SingleSrc, false, NOISE_SHORT>),
+                         dim3(p->wg_noise), dim3(256), 0, s, SingleSrc{d});
+    else
+      hipLaunchKernelGGL((noise_field_kernel<SingleSrc, false, NOISE_LONG>),
+                         dim3(p->wg_noise), dim3(256), 0, s, SingleSrc{d});
   }
   STEAM_LAUNCH_CHECK("noise_field_kernel");
   rc = run_operator<MODE_FLUX>(p, d, s);

@@ -212,26 +212,55 @@ __device__ __forceinline__ float2 kernel_bin_factor(const PassDesc& d, int kz) {
 #endif
 // STRIDE: a grid smaller than the field's line groups walks them with a
 // grid stride (a few workgroups per CU keep the VALU busy beside memory-bound
-// passes on another stream without taking their slots)
-template <class Src, bool STRIDE = false>
+// passes on another stream without taking their slots).  A workgroup takes
+// noise_lines_per_wg(nz) consecutive lines, noise_lines_per_pass(nz) of them
+// side by side per pass when lines are short (lane t then holds level
+// t mod nz of line t / nz, so a 48-level line keeps 240 of 256 lanes busy
+// rather than 48); a line of more than 128 levels is strided over by the
+// workgroup, with its x, y and keyed-noise set-up uniform.
+// LINES: NOISE_LONG or NOISE_SHORT when every instance of the launch has
+// lines of that kind (compiled for it alone), NOISE_ANY otherwise.
+enum { NOISE_ANY = 0, NOISE_LONG = 1, NOISE_SHORT = 2 };
+template <class Src, bool STRIDE = false, int LINES = NOISE_ANY>
 __global__ __launch_bounds__(256) void noise_field_kernel(Src src) {
   int wg = blockIdx.x;
   const PassDesc& d = src.get(PASS_NOISE, wg);
   const ConvGeom& g = d.g;
   const int nz = g.nz;
   const int64_t lines = (int64_t)g.nx * g.ny;
-  const int lpw = 4;
+  const int U = LINES == NOISE_LONG ? 1 : noise_lines_per_pass(nz);
+  const int lpw = LINES == NOISE_LONG ? 4 : noise_lines_per_wg(nz);
   const int64_t stride =
       STRIDE ? (int64_t)(d.wg_noise_n > 0 ? d.wg_noise_n : (int)gridDim.x) * lpw : lines;
+  if (LINES == NOISE_LONG || (LINES == NOISE_ANY && U == 1)) {
+    for (int64_t base = (int64_t)wg * lpw; base < lines; base += stride) {
+      for (int l = 0; l < lpw; ++l) {
+        const int64_t line = base + l;
+        if (line >= lines) break;
+        const int x = (int)(line / g.ny);
+        const int y = (int)(line - (int64_t)x * g.ny);
+        const LineNoise ln = noise_line(d.noise, x, y);
+        float* o = d.nfield + line * nz;
+        for (int z = threadIdx.x; z < nz; z += 256) o[z] = noise_at(d.noise, ln, z, nz);
+      }
+    }
+    return;
+  }
+  if constexpr (LINES == NOISE_LONG) return;
+  // short lines: a fixed (line slot, level) per lane; the lanes past U
+  // lines have no cell (the kernel has no barrier, so they leave)
+  const int sub = (int)threadIdx.x / nz;
+  const int z = (int)threadIdx.x - sub * nz;
+  if (sub >= U) return;
   for (int64_t base = (int64_t)wg * lpw; base < lines; base += stride) {
-    for (int l = 0; l < lpw; ++l) {
-      const int64_t line = base + l;
+    for (int l0 = 0; l0 < lpw; l0 += U) {
+      const int64_t line = base + l0 + sub;
       if (line >= lines) break;
-      const int x = (int)(line / g.ny);
-      const int y = (int)(line - (int64_t)x * g.ny);
-      const LineNoise ln = noise_line(d.noise, x, y);
-      float* o = d.nfield + line * nz;
-      for (int z = threadIdx.x; z < nz; z += 256) o[z] = noise_at(d.noise, ln, z, nz);
+      // lines < 2^24 (x/y extents of at most 4096)
+      const unsigned ul = (unsigned)line;
+      const int x = (int)(ul / (unsigned)g.ny);
+      const int y = (int)(ul - (unsigned)x * (unsigned)g.ny);
+      d.nfield[line * nz + z] = noise_at(d.noise, noise_line(d.noise, x, y), z, nz);
     }
   }
 }
