@@ -80,6 +80,12 @@ def parse():
     ap.add_argument("--pipelined-only", action="store_true",
                     help="only run the reported (pipelined) timed region of the class "
                          "advance, for profiling it alone; prints one JSON line")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only run the batched form (48 realisations); prints one "
+                         "JSON line")
+    ap.add_argument("--served-only", action="store_true",
+                    help="only run the served form (tile stream at --serve-rate and "
+                         "the --serve-sweep rates); prints one JSON line")
     ap.add_argument("--noise-ahead", type=int, default=2,
                     help="classes whose noise the pipelined form generates ahead")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -765,6 +771,17 @@ def main():
         from steam_amd import _lib
         _lib.require_gpu('cuda')
         print(json.dumps(refine_form()))
+        return
+    if args.batched_only:
+        from steam_amd import _lib
+        _lib.require_gpu('cuda')
+        print(json.dumps(batched_form(args.steps, args.warmup)))
+        return
+    if args.served_only:
+        from steam_amd import _lib
+        _lib.require_gpu('cuda')
+        print(json.dumps(served_form(args.serve_world, args.serve_requests,
+                                     args.serve_rate, args.serve_sweep)))
         return
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (got --gpus {args.gpus}); the "

@@ -337,6 +337,17 @@ int steam_batch_info(const steam_batch* batch, int32_t* launches,
  * instance's ledger stays in its workspace (steam_flux_ledger_read). */
 int steam_batch_advance(steam_batch* batch, steam_stream_t stream);
 
+/* steam_batch_advance without its descriptor upload, for a batch whose
+ * every instance reads a noise buffer: the advance runs on the descriptors
+ * that the batch's last steam_batch_noise uploaded (the caller orders that
+ * call before this one, and calls steam_batch_set_noise on the batch only
+ * after it).  A pipelined caller then writes a batch's staging copy for only
+ * one copy per step, its noise upload, so replays need not wait for a whole
+ * step before the next step's keys are written.  No counterpart in the
+ * reference, which has no batched or pipelined advance (one instance per
+ * call, steam/simulate.py:2066-2228). */
+int steam_batch_advance_after_noise(steam_batch* batch, steam_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * Scalar increments of the cascade (steam/simulate.py:1924-2029): the joint
  * pattern, per-level window reductions and the bounded amplitude-preserving

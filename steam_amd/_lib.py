@@ -120,6 +120,7 @@ _SIGNATURES = [
     ("steam_batch_set_noise", ctypes.c_int, [_P, _I32, ctypes.POINTER(NoiseParams)]),
     ("steam_batch_info", ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),
     ("steam_batch_advance", ctypes.c_int, [_P, _P]),
+    ("steam_batch_advance_after_noise", ctypes.c_int, [_P, _P]),
     ("steam_batch_mark_replay", ctypes.c_int, [_P, _P]),
     ("steam_batch_noise", ctypes.c_int, [_P, _P]),
     ("steam_scalar_pattern", ctypes.c_int,

@@ -217,6 +217,13 @@ class BatchedAdvance:
                                                    _lib.stream_handle(stream)),
                    "steam_batch_advance")
 
+    def advance_after_noise(self, stream=None):
+        """``advance`` without its descriptor upload: the descriptors are the
+        ones this batch's last ``generate_noise`` uploaded (every instance
+        reads a noise buffer); asynchronous."""
+        _lib.check(_lib.load().steam_batch_advance_after_noise(
+            self.handle, _lib.stream_handle(stream)), "steam_batch_advance_after_noise")
+
     def generate_noise(self, stream=None):
         """Write every instance's noise for its next advance into its noise
         buffer (batches created with ``noise_buffers``); asynchronous."""
@@ -285,6 +292,13 @@ class PipelinedBatch:
     captures into one graph (``capture``: one graph per parity, replayed in
     turn).  Results equal the plain batched advance with the same keys.
 
+    A batch's descriptors go up once per use, with its noise: step i's
+    advance runs on what batch i % 2's noise launch of step i - 1 uploaded.
+    So a replay of step i reads only batch (i + 1) % 2's staging copy, and
+    the keys of step i + 2 may be written as soon as step i's replay has
+    run, while step i + 1 is still queued: the host never drains the device
+    between steps.
+
     ``set_regions(regions)`` gives the keys of the next step whose noise is
     not yet generated (step i + 1 while step i is queued)."""
 
@@ -321,9 +335,10 @@ class PipelinedBatch:
 
     @property
     def launches_per_step(self):
-        """The advance's launches plus the next step's noise (its descriptor
-        upload and one noise launch: every instance shares the noise kernel)."""
-        return self.batches[0].launches_per_advance + 2
+        """The advance's launches (its descriptors are already on the device)
+        plus the next step's noise (its descriptor upload and one noise
+        launch: every instance shares the noise kernel)."""
+        return self.batches[0].launches_per_advance - 1 + 2
 
     def set_regions(self, regions):
         """Keys of step ``self.step + 1`` (the next noise to be generated)."""
@@ -334,7 +349,10 @@ class PipelinedBatch:
         self.side.wait_stream(main)          # buffers 1 - b were last read by step i - 1
         with torch.cuda.stream(self.side):
             self.batches[1 - b].generate_noise(self.side)
-        (self.first if first else self.batches[b]).advance(main)
+        if first:
+            self.first.advance(main)
+        else:
+            self.batches[b].advance_after_noise(main)
         main.wait_stream(self.side)
 
     def advance(self):
@@ -368,8 +386,9 @@ class PipelinedBatch:
 
     def replay(self):
         self.graphs[self.step % 2].replay()
-        for batch in self.batches:
-            batch.mark_replay()
+        # the replay's only descriptor upload read the staging copy of the
+        # batch whose noise it generated
+        self.batches[(self.step + 1) % 2].mark_replay()
         self.step += 1
 
     def ledgers(self):

@@ -50,6 +50,7 @@ struct steam_batch {
   std::vector<PassDesc> items;   // per-instance descriptors (host master copy)
   std::vector<ItemPlan> plans;
   std::vector<int> slot_item;    // flat descriptor slot -> instance
+  std::vector<std::vector<int>> item_slots;  // instance -> its flat slots
   std::vector<Group> groups[BP_COUNT];
   PassDesc* host = nullptr;      // pinned staging of the flat array
   PassDesc* dev = nullptr;
@@ -100,9 +101,11 @@ std::tuple<int, int, int> variant_of(const ItemPlan& ip, int pass) {
       return {ip.tile ? 2 : 3, radix_set(g.az), Z_NTH};
     case BP_Y:
       if (g.reg_y) return {1, g.reg_y, 256};
+      if (g.mr_y) return {3, g.mr_y, XYMR_NTH};
       return {2, radix_set(g.ay), FFT_NTH};
     case BP_X:
       if (g.reg_x) return {1, g.reg_x, (g.nx / 8 / xreg_lanes(g.reg_x)) * g.Tr};
+      if (g.mr_x) return {3, g.mr_x, XYMR_NTH};
       return {2, radix_set(g.ax), FFT_NTH};
     case BP_FIN: return {0, 0, FIN_NTH};
     default: return {ip.post ? 0 : -1, ip.post_rescale ? 1 : 0, FFT_NTH};
@@ -113,8 +116,8 @@ size_t lds_of(const ItemPlan& ip, int pass) {
   switch (pass) {
     case BP_Z: return ip.zreg ? 0 : ip.p->lds_z;
     case BP_ZINV: return ip.zreg ? 0 : ip.p->lds_zinv;
-    case BP_Y: return ip.p->g.reg_y ? 0 : ip.p->lds_y;
-    case BP_X: return ip.p->g.reg_x ? 0 : ip.p->lds_x;
+    case BP_Y: return (ip.p->g.reg_y || ip.p->g.mr_y) ? 0 : ip.p->lds_y;
+    case BP_X: return (ip.p->g.reg_x || ip.p->g.mr_x) ? 0 : ip.p->lds_x;
     default: return 0;
   }
 }
@@ -181,15 +184,17 @@ void build_groups(steam_batch* b) {
   }
   b->nslots = (int)flat.size();
   memcpy(b->host, flat.data(), flat.size() * sizeof(PassDesc));
+  b->item_slots.assign(b->n, {});
+  for (int s = 0; s < b->nslots; ++s) b->item_slots[b->slot_item[s]].push_back(s);
 }
 
-// refresh the flat array from the per-instance master copies (keeps each
-// slot's workgroup offsets)
-void refresh_slots(steam_batch* b) {
-  for (int s = 0; s < b->nslots; ++s) {
+// refresh instance i's slots of the flat array from its master copy (keeps
+// each slot's workgroup offsets)
+void refresh_item(steam_batch* b, int i) {
+  for (int s : b->item_slots[i]) {
     int wb[7];
     memcpy(wb, b->host[s].wg_base, sizeof(wb));
-    b->host[s] = b->items[b->slot_item[s]];
+    b->host[s] = b->items[i];
     memcpy(b->host[s].wg_base, wb, sizeof(wb));
   }
 }
@@ -227,6 +232,11 @@ int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
         if (gr.variant == 1) {
           STEAM_LOGN_DISPATCH(gr.arg, hipLaunchKernelGGL((ypass_reg_kernel<BatchSrc, LN_>), grid,
                                                          dim3(256), 0, s, src, inv));
+        } else if (gr.variant == 3) {
+          with_xyplan(gr.arg, [&](auto xp) {
+            hipLaunchKernelGGL((ypass_mr_kernel<BatchSrc, decltype(xp)>), grid, dim3(XYMR_NTH),
+                               0, s, src, inv);
+          });
         } else {
           STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((ypass_kernel<BatchSrc, RS_>), grid,
                                                        dim3(FFT_NTH), gr.lds, s, src, inv));
@@ -237,6 +247,11 @@ int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
           STEAM_LOGN_DISPATCH(gr.arg, hipLaunchKernelGGL(
               (xpass_reg_kernel<BatchSrc, 0, LN_, xreg_lanes(LN_)>), grid, dim3(gr.threads), 0, s,
               src));
+        } else if (gr.variant == 3) {
+          with_xyplan(gr.arg, [&](auto xp) {
+            hipLaunchKernelGGL((xpass_mr_kernel<BatchSrc, 0, decltype(xp)>), grid, dim3(XYMR_NTH),
+                               0, s, src);
+          });
         } else {
           STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((xpass_kernel<BatchSrc, 0, RS_>), grid,
                                                        dim3(FFT_NTH), gr.lds, s, src));
@@ -378,7 +393,7 @@ extern "C" int steam_batch_set_noise(steam_batch* b, int i,
     b->pending_upload = false;
   }
   b->items[i].noise = make_noise_dev(noise);
-  refresh_slots(b);
+  refresh_item(b, i);
   return STEAM_OK;
 }
 
@@ -419,17 +434,9 @@ extern "C" int steam_batch_noise(steam_batch* b, steam_stream_t stream) {
   return launch_pass(b, BP_NOISE_EXT, s, 0);
 }
 
-extern "C" int steam_batch_advance(steam_batch* b, steam_stream_t stream) {
-  if (!b) return set_error(STEAM_EINVAL, "steam_batch_advance: NULL batch");
-  hipStream_t s = (hipStream_t)stream;
-  STEAM_HIP(hipMemcpyAsync(b->dev, b->host, (size_t)b->nslots * sizeof(PassDesc),
-                           hipMemcpyHostToDevice, s));
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  STEAM_HIP(hipStreamIsCapturing(s, &cs));
-  if (cs == hipStreamCaptureStatusNone) {
-    STEAM_HIP(hipEventRecord(b->uploaded, s));
-    b->pending_upload = true;
-  }
+namespace {
+
+int advance_passes(steam_batch* b, hipStream_t s) {
   int rc;
   {
     ProfScope ps(STEAM_KF_NOISE, s);
@@ -464,4 +471,29 @@ extern "C" int steam_batch_advance(steam_batch* b, steam_stream_t stream) {
     if ((rc = launch_pass(b, BP_POST, s, 0))) return rc;
   }
   return STEAM_OK;
+}
+
+}  // namespace
+
+extern "C" int steam_batch_advance(steam_batch* b, steam_stream_t stream) {
+  if (!b) return set_error(STEAM_EINVAL, "steam_batch_advance: NULL batch");
+  hipStream_t s = (hipStream_t)stream;
+  STEAM_HIP(hipMemcpyAsync(b->dev, b->host, (size_t)b->nslots * sizeof(PassDesc),
+                           hipMemcpyHostToDevice, s));
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  STEAM_HIP(hipStreamIsCapturing(s, &cs));
+  if (cs == hipStreamCaptureStatusNone) {
+    STEAM_HIP(hipEventRecord(b->uploaded, s));
+    b->pending_upload = true;
+  }
+  return advance_passes(b, s);
+}
+
+extern "C" int steam_batch_advance_after_noise(steam_batch* b, steam_stream_t stream) {
+  if (!b) return set_error(STEAM_EINVAL, "steam_batch_advance_after_noise: NULL batch");
+  if (b->groups[BP_NOISE_EXT].empty() || !b->groups[BP_NOISE].empty())
+    return set_error(STEAM_EINVAL,
+                     "steam_batch_advance_after_noise: every instance must read a noise "
+                     "buffer (the descriptors come from the batch's steam_batch_noise)");
+  return advance_passes(b, (hipStream_t)stream);
 }

@@ -20,6 +20,7 @@ struct ConvGeom {
   int nyt_z, nyt_x;     // y-tiles per x line (z passes), per plane (x pass)
   int wg_z, wg_y, wg_x; // workgroups per pass
   int reg_y, reg_x;     // log2 length of the register FFT path (0: generic)
+  int mr_y, mr_x;       // length of the mixed-radix register path (0: none)
   int Tr;               // columns per workgroup in the register x pass
   int tile_z;           // single-block z passes with vector tile loads
   int zreg;             // single-block register z passes (compiled n2)

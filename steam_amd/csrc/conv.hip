@@ -44,6 +44,17 @@ static void factor_axis(int n, AxisPlan& ap) {
   if (m != 1) ap.radix[ap.nst++] = m;  // a prime, or the product of the rest
 }
 
+// the mixed-radix register x/y passes: on unless STEAM_XY_MIXED_RADIX=0 in
+// the environment (read once; the generic LDS passes then serve those
+// lengths, for measurement)
+static bool xy_mixed_radix() {
+  static const bool on = [] {
+    const char* e = getenv("STEAM_XY_MIXED_RADIX");
+    return !(e && *e == '0');
+  }();
+  return on;
+}
+
 // log2 of a power of two in [64, 2048], else 0 (no register FFT path)
 static int reg_log2(int n) {
   for (int l = 6; l <= 11; ++l)
@@ -92,15 +103,23 @@ static void set_tiles(ConvGeom& g, int nz_acc, int kahan, size_t& lds_z,
   const int64_t P = (int64_t)g.nblocks * g.nzc;
   g.reg_y = reg_log2(g.ny);
   g.reg_x = reg_log2(g.nx);
+  g.mr_y = (!g.reg_y && xy_mixed_radix() && xyplan_lanes(g.ny)) ? g.ny : 0;
+  g.mr_x = (!g.reg_x && xy_mixed_radix() && xyplan_lanes(g.nx)) ? g.nx : 0;
   g.wg_z = g.nx * g.nyt_z;
   if (g.reg_y) {
     const int lpw = 256 / (g.ny / 8);
+    g.wg_y = (int)((P * g.nx + lpw - 1) / lpw);
+  } else if (g.mr_y) {
+    const int lpw = XYMR_NTH / xyplan_lanes(g.ny);
     g.wg_y = (int)((P * g.nx + lpw - 1) / lpw);
   } else {
     g.wg_y = (int)((P * g.nx + g.Ty - 1) / g.Ty);
   }
   if (g.reg_x) {
     g.Tr = xreg_cols(g.reg_x);
+    g.wg_x = (int)(P * ((g.ny + g.Tr - 1) / g.Tr));
+  } else if (g.mr_x) {
+    g.Tr = XYMR_NTH / xyplan_lanes(g.nx);
     g.wg_x = (int)(P * ((g.ny + g.Tr - 1) / g.Tr));
   } else {
     g.Tr = 0;
@@ -287,7 +306,7 @@ extern "C" int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz,
   // [nzc][nx][ny] order after them and is then permuted
   // a spectrum buffer starts with its format word and the symmetry test's
   // counts (KSPEC_DATA_OFF bytes), then holds the spectrum
-  p->ktile = g.reg_x ? g.Tr : 0;
+  p->ktile = (g.reg_x || g.mr_x) ? g.Tr : 0;
   if (p->ktile) {
     const int64_t nyt = (ny + p->ktile - 1) / p->ktile;
     p->kstd_off = KSPEC_DATA_OFF + align256((int64_t)g.nzc * nyt * p->ktile * nx * 8);

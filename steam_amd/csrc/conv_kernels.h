@@ -1090,6 +1090,164 @@ inline bool zplan_lines(int n2, int& lines) {
   return with_zplan(n2, [&](auto zp) { lines = decltype(zp)::LINES * ZR_WAVES; });
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-radix register x/y passes: the x/y lengths that are not a power of
+// two but have a compiled plan (2^a 3^b 5^c lengths such as a batch's 96 and
+// 192 or a served tile's padded 80, 144, 288), from HBM to registers and
+// back as the radix-8 register passes do.  A line of N points is held by
+// P = N / V lanes, lane q holding the V values q + m P.
+// ---------------------------------------------------------------------------
+using XP288 = ZPlan<288, 18, 6, 6, 2, 2, 2>;
+using XP384 = ZPlan<384, 24, 8, 8, 6>;
+
+// call f(plan) for an x/y length with a mixed-radix register plan
+template <class F>
+inline bool with_xyplan(int n, F&& f) {
+  switch (n) {
+    case 32: f(ZP32{}); return true;
+    case 80: f(ZP80{}); return true;
+    case 96: f(ZP96{}); return true;
+    case 144: f(ZP144{}); return true;
+    case 160: f(ZP160{}); return true;
+    case 192: f(ZP192{}); return true;
+    case 288: f(XP288{}); return true;
+    case 320: f(ZP320{}); return true;
+    case 384: f(XP384{}); return true;
+    default: return false;
+  }
+}
+constexpr int XYMR_NTH = 256;
+// lanes per line of an x/y length's plan (0: no plan)
+inline int xyplan_lanes(int n) {
+  int p = 0;
+  with_xyplan(n, [&](auto xp) { p = decltype(xp)::P; });
+  return p;
+}
+
+// y pass: 256 / P contiguous y lines per workgroup, each wave-private
+// (a wave holds 64 / P of them), lines in LDS at the z plans' pitch
+template <class Src, class XP>
+__global__ __launch_bounds__(XYMR_NTH) void ypass_mr_kernel(Src src, int inv) {
+  constexpr int N = XP::N, V = XP::V, P = XP::P, LPW = XYMR_NTH / P;
+  __shared__ float2 lds[LPW * XP::PITCH + N];
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_Y, wg);
+  const ConvGeom& g = d.g;
+  const int tid = threadIdx.x;
+  const int q = tid % P, lw = tid / P;
+  float2* tw = lds + LPW * XP::PITCH;
+  for (int i = tid; i < N; i += XYMR_NTH) tw[i] = g.ay.tw[i];
+  __syncthreads();
+  const int planes = (d.mode == MODE_KERNEL) ? g.nzc : g.nblocks * g.nzc;
+  const int64_t lines = (int64_t)planes * g.nx;
+  const int64_t line = (int64_t)wg * LPW + lw;
+  const bool valid = line < lines;
+  float2* Sl = d.S + (valid ? line : 0) * N;
+  const unsigned lo = (unsigned)q * 8u;
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m)
+    v[m] = valid ? *lane_at(Sl, lo, m * P * 8) : make_float2(0.f, 0.f);
+  float2* line_lds = lds + lw * XP::PITCH;
+  if (inv)
+    XP::template fft_xy<true, false>(v, q, line_lds, 1, 0, tw);
+  else
+    XP::template fft_xy<false, false>(v, q, line_lds, 1, 0, tw);
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) *lane_at(Sl, lo, m * P * 8) = v[m];
+  }
+}
+
+// x pass: T = 256 / P adjacent y columns per workgroup (thread t % T is the
+// column, so a row access is a T*8-byte run); column lines in LDS at
+// zpad(i) * (T + 1) + t.  The kernel spectrum is in the pass's column tiles
+// [kz][y/T][x][T], or real and x-even [kz][y/T][nx/2 + 1][T] (KREAL: bin x
+// takes the value of min(x, nx - x); for x = q + m P that is x for m < V/2
+// and (V - m) P - q from m = V/2 on).  XMODE 0: forward, * spectrum,
+// inverse; 1: forward, * the kernel-build scale.
+template <int XMODE, class XP, bool KREAL>
+__device__ __forceinline__ void xpass_mr_body(const PassDesc& d, int wg, float2* lds,
+                                              const float2* tw) {
+  constexpr int N = XP::N, V = XP::V, P = XP::P, T = XYMR_NTH / P, LS = T + 1;
+  const ConvGeom& g = d.g;
+  const int tid = threadIdx.x;
+  const int t = tid % T, q = tid / T;
+  const int nyt = (g.ny + T - 1) / T;
+  const int p = wg / nyt;
+  const int y = (wg - p * nyt) * T + t;
+  const bool valid = y < g.ny;
+  const int ny = g.ny;
+  float2* Sb = d.S + (int64_t)p * N * ny;
+  const unsigned lb = (unsigned)(q * ny + (valid ? y : 0)) * 8u;
+  auto row = [&](int m) { return Sb + (int64_t)m * P * ny; };
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) v[m] = valid ? *lane_at(row(m), lb, 0) : make_float2(0.f, 0.f);
+  using KV = typename std::conditional<KREAL, float, float2>::type;
+  KV kv[V];
+  if (XMODE == 0) {
+    const int64_t tile = (int64_t)(p % g.nzc) * nyt + (wg - p * nyt);
+    if constexpr (KREAL) {
+      const float* Kb = reinterpret_cast<const float*>(d.K) + tile * (N / 2 + 1) * T;
+      const unsigned klo = (unsigned)(q * T + t) * 4u;
+      const unsigned khi = (unsigned)((P - q) * T + t) * 4u;
+#pragma unroll
+      for (int m = 0; m < V; ++m)
+        kv[m] = !valid ? 0.f
+                : m < V / 2 ? *lane_at(Kb + m * P * T, klo, 0)
+                            : *lane_at(Kb + (V - 1 - m) * P * T, khi, 0);
+    } else {
+      const float2* Kb = d.K + tile * N * T;
+      const unsigned klo = (unsigned)(q * T + t) * 8u;
+#pragma unroll
+      for (int m = 0; m < V; ++m)
+        kv[m] = valid ? *lane_at(Kb + m * P * T, klo, 0) : make_float2(0.f, 0.f);
+    }
+  }
+  XP::template fft_xy<false, true>(v, q, lds, LS, t, tw);
+  if (XMODE == 0) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      if constexpr (KREAL)
+        v[m] = make_float2(v[m].x * kv[m], v[m].y * kv[m]);
+      else
+        v[m] = cmul(v[m], kv[m]);
+    }
+    XP::template fft_xy<true, true>(v, q, lds, LS, t, tw);
+  } else {
+    const float2 f = kernel_bin_factor(d, p % g.nzc);
+#pragma unroll
+    for (int m = 0; m < V; ++m) v[m] = cmul(v[m], f);
+  }
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) *lane_at(row(m), lb, 0) = v[m];
+  }
+}
+
+template <class XP>
+struct XMrLds {
+  static constexpr int T = XYMR_NTH / XP::P;
+  static constexpr int LINE = (XP::N + XP::N / 8) * (T + 1);
+};
+
+template <class Src, int XMODE, class XP>
+__global__ __launch_bounds__(XYMR_NTH) void xpass_mr_kernel(Src src) {
+  constexpr int N = XP::N;
+  __shared__ float2 lds[XMrLds<XP>::LINE + N];
+  int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const PassDesc& d = src.get(PASS_X, wg);
+  float2* tw = lds + XMrLds<XP>::LINE;
+  for (int i = threadIdx.x; i < N; i += XYMR_NTH) tw[i] = d.g.ax.tw[i];
+  __syncthreads();
+  // the spectrum's format word is uniform over the launch's instance
+  if (XMODE == 0 && d.kfmt && *d.kfmt == KFMT_REAL_EVEN)
+    xpass_mr_body<XMODE, XP, true>(d, wg, lds, tw);
+  else
+    xpass_mr_body<XMODE, XP, false>(d, wg, lds, tw);
+}
+
 // The workgroup's plane tile: planes k = 0..N of its LW = ZR_WAVES * LINES
 // consecutive lines, tile[k * (LW + 1) + l], in the space of the wave lines.
 template <class ZP>

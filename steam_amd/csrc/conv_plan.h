@@ -81,6 +81,11 @@ inline void launch_ypass(const ConvGeom& g, size_t lds_y, hipStream_t s,
     STEAM_LOGN_DISPATCH(g.reg_y, hipLaunchKernelGGL((ypass_reg_kernel<SrcT, LN_>),
                                                     dim3(g.wg_y), dim3(256), 0, s,
                                                     src, inv));
+  } else if (g.mr_y) {
+    with_xyplan(g.mr_y, [&](auto xp) {
+      hipLaunchKernelGGL((ypass_mr_kernel<SrcT, decltype(xp)>), dim3(g.wg_y),
+                         dim3(XYMR_NTH), 0, s, src, inv);
+    });
   } else {
     const int rsy = radix_set(g.ay);
     STEAM_RS_DISPATCH(rsy, hipLaunchKernelGGL((ypass_kernel<SrcT, RS_>), dim3(g.wg_y),
@@ -96,6 +101,11 @@ inline void launch_xpass(const ConvGeom& g, size_t lds_x, hipStream_t s,
     STEAM_LOGN_DISPATCH(g.reg_x, hipLaunchKernelGGL(
         (xpass_reg_kernel<SrcT, XMODE, LN_, xreg_lanes(LN_)>), dim3(g.wg_x), dim3(threads),
         0, s, src));
+  } else if (g.mr_x) {
+    with_xyplan(g.mr_x, [&](auto xp) {
+      hipLaunchKernelGGL((xpass_mr_kernel<SrcT, XMODE, decltype(xp)>), dim3(g.wg_x),
+                         dim3(XYMR_NTH), 0, s, src);
+    });
   } else {
     const int rsx = radix_set(g.ax);
     STEAM_RS_DISPATCH(rsx, hipLaunchKernelGGL((xpass_kernel<SrcT, XMODE, RS_>),

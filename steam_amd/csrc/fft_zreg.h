@@ -273,6 +273,57 @@ struct ZStages {
   }
 };
 
+// The same stages for x/y lines, with the line in LDS at
+// lds[zpad(i) * ls + loff] and stage twiddles from a table of W_N^k (k < N).
+// WG: the lanes of a line span wavefronts (a column of the x pass), so an
+// exchange is bracketed by workgroup barriers; otherwise the line is
+// wave-private as in ZStages.  Every thread of the workgroup calls it.
+template <int N, int V, bool INV, bool WG, int NS, int R, int... REST>
+struct XYStages {
+  static __device__ __forceinline__ void run(float2 (&v)[V], int q, float2* lds, int ls,
+                                             int loff, const float2* tw) {
+    constexpr int P = N / V;
+    constexpr int NB = V / R;
+    static_assert(V % R == 0, "radix must divide the values per lane");
+    constexpr int STEP = N / (NS * R);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = q + b * P;
+      const int k = (NS == 1) ? 0 : j % NS;
+      float2 u[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 x = v[b + NB * r];
+        if (NS > 1 && r > 0) {
+          float2 w = tw[r * k * STEP];
+          if (INV) w.y = -w.y;
+          x = cmul(x, w);
+        }
+        u[r] = x;
+      }
+      Dft<R, INV>::run(u);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + NB * r] = u[r];
+    }
+    if constexpr (sizeof...(REST) > 0) {
+      if (WG) __syncthreads();  // the previous exchange's reads are done
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int j = q + b * P;
+        const int k = (NS == 1) ? 0 : j % NS;
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[zpad(base + r * NS) * ls + loff] = v[b + NB * r];
+      }
+      if (WG) __syncthreads(); else wave_lds_sync();
+#pragma unroll
+      for (int m = 0; m < V; ++m) v[m] = lds[zpad(q + m * P) * ls + loff];
+      if (!WG) wave_lds_sync();
+      XYStages<N, V, INV, WG, NS * R, REST...>::run(v, q, lds, ls, loff, tw);
+    }
+  }
+};
+
 // a z-line plan: length N, V values per lane, radices in stage order
 template <int N_, int V_, int... RS>
 struct ZPlan {
@@ -286,6 +337,12 @@ struct ZPlan {
   static __device__ __forceinline__ void fft(float2 (&v)[V_], int q, float2* line,
                                              const float2* tw2) {
     ZStages<N_, V_, INV, 1, RS...>::run(v, q, line, tw2);
+  }
+  // the same plan over an x/y line (XYStages)
+  template <bool INV, bool WG>
+  static __device__ __forceinline__ void fft_xy(float2 (&v)[V_], int q, float2* lds, int ls,
+                                                int loff, const float2* tw) {
+    XYStages<N_, V_, INV, WG, 1, RS...>::run(v, q, lds, ls, loff, tw);
   }
 };
 
