@@ -1166,10 +1166,19 @@ __global__ __launch_bounds__(XYMR_NTH) void ypass_mr_kernel(Src src, int inv) {
 // takes the value of min(x, nx - x); for x = q + m P that is x for m < V/2
 // and (V - m) P - q from m = V/2 on).  XMODE 0: forward, * spectrum,
 // inverse; 1: forward, * the kernel-build scale.
+template <class XP>
+struct XMrLds {
+  static constexpr int T = XYMR_NTH / XP::P;
+  // column stride: T + 1 (the lane groups of a wave start in different
+  // banks) unless that leaves room for only one workgroup per CU
+  static constexpr int LS = ((XP::N + XP::N / 8) * (T + 1) + XP::N) * 8 > 80 * 1024 ? T : T + 1;
+  static constexpr int LINE = (XP::N + XP::N / 8) * LS;
+};
+
 template <int XMODE, class XP, bool KREAL>
 __device__ __forceinline__ void xpass_mr_body(const PassDesc& d, int wg, float2* lds,
                                               const float2* tw) {
-  constexpr int N = XP::N, V = XP::V, P = XP::P, T = XYMR_NTH / P, LS = T + 1;
+  constexpr int N = XP::N, V = XP::V, P = XP::P, T = XYMR_NTH / P, LS = XMrLds<XP>::LS;
   const ConvGeom& g = d.g;
   const int tid = threadIdx.x;
   const int t = tid % T, q = tid / T;
@@ -1186,7 +1195,10 @@ __device__ __forceinline__ void xpass_mr_body(const PassDesc& d, int wg, float2*
   for (int m = 0; m < V; ++m) v[m] = valid ? *lane_at(row(m), lb, 0) : make_float2(0.f, 0.f);
   using KV = typename std::conditional<KREAL, float, float2>::type;
   KV kv[V];
-  if (XMODE == 0) {
+  // the spectrum loads go out with the line's (up to 24 values per lane) or
+  // after the forward transform (32: both sets would not fit the registers)
+  constexpr bool KPRE = V <= 24;
+  auto load_k = [&]() {
     const int64_t tile = (int64_t)(p % g.nzc) * nyt + (wg - p * nyt);
     if constexpr (KREAL) {
       const float* Kb = reinterpret_cast<const float*>(d.K) + tile * (N / 2 + 1) * T;
@@ -1204,9 +1216,11 @@ __device__ __forceinline__ void xpass_mr_body(const PassDesc& d, int wg, float2*
       for (int m = 0; m < V; ++m)
         kv[m] = valid ? *lane_at(Kb + m * P * T, klo, 0) : make_float2(0.f, 0.f);
     }
-  }
+  };
+  if (XMODE == 0 && KPRE) load_k();
   XP::template fft_xy<false, true>(v, q, lds, LS, t, tw);
   if (XMODE == 0) {
+    if (!KPRE) load_k();
 #pragma unroll
     for (int m = 0; m < V; ++m) {
       if constexpr (KREAL)
@@ -1226,14 +1240,10 @@ __device__ __forceinline__ void xpass_mr_body(const PassDesc& d, int wg, float2*
   }
 }
 
-template <class XP>
-struct XMrLds {
-  static constexpr int T = XYMR_NTH / XP::P;
-  static constexpr int LINE = (XP::N + XP::N / 8) * (T + 1);
-};
 
 template <class Src, int XMODE, class XP>
-__global__ __launch_bounds__(XYMR_NTH) void xpass_mr_kernel(Src src) {
+__global__ __launch_bounds__(XYMR_NTH) __attribute__((amdgpu_waves_per_eu(2))) void
+xpass_mr_kernel(Src src) {
   constexpr int N = XP::N;
   __shared__ float2 lds[XMrLds<XP>::LINE + N];
   int wg = xcd_remap(blockIdx.x, gridDim.x);
