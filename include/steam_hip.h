@@ -180,6 +180,14 @@ int steam_conv_nfft_candidates(int64_t nz, int64_t kz, int64_t* out,
  * which transforms with torch.fft (steam/utils.py:372-417). */
 int steam_conv_zplans(int32_t* out, int32_t max_out, int32_t* count);
 
+/* The x/y line lengths, other than the powers of two 64..2048 of the radix-8
+ * register passes, that have a compiled mixed-radix register plan (lines
+ * held in registers, exchanged through LDS); writes up to max_out values in
+ * increasing order.  Others take the LDS-staged Stockham passes.  No
+ * counterpart in the reference, which transforms with torch.fft
+ * (steam/utils.py:372-417). */
+int steam_conv_xyplans(int32_t* out, int32_t max_out, int32_t* count);
+
 typedef struct steam_conv_info {
   int64_t n_fft;           /* z transform length */
   int64_t n_blocks;        /* overlap-add blocks along z */

@@ -105,6 +105,7 @@ _SIGNATURES = [
      [_I64, _I64, ctypes.POINTER(_I64), _I32, ctypes.POINTER(_I32)]),
     ("steam_conv_plan_info", ctypes.c_int, [_P, ctypes.POINTER(ConvInfo)]),
     ("steam_conv_zplans", ctypes.c_int, [ctypes.POINTER(_I32), _I32, ctypes.POINTER(_I32)]),
+    ("steam_conv_xyplans", ctypes.c_int, [ctypes.POINTER(_I32), _I32, ctypes.POINTER(_I32)]),
     ("steam_conv_set_kernel", ctypes.c_int, [_P, _P, _P, _P, _P]),
     ("steam_conv_execute", ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     ("steam_flux_noise", ctypes.c_int, [_P, ctypes.POINTER(NoiseParams), _P, _P]),

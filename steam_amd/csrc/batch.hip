@@ -101,11 +101,11 @@ std::tuple<int, int, int> variant_of(const ItemPlan& ip, int pass) {
       return {ip.tile ? 2 : 3, radix_set(g.az), Z_NTH};
     case BP_Y:
       if (g.reg_y) return {1, g.reg_y, 256};
-      if (g.mr_y) return {3, g.mr_y, XYMR_NTH};
+      if (g.mr_y) return {3, g.mr_y, xyplan_threads(g.mr_y)};
       return {2, radix_set(g.ay), FFT_NTH};
     case BP_X:
       if (g.reg_x) return {1, g.reg_x, (g.nx / 8 / xreg_lanes(g.reg_x)) * g.Tr};
-      if (g.mr_x) return {3, g.mr_x, XYMR_NTH};
+      if (g.mr_x) return {3, g.mr_x, xyplan_threads(g.mr_x)};
       return {2, radix_set(g.ax), FFT_NTH};
     case BP_FIN: return {0, 0, FIN_NTH};
     default: return {ip.post ? 0 : -1, ip.post_rescale ? 1 : 0, FFT_NTH};
@@ -234,8 +234,8 @@ int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
                                                          dim3(256), 0, s, src, inv));
         } else if (gr.variant == 3) {
           with_xyplan(gr.arg, [&](auto xp) {
-            hipLaunchKernelGGL((ypass_mr_kernel<BatchSrc, decltype(xp)>), grid, dim3(XYMR_NTH),
-                               0, s, src, inv);
+            hipLaunchKernelGGL((ypass_mr_kernel<BatchSrc, decltype(xp)>), grid,
+                               dim3(xy_nth<decltype(xp)>()), 0, s, src, inv);
           });
         } else {
           STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((ypass_kernel<BatchSrc, RS_>), grid,
@@ -249,8 +249,8 @@ int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
               src));
         } else if (gr.variant == 3) {
           with_xyplan(gr.arg, [&](auto xp) {
-            hipLaunchKernelGGL((xpass_mr_kernel<BatchSrc, 0, decltype(xp)>), grid, dim3(XYMR_NTH),
-                               0, s, src);
+            hipLaunchKernelGGL((xpass_mr_kernel<BatchSrc, 0, decltype(xp)>), grid,
+                               dim3(xy_nth<decltype(xp)>()), 0, s, src);
           });
         } else {
           STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((xpass_kernel<BatchSrc, 0, RS_>), grid,

@@ -110,7 +110,7 @@ static void set_tiles(ConvGeom& g, int nz_acc, int kahan, size_t& lds_z,
     const int lpw = 256 / (g.ny / 8);
     g.wg_y = (int)((P * g.nx + lpw - 1) / lpw);
   } else if (g.mr_y) {
-    const int lpw = XYMR_NTH / xyplan_lanes(g.ny);
+    const int lpw = xyplan_threads(g.ny) / xyplan_lanes(g.ny);
     g.wg_y = (int)((P * g.nx + lpw - 1) / lpw);
   } else {
     g.wg_y = (int)((P * g.nx + g.Ty - 1) / g.Ty);
@@ -119,7 +119,7 @@ static void set_tiles(ConvGeom& g, int nz_acc, int kahan, size_t& lds_z,
     g.Tr = xreg_cols(g.reg_x);
     g.wg_x = (int)(P * ((g.ny + g.Tr - 1) / g.Tr));
   } else if (g.mr_x) {
-    g.Tr = XYMR_NTH / xyplan_lanes(g.nx);
+    g.Tr = xyplan_threads(g.nx) / xyplan_lanes(g.nx);
     g.wg_x = (int)(P * ((g.ny + g.Tr - 1) / g.Tr));
   } else {
     g.Tr = 0;
@@ -193,6 +193,19 @@ extern "C" int steam_conv_zplans(int32_t* out, int32_t max_out, int32_t* count) 
     int lines = 0;
     if (!zplan_lines(n2, lines)) continue;
     if (n < max_out) out[n] = n2;
+    ++n;
+  }
+  *count = std::min(n, std::max(0, (int)max_out));
+  return STEAM_OK;
+}
+
+extern "C" int steam_conv_xyplans(int32_t* out, int32_t max_out, int32_t* count) {
+  if (!count || (max_out > 0 && !out))
+    return set_error(STEAM_EINVAL, "steam_conv_xyplans: bad arguments");
+  int n = 0;
+  for (int len = 2; len <= FFT_LDSE; len += 2) {
+    if (!xyplan_lanes(len)) continue;
+    if (n < max_out) out[n] = len;
     ++n;
   }
   *count = std::min(n, std::max(0, (int)max_out));

@@ -1099,6 +1099,9 @@ inline bool zplan_lines(int n2, int& lines) {
 // ---------------------------------------------------------------------------
 using XP288 = ZPlan<288, 18, 6, 6, 2, 2, 2>;
 using XP384 = ZPlan<384, 24, 8, 8, 6>;
+// 400 points (a 384-cell served tile plus its halo): 50 values on 8 lanes,
+// the radix-2 stages first (fewer live registers than 10, 10, 2, 2)
+using XP400 = ZPlan<400, 50, 2, 2, 10, 10>;
 
 // call f(plan) for an x/y length with a mixed-radix register plan
 template <class F>
@@ -1113,22 +1116,34 @@ inline bool with_xyplan(int n, F&& f) {
     case 288: f(XP288{}); return true;
     case 320: f(ZP320{}); return true;
     case 384: f(XP384{}); return true;
+    case 400: f(XP400{}); return true;
     default: return false;
   }
 }
-constexpr int XYMR_NTH = 256;
-// lanes per line of an x/y length's plan (0: no plan)
+// threads per workgroup of a plan's x/y passes: 256, or 128 where 256 / P
+// wave-private lines would take more than 64 KB of LDS (two workgroups per CU)
+template <class XP>
+__host__ __device__ constexpr int xy_nth() {
+  return (256 / XP::P) * XP::PITCH * 8 > 64 * 1024 ? 128 : 256;
+}
+// lanes per line and threads per workgroup of an x/y length's plan (0: none)
 inline int xyplan_lanes(int n) {
   int p = 0;
   with_xyplan(n, [&](auto xp) { p = decltype(xp)::P; });
   return p;
 }
+inline int xyplan_threads(int n) {
+  int t = 0;
+  with_xyplan(n, [&](auto xp) { t = xy_nth<decltype(xp)>(); });
+  return t;
+}
 
 // y pass: 256 / P contiguous y lines per workgroup, each wave-private
 // (a wave holds 64 / P of them), lines in LDS at the z plans' pitch
 template <class Src, class XP>
-__global__ __launch_bounds__(XYMR_NTH) void ypass_mr_kernel(Src src, int inv) {
-  constexpr int N = XP::N, V = XP::V, P = XP::P, LPW = XYMR_NTH / P;
+__global__ __launch_bounds__(xy_nth<XP>()) void ypass_mr_kernel(Src src, int inv) {
+  constexpr int NTH = xy_nth<XP>();
+  constexpr int N = XP::N, V = XP::V, P = XP::P, LPW = NTH / P;
   __shared__ float2 lds[LPW * XP::PITCH + N];
   int wg = blockIdx.x;
   const PassDesc& d = src.get(PASS_Y, wg);
@@ -1136,7 +1151,7 @@ __global__ __launch_bounds__(XYMR_NTH) void ypass_mr_kernel(Src src, int inv) {
   const int tid = threadIdx.x;
   const int q = tid % P, lw = tid / P;
   float2* tw = lds + LPW * XP::PITCH;
-  for (int i = tid; i < N; i += XYMR_NTH) tw[i] = g.ay.tw[i];
+  for (int i = tid; i < N; i += NTH) tw[i] = g.ay.tw[i];
   __syncthreads();
   const int planes = (d.mode == MODE_KERNEL) ? g.nzc : g.nblocks * g.nzc;
   const int64_t lines = (int64_t)planes * g.nx;
@@ -1168,7 +1183,7 @@ __global__ __launch_bounds__(XYMR_NTH) void ypass_mr_kernel(Src src, int inv) {
 // inverse; 1: forward, * the kernel-build scale.
 template <class XP>
 struct XMrLds {
-  static constexpr int T = XYMR_NTH / XP::P;
+  static constexpr int T = xy_nth<XP>() / XP::P;
   // column stride: T + 1 (the lane groups of a wave start in different
   // banks) unless that leaves room for only one workgroup per CU
   static constexpr int LS = ((XP::N + XP::N / 8) * (T + 1) + XP::N) * 8 > 80 * 1024 ? T : T + 1;
@@ -1178,7 +1193,7 @@ struct XMrLds {
 template <int XMODE, class XP, bool KREAL>
 __device__ __forceinline__ void xpass_mr_body(const PassDesc& d, int wg, float2* lds,
                                               const float2* tw) {
-  constexpr int N = XP::N, V = XP::V, P = XP::P, T = XYMR_NTH / P, LS = XMrLds<XP>::LS;
+  constexpr int N = XP::N, V = XP::V, P = XP::P, T = XMrLds<XP>::T, LS = XMrLds<XP>::LS;
   const ConvGeom& g = d.g;
   const int tid = threadIdx.x;
   const int t = tid % T, q = tid / T;
@@ -1187,12 +1202,16 @@ __device__ __forceinline__ void xpass_mr_body(const PassDesc& d, int wg, float2*
   const int y = (wg - p * nyt) * T + t;
   const bool valid = y < g.ny;
   const int ny = g.ny;
+  // value m of the lane is row q + m P: the plane base (uniform) plus a
+  // 32-bit byte offset (a plane is under 4 GB), so no access needs a 64-bit
+  // row pointer of its own
   float2* Sb = d.S + (int64_t)p * N * ny;
   const unsigned lb = (unsigned)(q * ny + (valid ? y : 0)) * 8u;
-  auto row = [&](int m) { return Sb + (int64_t)m * P * ny; };
+  const unsigned rs = (unsigned)(P * ny) * 8u;
   float2 v[V];
 #pragma unroll
-  for (int m = 0; m < V; ++m) v[m] = valid ? *lane_at(row(m), lb, 0) : make_float2(0.f, 0.f);
+  for (int m = 0; m < V; ++m)
+    v[m] = valid ? *lane_at(Sb, lb + (unsigned)m * rs, 0) : make_float2(0.f, 0.f);
   using KV = typename std::conditional<KREAL, float, float2>::type;
   KV kv[V];
   // the spectrum loads go out with the line's (up to 24 values per lane) or
@@ -1220,15 +1239,41 @@ __device__ __forceinline__ void xpass_mr_body(const PassDesc& d, int wg, float2*
   if (XMODE == 0 && KPRE) load_k();
   XP::template fft_xy<false, true>(v, q, lds, LS, t, tw);
   if (XMODE == 0) {
-    if (!KPRE) load_k();
+    // the inverse reads its twiddles from a second copy of the table: the
+    // compiler cannot then keep the forward transform's twiddles live in
+    // registers for it (with 50 values per lane those would not fit)
+    const float2* twi = V > 24 ? tw + N : tw;
+    if constexpr (!KPRE && !KREAL) {
+      // the complex spectrum above 24 values per lane: loaded and applied
+      // in chunks of 10 (all of it at once would not fit the registers)
+      const int64_t tile = (int64_t)(p % g.nzc) * nyt + (wg - p * nyt);
+      const float2* Kb = d.K + tile * N * T;
+      const unsigned klo = (unsigned)(q * T + t) * 8u;
 #pragma unroll
-    for (int m = 0; m < V; ++m) {
-      if constexpr (KREAL)
-        v[m] = make_float2(v[m].x * kv[m], v[m].y * kv[m]);
-      else
-        v[m] = cmul(v[m], kv[m]);
+      for (int m0 = 0; m0 < V; m0 += 10) {
+        float2 kc[10];
+#pragma unroll
+        for (int i = 0; i < 10 && m0 + i < V; ++i)
+          kc[i] = valid ? *lane_at(Kb + (m0 + i) * P * T, klo, 0) : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int i = 0; i < 10 && m0 + i < V; ++i) v[m0 + i] = cmul(v[m0 + i], kc[i]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    } else {
+      if (!KPRE) load_k();
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        if constexpr (KREAL)
+          v[m] = make_float2(v[m].x * kv[m], v[m].y * kv[m]);
+        else
+          v[m] = cmul(v[m], kv[m]);
+      }
     }
-    XP::template fft_xy<true, true>(v, q, lds, LS, t, tw);
+    // likewise its LDS addresses: the lane's q and t pass through an empty
+    // asm, so they are recomputed rather than held from the forward transform
+    int qi = q, ti = t;
+    if constexpr (V > 24) asm volatile("" : "+v"(qi), "+v"(ti));
+    XP::template fft_xy<true, true>(v, qi, lds, LS, ti, twi);
   } else {
     const float2 f = kernel_bin_factor(d, p % g.nzc);
 #pragma unroll
@@ -1236,20 +1281,21 @@ __device__ __forceinline__ void xpass_mr_body(const PassDesc& d, int wg, float2*
   }
   if (valid) {
 #pragma unroll
-    for (int m = 0; m < V; ++m) *lane_at(row(m), lb, 0) = v[m];
+    for (int m = 0; m < V; ++m) *lane_at(Sb, lb + (unsigned)m * rs, 0) = v[m];
   }
 }
 
 
 template <class Src, int XMODE, class XP>
-__global__ __launch_bounds__(XYMR_NTH) __attribute__((amdgpu_waves_per_eu(2))) void
+__global__ __launch_bounds__(xy_nth<XP>()) __attribute__((amdgpu_waves_per_eu(2))) void
 xpass_mr_kernel(Src src) {
   constexpr int N = XP::N;
-  __shared__ float2 lds[XMrLds<XP>::LINE + N];
+  constexpr int NT = XP::V > 24 ? 2 * N : N;  // table copies (xpass_mr_body)
+  __shared__ float2 lds[XMrLds<XP>::LINE + NT];
   int wg = xcd_remap(blockIdx.x, gridDim.x);
   const PassDesc& d = src.get(PASS_X, wg);
   float2* tw = lds + XMrLds<XP>::LINE;
-  for (int i = threadIdx.x; i < N; i += XYMR_NTH) tw[i] = d.g.ax.tw[i];
+  for (int i = threadIdx.x; i < NT; i += xy_nth<XP>()) tw[i] = d.g.ax.tw[i < N ? i : i - N];
   __syncthreads();
   // the spectrum's format word is uniform over the launch's instance
   if (XMODE == 0 && d.kfmt && *d.kfmt == KFMT_REAL_EVEN)

@@ -84,7 +84,7 @@ inline void launch_ypass(const ConvGeom& g, size_t lds_y, hipStream_t s,
   } else if (g.mr_y) {
     with_xyplan(g.mr_y, [&](auto xp) {
       hipLaunchKernelGGL((ypass_mr_kernel<SrcT, decltype(xp)>), dim3(g.wg_y),
-                         dim3(XYMR_NTH), 0, s, src, inv);
+                         dim3(xy_nth<decltype(xp)>()), 0, s, src, inv);
     });
   } else {
     const int rsy = radix_set(g.ay);
@@ -104,7 +104,7 @@ inline void launch_xpass(const ConvGeom& g, size_t lds_x, hipStream_t s,
   } else if (g.mr_x) {
     with_xyplan(g.mr_x, [&](auto xp) {
       hipLaunchKernelGGL((xpass_mr_kernel<SrcT, XMODE, decltype(xp)>), dim3(g.wg_x),
-                         dim3(XYMR_NTH), 0, s, src);
+                         dim3(xy_nth<decltype(xp)>()), 0, s, src);
     });
   } else {
     const int rsx = radix_set(g.ax);

@@ -36,13 +36,16 @@ from .utils import conv_operator, kernel_key
 
 
 def fft_length(m):
-    """Smallest transform-friendly length >= m: a power of two when within
-    15% (register FFT path), else the smallest 2^a 3^b 5^c."""
-    p2 = 1
-    while p2 < m:
-        p2 *= 2
-    if p2 <= 1.15 * m:
-        return p2
+    """Transform length of a padded tile axis of at least m cells: the
+    smallest length with a register x/y FFT path (a power of two 64..2048 or
+    a compiled mixed-radix length, e.g. 80, 144, 288, 400) when it is within
+    30% of m, else the smallest 2^a 3^b 5^c (the LDS-staged passes)."""
+    from .utils import register_fft_lengths
+    for n in register_fft_lengths():
+        if n >= m:
+            if n <= 1.3 * m:
+                return n
+            break
     n = m
     while True:
         r = n

@@ -146,6 +146,23 @@ def conv_operator(field_shape, kernel_shape, n_fft=0, precision='plain'):
     return op
 
 
+_REGISTER_LENGTHS = []
+
+
+def register_fft_lengths():
+    """x/y line lengths served by register FFT passes, increasing: the
+    powers of two 64..2048 (radix-8 passes) and the library's compiled
+    mixed-radix lengths (``steam_conv_xyplans``).  A host query; no GPU."""
+    if not _REGISTER_LENGTHS:
+        lib = _lib.load()
+        out = (ctypes.c_int32 * 64)()
+        n = ctypes.c_int32()
+        _lib.check(lib.steam_conv_xyplans(out, 64, ctypes.byref(n)), "steam_conv_xyplans")
+        lengths = {1 << k for k in range(6, 12)} | {int(out[i]) for i in range(n.value)}
+        _REGISTER_LENGTHS.extend(sorted(lengths))
+    return list(_REGISTER_LENGTHS)
+
+
 def kernel_key(kernel):
     """Key of a kernel for the spectrum cache: a device tensor by its address,
     layout and version counter (in-place writes bump it), which the cache

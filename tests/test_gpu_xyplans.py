@@ -1,5 +1,5 @@
 """GPU parity of the mixed-radix register x/y passes: every x/y length with
-a compiled plan (32, 80, 96, 144, 160, 192, 288, 320, 384), taken along x,
+a compiled plan (32, 80, 96, 144, 160, 192, 288, 320, 384, 400), taken along x,
 along y and along both, against the float64 oracle's convolution.
 
 The kernel-spectrum formats: an asymmetric kernel keeps the complex
@@ -22,7 +22,7 @@ from oracle import model as M
 
 pytestmark = pytest.mark.gpu
 
-XY_PLANS = (32, 80, 96, 144, 160, 192, 288, 320, 384)
+XY_PLANS = (32, 80, 96, 144, 160, 192, 288, 320, 384, 400)
 CONV_REL_L2 = 1e-6
 
 
@@ -62,6 +62,16 @@ def test_mixed_radix_x_and_y_lengths_against_fp64(n):
     _conv_check((n, 64, 10), sym, "real_even")     # x plan beside the radix-8 y pass
 
 
+def test_plan_list_is_the_compiled_one():
+    import ctypes
+    from steam_amd import _lib
+    lib = _lib.load()
+    out = (ctypes.c_int32 * 64)()
+    n = ctypes.c_int32()
+    _lib.check(lib.steam_conv_xyplans(out, 64, ctypes.byref(n)), "steam_conv_xyplans")
+    assert tuple(out[i] for i in range(n.value)) == XY_PLANS
+
+
 def test_mixed_radix_ragged_last_column_tile():
     """A y extent that leaves the x pass's last column tile partly empty
     (tiles of 256 / P columns)."""
@@ -69,6 +79,7 @@ def test_mixed_radix_ragged_last_column_tile():
     _conv_check((96, 70, 9), rng.standard_normal((3, 5, 3)), "complex")
     _conv_check((192, 41, 6), M.turbulon_envelope(1, .5, .5, .5, support_factor=2,
                                                   dtype=np.float64), "real_even")
+    _conv_check((400, 37, 7), rng.standard_normal((5, 5, 3)), "complex")
 
 
 @pytest.mark.parametrize("shape", [(96, 96, 48), (192, 192, 96)])

@@ -12,20 +12,41 @@ def _smooth(k):
 
 
 def test_fft_length_is_smooth_and_not_shorter():
+    """A padded tile axis takes the smallest register-FFT length within 30%
+    (a power of two 64..2048 or a compiled mixed-radix length), else the
+    smallest 5-smooth length."""
     from steam_amd.serve import fft_length
+    from steam_amd.utils import register_fft_lengths
+    reg = register_fft_lengths()
+    assert reg == sorted(reg) and {64, 128, 256, 512, 1024, 2048} <= set(reg)
     for m in range(1, 700):
         n = fft_length(m)
-        assert n >= m
-        r = n
-        for f in (2, 3, 5):
-            while r % f == 0:
-                r //= f
-        assert r == 1
-        if n & (n - 1):  # not a power of two: the smallest 5-smooth length
+        assert n >= m and _smooth(n)
+        nxt = [r for r in reg if r >= m]
+        if nxt and nxt[0] <= 1.3 * m:
+            assert n == nxt[0]
+        else:
+            assert n not in reg or n == m
             assert not any(_smooth(k) for k in range(m, n))
-        else:            # a power of two within 15% of m
-            assert n <= 1.15 * m or n == 1
-    assert fft_length(78) == 80 and fft_length(142) == 144 and fft_length(120) == 128
+    # the served tiles (64, 128, 256, 384 cells plus a 7-cell halo per side)
+    assert [fft_length(t + 14) for t in (64, 128, 256, 384)] == [80, 144, 288, 400]
+    assert fft_length(120) == 128
+
+
+def test_xy_plan_list_is_what_the_library_compiles():
+    """The mixed-radix x/y lengths the library reports are 5-smooth, even,
+    and not powers of two (those take the radix-8 passes)."""
+    import ctypes
+    from steam_amd import _lib
+    lib = _lib.load()
+    out = (ctypes.c_int32 * 64)()
+    n = ctypes.c_int32()
+    _lib.check(lib.steam_conv_xyplans(out, 64, ctypes.byref(n)), "steam_conv_xyplans")
+    lens = [out[i] for i in range(n.value)]
+    assert lens == sorted(lens) and lens
+    assert set(lens) >= {32, 80, 96, 144, 160, 192, 288, 320, 384, 400}
+    for n in lens:
+        assert n % 2 == 0 and _smooth(n) and n & (n - 1) != 0 or n == 32
 
 
 def test_trace_is_poisson_with_neighbour_walk():
