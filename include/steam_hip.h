@@ -237,6 +237,9 @@ typedef struct steam_flux_ledger {
                                   steam/streaming.py:1233-1234; the in-RAM
                                   advance counts the whole array,
                                   steam/simulate.py:2198) */
+#define STEAM_UNIT_FLUX 16 /* the entering flux is 1 everywhere: flux_dev is
+                              written, never read (a served tile's first
+                              class; no fill of the buffer beforehand) */
 #define STEAM_SK_RAW 8 /* leave noise * entering flux in sk_dev, without the
                           1/<|noise|> scale: the streamed passes measure the
                           pattern on the raw product before the class's mean
@@ -295,7 +298,7 @@ typedef struct steam_batch_item {
   int64_t window[4];          /* x0, x1, y0, y1; all zero = whole field */
   float sigma_in;             /* pending rescale applied on read */
   int32_t flags;              /* STEAM_DEFER_RESCALE | STEAM_CHAIN_PENDING |
-                                 STEAM_CLIP_IN_WINDOW */
+                                 STEAM_CLIP_IN_WINDOW | STEAM_UNIT_FLUX */
   const steam_flux_ledger* ledger; /* NULL: measure; else apply-only with these
                                       reductions (copied at creation) */
 } steam_batch_item;

@@ -168,6 +168,7 @@ DEFER_RESCALE = 1
 CHAIN_PENDING = 2
 CLIP_IN_WINDOW = 4
 SK_RAW = 8
+UNIT_FLUX = 16
 
 EXPORTED = tuple(name for name, _, _ in _SIGNATURES)
 

@@ -88,7 +88,8 @@ class Instance:
     def __init__(self, flux, kernel, region, flux_noise_scale,
                  n_scale_classes_per_dyad=1, sparsity_factors=(1, 1, 1),
                  alpha=None, n_zero=0, zero_bottom=False, zero_top=False,
-                 window=None, defer_rescale=False, chain_pending=False, ledger=None):
+                 window=None, defer_rescale=False, chain_pending=False, ledger=None,
+                 unit_flux=False):
         if not (isinstance(flux, torch.Tensor) and flux.is_cuda
                 and flux.dtype == torch.float32 and flux.is_contiguous()):
             raise ValueError("batched instances need contiguous float32 device "
@@ -108,6 +109,8 @@ class Instance:
         self.chain_pending = chain_pending
         # a recorded FluxAdvanceLedger: the instance advances apply-only
         self.ledger = ledger
+        # the entering flux is 1 everywhere; the flux tensor is only written
+        self.unit_flux = unit_flux
 
     def native_noise(self, region=None):
         a, pcs, shift = class_scale_shift(self.flux_noise_scale,
@@ -172,7 +175,8 @@ class BatchedAdvance:
                     it.window[a] = int(inst.window[a])
             it.sigma_in = 1.0
             it.flags = ((_lib.DEFER_RESCALE if inst.defer_rescale else 0)
-                        | (_lib.CHAIN_PENDING if inst.chain_pending else 0))
+                        | (_lib.CHAIN_PENDING if inst.chain_pending else 0)
+                        | (_lib.UNIT_FLUX if getattr(inst, "unit_flux", False) else 0))
             if inst.ledger is not None:
                 native = inst.ledger.to_native()
                 self._ledgers.append(native)

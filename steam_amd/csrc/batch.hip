@@ -307,6 +307,10 @@ int make_item(const steam_batch_item& it, PassDesc& d, ItemPlan& ip) {
   d.sk = it.sk_dev;
   d.apply = it.ledger != nullptr;
   if (d.apply) d.led_in = ledger_to_dev(it.ledger);
+  d.unit_in = (it.flags & STEAM_UNIT_FLUX) != 0;
+  if (d.unit_in && (it.flags & STEAM_CHAIN_PENDING))
+    return set_error(STEAM_EINVAL, "steam_batch_create: a unit entering flux has no pending "
+                                   "rescale to chain");
   d.clip_win = (it.flags & STEAM_CLIP_IN_WINDOW) != 0;
   const int64_t* w = it.window;
   if (w[0] || w[1] || w[2] || w[3]) {
@@ -322,8 +326,10 @@ int make_item(const steam_batch_item& it, PassDesc& d, ItemPlan& ip) {
   ip.tile = !ip.zreg && p->g.tile_z && a16(d.out) && a16(d.nfield) && a16(d.sk);
   ip.gen_noise = it.noise_dev == nullptr;
   const bool defer = (it.flags & STEAM_DEFER_RESCALE) != 0;
-  ip.post = !defer || it.sk_dev != nullptr;
-  ip.post_rescale = !defer;
+  // apply mode: the inverse z pass stores the rescaled flux
+  if (d.apply && !defer) set_out_scale(d, d.led_in);
+  ip.post = (!defer && !d.out_scale) || it.sk_dev != nullptr;
+  ip.post_rescale = !defer && !d.out_scale;
   d.npart = ip.zreg ? p->g.wg_zr : p->g.wg_z;
   return STEAM_OK;
 }

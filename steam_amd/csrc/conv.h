@@ -103,6 +103,11 @@ struct PassDesc {
   int clip_win;         // count clipped cells inside the window only
   int wg_noise_n;       // workgroups a strided noise launch gives this instance
                         // (0: the whole grid)
+  int unit_in;          // the entering flux is 1 everywhere: the flux buffer
+                        // is written, never read
+  int out_scale;        // apply mode: the inverse z pass writes the rescaled
+  int out_flat;         // flux (flat ? out_em : flux * out_rs), the recorded
+  float out_rs, out_em; // ledger's rescale, so no post pass reads it again
   int wg_base[7];       // batched: first workgroup of z, y, x, zinv, post,
                         // finalize, noise passes
 };

@@ -548,6 +548,18 @@ DevLedger ledger_to_dev(const steam_flux_ledger* h) {
   return L;
 }
 
+// Apply mode: the rescale the finalize kernel derives from a recorded
+// ledger (the same float64 operations), set into the descriptor so the
+// inverse z pass stores the rescaled flux.
+void set_out_scale(PassDesc& d, const DevLedger& L) {
+  const double em = L.entering_total / std::fmax(L.entering_count, 1.0);
+  const double rm = L.realized_total / std::fmax(L.realized_count, 1.0);
+  d.out_scale = 1;
+  d.out_flat = rm > 0.0 ? 0 : 1;
+  d.out_rs = rm > 0.0 ? (float)(em / rm) : 1.0f;
+  d.out_em = (float)em;
+}
+
 void ledger_to_host(const DevLedger& L, steam_flux_ledger* h) {
   h->entering_total = L.entering_total;
   h->entering_count = (int64_t)L.entering_count;
@@ -612,6 +624,7 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
   const int chain_pending = (flags & STEAM_CHAIN_PENDING) != 0;
   const int clip_win = (flags & STEAM_CLIP_IN_WINDOW) != 0;
   const bool sk_scale = sk_dev && !(flags & STEAM_SK_RAW);
+  const int unit_in = (flags & STEAM_UNIT_FLUX) != 0;
   if (!p || !spectrum_dev || !workspace_dev || !flux_dev)
     return set_error(STEAM_EINVAL, "steam_flux_advance: NULL argument");
   int rc = check_noise_params(noise);
@@ -641,6 +654,13 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
     for (int i = 0; i < 4; ++i) d.win[i] = (int)window[i];
   }
   if (d.apply) d.led_in = ledger_to_dev(ledger);
+  d.unit_in = unit_in;
+  if (unit_in && chain_pending)
+    return set_error(STEAM_EINVAL, "steam_flux_advance: a unit entering flux has no pending "
+                                   "rescale to chain");
+  // apply mode knows its rescale up front: the inverse z pass stores the
+  // rescaled flux and no post pass rescales it
+  if (d.apply && !defer_rescale) set_out_scale(d, d.led_in);
   if (noise_dev) {
     d.nfield = const_cast<float*>(noise_dev);
   } else {
@@ -661,12 +681,13 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
     hipLaunchKernelGGL((finalize_kernel<SingleSrc>), dim3(1), dim3(FIN_NTH), 0, s, src);
   }
   STEAM_LAUNCH_CHECK("finalize_kernel");
-  if (!defer_rescale || sk_scale) {
+  const bool post_rescale = !defer_rescale && !d.out_scale;
+  if (post_rescale || sk_scale) {
     PassDesc dp = d;
     if (!sk_scale) dp.sk = nullptr;
     ProfScope ps(STEAM_KF_POST, s);
     hipLaunchKernelGGL((post_kernel<SingleSrc>), dim3(p->wg_post), dim3(FFT_NTH), 0,
-                       s, SingleSrc{dp}, defer_rescale ? 0 : 1);
+                       s, SingleSrc{dp}, post_rescale ? 1 : 0);
     STEAM_LAUNCH_CHECK("post_kernel");
   }
   if (ledger && mode == STEAM_MEASURE) {

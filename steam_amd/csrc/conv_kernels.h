@@ -167,6 +167,11 @@ __device__ __forceinline__ Pending pending_of(const PassDesc& d) {
 __device__ __forceinline__ float entering_value(const Pending& p, float raw) {
   return fmaf(p.a, raw, p.b);
 }
+// The advanced flux as stored: with out_scale (apply mode) the recorded
+// rescale applied as the post pass would apply it.
+__device__ __forceinline__ float out_value(const PassDesc& d, float v) {
+  return d.out_scale ? (d.out_flat ? d.out_em : __fmul_rn(v, d.out_rs)) : v;
+}
 
 // kp[x, y, z] of the rolled, folded kernel: taps i = x + offx (mod nx),
 // j = y + offy (mod ny), summed x-fold first then y-fold.
@@ -305,7 +310,7 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
           if (MODE == MODE_PLAIN) {
             v = d.in[line + z];
           } else if (MODE == MODE_FLUX) {
-            float f = entering_value(pd, d.out[line + z]);
+            float f = entering_value(pd, d.unit_in ? 1.f : d.out[line + z]);
             const float nse = d.nfield[line + z];
             float p = __fmul_rn(nse, f);
             if (win) {
@@ -718,14 +723,14 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
     } else {
       const bool win = xin && y >= d.win[2] && y < d.win[3];
       for (int z = tid; z < nz; z += Z_NTH) {
-        float f = entering_value(pd, d.out[line + z]);
+        float f = entering_value(pd, d.unit_in ? 1.f : d.out[line + z]);
         f = __fadd_rn(f, accz[t * nz + z]);
         if (f < 0.f) {
           if (win || !d.clip_win) acc[1] += 1.0;
           f = 0.f;
         }
         if (win) acc[0] += (double)f;
-        d.out[line + z] = f;
+        d.out[line + z] = out_value(d, f);
       }
     }
   }
@@ -808,7 +813,7 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
   for (int i = 0; i < ZT_U; ++i) {
     const int e = i * Z_NTH + tid;
     if (e < n4) {
-      fv[i] = in4[e];
+      fv[i] = (MODE == MODE_FLUX && d.unit_in) ? make_float4(1.f, 1.f, 1.f, 1.f) : in4[e];
       if (MODE == MODE_FLUX) nv[i] = nf4[e];
     }
   }
@@ -927,7 +932,7 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
 #pragma unroll
     for (int i = 0; i < ZT_U; ++i) {
       const int e = i * Z_NTH + tid;
-      if (e < n4) fv[i] = io4[e];
+      if (e < n4) fv[i] = d.unit_in ? make_float4(1.f, 1.f, 1.f, 1.f) : io4[e];
     }
   }
   const int64_t plane = (int64_t)g.nx * g.ny;
@@ -1017,7 +1022,8 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
           o[j] = v;
         }
         if (win) acc[0] += ((double)o[0] + (double)o[1]) + ((double)o[2] + (double)o[3]);
-        io4[e] = make_float4(o[0], o[1], o[2], o[3]);
+        io4[e] = make_float4(out_value(d, o[0]), out_value(d, o[1]), out_value(d, o[2]),
+                             out_value(d, o[3]));
       }
     }
   }
@@ -1444,13 +1450,14 @@ __device__ __forceinline__ void zfwd_reg_body(const PassDesc& d, int wg, float2*
     const int cb = ZLanes<ZP>::pair_bytes(m);
     v[m] = make_float2(0.f, 0.f);
     nv[m] = make_float2(0.f, 0.f);
+    const bool unit = MODE == MODE_FLUX && d.unit_in;
     if (m < ln.m_full) {
       if (ln.valid) {
-        v[m] = ld_pair<ODD>(srcp, lb, cb);
+        v[m] = unit ? make_float2(1.f, 1.f) : ld_pair<ODD>(srcp, lb, cb);
         if (MODE == MODE_FLUX) nv[m] = ld_pair<ODD>(nfp, lb, cb);
       }
     } else if (ln.valid && z0 < nz) {
-      v[m] = ld_pair_top<ODD>(srcp, lb, cb, z0, nz);
+      v[m] = unit ? make_float2(1.f, 1.f) : ld_pair_top<ODD>(srcp, lb, cb, z0, nz);
       if (MODE == MODE_FLUX) nv[m] = ld_pair_top<ODD>(nfp, lb, cb, z0, nz);
     }
   }
@@ -1613,7 +1620,9 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
   auto load_flux = [&](int m) {
     const int z0 = 2 * (q + m * P);
     const int cb = ZLanes<ZP>::pair_bytes(m);
-    if (m < ln.m_full)
+    if (d.unit_in)
+      fl[m] = make_float2(1.f, 1.f);
+    else if (m < ln.m_full)
       fl[m] = ln.valid ? ld_pair<ODD>(outp, lb, cb) : make_float2(0.f, 0.f);
     else
       fl[m] = (ln.valid && z0 < nz) ? ld_pair_top<ODD>(outp, lb, cb, z0, nz)
@@ -1678,10 +1687,11 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
       if (count_clip) nclip += clipped;
       if (win) acc0 += (double)o0 + (double)o1;
       const int cb = ZLanes<ZP>::pair_bytes(m);
+      const float2 ov = make_float2(out_value(d, o0), out_value(d, o1));
       if (full)
-        st_pair<ODD>(outp, lb, cb, make_float2(o0, o1));
+        st_pair<ODD>(outp, lb, cb, ov);
       else
-        st_pair_top<ODD>(outp, lb, cb, z0, nz, make_float2(o0, o1));
+        st_pair_top<ODD>(outp, lb, cb, z0, nz, ov);
     }
   }
   if (!d.apply) {

@@ -32,6 +32,7 @@ int noise_wgs_cap();
 NoiseDev make_noise_dev(const steam_noise_params* np);
 int check_noise_params(const steam_noise_params* np);
 DevLedger ledger_to_dev(const steam_flux_ledger* h);
+void set_out_scale(PassDesc& d, const DevLedger& L);
 void ledger_to_host(const DevLedger& L, steam_flux_ledger* h);
 
 }  // namespace steam

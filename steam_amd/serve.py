@@ -101,7 +101,7 @@ class WorldClass:
         ledger: apply-only (the world's means); without: measure mode over
         the tile interior, returning the tile's ledger too."""
         po, ps, (ix, iy, iz) = self.padded_box(origin, shape)
-        flux = torch.ones(ps, dtype=torch.float32, device='cuda')
+        flux = torch.empty(ps, dtype=torch.float32, device='cuda')
         op = conv_operator(ps, tuple(self.kernel.shape))
         spec = op.spectrum(self.kernel, key=kernel_key(self.kernel))
         window = None if ledger is not None else (ix, ix + shape[0], iy, iy + shape[1])
@@ -110,7 +110,7 @@ class WorldClass:
         led, _ = advance_flux_device(flux, self.region(po), self.kernel,
                                      self.flux_noise_scale, 1, (1, 1, 1),
                                      alpha=self.alpha, ledger=ledger, op=op,
-                                     spec=spec, window=window)
+                                     spec=spec, window=window, unit_flux=True)
         out = flux[ix:ix + shape[0], iy:iy + shape[1], iz:iz + shape[2]]
         return out, led
 
@@ -179,8 +179,9 @@ class _InFlight:
 
 class _SlotPool:
     """The device buffers of one padded tile shape on one stream: ``slots``
-    flux buffers (views of one tensor, so one fill resets them all) and
-    workspaces, and a batched advance over the first c slots for each group
+    flux buffers (views of one tensor; each advance starts from a unit flux
+    it does not read, so they are never filled) and workspaces, and a
+    batched advance over the first c slots for each group
     size c, created on first use and kept (their descriptors are refreshed
     per group from pinned memory, nothing is allocated per request)."""
 
@@ -203,7 +204,8 @@ class _SlotPool:
         if b is None:
             w = self.world
             insts = [Instance(self.flux[i], w.kernel, w.region((0, 0, 0)), w.flux_noise_scale,
-                              alpha=w.alpha, ledger=w.ledger) for i in range(count)]
+                              alpha=w.alpha, ledger=w.ledger, unit_flux=True)
+                     for i in range(count)]
             b = BatchedAdvance(insts, autotune=False, workspaces=self.workspaces[:count])
             self.batches[count] = b
         return b
@@ -277,7 +279,7 @@ class TileServer:
         pool = self._pool(boxes[0][1], stream)
         b = pool.batch(len(reqs))
         with torch.cuda.stream(stream):
-            pool.flux[:len(reqs)].fill_(1.0)
+            # the advance starts from a unit flux it does not read (no fill);
             # the stream is free: the batch's previous upload has been read
             b.set_regions([w.region(po) for po, _, _ in boxes])
             b.advance(stream)

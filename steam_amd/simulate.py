@@ -332,7 +332,8 @@ def advance_flux_device(flux_t, region, kernel_t, flux_noise_scale,
                         ledger=None, alpha=None, sigma_in=1.0, sk_t=None,
                         defer_rescale=False, precision='plain', spec=None,
                         op=None, sync_ledger=True, chain_pending=False,
-                        noise_t=None, clip_in_window=False, sk_raw=False):
+                        noise_t=None, clip_in_window=False, sk_raw=False,
+                        unit_flux=False):
     """Class advance of a device-resident flux (float32 tensor, in place).
 
     ``sigma_in`` is a pending float32 rescale applied on read; with
@@ -342,9 +343,11 @@ def advance_flux_device(flux_t, region, kernel_t, flux_noise_scale,
     ``noise_t`` is the class's noise field from generate_noise_device (None
     generates it inside the advance).  ``clip_in_window`` counts clipped
     cells inside ``window`` only (the streamed passes' count); ``sk_raw``
-    leaves noise * entering flux in ``sk_t`` without the 1/<|noise|> scale.  Returns (FluxAdvanceLedger, native
-    ledger); both None when ``sync_ledger`` is False in measure mode (the
-    ledger then stays on the device).
+    leaves noise * entering flux in ``sk_t`` without the 1/<|noise|> scale.
+    ``unit_flux``: the entering flux is 1 everywhere and ``flux_t`` is only
+    written (a first class needs no fill).  Returns (FluxAdvanceLedger,
+    native ledger); both None when ``sync_ledger`` is False in measure mode
+    (the ledger then stays on the device).
     """
     lib = _lib.load()
     a, pcs, shift = class_scale_shift(flux_noise_scale, n_scale_classes_per_dyad,
@@ -366,7 +369,8 @@ def advance_flux_device(flux_t, region, kernel_t, flux_noise_scale,
     flags = ((_lib.DEFER_RESCALE if defer_rescale else 0)
              | (_lib.CHAIN_PENDING if chain_pending else 0)
              | (_lib.CLIP_IN_WINDOW if clip_in_window else 0)
-             | (_lib.SK_RAW if sk_raw else 0))
+             | (_lib.SK_RAW if sk_raw else 0)
+             | (_lib.UNIT_FLUX if unit_flux else 0))
     _lib.check(lib.steam_flux_advance(
         op.plan, _lib.ptr(spec), _lib.ptr(ws), _lib.ptr(flux_t),
         float(np.float32(sigma_in)), ctypes.byref(p), _lib.ptr(noise_t),

@@ -242,3 +242,36 @@ def test_plain_cpu_config_with_the_power_law_kernel():
     assert _rel(fg, fo) < ADVANCE_REL_L2
     assert _rel(sk, so) < ADVANCE_REL_L2
     assert led.noise_abs.count == lo['noise_abs'][1]
+
+
+@pytest.mark.parametrize("shape,n_fft", [((48, 40, 48), 0),     # register z passes
+                                         ((40, 48, 44), 60),    # vector-tile z passes
+                                         ((32, 40, 40), 24)])   # multi-block generic
+def test_unit_entering_flux_and_fused_apply_rescale_are_bit_identical(shape, n_fft):
+    """A unit entering flux that the advance does not read gives what a
+    buffer filled with ones gives, bit for bit, in measure and apply mode;
+    the apply mode's rescale, stored by the inverse z pass, equals the
+    measure mode's post pass bit for bit (each z-pass variant)."""
+    import torch
+    from steam_amd import simulate as S
+    from steam_amd.utils import ConvOperator
+    key = M.class_key(np.random.SeedSequence(9))
+    region = S.NoiseRegion(tuple(np.uint64(k) for k in key), (3, 5, 0), (64, 64, shape[2]))
+    kern = S._turbulon_envelope(1, .5, .5, .5, as_tensor=True)
+    op = ConvOperator(shape, tuple(kern.shape), n_fft=n_fft)
+    print(f"{shape} n_fft {op.n_fft} blocks {op.n_blocks}", flush=True)
+    spec = op.spectrum(kern)
+    ones = torch.ones(shape, device='cuda')
+    unit = torch.full(shape, float('nan'), device='cuda')   # never read
+    sk1, sk2 = torch.empty_like(ones), torch.empty_like(ones)
+    l1, _ = S.advance_flux_device(ones, region, kern, 0.3, 1, (1, 1, 1), alpha=1.7,
+                                  op=op, spec=spec, sk_t=sk1)
+    l2, _ = S.advance_flux_device(unit, region, kern, 0.3, 1, (1, 1, 1), alpha=1.7,
+                                  op=op, spec=spec, sk_t=sk2, unit_flux=True)
+    assert torch.equal(ones, unit) and torch.equal(sk1, sk2)
+    assert (l1.entering.total, l1.realized.total, l1.n_clipped) == \
+        (l2.entering.total, l2.realized.total, l2.n_clipped)
+    applied = torch.full(shape, float('nan'), device='cuda')
+    S.advance_flux_device(applied, region, kern, 0.3, 1, (1, 1, 1), alpha=1.7,
+                          op=op, spec=spec, ledger=l1, unit_flux=True)
+    assert torch.equal(applied, ones)
