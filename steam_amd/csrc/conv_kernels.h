@@ -1444,21 +1444,37 @@ __device__ __forceinline__ void zfwd_reg_body(const PassDesc& d, int wg, float2*
   const float* nfp = d.nfield + base;
   const unsigned lb = ln.lb;
   float2 v[V], nv[V];
+  // a unit entering flux (uniform over the launch): the flux is not read
+  const bool unit = MODE == MODE_FLUX && d.unit_in;
+  if (unit) {
 #pragma unroll
-  for (int m = 0; m < V; ++m) {
-    const int z0 = 2 * (q + m * P);
-    const int cb = ZLanes<ZP>::pair_bytes(m);
-    v[m] = make_float2(0.f, 0.f);
-    nv[m] = make_float2(0.f, 0.f);
-    const bool unit = MODE == MODE_FLUX && d.unit_in;
-    if (m < ln.m_full) {
-      if (ln.valid) {
-        v[m] = unit ? make_float2(1.f, 1.f) : ld_pair<ODD>(srcp, lb, cb);
-        if (MODE == MODE_FLUX) nv[m] = ld_pair<ODD>(nfp, lb, cb);
+    for (int m = 0; m < V; ++m) {
+      const int z0 = 2 * (q + m * P);
+      const int cb = ZLanes<ZP>::pair_bytes(m);
+      v[m] = make_float2(1.f, 1.f);
+      nv[m] = make_float2(0.f, 0.f);
+      if (m < ln.m_full) {
+        if (ln.valid) nv[m] = ld_pair<ODD>(nfp, lb, cb);
+      } else if (ln.valid && z0 < nz) {
+        nv[m] = ld_pair_top<ODD>(nfp, lb, cb, z0, nz);
       }
-    } else if (ln.valid && z0 < nz) {
-      v[m] = unit ? make_float2(1.f, 1.f) : ld_pair_top<ODD>(srcp, lb, cb, z0, nz);
-      if (MODE == MODE_FLUX) nv[m] = ld_pair_top<ODD>(nfp, lb, cb, z0, nz);
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      const int z0 = 2 * (q + m * P);
+      const int cb = ZLanes<ZP>::pair_bytes(m);
+      v[m] = make_float2(0.f, 0.f);
+      nv[m] = make_float2(0.f, 0.f);
+      if (m < ln.m_full) {
+        if (ln.valid) {
+          v[m] = ld_pair<ODD>(srcp, lb, cb);
+          if (MODE == MODE_FLUX) nv[m] = ld_pair<ODD>(nfp, lb, cb);
+        }
+      } else if (ln.valid && z0 < nz) {
+        v[m] = ld_pair_top<ODD>(srcp, lb, cb, z0, nz);
+        if (MODE == MODE_FLUX) nv[m] = ld_pair_top<ODD>(nfp, lb, cb, z0, nz);
+      }
     }
   }
   stage_tw2(g, tw2, N);
@@ -1620,15 +1636,19 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
   auto load_flux = [&](int m) {
     const int z0 = 2 * (q + m * P);
     const int cb = ZLanes<ZP>::pair_bytes(m);
-    if (d.unit_in)
-      fl[m] = make_float2(1.f, 1.f);
-    else if (m < ln.m_full)
+    if (m < ln.m_full)
       fl[m] = ln.valid ? ld_pair<ODD>(outp, lb, cb) : make_float2(0.f, 0.f);
     else
       fl[m] = (ln.valid && z0 < nz) ? ld_pair_top<ODD>(outp, lb, cb, z0, nz)
                                     : make_float2(0.f, 0.f);
   };
-  if (MODE == MODE_FLUX && STEAM_ZR_PREFETCH) {
+  // a unit entering flux (uniform over the launch): the flux is not read
+  const bool unit = MODE == MODE_FLUX && d.unit_in;
+  if (unit) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) fl[m] = make_float2(1.f, 1.f);
+  }
+  if (MODE == MODE_FLUX && STEAM_ZR_PREFETCH && !unit) {
 #pragma unroll
     for (int m = 0; m < V; ++m) load_flux(m);
   }
@@ -1664,7 +1684,7 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
     }
     return;
   }
-  if (!STEAM_ZR_PREFETCH) {
+  if (!STEAM_ZR_PREFETCH && !unit) {
 #pragma unroll
     for (int m = 0; m < V; ++m) load_flux(m);
   }
