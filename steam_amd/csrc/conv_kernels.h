@@ -1451,11 +1451,16 @@ __device__ __forceinline__ void zfwd_reg_body(const PassDesc& d, int wg, float2*
     for (int m = 0; m < V; ++m) {
       const int z0 = 2 * (q + m * P);
       const int cb = ZLanes<ZP>::pair_bytes(m);
-      v[m] = make_float2(1.f, 1.f);
+      // levels past the top stay zero: they are the transform's padding
+      v[m] = make_float2(0.f, 0.f);
       nv[m] = make_float2(0.f, 0.f);
       if (m < ln.m_full) {
-        if (ln.valid) nv[m] = ld_pair<ODD>(nfp, lb, cb);
+        if (ln.valid) {
+          v[m] = make_float2(1.f, 1.f);
+          nv[m] = ld_pair<ODD>(nfp, lb, cb);
+        }
       } else if (ln.valid && z0 < nz) {
+        v[m] = make_float2(1.f, 1.f);
         nv[m] = ld_pair_top<ODD>(nfp, lb, cb, z0, nz);
       }
     }
