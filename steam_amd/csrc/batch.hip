@@ -39,7 +39,7 @@ struct Group {
 // one instance's choices, fixed at creation
 struct ItemPlan {
   const steam_conv_plan* p;
-  bool zreg, tile, gen_noise, post, post_rescale;
+  bool zreg, tile, gen_noise, post, post_rescale, xf;
   int wg_noise_ext;  // workgroups of its strided noise launch (steam_batch_noise)
 };
 
@@ -97,7 +97,9 @@ std::tuple<int, int, int> variant_of(const ItemPlan& ip, int pass) {
     case BP_NOISE_EXT: return {ip.gen_noise ? -1 : 0, 0, 256};
     case BP_Z:
     case BP_ZINV:
-      if (ip.zreg) return {1, 2 * g.n2 + (g.nz & 1), ZR_NTH};  // plan length, odd nz
+      // plan length, odd nz, the served flux options (unit entering flux,
+      // stored apply rescale)
+      if (ip.zreg) return {1, 4 * g.n2 + 2 * (g.nz & 1) + (ip.xf ? 1 : 0), ZR_NTH};
       return {ip.tile ? 2 : 3, radix_set(g.az), Z_NTH};
     case BP_Y:
       if (g.reg_y) return {1, g.reg_y, 256};
@@ -212,13 +214,18 @@ int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
         break;
       case BP_Z:
         if (gr.variant == 1) {
-          with_zplan(gr.arg >> 1, [&](auto zp) {
-            if (gr.arg & 1)
-              hipLaunchKernelGGL((zfwd_reg_kernel<BatchSrc, MODE_FLUX, decltype(zp), true>), grid,
-                                 dim3(ZR_NTH), 0, s, src);
-            else
-              hipLaunchKernelGGL((zfwd_reg_kernel<BatchSrc, MODE_FLUX, decltype(zp), false>),
-                                 grid, dim3(ZR_NTH), 0, s, src);
+          with_zplan(gr.arg >> 2, [&](auto zp) {
+            using ZP = decltype(zp);
+            switch (gr.arg & 3) {
+              case 0: hipLaunchKernelGGL((zfwd_reg_kernel<BatchSrc, MODE_FLUX, ZP, false>), grid,
+                                         dim3(ZR_NTH), 0, s, src); break;
+              case 1: hipLaunchKernelGGL((zfwd_reg_kernel<BatchSrc, MODE_FLUX, ZP, false, true>),
+                                         grid, dim3(ZR_NTH), 0, s, src); break;
+              case 2: hipLaunchKernelGGL((zfwd_reg_kernel<BatchSrc, MODE_FLUX, ZP, true>), grid,
+                                         dim3(ZR_NTH), 0, s, src); break;
+              default: hipLaunchKernelGGL((zfwd_reg_kernel<BatchSrc, MODE_FLUX, ZP, true, true>),
+                                          grid, dim3(ZR_NTH), 0, s, src); break;
+            }
           });
         } else if (gr.variant == 2) {
           STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((zfwd_tile_kernel<BatchSrc, MODE_FLUX, RS_>),
@@ -259,13 +266,18 @@ int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
         break;
       case BP_ZINV:
         if (gr.variant == 1) {
-          with_zplan(gr.arg >> 1, [&](auto zp) {
-            if (gr.arg & 1)
-              hipLaunchKernelGGL((zinv_reg_kernel<BatchSrc, MODE_FLUX, decltype(zp), true>), grid,
-                                 dim3(ZR_NTH), 0, s, src);
-            else
-              hipLaunchKernelGGL((zinv_reg_kernel<BatchSrc, MODE_FLUX, decltype(zp), false>),
-                                 grid, dim3(ZR_NTH), 0, s, src);
+          with_zplan(gr.arg >> 2, [&](auto zp) {
+            using ZP = decltype(zp);
+            switch (gr.arg & 3) {
+              case 0: hipLaunchKernelGGL((zinv_reg_kernel<BatchSrc, MODE_FLUX, ZP, false>), grid,
+                                         dim3(ZR_NTH), 0, s, src); break;
+              case 1: hipLaunchKernelGGL((zinv_reg_kernel<BatchSrc, MODE_FLUX, ZP, false, true>),
+                                         grid, dim3(ZR_NTH), 0, s, src); break;
+              case 2: hipLaunchKernelGGL((zinv_reg_kernel<BatchSrc, MODE_FLUX, ZP, true>), grid,
+                                         dim3(ZR_NTH), 0, s, src); break;
+              default: hipLaunchKernelGGL((zinv_reg_kernel<BatchSrc, MODE_FLUX, ZP, true, true>),
+                                          grid, dim3(ZR_NTH), 0, s, src); break;
+            }
           });
         } else if (gr.variant == 2) {
           STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((zinv_tile_kernel<BatchSrc, MODE_FLUX, RS_>),
@@ -330,6 +342,7 @@ int make_item(const steam_batch_item& it, PassDesc& d, ItemPlan& ip) {
   if (d.apply && !defer) set_out_scale(d, d.led_in);
   ip.post = (!defer && !d.out_scale) || it.sk_dev != nullptr;
   ip.post_rescale = !defer && !d.out_scale;
+  ip.xf = d.unit_in || d.out_scale;
   d.npart = ip.zreg ? p->g.wg_zr : p->g.wg_z;
   return STEAM_OK;
 }

@@ -1429,7 +1429,10 @@ struct ZLanes {
   }
 };
 
-template <int MODE, class ZP, bool ODD, bool FULL>
+// XF: the served flux options (a unit entering flux, the apply mode's stored
+// rescale) are tested at run time; without it they compile away, so the
+// other advances run the kernels without them.
+template <int MODE, class ZP, bool ODD, bool FULL, bool XF>
 __device__ __forceinline__ void zfwd_reg_body(const PassDesc& d, int wg, float2* lds_lines,
                                               float2* tw2, double* red) {
   constexpr int N = ZP::N, V = ZP::V, P = ZP::P, LPW = ZP::LINES;
@@ -1445,7 +1448,7 @@ __device__ __forceinline__ void zfwd_reg_body(const PassDesc& d, int wg, float2*
   const unsigned lb = ln.lb;
   float2 v[V], nv[V];
   // a unit entering flux (uniform over the launch): the flux is not read
-  const bool unit = MODE == MODE_FLUX && d.unit_in;
+  const bool unit = XF && MODE == MODE_FLUX && d.unit_in;
   if (unit) {
 #pragma unroll
     for (int m = 0; m < V; ++m) {
@@ -1574,7 +1577,7 @@ __device__ __forceinline__ void zfwd_reg_body(const PassDesc& d, int wg, float2*
   (void)LPW;
 }
 
-template <class Src, int MODE, class ZP, bool ODD>
+template <class Src, int MODE, class ZP, bool ODD, bool XF = false>
 __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
   constexpr int N = ZP::N;
   __shared__ float2 lds_lines[ZTile<ZP>::LDS];
@@ -1585,12 +1588,12 @@ __global__ __launch_bounds__(ZR_NTH) void zfwd_reg_kernel(Src src) {
   const ConvGeom& g = d.g;
   // a workgroup whose lines all exist takes the unmasked body
   if ((int64_t)(wg + 1) * ZTile<ZP>::LW <= (int64_t)g.nx * g.ny)
-    zfwd_reg_body<MODE, ZP, ODD, true>(d, wg, lds_lines, tw2, red);
+    zfwd_reg_body<MODE, ZP, ODD, true, XF>(d, wg, lds_lines, tw2, red);
   else
-    zfwd_reg_body<MODE, ZP, ODD, false>(d, wg, lds_lines, tw2, red);
+    zfwd_reg_body<MODE, ZP, ODD, false, XF>(d, wg, lds_lines, tw2, red);
 }
 
-template <int MODE, class ZP, bool ODD, bool FULL>
+template <int MODE, class ZP, bool ODD, bool FULL, bool XF>
 __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2* lds_lines,
                                               float2* tw2, double* red) {
   constexpr int N = ZP::N, V = ZP::V, P = ZP::P;
@@ -1648,7 +1651,7 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
                                     : make_float2(0.f, 0.f);
   };
   // a unit entering flux (uniform over the launch): the flux is not read
-  const bool unit = MODE == MODE_FLUX && d.unit_in;
+  const bool unit = XF && MODE == MODE_FLUX && d.unit_in;
   if (unit) {
 #pragma unroll
     for (int m = 0; m < V; ++m) fl[m] = make_float2(1.f, 1.f);
@@ -1694,6 +1697,29 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
     for (int m = 0; m < V; ++m) load_flux(m);
   }
   const Pending pd = pending_of(d);
+  if (XF && d.out_scale) {
+    // apply mode (uniform over the launch): the recorded rescale, stored as
+    // the post pass would store it; no partial sums
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      const int z0 = 2 * (q + m * P);
+      const bool full = m < ln.m_full;
+      float o0 = __fadd_rn(entering_value(pd, fl[m].x), v[m].x);
+      float o1 = __fadd_rn(entering_value(pd, fl[m].y), v[m].y);
+      if (ODD && !full && z0 + 1 >= nz) o1 = 0.f;
+      o0 = (o0 < 0.f) ? 0.f : o0;
+      o1 = (o1 < 0.f) ? 0.f : o1;
+      if (full ? ln.valid : (ln.valid && z0 < nz)) {
+        const int cb = ZLanes<ZP>::pair_bytes(m);
+        const float2 ov = make_float2(out_value(d, o0), out_value(d, o1));
+        if (full)
+          st_pair<ODD>(outp, lb, cb, ov);
+        else
+          st_pair_top<ODD>(outp, lb, cb, z0, nz, ov);
+      }
+    }
+    return;
+  }
   const bool win = ln.in_window(g, d.win);
   const bool count_clip = win || !d.clip_win;
   double acc0 = 0.0;
@@ -1712,11 +1738,10 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
       if (count_clip) nclip += clipped;
       if (win) acc0 += (double)o0 + (double)o1;
       const int cb = ZLanes<ZP>::pair_bytes(m);
-      const float2 ov = make_float2(out_value(d, o0), out_value(d, o1));
       if (full)
-        st_pair<ODD>(outp, lb, cb, ov);
+        st_pair<ODD>(outp, lb, cb, make_float2(o0, o1));
       else
-        st_pair_top<ODD>(outp, lb, cb, z0, nz, ov);
+        st_pair_top<ODD>(outp, lb, cb, z0, nz, make_float2(o0, o1));
     }
   }
   if (!d.apply) {
@@ -1729,7 +1754,7 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
   }
 }
 
-template <class Src, int MODE, class ZP, bool ODD>
+template <class Src, int MODE, class ZP, bool ODD, bool XF = false>
 __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src src) {
   constexpr int N = ZP::N;
   __shared__ float2 lds_lines[ZTile<ZP>::LDS];
@@ -1739,9 +1764,9 @@ __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src sr
   const PassDesc& d = src.get(PASS_ZINV, wg);
   const ConvGeom& g = d.g;
   if ((int64_t)(wg + 1) * ZTile<ZP>::LW <= (int64_t)g.nx * g.ny)
-    zinv_reg_body<MODE, ZP, ODD, true>(d, wg, lds_lines, tw2, red);
+    zinv_reg_body<MODE, ZP, ODD, true, XF>(d, wg, lds_lines, tw2, red);
   else
-    zinv_reg_body<MODE, ZP, ODD, false>(d, wg, lds_lines, tw2, red);
+    zinv_reg_body<MODE, ZP, ODD, false, XF>(d, wg, lds_lines, tw2, red);
 }
 
 // ---------------------------------------------------------------------------

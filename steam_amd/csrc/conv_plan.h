@@ -128,16 +128,25 @@ inline int run_operator(const steam_conv_plan* p, PassDesc& d, hipStream_t s) {
   const bool tile = !zreg && g.tile_z && a16(d.in) && a16(d.out) && a16(d.nfield) &&
                     a16(d.sk);
   d.npart = zreg ? g.wg_zr : g.wg_z;
+  // the served flux options take the register z passes compiled with them
+  const bool xf = MODE == MODE_FLUX && (d.unit_in || d.out_scale);
   SingleSrc src{d};
   {
     ProfScope ps(STEAM_KF_ZFWD, s);
     if (zreg) {
       with_zplan(g.n2, [&](auto zp) {
-        if (odd)
-          hipLaunchKernelGGL((zfwd_reg_kernel<SingleSrc, MODE, decltype(zp), true>),
+        using ZP = decltype(zp);
+        if (xf && odd)
+          hipLaunchKernelGGL((zfwd_reg_kernel<SingleSrc, MODE, ZP, true, true>),
+                             dim3(g.wg_zr), dim3(ZR_NTH), 0, s, src);
+        else if (xf)
+          hipLaunchKernelGGL((zfwd_reg_kernel<SingleSrc, MODE, ZP, false, true>),
+                             dim3(g.wg_zr), dim3(ZR_NTH), 0, s, src);
+        else if (odd)
+          hipLaunchKernelGGL((zfwd_reg_kernel<SingleSrc, MODE, ZP, true>),
                              dim3(g.wg_zr), dim3(ZR_NTH), 0, s, src);
         else
-          hipLaunchKernelGGL((zfwd_reg_kernel<SingleSrc, MODE, decltype(zp), false>),
+          hipLaunchKernelGGL((zfwd_reg_kernel<SingleSrc, MODE, ZP, false>),
                              dim3(g.wg_zr), dim3(ZR_NTH), 0, s, src);
       });
     } else if (tile) {
@@ -168,11 +177,18 @@ inline int run_operator(const steam_conv_plan* p, PassDesc& d, hipStream_t s) {
     ProfScope ps(STEAM_KF_ZINV, s);
     if (zreg) {
       with_zplan(g.n2, [&](auto zp) {
-        if (odd)
-          hipLaunchKernelGGL((zinv_reg_kernel<SingleSrc, MODE, decltype(zp), true>),
+        using ZP = decltype(zp);
+        if (xf && odd)
+          hipLaunchKernelGGL((zinv_reg_kernel<SingleSrc, MODE, ZP, true, true>),
+                             dim3(g.wg_zr), dim3(ZR_NTH), 0, s, src);
+        else if (xf)
+          hipLaunchKernelGGL((zinv_reg_kernel<SingleSrc, MODE, ZP, false, true>),
+                             dim3(g.wg_zr), dim3(ZR_NTH), 0, s, src);
+        else if (odd)
+          hipLaunchKernelGGL((zinv_reg_kernel<SingleSrc, MODE, ZP, true>),
                              dim3(g.wg_zr), dim3(ZR_NTH), 0, s, src);
         else
-          hipLaunchKernelGGL((zinv_reg_kernel<SingleSrc, MODE, decltype(zp), false>),
+          hipLaunchKernelGGL((zinv_reg_kernel<SingleSrc, MODE, ZP, false>),
                              dim3(g.wg_zr), dim3(ZR_NTH), 0, s, src);
       });
     } else if (tile) {
