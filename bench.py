@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=512)
@@ -968,7 +968,7 @@ def main():
         torch.cuda.synchronize()
         plain = voxels * reps / (time.perf_counter() - p0)
 
-    batched = None if args.no_batched else batched_form(10, 2)
+    batched = None if args.no_batched else batched_form(40, 4)
     served = None if args.no_serve else served_form(args.serve_world,
                                                      args.serve_requests,
                                                      args.serve_rate,

@@ -204,7 +204,7 @@ extern "C" int steam_conv_xyplans(int32_t* out, int32_t max_out, int32_t* count)
     return set_error(STEAM_EINVAL, "steam_conv_xyplans: bad arguments");
   int n = 0;
   for (int len = 2; len <= FFT_LDSE; len += 2) {
-    if (!xyplan_lanes(len)) continue;
+    if (!xyplan_lanes(len) || reg_log2(len)) continue;
     if (n < max_out) out[n] = len;
     ++n;
   }

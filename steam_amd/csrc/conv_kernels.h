@@ -1248,7 +1248,8 @@ __device__ __forceinline__ void xpass_mr_body(const PassDesc& d, int wg, float2*
     // the inverse reads its twiddles from a second copy of the table: the
     // compiler cannot then keep the forward transform's twiddles live in
     // registers for it (with 50 values per lane those would not fit)
-    const float2* twi = V > 24 ? tw + N : tw;
+    constexpr bool REDERIVE = V > 24;
+    const float2* twi = REDERIVE ? tw + N : tw;
     if constexpr (!KPRE && !KREAL) {
       // the complex spectrum above 24 values per lane: loaded and applied
       // in chunks of 10 (all of it at once would not fit the registers)
@@ -1278,7 +1279,7 @@ __device__ __forceinline__ void xpass_mr_body(const PassDesc& d, int wg, float2*
     // likewise its LDS addresses: the lane's q and t pass through an empty
     // asm, so they are recomputed rather than held from the forward transform
     int qi = q, ti = t;
-    if constexpr (V > 24) asm volatile("" : "+v"(qi), "+v"(ti));
+    if constexpr (REDERIVE) asm volatile("" : "+v"(qi), "+v"(ti));
     XP::template fft_xy<true, true>(v, qi, lds, LS, ti, twi);
   } else {
     const float2 f = kernel_bin_factor(d, p % g.nzc);
@@ -1293,8 +1294,8 @@ __device__ __forceinline__ void xpass_mr_body(const PassDesc& d, int wg, float2*
 
 
 template <class Src, int XMODE, class XP>
-__global__ __launch_bounds__(xy_nth<XP>()) __attribute__((amdgpu_waves_per_eu(2))) void
-xpass_mr_kernel(Src src) {
+__global__ __launch_bounds__(xy_nth<XP>()) __attribute__((amdgpu_waves_per_eu(2)))
+void xpass_mr_kernel(Src src) {
   constexpr int N = XP::N;
   constexpr int NT = XP::V > 24 ? 2 * N : N;  // table copies (xpass_mr_body)
   __shared__ float2 lds[XMrLds<XP>::LINE + NT];
