@@ -107,6 +107,43 @@ def copy_calibration():
                       "launches": 3}))
 
 
+def valu_ceiling(reps=3):
+    """Measured compute-only VALU ceiling: wave-level fma instructions issued
+    per second by the library's probe (8 independent fma chains per lane, 8
+    waves per CU), best of ``reps``."""
+    import torch
+    from steam_amd import _lib
+    lib = _lib.load()
+    wgs, iters = 2048, 20000
+    out = torch.empty(wgs * 256, dtype=torch.float32, device='cuda')
+    n = ctypes.c_int64()
+    best = None
+    for _ in range(reps + 1):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        _lib.check(lib.steam_valu_probe(wgs, iters, _lib.ptr(out), ctypes.byref(n),
+                                        _lib.stream_handle()), "steam_valu_probe")
+        b.record()
+        b.synchronize()
+        rate = n.value / (a.elapsed_time(b) * 1e-3)
+        best = rate if best is None else max(best, rate)
+    return best
+
+
+def read_valu_per_wave(path, kernel_prefix):
+    """VALU instructions per wave and waves per dispatch of a kernel, from the
+    committed SQ-counter summary."""
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith(kernel_prefix):
+                f = line[74:].split()
+                return float(f[4]), float(f[3])
+    return None, None
+
+
 def read_traffic(path):
     """Per-family HBM bytes per launch from the committed rocprofv3 summary
     (FETCH_SIZE / WRITE_SIZE scaled by the copy calibration)."""
@@ -976,6 +1013,25 @@ def main():
 
     full = None if args.no_full_simulate else full_simulate_form()
 
+    # the VALU-bound noise kernel against the measured VALU ceiling: its
+    # vector instructions per wave (committed SQ counters of the full-grid
+    # launch of the serial region) over its measured serial duration
+    noise_valu = None
+    vpw, waves = read_valu_per_wave(
+        os.path.join(ROOT, "profiles", "sq_counters_flux_advance_512x512x256.txt"),
+        "void steam::noise_field_kernel<steam::SingleSrc, false")
+    if vpw and fam.get("noise"):
+        ceiling = valu_ceiling()
+        achieved = vpw * waves / (fam["noise"]["avg_ms"] * 1e-3)
+        noise_valu = {"bound": "valu", "valu_per_wave": vpw, "waves": waves,
+                      "valu_per_centre": round(vpw * waves * 64 / voxels, 2),
+                      "achieved_wave_instr_per_s": achieved,
+                      "ceiling_wave_instr_per_s": ceiling,
+                      "frac": round(achieved / ceiling, 4),
+                      "note": "SQ_INSTS_VALU counts an instruction once whatever its rate; "
+                              "the noise's 32x32->64 multiplies and transcendentals take "
+                              "several issue slots each, so frac understates its VALU use"}
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_sample or list(shape), c, alpha,
@@ -1000,6 +1056,7 @@ def main():
                    "n_fft": info.n_fft, "n_blocks": info.n_blocks,
                    "parallelism": "single GPU"},
         "roofline": roofline,
+        "noise_valu": noise_valu,
         "cpu_baseline": cpu,
         "plain_dropin_voxels_per_s": plain,
         "serial_resident_voxels_per_s": voxels / (serial_ms * 1e-3),

@@ -562,6 +562,15 @@ int steam_sum_f64(const float* data_dev, int64_t nx, int64_t ny, int64_t nz,
 #define STEAM_KF_NOISE 6    /* multiplier-noise field */
 #define STEAM_KF_COUNT 7
 
+/* Compute-only VALU ceiling probe: workgroups of 256 threads, each lane
+ * running 8 independent float32 fma chains for iters iterations; writes
+ * workgroups * 256 floats to out_dev and the number of wave-level fma
+ * instructions issued to *wave_fmas.  Time it with events on stream to get
+ * the measured issue rate of vector instructions, the ceiling of the
+ * VALU-bound noise kernel.  No counterpart in the reference. */
+int steam_valu_probe(int32_t workgroups, int32_t iters, float* out_dev,
+                     int64_t* wave_fmas, steam_stream_t stream);
+
 /* Start (on != 0) or stop recording; starting clears earlier records. */
 int steam_profile_enable(int on);
 /* Synchronise on the recorded events and report one family's launch count

@@ -157,6 +157,7 @@ _SIGNATURES = [
     ("steam_scale_inplace", ctypes.c_int, [_P, _I64, _F32, _P]),
     ("steam_sum_f64", ctypes.c_int,
      [_P, _I64, _I64, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_F64), _P]),
+    ("steam_valu_probe", ctypes.c_int, [_I32, _I32, _P, ctypes.POINTER(ctypes.c_int64), _P]),
     ("steam_profile_enable", ctypes.c_int, [ctypes.c_int]),
     ("steam_profile_read", ctypes.c_int,
      [ctypes.c_int, ctypes.POINTER(_I64), ctypes.POINTER(_F64)]),

@@ -104,9 +104,38 @@ __global__ __launch_bounds__(256) void part_sum_kernel(const double* part,
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// VALU ceiling probe: every lane runs 8 independent fma chains, so a wave
+// issues vector instructions back to back (8 fma per iteration; the loop
+// counter is scalar).  The sum of the chains is stored so nothing is dead.
+__global__ __launch_bounds__(256) void valu_probe_kernel(int iters, float b, float c,
+                                                         float* __restrict__ out) {
+  float a[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a[i] = (float)(threadIdx.x + i);
+  for (int k = 0; k < iters; ++k) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = fmaf(a[i], b, c);
+  }
+  float sum = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) sum += a[i];
+  out[(int64_t)blockIdx.x * 256 + threadIdx.x] = sum;
+}
+
 }  // namespace steam
 
 using namespace steam;
+
+extern "C" int steam_valu_probe(int32_t workgroups, int32_t iters, float* out_dev,
+                                int64_t* wave_fmas, steam_stream_t stream) {
+  if (workgroups < 1 || iters < 1 || !out_dev || !wave_fmas)
+    return set_error(STEAM_EINVAL, "steam_valu_probe: bad arguments");
+  hipLaunchKernelGGL(valu_probe_kernel, dim3(workgroups), dim3(256), 0, (hipStream_t)stream,
+                     iters, 0.999f, 0.001f, out_dev);
+  STEAM_LAUNCH_CHECK("valu_probe_kernel");
+  *wave_fmas = (int64_t)workgroups * 4 * (int64_t)iters * 8;
+  return STEAM_OK;
+}
 
 extern "C" int steam_profile_enable(int on) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
