@@ -57,6 +57,12 @@ struct steam_batch {
   int nslots = 0;
   hipEvent_t uploaded = nullptr;
   bool pending_upload = false;
+  // a pass whose instances fall into several kernel variants runs them on
+  // parallel branches: the largest group on the caller's stream, the others
+  // on these side streams, forked from and joined back into it
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -201,8 +207,8 @@ void refresh_item(steam_batch* b, int i) {
   }
 }
 
-int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
-  for (const Group& gr : b->groups[pass]) {
+int launch_group(steam_batch* b, int pass, const Group& gr, hipStream_t s, int inv) {
+  {
     BatchSrc src{b->dev + gr.offset, gr.count};
     const dim3 grid(gr.wgs);
     switch (pass) {
@@ -299,6 +305,46 @@ int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
   return STEAM_OK;
 }
 
+// One pass: its variant groups are independent (different instances), so
+// with several groups the largest runs on s and the others on the side
+// streams, forked from s before and joined into it after (a captured graph
+// records them as parallel branches).  Short launches of small members then
+// overlap the large ones instead of running one after another.
+int launch_pass(steam_batch* b, int pass, hipStream_t s, int inv) {
+  const std::vector<Group>& gs = b->groups[pass];
+  if (gs.size() < 2 || !b->side[0]) {
+    for (const Group& gr : gs) {
+      int rc = launch_group(b, pass, gr, s, inv);
+      if (rc) return rc;
+    }
+    return STEAM_OK;
+  }
+  size_t big = 0;
+  for (size_t i = 1; i < gs.size(); ++i)
+    if (gs[i].wgs > gs[big].wgs) big = i;
+  STEAM_HIP(hipEventRecord(b->fork, s));
+  bool used[2] = {false, false};
+  int k = 0;
+  for (size_t i = 0; i < gs.size(); ++i) {
+    if (i == big) continue;
+    const int j = k++ % 2;
+    if (!used[j]) {
+      STEAM_HIP(hipStreamWaitEvent(b->side[j], b->fork, 0));
+      used[j] = true;
+    }
+    int rc = launch_group(b, pass, gs[i], b->side[j], inv);
+    if (rc) return rc;
+  }
+  int rc = launch_group(b, pass, gs[big], s, inv);
+  if (rc) return rc;
+  for (int j = 0; j < 2; ++j) {
+    if (!used[j]) continue;
+    STEAM_HIP(hipEventRecord(b->join[j], b->side[j]));
+    STEAM_HIP(hipStreamWaitEvent(s, b->join[j], 0));
+  }
+  return STEAM_OK;
+}
+
 int make_item(const steam_batch_item& it, PassDesc& d, ItemPlan& ip) {
   const steam_conv_plan* p = it.plan;
   if (!p || !it.spectrum_dev || !it.workspace_dev || !it.flux_dev)
@@ -386,6 +432,22 @@ extern "C" int steam_batch_create(const steam_batch_item* items, int n,
     return check_hip(e, "hipEventCreate(batch)");
   }
   build_groups(b);
+  // side streams (default priority) and the fork/join events, only for a
+  // batch with a pass of several kernel variants
+  bool several = false;
+  for (int p = 0; p < BP_COUNT; ++p) several = several || b->groups[p].size() > 1;
+  if (several) {
+    e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    for (int j = 0; j < 2 && e == hipSuccess; ++j) {
+      e = hipStreamCreateWithFlags(&b->side[j], hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join[j], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+      const int rc = check_hip(e, "hipStreamCreate(batch side streams)");
+      steam_batch_destroy(b);
+      return rc;
+    }
+  }
   *out = b;
   return STEAM_OK;
 }
@@ -394,6 +456,14 @@ extern "C" int steam_batch_destroy(steam_batch* b) {
   if (!b) return STEAM_OK;
   if (b->pending_upload) (void)hipEventSynchronize(b->uploaded);
   if (b->uploaded) (void)hipEventDestroy(b->uploaded);
+  for (int j = 0; j < 2; ++j) {
+    if (b->side[j]) {
+      (void)hipStreamSynchronize(b->side[j]);
+      (void)hipStreamDestroy(b->side[j]);
+    }
+    if (b->join[j]) (void)hipEventDestroy(b->join[j]);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->dev) (void)hipFree(b->dev);
   if (b->host) (void)hipHostFree(b->host);
   delete b;
