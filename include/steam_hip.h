@@ -239,7 +239,9 @@ typedef struct steam_flux_ledger {
                                   steam/simulate.py:2198) */
 #define STEAM_UNIT_FLUX 16 /* the entering flux is 1 everywhere: flux_dev is
                               written, never read (a served tile's first
-                              class; no fill of the buffer beforehand) */
+                              class; no fill of the buffer beforehand).  In
+                              apply mode the recorded rescale is then stored
+                              by the advance itself (no rescale pass) */
 #define STEAM_SK_RAW 8 /* leave noise * entering flux in sk_dev, without the
                           1/<|noise|> scale: the streamed passes measure the
                           pattern on the raw product before the class's mean

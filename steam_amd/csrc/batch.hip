@@ -384,8 +384,9 @@ int make_item(const steam_batch_item& it, PassDesc& d, ItemPlan& ip) {
   ip.tile = !ip.zreg && p->g.tile_z && a16(d.out) && a16(d.nfield) && a16(d.sk);
   ip.gen_noise = it.noise_dev == nullptr;
   const bool defer = (it.flags & STEAM_DEFER_RESCALE) != 0;
-  // apply mode: the inverse z pass stores the rescaled flux
-  if (d.apply && !defer) set_out_scale(d, d.led_in);
+  // a served tile (unit entering flux, apply mode): the inverse z pass
+  // stores the rescaled flux (other apply-mode items keep the post pass)
+  if (d.apply && !defer && d.unit_in) set_out_scale(d, d.led_in);
   ip.post = (!defer && !d.out_scale) || it.sk_dev != nullptr;
   ip.post_rescale = !defer && !d.out_scale;
   ip.xf = d.unit_in || d.out_scale;

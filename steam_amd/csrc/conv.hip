@@ -658,9 +658,11 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
   if (unit_in && chain_pending)
     return set_error(STEAM_EINVAL, "steam_flux_advance: a unit entering flux has no pending "
                                    "rescale to chain");
-  // apply mode knows its rescale up front: the inverse z pass stores the
-  // rescaled flux and no post pass rescales it
-  if (d.apply && !defer_rescale) set_out_scale(d, d.led_in);
+  // a served tile (unit entering flux, apply mode) knows its rescale up
+  // front: the inverse z pass stores the rescaled flux and no post pass
+  // rescales it.  Other apply-mode advances keep the post pass, so a replay
+  // runs the kernels of the run it replays
+  if (d.apply && !defer_rescale && unit_in) set_out_scale(d, d.led_in);
   if (noise_dev) {
     d.nfield = const_cast<float*>(noise_dev);
   } else {

@@ -481,3 +481,30 @@ def test_file_api_simulate_then_refine(tmp_path):
              plain, seed=42)
     with pytest.raises(ValueError, match="flux_state"):
         refine(plain, 4, 12, 4, 12, 62.5, 62.5)
+
+
+def test_gpu_nest_replays_bit_for_bit_at_the_cost_comparison_size():
+    """The 2048x128x115 root of the reference's cost comparison
+    (paper/supplement.tex:713-716; synthetic linear profiles), refined over
+    x [512, 1536), y [32, 96) to 1.5 km: the nest replayed apply-only from its
+    ledger equals the nest bit for bit (a replay runs the kernels of the run
+    it replays: at these sizes the register z plans differ from the small
+    fixtures')."""
+    from steam_amd import cascade as C
+    from steam_amd.refine import refine_fields
+    z = np.arange(201) * 100.0
+    hp = 340e3 - 20e3 * z / z.max()
+    qp = 0.018 - 0.016 * z / z.max()
+    print("root 2048x128x115", flush=True)
+    root = C.simulate_fields(hp, qp, 2048, 128, 3000.0, 3000.0, 384000.0, 10.0, 20000.0,
+                             100.0, seed=0, save_for_refinement=True, as_tensors=True)
+    print("nest and its replay", flush=True)
+    nest = refine_fields(root['parent'], 512, 1536, 32, 96, 1500.0, 1500.0, as_tensors=True)
+    replay = refine_fields(root['parent'], 512, 1536, 32, 96, 1500.0, 1500.0,
+                           as_tensors=True, ledger=nest['ledger'])
+    for k in ('h', 'qt', 'flux'):
+        assert torch.equal(replay[k], nest[k]), k
+    again = C.simulate_fields(hp, qp, 2048, 128, 3000.0, 3000.0, 384000.0, 10.0, 20000.0,
+                              100.0, seed=0, as_tensors=True, ledger=root['ledger'])
+    for k in ('h', 'qt', 'flux'):
+        assert torch.equal(again[k], root[k]), k

@@ -249,9 +249,9 @@ def test_plain_cpu_config_with_the_power_law_kernel():
                                          ((32, 40, 40), 24)])   # multi-block generic
 def test_unit_entering_flux_and_fused_apply_rescale_are_bit_identical(shape, n_fft):
     """A unit entering flux that the advance does not read gives what a
-    buffer filled with ones gives, bit for bit, in measure and apply mode;
-    the apply mode's rescale, stored by the inverse z pass, equals the
-    measure mode's post pass bit for bit (each z-pass variant)."""
+    buffer filled with ones gives, bit for bit, in measure mode; with it the
+    apply mode's rescale, stored by the inverse z pass, equals the measure
+    mode's post pass bit for bit (each z-pass variant)."""
     import torch
     from steam_amd import simulate as S
     from steam_amd.utils import ConvOperator
