@@ -207,13 +207,15 @@ __device__ __forceinline__ float2 kernel_bin_factor(const PassDesc& d, int kz) {
 // along z (coalesced stores).
 // ---------------------------------------------------------------------------
 // workgroups of a noise field generated ahead of its advance (0: one per
-// four lines), two per CU of the 256.  Generated two classes ahead beside
-// the transforms, a 512x512x256 step took 0.914 ms at 512 on two boxes
-// (640: 0.927; 384: 0.918 on one box, 1.03-1.16 on another, where the noise
-// no longer kept up with the transforms; the whole grid competes with them
-// for slots: 1024 gave 0.975)
+// four lines), one per CU of the 256.  Generated two classes ahead beside
+// the transforms, a 512x512x256 step over 100 steps took 0.852-0.854 ms at
+// 256, 0.894 at 320, 0.864-0.870 at 384 and 448, 0.860-0.864 at 512 and
+// 0.858-0.862 at 640 (two rounds on one box).  With the noise kernel at 0.41 ms
+// alone 512 had been best (0.914 ms; 384 fell behind the transforms on some
+// boxes); at 0.33 ms one workgroup per CU keeps up and takes fewer of the
+// passes' slots.
 #ifndef STEAM_NOISE_WGS
-#define STEAM_NOISE_WGS 512
+#define STEAM_NOISE_WGS 256
 #endif
 // STRIDE: a grid smaller than the field's line groups walks them with a
 // grid stride (a few workgroups per CU keep the VALU busy beside memory-bound
