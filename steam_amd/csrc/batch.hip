@@ -131,13 +131,14 @@ size_t lds_of(const ItemPlan& ip, int pass) {
 }
 
 // Workgroup cap of a batch's noise generated ahead: STEAM_BATCH_NOISE_WGS
-// from the environment when set (read once), else 768 (three per CU; a
-// 48-member batch of 146M cells measured 3.43 ms per pipelined step at 768
-// against 5.8 at 384 and 3.68 at 1536)
+// from the environment when set (read once), else 1024 (four per CU; the
+// 48-member batch of 146M cells over 40 pipelined steps: 2.06-2.09 ms at
+// 1024, 2.14-2.15 at 768, 2.14-2.17 at 1280, 2.19-2.20 at 1536, 2.21-2.25
+// at 2048, 2.70 at 512 and 4.0 at 256, where the noise falls behind)
 static int batch_noise_wgs_cap() {
   static const int cap = [] {
     const char* e = getenv("STEAM_BATCH_NOISE_WGS");
-    return e && *e ? atoi(e) : 768;
+    return e && *e ? atoi(e) : 1024;
   }();
   return cap;
 }
