@@ -629,49 +629,78 @@ __global__ void ba_init_kernel(BaVec v, int nz, int n_rescale) {
     v.flags[1] = 0;
   }
 }
-// demean = done ? 0 : level mean; recorded for this step
-__global__ void ba_demean_kernel(BaVec v, int nz, int step, int n_rescale) {
+// One level's run partials joined exactly as level_combine_kernel joins
+// them (lane l takes runs l, l + 64, ... in order, then a fixed butterfly).
+__device__ __forceinline__ double join_level_runs(const double* __restrict__ row, int groups,
+                                                  int op, int lane) {
+  double acc = lv_init(op);
+#pragma unroll 8
+  for (int g = lane; g < groups; g += 64) acc = lv_join(op, acc, row[g]);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) acc = lv_join(op, acc, __shfl_xor(acc, off, 64));
+  return acc;
+}
+
+// A rescale step's first combine (the level mean of the pending-scaled d)
+// and its demean: shift = done ? 0 : level mean, recorded for the step.
+// One wave per level.
+__global__ __launch_bounds__(256) void ba_combine_demean_kernel(const double* __restrict__ part,
+                                                                int groups, int nz, double div,
+                                                                BaVec v, int step,
+                                                                int n_rescale) {
   if (v.flags[0]) return;
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  if (z < nz) {
-    const double dm = v.done[z] != 0.0 ? 0.0 : v.mean[z];
+  const int lane = threadIdx.x & 63;
+  const int z = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (z >= nz) return;
+  const double acc = join_level_runs(part + (int64_t)z * groups, groups, LV_SCALED_SUM, lane);
+  if (lane == 0) {
+    const double mean = acc / div;
+    v.mean[z] = mean;
+    const double dm = v.done[z] != 0.0 ? 0.0 : mean;
     v.shift[z] = dm;
     v.hist_demean[(int64_t)z * n_rescale + step] = dm;
   }
 }
-// m_abs = level mean |d|; scale = min(a0 / max(m_abs, 1e-300), 2);
-// done |= m_abs <= 0 | |scale - 1| < 1e-4; applied = done ? 1 : scale
-__global__ void ba_rescale_kernel(BaVec v, int nz) {
+
+// A rescale step's second combine (m_abs, the level mean of |d| after the
+// scale, demean and clip), then in the same workgroup: scale =
+// min(a0 / max(m_abs, 1e-300), 2); done |= m_abs <= 0 | |scale - 1| < 1e-4;
+// applied = done ? 1 : scale.  When every level is done the stop flag is
+// raised (the step's scale is then neither applied nor recorded), else the
+// scale is recorded and left pending for the next pass over d; n_loop =
+// step + 1 either way.
+constexpr int BA_CK_NTH = 1024;
+__global__ __launch_bounds__(BA_CK_NTH) void ba_combine_rescale_check_kernel(
+    const double* __restrict__ part, int groups, int nz, double div, BaVec v, int step,
+    int n_rescale) {
   if (v.flags[0]) return;
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  if (z < nz) {
-    const double m_abs = v.mean[z];
-    const double s = fmin(v.a0[z] / fmax(m_abs, 1e-300), 2.0);
-    const bool d = v.done[z] != 0.0 || m_abs <= 0.0 || fabs(s - 1.0) < 1e-4;
-    v.done[z] = d ? 1.0 : 0.0;
-    v.scale[z] = d ? 1.0 : s;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int z = w; z < nz; z += BA_CK_NTH / 64) {
+    const double acc = join_level_runs(part + (int64_t)z * groups, groups, LV_APPLY_ABS, lane);
+    if (lane == 0) {
+      const double m_abs = acc / div;
+      v.mean[z] = m_abs;
+      const double sc = fmin(v.a0[z] / fmax(m_abs, 1e-300), 2.0);
+      const bool d = v.done[z] != 0.0 || m_abs <= 0.0 || fabs(sc - 1.0) < 1e-4;
+      v.done[z] = d ? 1.0 : 0.0;
+      v.scale[z] = d ? 1.0 : sc;
+    }
   }
-}
-// one workgroup: n_loop = step + 1; when every level is done raise stop
-// (the step's scale is then neither applied nor recorded), else record it
-// and leave it pending, to be applied by the next pass over d
-__global__ __launch_bounds__(256) void ba_check_kernel(BaVec v, int nz, int step, int n_rescale) {
-  if (v.flags[0]) return;
   __shared__ int all_done;
   if (threadIdx.x == 0) all_done = 1;
   __syncthreads();
-  for (int z = threadIdx.x; z < nz; z += blockDim.x)
+  for (int z = threadIdx.x; z < nz; z += BA_CK_NTH)
     if (v.done[z] == 0.0) all_done = 0;
   __syncthreads();
   if (all_done) {
-    for (int z = threadIdx.x; z < nz; z += blockDim.x) v.pend[z] = 1.0;
+    for (int z = threadIdx.x; z < nz; z += BA_CK_NTH) v.pend[z] = 1.0;
     if (threadIdx.x == 0) {
       v.flags[1] = step + 1;
       v.flags[0] = 1;
     }
     return;
   }
-  for (int z = threadIdx.x; z < nz; z += blockDim.x) {
+  for (int z = threadIdx.x; z < nz; z += BA_CK_NTH) {
     v.hist_scale[(int64_t)z * n_rescale + step] = v.scale[z];
     v.pend[z] = v.scale[z];
   }
@@ -1202,13 +1231,17 @@ extern "C" int steam_bounded_add(float* pert_dev, const float* mean_z_dev, float
     apply.win[1] = (int)nx;
     apply.win[2] = 0;
     apply.win[3] = (int)ny;
+    // a step is four launches: each pass's partials, and each combine with
+    // the step's per-level work (demean; rescale and the all-done check)
+    const unsigned gzw = (unsigned)((nz + 3) / 4);
     for (int step = 0; step < n_rescale; ++step) {
-      if ((rc = level_reduce(scaled, v.mean, npts, red, s))) return rc;
-      hipLaunchKernelGGL(ba_demean_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz, step,
-                         n_rescale);
-      if ((rc = level_reduce(apply, v.mean, npts, red, s))) return rc;
-      hipLaunchKernelGGL(ba_rescale_kernel, dim3(gz), dim3(256), 0, s, v, (int)nz);
-      hipLaunchKernelGGL(ba_check_kernel, dim3(1), dim3(256), 0, s, v, (int)nz, step, n_rescale);
+      int64_t groups = 0;
+      if ((rc = level_partials(scaled, red, s, &groups))) return rc;
+      hipLaunchKernelGGL(ba_combine_demean_kernel, dim3(gzw), dim3(256), 0, s, red, (int)groups,
+                         (int)nz, npts, v, step, n_rescale);
+      if ((rc = level_partials(apply, red, s, &groups))) return rc;
+      hipLaunchKernelGGL(ba_combine_rescale_check_kernel, dim3(1), dim3(BA_CK_NTH), 0, s, red,
+                         (int)groups, (int)nz, npts, v, step, n_rescale);
     }
     // the final demean and clip run whatever the loop did; then e = d + phi
     // in place, and mu is bisected in that shifted space
