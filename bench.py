@@ -392,6 +392,47 @@ def served_form(world, n_req, rate, sweep=None):
     }
 
 
+class NoiseAhead:
+    """The pipelined form's noise: class i's field is generated ``ahead``
+    classes in advance on a side stream into a ring of ``ahead + 1`` buffers
+    (the noise is flux-independent and VALU-bound), while the advances run
+    their memory-bound FFT passes on the current stream; the side stream
+    waits only for the advance that last read the buffer it refills."""
+
+    def __init__(self, op, regions, c, alpha, shape, ahead):
+        import torch
+        from steam_amd import simulate as S
+        self._gen_args = (S, op, regions, c, alpha)
+        self.ahead, self.nbuf = ahead, ahead + 1
+        self.main = torch.cuda.current_stream()
+        self.side = torch.cuda.Stream()
+        self.bufs = [torch.empty(shape, dtype=torch.float32, device='cuda')
+                     for _ in range(self.nbuf)]
+        self.ev_noise = [torch.cuda.Event() for _ in range(self.nbuf)]
+        self.ev_adv = [torch.cuda.Event() for _ in range(self.nbuf)]
+        self.queued = -1
+
+    def _gen(self, i):
+        S, op, regions, c, alpha = self._gen_args
+        b = i % self.nbuf
+        if i >= self.nbuf:
+            self.side.wait_event(self.ev_adv[b])
+        S.generate_noise_device(op, regions[i], c, 1, (1, 1, 1), self.bufs[b],
+                                alpha=alpha, stream=self.side)
+        self.ev_noise[b].record(self.side)
+        self.queued = i
+
+    def step(self, i, advance):
+        """Queue class i: its noise (and the next ones' up to ``ahead``) on
+        the side stream, then ``advance(noise_t)`` on the current stream."""
+        b = i % self.nbuf
+        while self.queued < i + self.ahead:
+            self._gen(self.queued + 1)
+        self.main.wait_event(self.ev_noise[b])
+        advance(self.bufs[b])
+        self.ev_adv[b].record(self.main)
+
+
 def precision_form(steps=10):
     """The precision workload: float32 FFTs, spectral multiply and cascade
     arithmetic; float64 only in the kernel-correction sums and the volume
@@ -438,7 +479,27 @@ def precision_form(steps=10):
                               chain_pending=True, precision='kahan')
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
-    say(f"GPU {ms:.3f} ms per advance; running the FP64 oracle")
+    # the same chained advances with each class's noise generated two
+    # classes ahead beside the advance before (fresh keys per class)
+    seeds = np.random.SeedSequence(7).spawn(2 * steps + 8)
+    regions = [S.NoiseRegion(S.class_key(sq), (0, 0, 0), shape) for sq in seeds]
+    ahead_noise = NoiseAhead(op, regions, c, alpha, shape, 2)
+
+    def piped(i):
+        ahead_noise.step(i, lambda noise_t: S.advance_flux_device(
+            work, regions[i], kern, c, 1, (1, 1, 1), alpha=alpha, op=op, spec=spec,
+            defer_rescale=True, sync_ledger=False, chain_pending=True, precision='kahan',
+            noise_t=noise_t))
+    for i in range(2):
+        piped(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(2, 2 + steps):
+        piped(i)
+    torch.cuda.synchronize()
+    ms_piped = (time.perf_counter() - t0) * 1e3 / steps
+    del ahead_noise
+    say(f"GPU {ms:.3f} ms per advance, {ms_piped:.3f} pipelined; running the FP64 oracle")
     kern64 = M.turbulon_envelope(1, 0.5, 0.5, 0.5, support_factor=32, dtype=np.float64)
     t0 = time.perf_counter()
     f64, s64, led, _ = M.advance_flux(flux0, key, (0, 0, 0), shape, kern64, c, 1,
@@ -499,6 +560,10 @@ def precision_form(steps=10):
             "kernel": list(kern.shape), "alpha": alpha, "C1": c1, "c": c,
             "n_fft": op.n_fft, "n_blocks": op.n_blocks,
             "ms_per_step": ms, "voxels_per_s": vox / (ms * 1e-3),
+            "ms_per_step_pipelined": ms_piped,
+            "voxels_per_s_pipelined": vox / (ms_piped * 1e-3),
+            "pipelined": "each class's noise generated two classes ahead on a side stream "
+                         "(ms_per_step: the noise generated inside each advance)",
             "flux_rel_l2": fl2, "flux_max_abs": fmax,
             "sk_rel_l2": sl2, "sk_max_abs": smax,
             "flux_max_abs_relative_to_max": fmax / float(np.abs(f64).max()),
@@ -889,33 +954,12 @@ def main():
     # advance into a ring of ahead + 1 buffers, while class i runs its
     # memory-bound FFT passes on the main stream; the side stream only waits
     # for the advance that last read the buffer it refills
-    ahead = max(1, args.noise_ahead)
-    nbuf = ahead + 1
-    main = torch.cuda.current_stream()
-    side = torch.cuda.Stream()
-    bufs = [torch.empty(shape, dtype=torch.float32, device='cuda') for _ in range(nbuf)]
-    ev_noise = [torch.cuda.Event() for _ in range(nbuf)]
-    ev_adv = [torch.cuda.Event() for _ in range(nbuf)]
-    queued = [-1]
-
-    def gen(i):
-        b = i % nbuf
-        if i >= nbuf:
-            side.wait_event(ev_adv[b])
-        S.generate_noise_device(op, regions[i], c, 1, (1, 1, 1), bufs[b],
-                                alpha=alpha, stream=side)
-        ev_noise[b].record(side)
-        queued[0] = i
+    ahead_noise = NoiseAhead(op, regions, c, alpha, shape, max(1, args.noise_ahead))
 
     def piped_step(i):
-        b = i % nbuf
-        while queued[0] < i + ahead:
-            gen(queued[0] + 1)
-        main.wait_event(ev_noise[b])
-        S.advance_flux_device(flux, regions[i], kern, c, 1, (1, 1, 1), alpha=alpha,
-                              defer_rescale=True, sync_ledger=False, op=op,
-                              spec=spec, chain_pending=True, noise_t=bufs[b])
-        ev_adv[b].record(main)
+        ahead_noise.step(i, lambda noise_t: S.advance_flux_device(
+            flux, regions[i], kern, c, 1, (1, 1, 1), alpha=alpha, defer_rescale=True,
+            sync_ledger=False, op=op, spec=spec, chain_pending=True, noise_t=noise_t))
 
     # pipelined timed region (the reported value), uninstrumented; every
     # pipelined advance chains the pending rescale the previous one left, so
