@@ -39,7 +39,7 @@ struct Group {
 // one instance's choices, fixed at creation
 struct ItemPlan {
   const steam_conv_plan* p;
-  bool zreg, tile, gen_noise, post, post_rescale, xf;
+  bool zreg, tile, gen_noise, post, post_rescale, xf, apply;
   int wg_noise_ext;  // workgroups of its strided noise launch (steam_batch_noise)
 };
 
@@ -115,7 +115,7 @@ std::tuple<int, int, int> variant_of(const ItemPlan& ip, int pass) {
       if (g.reg_x) return {1, g.reg_x, (g.nx / 8 / xreg_lanes(g.reg_x)) * g.Tr};
       if (g.mr_x) return {3, g.mr_x, xyplan_threads(g.mr_x)};
       return {2, radix_set(g.ax), FFT_NTH};
-    case BP_FIN: return {0, 0, FIN_NTH};
+    case BP_FIN: return {ip.apply ? 1 : 0, 0, ip.apply ? FIN_APPLY_NTH : FIN_NTH};
     default: return {ip.post ? 0 : -1, ip.post_rescale ? 1 : 0, FFT_NTH};
   }
 }
@@ -295,7 +295,11 @@ int launch_group(steam_batch* b, int pass, const Group& gr, hipStream_t s, int i
         }
         break;
       case BP_FIN:
-        hipLaunchKernelGGL((finalize_kernel<BatchSrc>), grid, dim3(FIN_NTH), 0, s, src);
+        if (gr.variant == 1)
+          hipLaunchKernelGGL((finalize_apply_kernel<BatchSrc>), grid, dim3(FIN_APPLY_NTH), 0, s,
+                             src);
+        else
+          hipLaunchKernelGGL((finalize_kernel<BatchSrc>), grid, dim3(FIN_NTH), 0, s, src);
         break;
       default:
         hipLaunchKernelGGL((post_kernel<BatchSrc>), grid, dim3(FFT_NTH), 0, s, src, gr.arg);
@@ -391,6 +395,7 @@ int make_item(const steam_batch_item& it, PassDesc& d, ItemPlan& ip) {
   ip.post = (!defer && !d.out_scale) || it.sk_dev != nullptr;
   ip.post_rescale = !defer && !d.out_scale;
   ip.xf = d.unit_in || d.out_scale;
+  ip.apply = d.apply;
   d.npart = ip.zreg ? p->g.wg_zr : p->g.wg_z;
   return STEAM_OK;
 }

@@ -680,7 +680,11 @@ extern "C" int steam_flux_advance(const steam_conv_plan* p,
   SingleSrc src{d};
   {
     ProfScope ps(STEAM_KF_FINALIZE, s);
-    hipLaunchKernelGGL((finalize_kernel<SingleSrc>), dim3(1), dim3(FIN_NTH), 0, s, src);
+    if (d.apply)
+      hipLaunchKernelGGL((finalize_apply_kernel<SingleSrc>), dim3(1), dim3(FIN_APPLY_NTH), 0, s,
+                         src);
+    else
+      hipLaunchKernelGGL((finalize_kernel<SingleSrc>), dim3(1), dim3(FIN_NTH), 0, s, src);
   }
   STEAM_LAUNCH_CHECK("finalize_kernel");
   const bool post_rescale = !defer_rescale && !d.out_scale;

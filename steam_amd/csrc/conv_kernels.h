@@ -1780,6 +1780,27 @@ __global__ __launch_bounds__(ZR_NTH) STEAM_ZINV_ATTR void zinv_reg_kernel(Src sr
 // sum is a shuffle tree per wave plus one pass over the waves
 constexpr int FIN_NTH = 1024;
 
+// The advance's device ledger from its sums: the means, the rescale
+// entering / realized (or flatten to the entering mean when nothing is
+// left), and 1 / <|noise|> for S_k.
+__device__ __forceinline__ void finalize_store(const PassDesc& d, DevLedger L) {
+  const double em = L.entering_total / fmax(L.entering_count, 1.0);
+  const double rm = L.realized_total / fmax(L.realized_count, 1.0);
+  const double ma = L.noise_abs_total / fmax(L.noise_abs_count, 1.0);
+  if (rm > 0.0) {
+    L.rescale = (float)(em / rm);
+    L.flatten = 0;
+  } else {
+    L.rescale = 1.0f;
+    L.flatten = 1;
+  }
+  L.entering_mean_f = (float)em;
+  L.have_mean_abs = ma > 0.0;
+  L.inv_mean_abs = ma > 0.0 ? (float)(1.0 / ma) : 1.0f;
+  L.pad = 0;
+  *d.led = L;
+}
+
 template <class Src>
 __global__ __launch_bounds__(FIN_NTH) void finalize_kernel(Src src) {
   int wg = blockIdx.x;
@@ -1812,21 +1833,21 @@ __global__ __launch_bounds__(FIN_NTH) void finalize_kernel(Src src) {
     if (tid != 0) return;
     L = d.led_in;
   }
-  const double em = L.entering_total / fmax(L.entering_count, 1.0);
-  const double rm = L.realized_total / fmax(L.realized_count, 1.0);
-  const double ma = L.noise_abs_total / fmax(L.noise_abs_count, 1.0);
-  if (rm > 0.0) {
-    L.rescale = (float)(em / rm);
-    L.flatten = 0;
-  } else {
-    L.rescale = 1.0f;
-    L.flatten = 1;
-  }
-  L.entering_mean_f = (float)em;
-  L.have_mean_abs = ma > 0.0;
-  L.inv_mean_abs = ma > 0.0 ? (float)(1.0 / ma) : 1.0f;
-  L.pad = 0;
-  *d.led = L;
+  finalize_store(d, L);
+}
+
+// The apply-mode finalize (the ledger is given, so there is nothing to
+// reduce): one wave per instance.  A 1024-thread workgroup would wait for
+// a whole CU's wave slots to free up, which beside other streams' kernels
+// (the tile server's) held it for hundreds of microseconds.
+constexpr int FIN_APPLY_NTH = 64;
+
+template <class Src>
+__global__ __launch_bounds__(FIN_APPLY_NTH) void finalize_apply_kernel(Src src) {
+  if (threadIdx.x != 0) return;
+  int wg = blockIdx.x;
+  const PassDesc& d = src.get(PASS_FIN, wg);
+  finalize_store(d, d.led_in);
 }
 
 // ---------------------------------------------------------------------------
