@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=1000.0,
                     help="tile requests per second of the served form")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-sweep", type=float, nargs="*", default=[500.0, 2000.0, 4000.0],
+    ap.add_argument("--serve-sweep", type=float, nargs="*", default=[500.0, 2000.0, 4000.0, 5000.0],
                     help="also serve the trace at these rates (req/s) and report "
                          "throughput and p50/p99 per rate")
     ap.add_argument("--serve-world", type=int, nargs=3, default=[2048, 2048, 256])

@@ -317,6 +317,19 @@ int steam_batch_destroy(steam_batch* batch);
 int steam_batch_set_noise(steam_batch* batch, int32_t i,
                           const steam_noise_params* noise);
 
+/* Store only a box of instance i's advanced flux, dense (x, y, z order) in
+ * out_dev, for its next advances; the flux grid itself is then not written.
+ * origin[3] and shape[3] are the box's corner and extent in the instance's
+ * grid.  Only for an apply-mode instance with a unit entering flux
+ * (STEAM_UNIT_FLUX), no S_k output and a register z plan: a served tile,
+ * whose interior is the box, so no copy out of the padded grid follows the
+ * advance.  out_dev NULL stores the whole grid again.  Waits for the
+ * previous advance's descriptor upload, as steam_batch_set_noise does.  No
+ * counterpart in the reference, whose tiles are cut from whole fields
+ * (steam/streaming.py:822-847). */
+int steam_batch_set_crop(steam_batch* batch, int32_t i, float* out_dev,
+                         const int64_t* origin, const int64_t* shape);
+
 /* The noise fields of the instances created with a noise_dev buffer, for
  * their next advance: every such instance's multiplier noise (its current
  * keyed-noise parameters) written into its buffer, one launch per variant,

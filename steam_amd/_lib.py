@@ -119,6 +119,8 @@ _SIGNATURES = [
      [ctypes.POINTER(BatchItem), _I32, ctypes.POINTER(_P)]),
     ("steam_batch_destroy", ctypes.c_int, [_P]),
     ("steam_batch_set_noise", ctypes.c_int, [_P, _I32, ctypes.POINTER(NoiseParams)]),
+    ("steam_batch_set_crop", ctypes.c_int,
+     [_P, _I32, _P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     ("steam_batch_info", ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),
     ("steam_batch_advance", ctypes.c_int, [_P, _P]),
     ("steam_batch_advance_after_noise", ctypes.c_int, [_P, _P]),

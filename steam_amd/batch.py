@@ -214,6 +214,29 @@ class BatchedAdvance:
             _lib.check(lib.steam_batch_set_noise(self.handle, i, ctypes.byref(p)),
                        "steam_batch_set_noise")
 
+    def set_crops(self, crops):
+        """Store only a box of each instance's advanced flux for the next
+        advances: ``crops[i]`` is ``(out, origin)``, ``out`` a contiguous
+        float32 device tensor of the box's shape and ``origin`` its corner in
+        the instance's grid (the grid itself is then not written), or None
+        for the whole grid.  Only for apply-mode instances with a unit
+        entering flux, no S_k and a register z plan (served tiles); others
+        raise ValueError."""
+        lib = _lib.load()
+        for i, c in enumerate(crops):
+            if c is None:
+                _lib.check(lib.steam_batch_set_crop(self.handle, i, None, None, None),
+                           "steam_batch_set_crop")
+                continue
+            out, origin = c
+            if (out.dtype != torch.float32 or not out.is_cuda or not out.is_contiguous()
+                    or out.dim() != 3):
+                raise ValueError("a crop is a contiguous float32 (x, y, z) device tensor")
+            o = (ctypes.c_int64 * 3)(*[int(v) for v in origin])
+            n = (ctypes.c_int64 * 3)(*[int(v) for v in out.shape])
+            _lib.check(lib.steam_batch_set_crop(self.handle, i, _lib.ptr(out), o, n),
+                       "steam_batch_set_crop")
+
     def advance(self, stream=None):
         """Advance every instance by one class on ``stream`` (default:
         current); asynchronous."""

@@ -493,6 +493,36 @@ extern "C" int steam_batch_set_noise(steam_batch* b, int i,
   return STEAM_OK;
 }
 
+extern "C" int steam_batch_set_crop(steam_batch* b, int i, float* out_dev,
+                                    const int64_t* origin, const int64_t* shape) {
+  if (!b || i < 0 || i >= b->n)
+    return set_error(STEAM_EINVAL, "steam_batch_set_crop: bad batch or index");
+  PassDesc& d = b->items[i];
+  if (out_dev) {
+    if (!origin || !shape) return set_error(STEAM_EINVAL, "steam_batch_set_crop: NULL box");
+    if (!d.out_scale || d.sk || !b->plans[i].zreg)
+      return set_error(STEAM_EINVAL,
+                       "steam_batch_set_crop: only an apply-mode instance with a unit entering "
+                       "flux, no S_k and a register z plan stores a crop");
+    const int64_t dims[3] = {d.g.nx, d.g.ny, d.g.nz};
+    for (int a = 0; a < 3; ++a)
+      if (origin[a] < 0 || shape[a] < 1 || origin[a] + shape[a] > dims[a])
+        return set_error(STEAM_EINVAL, "steam_batch_set_crop: box outside the grid");
+  }
+  // the previous advance's upload must have read the staging copy
+  if (b->pending_upload) {
+    STEAM_HIP(hipEventSynchronize(b->uploaded));
+    b->pending_upload = false;
+  }
+  d.crop_out = out_dev;
+  for (int a = 0; a < 3; ++a) {
+    d.crop_o[a] = out_dev ? (int)origin[a] : 0;
+    d.crop_n[a] = out_dev ? (int)shape[a] : 0;
+  }
+  refresh_item(b, i);
+  return STEAM_OK;
+}
+
 extern "C" int steam_batch_mark_replay(steam_batch* b, steam_stream_t stream) {
   if (!b) return set_error(STEAM_EINVAL, "steam_batch_mark_replay: NULL batch");
   // a replay's captured upload reads the staging copy when the replay runs:

@@ -108,6 +108,9 @@ struct PassDesc {
   int out_scale;        // apply mode: the inverse z pass writes the rescaled
   int out_flat;         // flux (flat ? out_em : flux * out_rs), the recorded
   float out_rs, out_em; // ledger's rescale, so no post pass reads it again
+  float* crop_out;      // with out_scale: only the box crop_o + [0, crop_n) of
+  int crop_o[3];        // the advanced flux is stored, dense in crop_out, and
+  int crop_n[3];        // the grid itself is not written (NULL: the grid)
   int wg_base[7];       // batched: first workgroup of z, y, x, zinv, post,
                         // finalize, noise passes
 };

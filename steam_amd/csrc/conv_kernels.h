@@ -1703,6 +1703,28 @@ __device__ __forceinline__ void zinv_reg_body(const PassDesc& d, int wg, float2*
   if (XF && d.out_scale) {
     // apply mode (uniform over the launch): the recorded rescale, stored as
     // the post pass would store it; no partial sums
+    if (d.crop_out) {
+      // a crop (uniform over the launch): the lane's line, if inside the
+      // crop's x/y extent, stores the levels inside its z extent, each as
+      // a float (the crop's lines need not start on an even level)
+      const unsigned li = (unsigned)(ln.line0 + ln.l);
+      const int cx = (int)(li / (unsigned)g.ny) - d.crop_o[0];
+      const int cy = (int)(li % (unsigned)g.ny) - d.crop_o[1];
+      const bool cl = ln.valid && cx >= 0 && cx < d.crop_n[0] && cy >= 0 && cy < d.crop_n[1];
+      const int zc0 = d.crop_o[2], zc1 = d.crop_o[2] + d.crop_n[2];
+      float* co = d.crop_out + ((int64_t)cx * d.crop_n[1] + cy) * d.crop_n[2] - zc0;
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        const int z0 = 2 * (q + m * P);
+        float o0 = __fadd_rn(entering_value(pd, fl[m].x), v[m].x);
+        float o1 = __fadd_rn(entering_value(pd, fl[m].y), v[m].y);
+        o0 = (o0 < 0.f) ? 0.f : o0;
+        o1 = (o1 < 0.f) ? 0.f : o1;
+        if (cl && z0 >= zc0 && z0 < zc1) co[z0] = out_value(d, o0);
+        if (cl && z0 + 1 >= zc0 && z0 + 1 < zc1) co[z0 + 1] = out_value(d, o1);
+      }
+      return;
+    }
 #pragma unroll
     for (int m = 0; m < V; ++m) {
       const int z0 = 2 * (q + m * P);

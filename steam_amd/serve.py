@@ -193,6 +193,9 @@ class _SlotPool:
         self.workspaces = [torch.empty(op.workspace_bytes, dtype=torch.uint8, device='cuda')
                            for _ in range(slots)]
         self.batches = {}
+        # whether the batches store each tile's interior straight into its
+        # result (a register z plan), else it is copied out of the slot
+        self.crops = True
 
     @property
     def nbytes(self):
@@ -282,11 +285,23 @@ class TileServer:
             # the advance starts from a unit flux it does not read (no fill);
             # the stream is free: the batch's previous upload has been read
             b.set_regions([w.region(po) for po, _, _ in boxes])
+            outs = None
+            if pool.crops:
+                # the inverse z pass stores each tile's interior in its result
+                outs = [torch.empty(tuple(r["shape"]), dtype=torch.float32, device='cuda')
+                        for r in reqs]
+                try:
+                    b.set_crops([(o, inner) for o, (_, _, inner) in zip(outs, boxes)])
+                except ValueError:
+                    pool.crops = False
+                    b.set_crops([None] * len(reqs))
+                    outs = None
             b.advance(stream)
-            outs = []
-            for i, (r, (_, _, (ix, iy, iz))) in enumerate(zip(reqs, boxes)):
-                tx, ty, tz = r["shape"]
-                outs.append(pool.flux[i, ix:ix + tx, iy:iy + ty, iz:iz + tz].contiguous())
+            if outs is None:
+                outs = []
+                for i, (r, (_, _, (ix, iy, iz))) in enumerate(zip(reqs, boxes)):
+                    tx, ty, tz = r["shape"]
+                    outs.append(pool.flux[i, ix:ix + tx, iy:iy + ty, iz:iz + tz].contiguous())
             ev = torch.cuda.Event()
             ev.record(stream)
         return pool, outs, ev

@@ -206,3 +206,59 @@ def test_pipelined_batch_equals_the_batched_advance():
         np.testing.assert_array_equal(a.flux.cpu().numpy(), b.flux.cpu().numpy())
     for la, lb in zip(pipe.ledgers(), plain.ledgers()):
         assert la.realized == lb.realized and la.n_clipped == lb.n_clipped
+
+
+def test_crops_store_a_box_of_the_whole_grid_bit_for_bit():
+    """An apply-mode instance with a unit entering flux and a register z plan
+    (a served tile) stores only a box of its advanced flux, dense in a
+    caller tensor: the box equals the same box of its whole-grid advance bit
+    for bit, odd z corners and odd level counts included.  A measure-mode
+    instance and a box outside the grid are refused."""
+    from steam_amd import simulate as S
+    from steam_amd.batch import BatchedAdvance, Instance
+    kern = S._turbulon_envelope(1, .5, .5, .5, as_tensor=True)
+    shapes = [(64, 48, 40), (48, 40, 31), (64, 64, 32)]
+    boxes = [((5, 7, 3), (40, 30, 33)), ((0, 0, 1), (48, 40, 30)), ((10, 20, 0), (8, 8, 32))]
+    regions = []
+    for i, shape in enumerate(shapes):
+        key = M.class_key(np.random.SeedSequence(300 + i))
+        regions.append(S.NoiseRegion(tuple(np.uint64(k) for k in key), (0, 0, 0), shape))
+    # the ledgers: a measure-mode advance of each grid from a unit flux
+    measure = [Instance(torch.ones(sh, device='cuda'), kern, r, 0.3, alpha=1.8)
+               for sh, r in zip(shapes, regions)]
+    mb = BatchedAdvance(measure, autotune=False)
+    mb.advance()
+    ledgers = mb.ledgers()
+    with pytest.raises(ValueError):
+        mb.set_crops([(torch.empty(shapes[0], device='cuda'), (0, 0, 0)), None, None])
+
+    def apply_batch():
+        insts = [Instance(torch.empty(sh, device='cuda'), kern, r, 0.3, alpha=1.8,
+                          ledger=led, unit_flux=True)
+                 for sh, r, led in zip(shapes, regions, ledgers)]
+        return insts, BatchedAdvance(insts, autotune=False)
+
+    insts, b = apply_batch()
+    b.advance()
+    torch.cuda.synchronize()
+    whole = [inst.flux.cpu().numpy() for inst in insts]
+    outs = [torch.full(n, -1.0, device='cuda') for _, n in boxes]
+    b.set_crops([(o, org) for o, (org, _) in zip(outs, boxes)])
+    for inst in insts:
+        inst.flux.fill_(-2.0)
+    b.advance()
+    torch.cuda.synchronize()
+    for i, ((o, n), out) in enumerate(zip(boxes, outs)):
+        ref = whole[i][o[0]:o[0] + n[0], o[1]:o[1] + n[1], o[2]:o[2] + n[2]]
+        print(f"instance {i}: box {o} + {n}", flush=True)
+        np.testing.assert_array_equal(out.cpu().numpy(), ref)
+        # the grid itself is not written while a crop is set
+        assert np.all(insts[i].flux.cpu().numpy() == -2.0)
+    # back to whole grids
+    b.set_crops([None] * len(insts))
+    b.advance()
+    torch.cuda.synchronize()
+    for inst, ref in zip(insts, whole):
+        np.testing.assert_array_equal(inst.flux.cpu().numpy(), ref)
+    with pytest.raises(ValueError):
+        b.set_crops([(torch.empty((8, 8, 8), device='cuda'), (60, 0, 0)), None, None])
