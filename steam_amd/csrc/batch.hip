@@ -110,11 +110,11 @@ std::tuple<int, int, int> variant_of(const ItemPlan& ip, int pass) {
     case BP_Y:
       if (g.reg_y) return {1, g.reg_y, 256};
       if (g.mr_y) return {3, g.mr_y, xyplan_threads(g.mr_y)};
-      return {2, radix_set(g.ay), FFT_NTH};
+      return {2, radix_set(g.ay), generic_xy_nth(radix_set(g.ay))};
     case BP_X:
       if (g.reg_x) return {1, g.reg_x, (g.nx / 8 / xreg_lanes(g.reg_x)) * g.Tr};
       if (g.mr_x) return {3, g.mr_x, xyplan_threads(g.mr_x)};
-      return {2, radix_set(g.ax), FFT_NTH};
+      return {2, radix_set(g.ax), generic_xy_nth(radix_set(g.ax))};
     case BP_FIN: return {ip.apply ? 1 : 0, 0, ip.apply ? FIN_APPLY_NTH : FIN_NTH};
     default: return {ip.post ? 0 : -1, ip.post_rescale ? 1 : 0, FFT_NTH};
   }
@@ -252,8 +252,9 @@ int launch_group(steam_batch* b, int pass, const Group& gr, hipStream_t s, int i
                                dim3(xy_nth<decltype(xp)>()), 0, s, src, inv);
           });
         } else {
-          STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((ypass_kernel<BatchSrc, RS_>), grid,
-                                                       dim3(FFT_NTH), gr.lds, s, src, inv));
+          STEAM_RS_DISPATCH_XY(gr.arg, hipLaunchKernelGGL((ypass_kernel<BatchSrc, RS_>), grid,
+                                                          dim3(generic_xy_nth(RS_)), gr.lds, s,
+                                                          src, inv));
         }
         break;
       case BP_X:
@@ -267,8 +268,9 @@ int launch_group(steam_batch* b, int pass, const Group& gr, hipStream_t s, int i
                                dim3(xy_nth<decltype(xp)>()), 0, s, src);
           });
         } else {
-          STEAM_RS_DISPATCH(gr.arg, hipLaunchKernelGGL((xpass_kernel<BatchSrc, 0, RS_>), grid,
-                                                       dim3(FFT_NTH), gr.lds, s, src));
+          STEAM_RS_DISPATCH_XY(gr.arg, hipLaunchKernelGGL((xpass_kernel<BatchSrc, 0, RS_>), grid,
+                                                          dim3(generic_xy_nth(RS_)), gr.lds, s,
+                                                          src));
         }
         break;
       case BP_ZINV:

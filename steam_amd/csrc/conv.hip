@@ -3,6 +3,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <complex>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -42,6 +43,95 @@ static void factor_axis(int n, AxisPlan& ap) {
   for (int p = 11; m != 1 && p * p <= m && ap.nst < FFT_MAXST - 1; p += 2)
     while (m % p == 0 && m != p && ap.nst < FFT_MAXST - 1) { ap.radix[ap.nst++] = p; m /= p; }
   if (m != 1) ap.radix[ap.nst++] = m;  // a prime, or the product of the rest
+}
+
+// A direct stage of radix R costs R complex products per element; the
+// Bluestein form costs two transforms of a smooth length M >= 2n - 1 per
+// transform, in the LDS-staged passes.  Measured x passes (140 x 97 planes):
+// 2060 = 4 5 103 took 5.27 ms direct and 4.77 ms Bluestein, 1036 = 4 7 37
+// 0.82 ms direct and 3.26 ms Bluestein, so it takes over from radix 128 on,
+// where a direct stage would cost hundreds of products per element (a
+// prime line: n of them).  STEAM_BLUESTEIN=0 in the environment keeps the
+// direct stages (read once; for measurement).
+constexpr int BLUE_MIN_RADIX = 128;
+
+static bool bluestein_on() {
+  static const bool on = [] {
+    const char* e = getenv("STEAM_BLUESTEIN");
+    return !(e && *e == '0');
+  }();
+  return on;
+}
+
+// The x/y form of a length: its stages, and, when they hold a long direct
+// stage, the Bluestein length and its stages (the smallest M >= 2n - 1 that
+// factors into radices 2..8 within the stage slots).  Tables come later.
+static void factor_xy_axis(int n, AxisPlan& ap) {
+  factor_axis(n, ap);
+  ap.bm = 0;
+  ap.bnst = 0;
+  int worst = 0;
+  for (int s = 0; s < ap.nst; ++s)
+    worst = std::max(worst, ap.radix[s] == 0 ? n : ap.radix[s]);
+  if (worst < BLUE_MIN_RADIX || !bluestein_on()) return;
+  for (int M = 2 * n - 1; M <= FFT_LDSE * 2; ++M) {
+    AxisPlan mp;
+    factor_axis(M, mp);
+    bool smooth = mp.nst > 0;
+    for (int s = 0; s < mp.nst; ++s) smooth = smooth && mp.radix[s] >= 2 && mp.radix[s] <= 8;
+    if (!smooth) continue;
+    ap.bm = M;
+    ap.bnst = mp.nst;
+    for (int s = 0; s < FFT_MAXST; ++s) ap.bradix[s] = s < mp.nst ? mp.radix[s] : 0;
+    return;
+  }
+}
+
+// Forward DFT in float64 of a 7-smooth length (recursive decimation in time
+// over the smallest prime factor), for the Bluestein tables.
+static void dft64(std::vector<std::complex<double>>& x) {
+  const int n = (int)x.size();
+  if (n <= 1) return;
+  int p = 2;
+  while (n % p) ++p;
+  const int q = n / p;
+  std::vector<std::vector<std::complex<double>>> sub(p, std::vector<std::complex<double>>(q));
+  for (int r = 0; r < p; ++r)
+    for (int j = 0; j < q; ++j) sub[r][j] = x[j * p + r];
+  for (int r = 0; r < p; ++r) dft64(sub[r]);
+  for (int k = 0; k < n; ++k) {
+    std::complex<double> acc = 0.0;
+    for (int r = 0; r < p; ++r) {
+      const double a = -2.0 * M_PI * (double)((int64_t)r * k % n) / (double)n;
+      acc += sub[r][k % q] * std::complex<double>(cos(a), sin(a));
+    }
+    x[k] = acc;
+  }
+}
+
+// The Bluestein tables of an axis: chirp[m] = exp(-pi i m^2 / n) (m^2 taken
+// mod 2n exactly), the transform of the wrapped conjugate chirp over M
+// divided by M, and the M-point twiddles; appended to tw.
+static void bluestein_tables(const AxisPlan& ap, std::vector<float2>& tw) {
+  const int n = ap.n, M = ap.bm;
+  std::vector<std::complex<double>> c(n), b(M, 0.0);
+  for (int m = 0; m < n; ++m) {
+    const int64_t e = ((int64_t)m * m) % (2 * (int64_t)n);
+    const double a = -M_PI * (double)e / (double)n;
+    c[m] = std::complex<double>(cos(a), sin(a));
+  }
+  for (int m = 0; m < n; ++m) {
+    b[m] = std::conj(c[m]);
+    if (m > 0) b[M - m] = std::conj(c[m]);
+  }
+  dft64(b);
+  for (int m = 0; m < n; ++m) tw.push_back(make_float2((float)c[m].real(), (float)c[m].imag()));
+  for (int m = 0; m < M; ++m)
+    tw.push_back(make_float2((float)(b[m].real() / M), (float)(b[m].imag() / M)));
+  for (int m = 0; m < M; ++m) {
+    const double a = -2.0 * M_PI * (double)m / (double)M;
+    tw.push_back(make_float2((float)cos(a), (float)sin(a)));
+  }
 }
 
 // the mixed-radix register x/y passes: on unless STEAM_XY_MIXED_RADIX=0 in
@@ -93,10 +183,21 @@ static void set_tiles(ConvGeom& g, int nz_acc, int kahan, size_t& lds_z,
   g.Tz = Tz;
   g.logTz = ilog2(Tz);
   g.Ty = std::max(1, std::min(16, floor_pow2(FFT_LDSE / g.ny)));
-  g.logTy = ilog2(g.Ty);
   int Tx = std::min(16, floor_pow2(std::max<int64_t>(1, FFT_LDSE / g.nx)));
   Tx = std::min(Tx, 1 << ilog2(g.ny));
   g.Tx = std::max(1, Tx);
+  // a Bluestein axis takes one scratch line of bm values after the lines:
+  // fewer lines per workgroup until it fits 64 KiB, else the direct stage
+  constexpr size_t kLds = 64 * 1024;
+  if (g.ay.bm) {
+    while (g.Ty > 1 && (size_t)((g.Ty + 1) * g.ny + g.ay.bm) * 8 > kLds) g.Ty /= 2;
+    if ((size_t)((g.Ty + 1) * g.ny + g.ay.bm) * 8 > kLds) g.ay.bm = 0;
+  }
+  if (g.ax.bm) {
+    while (g.Tx > 1 && (size_t)(g.nx * g.Tx + g.ax.bm) * 8 > kLds) g.Tx /= 2;
+    if ((size_t)(g.nx * g.Tx + g.ax.bm) * 8 > kLds) g.ax.bm = 0;
+  }
+  g.logTy = ilog2(g.Ty);
   g.logTx = ilog2(g.Tx);
   g.nyt_z = (g.ny + g.Tz - 1) / g.Tz;
   g.nyt_x = (g.ny + g.Tx - 1) / g.Tx;
@@ -135,8 +236,8 @@ static void set_tiles(ConvGeom& g, int nz_acc, int kahan, size_t& lds_z,
     lds_z = std::max(lds_z + (size_t)(g.n2 + g.n2 / 2 + 1) * 8, (size_t)3 * 8 * (Z_NTH / 64));
     lds_zinv = lds_z;
   }
-  lds_y = (size_t)(g.Ty + 1) * g.ny * 8;
-  lds_x = (size_t)g.nx * g.Tx * 8;
+  lds_y = (size_t)((g.Ty + 1) * g.ny + g.ay.bm) * 8;
+  lds_x = (size_t)(g.nx * g.Tx + g.ax.bm) * 8;
 }
 
 // model cost of a z transform length: spectral planes nblocks*(n/2+1), each
@@ -250,9 +351,10 @@ extern "C" int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz,
   g.nzc = g.n2 + 1;
   g.L = (int)(n_fft - kz + 1);
   g.nblocks = (int)((nz + g.L - 1) / g.L);
-  factor_axis((int)nx, g.ax);
-  factor_axis((int)ny, g.ay);
+  factor_xy_axis((int)nx, g.ax);
+  factor_xy_axis((int)ny, g.ay);
   factor_axis(g.n2, g.az);
+  g.az.bm = 0;
   const int kah = precision == STEAM_PREC_KAHAN ? 1 : 0;
   set_tiles(g, (int)nz, kah, p->lds_z, p->lds_zinv, p->lds_y, p->lds_x);
   if (p->lds_zinv > 64 * 1024) {
@@ -270,9 +372,10 @@ extern "C" int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz,
     set_tiles(p->gk, 1, 0, p->lds_zk, unused, p->lds_yk, p->lds_xk);
   }
 
-  // twiddles: [x: nx][y: ny][z: n2][r2c: n2+1]
-  const int64_t ntw = nx + ny + g.n2 + g.n2 + 1;
-  std::vector<float2> tw(ntw);
+  // twiddles: [x: nx][y: ny][z: n2][r2c: n2+1], then for a Bluestein x and
+  // y axis [chirp: n][chirp transform / M: M][M-point twiddles: M]
+  const int64_t ntw0 = nx + ny + g.n2 + g.n2 + 1;
+  std::vector<float2> tw(ntw0);
   auto fill = [&](int64_t off, int64_t n, int64_t period) {
     for (int64_t m = 0; m < n; ++m) {
       double a = -2.0 * M_PI * (double)m / (double)period;
@@ -283,6 +386,11 @@ extern "C" int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz,
   fill(nx, ny, ny);
   fill(nx + ny, g.n2, g.n2);
   fill(nx + ny + g.n2, g.n2 + 1, g.nfft);
+  const int64_t bx_off = (int64_t)tw.size();
+  if (g.ax.bm) bluestein_tables(g.ax, tw);
+  const int64_t by_off = (int64_t)tw.size();
+  if (g.ay.bm) bluestein_tables(g.ay, tw);
+  const int64_t ntw = (int64_t)tw.size();
   hipError_t e = hipMalloc(&p->tw_dev, ntw * sizeof(float2));
   if (e != hipSuccess) {
     delete p;
@@ -298,8 +406,16 @@ extern "C" int steam_conv_plan_create(int64_t nx, int64_t ny, int64_t nz,
   g.ay.tw = p->tw_dev + nx;
   g.az.tw = p->tw_dev + nx + ny;
   g.wr2c = p->tw_dev + nx + ny + g.n2;
-  p->gk.ax.tw = g.ax.tw;
-  p->gk.ay.tw = g.ay.tw;
+  auto blue_ptrs = [&](AxisPlan& ap, int64_t off) {
+    if (!ap.bm) return;
+    ap.chirp = p->tw_dev + off;
+    ap.bk = p->tw_dev + off + ap.n;
+    ap.btw = p->tw_dev + off + ap.n + ap.bm;
+  };
+  blue_ptrs(g.ax, bx_off);
+  blue_ptrs(g.ay, by_off);
+  p->gk.ax = g.ax;
+  p->gk.ay = g.ay;
   p->gk.az.tw = g.az.tw;
   p->gk.wr2c = g.wr2c;
 

@@ -381,8 +381,13 @@ __global__ __launch_bounds__(Z_NTH) __attribute__((amdgpu_waves_per_eu(4))) void
 // ---------------------------------------------------------------------------
 // ypass: contiguous y-lines (Ty per workgroup), in place
 // ---------------------------------------------------------------------------
+// threads of a generic x/y pass workgroup: the Bluestein set's length-M
+// scratch transform (M < 2^13) takes 512 at 16 values each
+constexpr int generic_xy_nth(int rs) { return rs == 3 ? 512 : FFT_NTH; }
+
 template <class Src, int RS>
-__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void ypass_kernel(Src src, int inv) {
+__global__ __launch_bounds__(generic_xy_nth(RS)) __attribute__((amdgpu_waves_per_eu(4))) void ypass_kernel(Src src, int inv) {
+  constexpr int NT = generic_xy_nth(RS);
   int wg = blockIdx.x;
   const PassDesc& d = src.get(PASS_Y, wg);
   const ConvGeom& g = d.g;
@@ -396,16 +401,16 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
   for (int t = 0; t < g.Ty; ++t) {
     const int64_t l = l0 + t;
     const float2* srcl = d.S + l * ny;
-    for (int n = tid; n < ny; n += FFT_NTH)
+    for (int n = tid; n < ny; n += NT)
       smem[n * ldn + t] = (l < lines) ? srcl[n] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lines<true, RS>(smem, g.ay, g.Ty, g.logTy, ldn, 1, inv != 0, tid);
+  fft_lines<true, RS, false, FFT_EMAX, NT>(smem, g.ay, g.Ty, g.logTy, ldn, 1, inv != 0, tid);
   for (int t = 0; t < g.Ty; ++t) {
     const int64_t l = l0 + t;
     if (l >= lines) break;
     float2* dstl = d.S + l * ny;
-    for (int n = tid; n < ny; n += FFT_NTH) dstl[n] = smem[n * ldn + t];
+    for (int n = tid; n < ny; n += NT) dstl[n] = smem[n * ldn + t];
   }
 }
 
@@ -413,7 +418,8 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
 // xpass: x-lines in column tiles of Tx; XMODE 0: fwd * K inv, 1: fwd * scale
 // ---------------------------------------------------------------------------
 template <class Src, int XMODE, int RS>
-__global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) void xpass_kernel(Src src) {
+__global__ __launch_bounds__(generic_xy_nth(RS)) __attribute__((amdgpu_waves_per_eu(4))) void xpass_kernel(Src src) {
+  constexpr int NT = generic_xy_nth(RS);
   int wg = xcd_remap(blockIdx.x, gridDim.x);
   const PassDesc& d = src.get(PASS_X, wg);
   const ConvGeom& g = d.g;
@@ -424,31 +430,31 @@ __global__ __launch_bounds__(FFT_NTH) __attribute__((amdgpu_waves_per_eu(4))) vo
   const int nx = g.nx, ny = g.ny, Tx = g.Tx, logTx = g.logTx;
   float2* Sp = d.S + (int64_t)p * nx * ny;
   const int ne = nx * Tx;
-  for (int e = tid; e < ne; e += FFT_NTH) {
+  for (int e = tid; e < ne; e += NT) {
     const int t = e & (Tx - 1);
     const int n = e >> logTx;
     const int y = y0 + t;
     smem[e] = (y < ny) ? Sp[(int64_t)n * ny + y] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lines<true, RS>(smem, g.ax, Tx, logTx, Tx, 1, false, tid);
+  fft_lines<true, RS, false, FFT_EMAX, NT>(smem, g.ax, Tx, logTx, Tx, 1, false, tid);
   if (XMODE == 0) {
     const int kzp = p % g.nzc;
     const float2* Kp = d.K + (int64_t)kzp * nx * ny;
-    for (int e = tid; e < ne; e += FFT_NTH) {
+    for (int e = tid; e < ne; e += NT) {
       const int t = e & (Tx - 1);
       const int n = e >> logTx;
       const int y = y0 + t;
       if (y < ny) smem[e] = cmul(smem[e], Kp[(int64_t)n * ny + y]);
     }
     __syncthreads();
-    fft_lines<true, RS>(smem, g.ax, Tx, logTx, Tx, 1, true, tid);
+    fft_lines<true, RS, false, FFT_EMAX, NT>(smem, g.ax, Tx, logTx, Tx, 1, true, tid);
   } else {
     const float2 f = kernel_bin_factor(d, p % g.nzc);
-    for (int e = tid; e < ne; e += FFT_NTH) smem[e] = cmul(smem[e], f);
+    for (int e = tid; e < ne; e += NT) smem[e] = cmul(smem[e], f);
     __syncthreads();
   }
-  for (int e = tid; e < ne; e += FFT_NTH) {
+  for (int e = tid; e < ne; e += NT) {
     const int t = e & (Tx - 1);
     const int n = e >> logTx;
     const int y = y0 + t;

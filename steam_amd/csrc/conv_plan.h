@@ -65,6 +65,15 @@ inline PassDesc base_desc(const steam_conv_plan* p, void* ws) {
     default: { constexpr int RS_ = 2; __VA_ARGS__; } break; \
   }
 
+// the x/y passes add the Bluestein set
+#define STEAM_RS_DISPATCH_XY(rs, ...)       \
+  switch (rs) {                             \
+    case 0: { constexpr int RS_ = 0; __VA_ARGS__; } break; \
+    case 1: { constexpr int RS_ = 1; __VA_ARGS__; } break; \
+    case 3: { constexpr int RS_ = 3; __VA_ARGS__; } break; \
+    default: { constexpr int RS_ = 2; __VA_ARGS__; } break; \
+  }
+
 #define STEAM_LOGN_DISPATCH(logn, ...)                     \
   switch (logn) {                                          \
     case 6: { constexpr int LN_ = 6; __VA_ARGS__; } break;  \
@@ -89,8 +98,8 @@ inline void launch_ypass(const ConvGeom& g, size_t lds_y, hipStream_t s,
     });
   } else {
     const int rsy = radix_set(g.ay);
-    STEAM_RS_DISPATCH(rsy, hipLaunchKernelGGL((ypass_kernel<SrcT, RS_>), dim3(g.wg_y),
-                                              dim3(FFT_NTH), lds_y, s, src, inv));
+    STEAM_RS_DISPATCH_XY(rsy, hipLaunchKernelGGL((ypass_kernel<SrcT, RS_>), dim3(g.wg_y),
+                                                 dim3(generic_xy_nth(RS_)), lds_y, s, src, inv));
   }
 }
 
@@ -109,9 +118,9 @@ inline void launch_xpass(const ConvGeom& g, size_t lds_x, hipStream_t s,
     });
   } else {
     const int rsx = radix_set(g.ax);
-    STEAM_RS_DISPATCH(rsx, hipLaunchKernelGGL((xpass_kernel<SrcT, XMODE, RS_>),
-                                              dim3(g.wg_x), dim3(FFT_NTH), lds_x, s,
-                                              src));
+    STEAM_RS_DISPATCH_XY(rsx, hipLaunchKernelGGL((xpass_kernel<SrcT, XMODE, RS_>),
+                                                 dim3(g.wg_x), dim3(generic_xy_nth(RS_)), lds_x,
+                                                 s, src));
   }
 }
 

@@ -28,6 +28,15 @@ struct AxisPlan {
   int radix[FFT_MAXST];    // 2,3,4,5,7,8; above 8: direct stage of that radix;
                            // 0: direct DFT of the whole line
   const float2* tw;        // n entries exp(-2 pi i m / n)
+  // Bluestein (chirp-z) form, for an x/y length whose direct stage would be
+  // long: bm > 0 is the smooth length of the circular convolution that
+  // replaces it (bm >= 2n - 1), with its own stages and tables
+  int bm, bnst;
+  int bradix[FFT_MAXST];
+  const float2* btw;       // bm entries exp(-2 pi i m / bm)
+  const float2* chirp;     // n entries exp(-pi i m^2 / n)
+  const float2* bk;        // bm entries: the transform of the wrapped conjugate
+                           // chirp, divided by bm
 };
 
 // constant rotations for the odd radices: cos/sin(2 pi m / R)
@@ -233,11 +242,13 @@ __device__ __noinline__ void fft_direct(float2* buf, int N, int R, int Ns, int T
 }
 
 // Radix sets a kernel is instantiated for: 0 = {2,4,8}, 1 = {2,3,4,5,8},
-// 2 = every stage kind.  The stage loop is unrolled into fixed slots so the
+// 2 = every stage kind, 3 = the Bluestein form (its own kernels, so the
+// others do not carry its registers).  The stage loop is unrolled into fixed slots so the
 // register allocator sees straight-line code (a rolled loop around the radix
 // switch keeps every case's registers live); the rare radix-7 / direct set
 // calls out-of-line stages instead.
 inline int radix_set(const AxisPlan& ap) {
+  if (ap.bm > 0) return 3;  // the Bluestein form (x/y only)
   int rs = 0;
   for (int s = 0; s < ap.nst; ++s) {
     int r = ap.radix[s];
@@ -247,11 +258,11 @@ inline int radix_set(const AxisPlan& ap) {
   return rs;
 }
 
-// Transform T lines of length ap.n held in LDS.  Caller synchronises before.
+// The stages of a length (its radix list).  Caller synchronises before.
 template <bool TFAST, int RSET, bool CALLS = false, int EM = FFT_EMAX, int NTH = FFT_NTH>
-__device__ __forceinline__ void fft_lines(float2* buf, const AxisPlan& ap, int T,
-                                          int logT, int ldn, int ldt, bool inv,
-                                          int tid) {
+__device__ __forceinline__ void fft_lines_stages(float2* buf, const AxisPlan& ap, int T,
+                                                 int logT, int ldn, int ldt, bool inv,
+                                                 int tid) {
   int Ns = 1;
 #pragma unroll
   for (int s = 0; s < 6; ++s) {
@@ -280,6 +291,106 @@ __device__ __forceinline__ void fft_lines(float2* buf, const AxisPlan& ap, int T
     }
     Ns *= (R == 0) ? ap.n : R;
   }
+}
+
+// Bluestein's chirp-z transform of T lines of length N = ap.n, one line at a
+// time through a scratch line of M = ap.bm values placed after the lines
+// (buf + N * ldn; the launch sizes its LDS for it).  With c[m] =
+// exp(-pi i m^2 / N), nk = (n^2 + k^2 - (k - n)^2) / 2 gives
+//   X[k] = c[k] sum_n (x[n] c[n]) conj(c[k - n]),
+// a linear convolution over |k - n| < N, taken circularly at length M >= 2N - 1
+// by forward and unnormalised inverse transforms of M (its stages are 2..8
+// radices) with the conjugate chirp's transform (/ M) from the table.  The
+// inverse transform is conj(DFT(conj x)).  Cost: two length-M transforms per
+// line, against N R complex products for a direct stage of radix R.
+// What the Bluestein form reads of its axis plan, passed by value (a
+// reference into the kernel's argument block made the caller copy the whole
+// descriptor to scratch memory).
+struct BlueArgs {
+  int n, m, nst;
+  int radix[FFT_MAXST];
+  const float2* btw;
+  const float2* chirp;
+  const float2* bk;
+};
+
+__device__ __forceinline__ BlueArgs blue_args(const AxisPlan& ap) {
+  BlueArgs b;
+  b.n = ap.n;
+  b.m = ap.bm;
+  b.nst = ap.bnst;
+#pragma unroll
+  for (int s = 0; s < FFT_MAXST; ++s) b.radix[s] = ap.bradix[s];
+  b.btw = ap.btw;
+  b.chirp = ap.chirp;
+  b.bk = ap.bk;
+  return b;
+}
+
+// The stages of the Bluestein length on one scratch line, inlined (called
+// out of line with 32 values per thread they spilled to scratch memory).
+template <int NTH>
+__device__ __forceinline__ void fft_blue_stages(float2* sc, const BlueArgs& ap, bool inv,
+                                                int tid) {
+  constexpr int EM = 8192 / NTH;  // values per thread: lengths below 2^13
+  const int M = ap.m;
+  int Ns = 1;
+  // a rolled loop: each radix's stage body appears once
+#pragma nounroll
+  for (int s = 0; s < ap.nst; ++s) {
+    const int R = ap.radix[s];
+    switch (R) {
+      case 2: fft_stage<2, false, EM, NTH>(sc, M, 1, 0, 1, 0, Ns, ap.btw, inv, tid); break;
+      case 3: fft_stage<3, false, EM, NTH>(sc, M, 1, 0, 1, 0, Ns, ap.btw, inv, tid); break;
+      case 4: fft_stage<4, false, EM, NTH>(sc, M, 1, 0, 1, 0, Ns, ap.btw, inv, tid); break;
+      case 5: fft_stage<5, false, EM, NTH>(sc, M, 1, 0, 1, 0, Ns, ap.btw, inv, tid); break;
+      case 7: fft_stage<7, false, EM, NTH>(sc, M, 1, 0, 1, 0, Ns, ap.btw, inv, tid); break;
+      default: fft_stage<8, false, EM, NTH>(sc, M, 1, 0, 1, 0, Ns, ap.btw, inv, tid); break;
+    }
+    Ns *= R;
+  }
+}
+
+template <int NTH>
+__device__ __noinline__ void fft_bluestein(float2* buf, BlueArgs ap, int T, int ldn, int ldt,
+                                           bool inv, int tid) {
+  const int N = ap.n, M = ap.m;
+  float2* sc = buf + (int64_t)N * ldn;
+  for (int t = 0; t < T; ++t) {
+    float2* line = buf + t * ldt;
+    for (int m = tid; m < M; m += NTH) {
+      float2 v = make_float2(0.f, 0.f);
+      if (m < N) {
+        float2 x = line[m * ldn];
+        if (inv) x.y = -x.y;
+        v = cmul(x, ap.chirp[m]);
+      }
+      sc[m] = v;
+    }
+    __syncthreads();
+    fft_blue_stages<NTH>(sc, ap, false, tid);
+    for (int m = tid; m < M; m += NTH) sc[m] = cmul(sc[m], ap.bk[m]);
+    __syncthreads();
+    fft_blue_stages<NTH>(sc, ap, true, tid);
+    for (int k = tid; k < N; k += NTH) {
+      float2 y = cmul(sc[k], ap.chirp[k]);
+      if (inv) y.y = -y.y;
+      line[k * ldn] = y;
+    }
+    __syncthreads();
+  }
+}
+
+// Transform T lines of length ap.n held in LDS.  Caller synchronises before.
+// Radix set 3 (an axis with a Bluestein form, ap.bm > 0) takes that form.
+template <bool TFAST, int RSET, bool CALLS = false, int EM = FFT_EMAX, int NTH = FFT_NTH>
+__device__ __forceinline__ void fft_lines(float2* buf, const AxisPlan& ap, int T,
+                                          int logT, int ldn, int ldt, bool inv,
+                                          int tid) {
+  if constexpr (RSET == 3)
+    fft_bluestein<NTH>(buf, blue_args(ap), T, ldn, ldt, inv, tid);
+  else
+    fft_lines_stages<TFAST, RSET, CALLS, EM, NTH>(buf, ap, T, logT, ldn, ldt, inv, tid);
 }
 
 }  // namespace steam

@@ -42,6 +42,12 @@ def _rel(a, b):
     ((22, 26, 8), (5, 5, 3)), ((260, 16, 6), (7, 7, 3)), ((51, 88, 6), (9, 5, 3)),
     ((2060, 4, 4), (13, 3, 3)), ((8, 286, 5), (3, 13, 3)), ((37, 41, 7), (5, 5, 5)),
     ((143, 8, 4), (9, 3, 3)), ((1034, 4, 4), (5, 3, 3)), ((8, 1573, 3), (3, 7, 3)),
+    # lengths with a prime factor of 128 or more: the Bluestein form (x, y,
+    # both, a folded kernel; 2053 and 1031 prime, 1548 = 4 * 3^2 * 43, 1542 =
+    # 2 * 3 * 257)
+    ((257, 8, 4), (5, 3, 3)), ((8, 263, 5), (3, 5, 3)), ((131, 137, 4), (7, 9, 3)),
+    ((2053, 4, 4), (13, 3, 3)), ((6, 1031, 3), (3, 9, 3)), ((1542, 6, 3), (5, 3, 3)),
+    ((139, 12, 5), (300, 5, 3)),
 ])
 def test_convolution_matches_oracle(fs, ks):
     from steam_amd.utils import convolve_fft_xy_oa_z
