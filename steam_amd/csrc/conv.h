@@ -25,6 +25,9 @@ struct ConvGeom {
   int tile_z;           // single-block z passes with vector tile loads
   int zreg;             // single-block register z passes (compiled n2)
   int wg_zr;            // workgroups of the register z passes
+  int twl_y, twl_x;     // generic y / x pass with a direct or radix-7 stage:
+                        // the axis twiddle table is staged in LDS after the
+                        // lines (its reads are lane-scattered there)
   AxisPlan ax, ay, az;
   const float2* wr2c;   // n2+1 entries exp(-2 pi i k / nfft)
 };

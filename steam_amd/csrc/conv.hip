@@ -47,11 +47,12 @@ static void factor_axis(int n, AxisPlan& ap) {
 
 // A direct stage of radix R costs R complex products per element; the
 // Bluestein form costs two transforms of a smooth length M >= 2n - 1 per
-// transform, in the LDS-staged passes.  Measured x passes (140 x 97 planes):
-// 2060 = 4 5 103 took 5.27 ms direct and 4.77 ms Bluestein, 1036 = 4 7 37
-// 0.82 ms direct and 3.26 ms Bluestein, so it takes over from radix 128 on,
-// where a direct stage would cost hundreds of products per element (a
-// prime line: n of them).  STEAM_BLUESTEIN=0 in the environment keeps the
+// transform, in the LDS-staged passes.  Measured x passes (140 x 97 planes,
+// twiddles read from LDS): 2060 = 4 5 103 took 3.04 ms direct and 4.77 ms
+// Bluestein, 1036 = 4 7 37 0.65 ms direct and 3.26 ms Bluestein, a 2053-point
+// (prime) line 117.8 ms direct and 5.1 ms Bluestein; so it takes over from
+// radix 128 on, where a direct stage costs hundreds of products per element
+// (a prime line: n of them).  STEAM_BLUESTEIN=0 in the environment keeps the
 // direct stages (read once; for measurement).
 constexpr int BLUE_MIN_RADIX = 128;
 
@@ -238,6 +239,12 @@ static void set_tiles(ConvGeom& g, int nz_acc, int kahan, size_t& lds_z,
   }
   lds_y = (size_t)((g.Ty + 1) * g.ny + g.ay.bm) * 8;
   lds_x = (size_t)(g.nx * g.Tx + g.ax.bm) * 8;
+  // the generic passes of a length with a direct or radix-7 stage read the
+  // axis twiddles from LDS when they fit beside the lines
+  g.twl_y = radix_set(g.ay) == 2 && lds_y + (size_t)g.ny * 8 <= kLds;
+  g.twl_x = radix_set(g.ax) == 2 && lds_x + (size_t)g.nx * 8 <= kLds;
+  if (g.twl_y) lds_y += (size_t)g.ny * 8;
+  if (g.twl_x) lds_x += (size_t)g.nx * 8;
 }
 
 // model cost of a z transform length: spectral planes nblocks*(n/2+1), each

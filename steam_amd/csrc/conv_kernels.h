@@ -404,8 +404,14 @@ __global__ __launch_bounds__(generic_xy_nth(RS)) __attribute__((amdgpu_waves_per
     for (int n = tid; n < ny; n += NT)
       smem[n * ldn + t] = (l < lines) ? srcl[n] : make_float2(0.f, 0.f);
   }
+  AxisPlan ap = g.ay;
+  if (RS == 2 && g.twl_y) {
+    float2* twl = smem + ny * ldn;
+    for (int n = tid; n < ny; n += NT) twl[n] = g.ay.tw[n];
+    ap.tw = twl;
+  }
   __syncthreads();
-  fft_lines<true, RS, false, FFT_EMAX, NT>(smem, g.ay, g.Ty, g.logTy, ldn, 1, inv != 0, tid);
+  fft_lines<true, RS, false, FFT_EMAX, NT>(smem, ap, g.Ty, g.logTy, ldn, 1, inv != 0, tid);
   for (int t = 0; t < g.Ty; ++t) {
     const int64_t l = l0 + t;
     if (l >= lines) break;
@@ -436,8 +442,14 @@ __global__ __launch_bounds__(generic_xy_nth(RS)) __attribute__((amdgpu_waves_per
     const int y = y0 + t;
     smem[e] = (y < ny) ? Sp[(int64_t)n * ny + y] : make_float2(0.f, 0.f);
   }
+  AxisPlan ap = g.ax;
+  if (RS == 2 && g.twl_x) {
+    float2* twl = smem + ne;
+    for (int n = tid; n < nx; n += NT) twl[n] = g.ax.tw[n];
+    ap.tw = twl;
+  }
   __syncthreads();
-  fft_lines<true, RS, false, FFT_EMAX, NT>(smem, g.ax, Tx, logTx, Tx, 1, false, tid);
+  fft_lines<true, RS, false, FFT_EMAX, NT>(smem, ap, Tx, logTx, Tx, 1, false, tid);
   if (XMODE == 0) {
     const int kzp = p % g.nzc;
     const float2* Kp = d.K + (int64_t)kzp * nx * ny;
@@ -448,7 +460,7 @@ __global__ __launch_bounds__(generic_xy_nth(RS)) __attribute__((amdgpu_waves_per
       if (y < ny) smem[e] = cmul(smem[e], Kp[(int64_t)n * ny + y]);
     }
     __syncthreads();
-    fft_lines<true, RS, false, FFT_EMAX, NT>(smem, g.ax, Tx, logTx, Tx, 1, true, tid);
+    fft_lines<true, RS, false, FFT_EMAX, NT>(smem, ap, Tx, logTx, Tx, 1, true, tid);
   } else {
     const float2 f = kernel_bin_factor(d, p % g.nzc);
     for (int e = tid; e < ne; e += NT) smem[e] = cmul(smem[e], f);
