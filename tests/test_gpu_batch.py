@@ -262,3 +262,26 @@ def test_crops_store_a_box_of_the_whole_grid_bit_for_bit():
         np.testing.assert_array_equal(inst.flux.cpu().numpy(), ref)
     with pytest.raises(ValueError):
         b.set_crops([(torch.empty((8, 8, 8), device='cuda'), (60, 0, 0)), None, None])
+
+
+def test_members_on_bluestein_axes_equal_their_single_advance():
+    """Members whose x or y length is prime (the Bluestein form of the x/y
+    passes, 512-thread workgroups) beside an ordinary member: the batched
+    advance equals each member's single advance bit for bit."""
+    from steam_amd import simulate as S
+    from steam_amd.batch import BatchedAdvance, Instance
+    kern = S._turbulon_envelope(1, .5, .5, .5, as_tensor=True)
+    rng = np.random.default_rng(5)
+    insts = []
+    for i, shape in enumerate([(40, 131, 12), (137, 24, 10), (32, 32, 24)]):
+        flux = torch.from_numpy(np.abs(1 + 0.2 * rng.standard_normal(shape))
+                                .astype(np.float32)).cuda()
+        key = M.class_key(np.random.SeedSequence(500 + i))
+        region = S.NoiseRegion(tuple(np.uint64(k) for k in key), (0, 0, 0), shape)
+        insts.append(Instance(flux, kern, region, 0.3, alpha=1.8))
+    batch = BatchedAdvance(insts, autotune=False)
+    refs = [_single(inst, op, sp) for inst, op, sp in zip(insts, batch.ops, batch.specs)]
+    batch.advance()
+    torch.cuda.synchronize()
+    for inst, ref in zip(insts, refs):
+        np.testing.assert_array_equal(inst.flux.cpu().numpy(), ref.cpu().numpy())
