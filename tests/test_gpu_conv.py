@@ -141,6 +141,38 @@ def test_identity_zero_and_shift():
                                atol=1e-5)
 
 
+def test_unit_kernel_zero_kernel_and_z_boundary():
+    """The reference's edge cases (tests/test_convolution.py:79-163): a 1x1x1
+    kernel scales, a zero kernel gives exact zeros, z is zero-padded (a z roll
+    is not an output roll, and a spike at z = 0 spreads over fewer levels than
+    an interior one).  The FFT leaves rounding residue of order 1e-8 outside a
+    spike's support, so support is counted above 1e-5."""
+    from steam_amd.utils import convolve_fft_xy_oa_z
+    rng = np.random.default_rng(99)
+    f = rng.standard_normal((8, 8, 6)).astype(np.float32)
+    np.testing.assert_allclose(
+        convolve_fft_xy_oa_z(f, np.array([[[2.5]]], np.float32)), 2.5 * f,
+        rtol=1e-5, atol=1e-5)
+    np.testing.assert_array_equal(
+        convolve_fft_xy_oa_z(f, np.zeros((3, 3, 3), np.float32)), 0.0)
+
+    f = rng.standard_normal((8, 8, 12)).astype(np.float32)
+    k = rng.standard_normal((3, 3, 5)).astype(np.float32)
+    rolled = np.roll(convolve_fft_xy_oa_z(f, k), 2, axis=2)
+    assert not np.allclose(convolve_fft_xy_oa_z(np.roll(f, 2, axis=2), k), rolled,
+                           atol=1e-3)
+
+    k = np.ones((1, 1, 5), np.float32)
+    support = []
+    for z in (10, 0):
+        spike = np.zeros((8, 8, 20), np.float32)
+        spike[4, 4, z] = 1.0
+        out = convolve_fft_xy_oa_z(spike, k)
+        np.testing.assert_allclose(out, M.conv_fft(spike, k), atol=1e-6)
+        support.append(int(np.count_nonzero(np.abs(out[4, 4, :]) > 1e-5)))
+    assert support == [5, 3]
+
+
 def test_device_tensor_in_device_tensor_out():
     from steam_amd.utils import convolve_fft_xy_oa_z
     rng = np.random.default_rng(8)
