@@ -19,9 +19,16 @@ static std::mutex g_prof_mu;
 static std::vector<ProfRec> g_prof;
 static std::vector<hipEvent_t> g_pending;  // begin events awaiting their end
 
+// Timing-only events: without the system-scope fence a record does not
+// write back and invalidate the caches, which would otherwise be counted in
+// the span of the launch it brackets and slow the launch after it.
+static hipError_t prof_event(hipEvent_t* e) {
+  return hipEventCreateWithFlags(e, hipEventDisableSystemFence);
+}
+
 void prof_begin(int fam, hipStream_t s) {
   hipEvent_t e;
-  if (hipEventCreate(&e) != hipSuccess) return;
+  if (prof_event(&e) != hipSuccess) return;
   (void)hipEventRecord(e, s);
   std::lock_guard<std::mutex> lk(g_prof_mu);
   g_pending.push_back(e);
@@ -30,7 +37,7 @@ void prof_begin(int fam, hipStream_t s) {
 
 void prof_end(int fam, hipStream_t s) {
   hipEvent_t e;
-  if (hipEventCreate(&e) != hipSuccess) return;
+  if (prof_event(&e) != hipSuccess) return;
   (void)hipEventRecord(e, s);
   std::lock_guard<std::mutex> lk(g_prof_mu);
   if (g_pending.empty()) return;
